@@ -1,0 +1,236 @@
+/*
+ * vrgi.h — C ABI of the MI355X (gfx950) voxel path-tracing hot path.
+ *
+ * This ABI replaces the public Rust API of the reference's render_system
+ * (pimpale/vulkan-voxel-restir-gi):
+ *   Scene<K> / SceneUploader / SceneUploadedObjectHandle  (src/render_system/scene.rs)
+ *   Renderer                                               (src/render_system/interactive_rendering.rs)
+ *   Vertex3D / LightVertex3D / InstanceData                (src/render_system/vertex.rs)
+ *   BvhNode                                                (src/render_system/bvh/mod.rs)
+ *   build_bl_bvh / build_tl_bvh                            (src/render_system/bvh/build.rs)
+ * Every entry point returns 0 on success or a negative VRGI_E_* code; the
+ * message of the last failure on the calling thread is vrgi_last_error().
+ * Plain pointers and sizes only: no torch, no HIP types.
+ */
+#ifndef VRGI_H
+#define VRGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VRGI_ABI_VERSION 1u
+
+/* status codes */
+#define VRGI_OK 0
+#define VRGI_E_INVALID (-1)   /* bad argument (null, size, unknown key, n_verts % 3 != 0) */
+#define VRGI_E_DEVICE (-2)    /* HIP runtime failure (allocation, launch) */
+#define VRGI_E_STATE (-3)     /* call not valid in the current state */
+#define VRGI_E_LIMIT (-4)     /* size beyond a documented maximum */
+
+/* Vertex3D (vertex.rs:3-12): 24 bytes, non-indexed, 3 per triangle. */
+typedef struct vrgi_vertex3d {
+    float position[3];
+    uint32_t t; /* material slot = block * 6 + face (block.rs:116-119) */
+    float uv[2];
+} vrgi_vertex3d;
+
+/* LightVertex3D (vertex.rs:23-28): 16 bytes, in light BL-BVH leaf order. */
+typedef struct vrgi_light_vertex3d {
+    float position[3];
+    uint32_t bvh_node_idx;
+} vrgi_light_vertex3d;
+
+/* BvhNode (bvh/mod.rs:6-18): 40 bytes, scalar layout. left == 0xFFFFFFFF marks a
+ * leaf whose right_or_prim holds the primitive id; both 0xFFFFFFFF is the dummy
+ * node of an empty BVH (bvh/mod.rs:27-38). */
+typedef struct vrgi_bvh_node {
+    uint32_t left;
+    uint32_t right_or_prim;
+    float min[3];
+    float power;
+    float max[3];
+    uint32_t parent;
+} vrgi_bvh_node;
+
+/* One material slot = the three constant texels the reference samples per slot
+ * (raytrace.rs:465-478): reflectivity rgba (alpha = 1 - transmit probability),
+ * emissivity rgb (scaled by 1000 in the shader), metallicity (r). */
+typedef struct vrgi_material {
+    float reflectivity[4];
+    float emissivity[4];
+    float metallicity[4];
+} vrgi_material;
+
+/* Renderer configuration. vrgi_config_reference() fills the reference's constants
+ * (interactive_rendering.rs:766-767, restir_spatial_resampling.rs:313-314). */
+typedef struct vrgi_config {
+    uint32_t width, height;       /* render extent (Renderer::rebuild) */
+    uint32_t num_bounces;         /* B; reference 3 */
+    uint32_t rr_start_bounce;     /* Russian roulette from this bounce on; 0xFFFFFFFF = off (reference) */
+    float rr_min_survival;        /* lower clamp of the survival probability */
+    uint32_t restir_mode;         /* 0 = ReSTIR GI (reference), 1 = ReSTIR DI */
+    float restir_spatial_radius;  /* neighbour jitter = radius * (u - 0.5); reference 15 */
+    uint32_t di_candidates;       /* DI mode: light candidates per pixel */
+    uint32_t seed;                /* salt of the counter-based per-frame seeds */
+    uint32_t flags;               /* VRGI_FLAG_* */
+} vrgi_config;
+
+#define VRGI_FLAG_REBUILD_EVERY_FRAME 1u /* rebuild the geometry BVH on every render() */
+#define VRGI_FLAG_KEEP_BOUNCES 2u        /* keep per-bounce buffers readable after render() */
+
+typedef struct vrgi_camera {
+    float eye[3], front[3], right[3], up[3]; /* Camera::eye_front_right_up (camera.rs:115-122) */
+} vrgi_camera;
+
+/* RenderingPreferences (camera.rs:37-56). */
+typedef struct vrgi_prefs {
+    uint32_t nee_type;                  /* 0 none, 1 NEE every bounce, 2 NEE at bounce 1 only */
+    uint32_t debug_view;                /* 0 path tracer, 1 ReSTIR, 2 debug */
+    uint32_t restir_spatial_iterations; /* spatial neighbours, 0..20 */
+} vrgi_prefs;
+
+/* Result of one render(). Device pointers stay valid until the next render/resize. */
+typedef struct vrgi_frame_out {
+    const float *radiance;  /* [H][W][3] plain path tracer, L_out at bounce 0 */
+    const float *restir;    /* [H][W][3] ReSTIR final radiance */
+    const uint8_t *bgra8;   /* [H][W][4] postprocessed view */
+    uint64_t frame_index;   /* frame counter of this frame (ReSTIR slot = frame_index % 3) */
+} vrgi_frame_out;
+
+typedef struct vrgi_frame_stats {
+    uint64_t frames;        /* frames counted since the last reset */
+    uint64_t rays_closest;  /* closest-hit rays traced (live rays, all bounces) */
+    uint64_t rays_light;    /* light-geometry any-hit rays (NEE pdf) */
+    uint64_t bvh_nodes;     /* nodes of the current geometry BVH */
+    uint32_t bvh_depth;     /* depth of the current geometry BVH */
+    uint32_t pad;
+} vrgi_frame_stats;
+
+typedef struct vrgi_ctx vrgi_ctx;
+typedef struct vrgi_object vrgi_object;
+
+const char *vrgi_last_error(void);
+uint32_t vrgi_abi_version(void);
+void vrgi_config_reference(vrgi_config *cfg, uint32_t width, uint32_t height);
+
+/* Scene::new + Renderer::new (scene.rs:112, interactive_rendering.rs:447).
+ * materials: n_slots texture slots; texture_luminances: utils::get_texture_luminances
+ * (utils.rs:223-235), one per slot, feeding light power = luminance * area. */
+int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials,
+                const float *texture_luminances, uint32_t n_slots, int device, vrgi_ctx **out);
+int vrgi_destroy(vrgi_ctx *ctx);
+
+/* SceneUploader::upload_object (scene.rs:547-725). Thread-safe. n_verts == 0
+ * yields the Empty handle (*out == NULL, status 0); n_verts % 3 != 0 is VRGI_E_INVALID.
+ * Builds the object's light BL-BVH (build_bl_bvh over its emissive triangles). */
+int vrgi_upload_object(vrgi_ctx *ctx, const vrgi_vertex3d *verts, size_t n_verts,
+                       vrgi_object **out);
+int vrgi_object_retain(vrgi_object *obj);
+int vrgi_object_release(vrgi_object *obj);
+/* light BL-BVH of an uploaded object: node count and host copy (test hook). */
+int vrgi_object_light_bvh(vrgi_object *obj, vrgi_bvh_node *dst, size_t cap, size_t *n_nodes);
+
+/* Scene::add_object / update_object / remove_object (scene.rs:150-225).
+ * transform: InstanceData.transform, mat4x3 column-major (4 columns of xyz). */
+int vrgi_scene_add(vrgi_ctx *ctx, uint32_t key, vrgi_object *obj, const float transform[12]);
+int vrgi_scene_update(vrgi_ctx *ctx, uint32_t key, const float transform[12]);
+int vrgi_scene_remove(vrgi_ctx *ctx, uint32_t key);
+/* Scene::dispose_old_objects (scene.rs:227). */
+int vrgi_dispose_old(vrgi_ctx *ctx);
+
+/* Renderer::rebuild(extent) (interactive_rendering.rs:818). */
+int vrgi_resize(vrgi_ctx *ctx, uint32_t width, uint32_t height);
+
+/* Renderer::render (interactive_rendering.rs:952-1669): (re)builds the BVHs when
+ * the scene changed, then raygen, B x (traverse + shade + NEE), nee_pdf,
+ * outgoing radiance, ReSTIR temporal / spatial / finalize, postprocess.
+ * Work is queued on the context's stream; vrgi_sync waits for it. */
+int vrgi_render(vrgi_ctx *ctx, const vrgi_camera *cam, const vrgi_prefs *prefs,
+                vrgi_frame_out *out);
+int vrgi_sync(vrgi_ctx *ctx);
+int vrgi_stats(vrgi_ctx *ctx, vrgi_frame_stats *st, int reset);
+
+/* ---- readback / test hooks (host copies; each synchronises the stream) ---- */
+enum vrgi_buffer_id {
+    VRGI_BUF_GEOM_BVH = 0,       /* vrgi_bvh_node[2N-1], geometry BVH (build_bl_bvh semantics) */
+    VRGI_BUF_GEOM_PRIM_ORDER,    /* uint32[N] global triangle id per leaf position */
+    VRGI_BUF_LIGHT_TL_BVH,       /* vrgi_bvh_node[], light top-level BVH (build_tl_bvh) */
+    VRGI_BUF_LIGHT_GEOM_BVH,     /* vrgi_bvh_node[], BVH over world-space light triangles */
+    VRGI_BUF_PATH_LEN,           /* uint32[P] live bounces per pixel */
+    VRGI_BUF_BOUNCE_ORIGIN,      /* float[B][P][3] origin written by bounce b (= ray origin b+1) */
+    VRGI_BUF_BOUNCE_DIRECTION,   /* float[B][P][3] direction written by bounce b */
+    VRGI_BUF_BOUNCE_NORMAL,      /* float[B][P][3] */
+    VRGI_BUF_BOUNCE_EMISSIVITY,  /* float[B][P][3] */
+    VRGI_BUF_BOUNCE_REFLECTIVITY,/* float[B][P][3] */
+    VRGI_BUF_BOUNCE_MIS_WEIGHT,  /* float[B][P] */
+    VRGI_BUF_BOUNCE_BSDF_PDF,    /* float[B][P] */
+    VRGI_BUF_BOUNCE_NEE_PDF,     /* float[B][P] */
+    VRGI_BUF_RADIANCE,           /* float[P][3] */
+    VRGI_BUF_RESTIR,             /* float[P][3] */
+    VRGI_BUF_TEMPORAL_RESERVOIR, /* float[P][20] slot (frame % 3) after the last frame */
+    VRGI_BUF_SPATIAL_RESERVOIR,  /* float[P][20] */
+    VRGI_BUF_BGRA8,              /* uint8[P][4] */
+    VRGI_BUF_HIT_PRIM,           /* uint32[B][P] global triangle id hit at bounce b (0xFFFFFFFF miss) */
+    VRGI_BUF_HIT_T,              /* float[B][P] hit distance at bounce b */
+    VRGI_BUF_DI_LIGHT,           /* uint32[P] DI reservoir light (global light triangle id) */
+    VRGI_BUF_DI_WEIGHTS,         /* float[P][4] DI reservoir (w_sum, W, M, p_hat) */
+    VRGI_BUF_COUNT
+};
+/* bytes needed for buffer `which` in the current state (0 if not available). */
+int vrgi_buffer_size(vrgi_ctx *ctx, int which, size_t *bytes);
+int vrgi_read_buffer(vrgi_ctx *ctx, int which, void *dst, size_t bytes);
+
+/* ---- standalone operators (the drop-in pieces, usable without a scene) ---- */
+
+/* build_bl_bvh (build.rs:249-372) on the GPU over n triangles (tri_xyz: 9n floats),
+ * prim_luminances (n), prim_index_ids (n). nodes: 2n-1 entries; prim_order: n
+ * entries (prim index per leaf position); root_box[6] and root_power as the
+ * function's returned (Aabb, f32). Host pointers. */
+int vrgi_build_bl_bvh(vrgi_ctx *ctx, const float *tri_xyz, const float *prim_luminances,
+                      const uint32_t *prim_index_ids, uint64_t n, vrgi_bvh_node *nodes,
+                      uint32_t *prim_order, float root_box[6], float *root_power);
+
+/* build_tl_bvh (build.rs:374-480): m instance boxes (aabbs: 6m = min xyz, max xyz),
+ * transforms (12m, column-major mat4x3), powers (m), ids (m). nodes: 2m-1. */
+int vrgi_build_tl_bvh(vrgi_ctx *ctx, const float *transforms, const float *aabbs,
+                      const float *powers, const uint32_t *ids, uint64_t m,
+                      vrgi_bvh_node *nodes);
+
+/* Closest hit of n rays against the current scene's geometry BVH (replaces the
+ * ray query of raytrace.rs:354-388). t in [tmin, tmax]; ties broken by the lower
+ * global triangle id. Outputs: hit triangle id (0xFFFFFFFF = miss), t, and the
+ * barycentrics (u, v) of vertices 1 and 2. Host pointers. */
+int vrgi_trace_closest(vrgi_ctx *ctx, const float *origins, const float *dirs, uint64_t n,
+                       float tmin, float tmax, uint32_t *hit_prim, float *hit_t,
+                       float *hit_uv);
+
+/* ---- synthetic scenes (caller side: the game_system worldgen + chunk mesher) ---- */
+
+/* Seeded voxel world of one of the benchmark workloads: kind 0 = 64x64x64
+ * value-noise terrain (4 block types, 1% emissive surface voxels); kind 1 =
+ * 256x128x256 terrain + 200 boxes (0.5% emissive). blocks: u16 per voxel,
+ * index x*(dy*dz) + y*dz + z; air = n_block_types. */
+int vrgi_world_dims(int kind, uint32_t dims[3], uint32_t *n_block_types);
+int vrgi_world_generate(int kind, uint64_t seed, uint16_t *blocks);
+/* material table of a generated world (6 slots per block type) and luminances. */
+int vrgi_world_materials(int kind, uint64_t seed, vrgi_material *mats, float *lum,
+                         uint32_t cap_slots, uint32_t *n_slots);
+/* translucent flag per block type of a generated world (air excluded). */
+int vrgi_world_translucent(int kind, uint8_t *translucent, uint32_t cap);
+
+/* chunk::gen_mesh (chunk.rs:158-292) for the 32^3 chunk at chunk coordinates
+ * (cx, cy, cz) of a world of dims; voxels outside the world are air. Writes at
+ * most cap vertices, returns the vertex count in *n_out (chunk-local positions). */
+int vrgi_gen_mesh_chunk(const uint16_t *blocks, const uint32_t dims[3],
+                        uint32_t n_block_types, const uint8_t *translucent, uint32_t cx,
+                        uint32_t cy, uint32_t cz, vrgi_vertex3d *out, size_t cap,
+                        size_t *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VRGI_H */

@@ -1,0 +1,115 @@
+// oracle.h — CPU restatement of the reference's hot path (TEST INFRASTRUCTURE).
+//
+// This is the checker, not the product: only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load it. It restates, in plain C++ with FP32
+// exact semantics (no FMA contraction, correctly rounded / and sqrt):
+//   - the per-pixel RNG                raytrace.rs:108-149
+//   - Aabb and the binned-SAH builder  bvh/aabb.rs:4-92, bvh/build.rs:36-480
+//   - the exposed-face mesher          game_system/chunk.rs:158-292
+//   - light gather / light vertices    scene.rs:509-571, utils.rs:223-235
+//   - light TL-BVH + InstanceData      scene.rs:243-338
+//   - raygen, raytrace, nee_pdf, outgoing_radiance, ReSTIR temporal / spatial /
+//     finalize, postprocess            shader/*.rs
+// The Vulkan ray queries (raytrace.rs:354-388, nee_pdf.rs:305-334) are replaced by
+// the Woop-Benthin-Wald watertight ray/triangle test (JCGT 2(1), 2013) over a
+// binned-SAH BVH built with build_bl_bvh semantics; ties between equal t are broken
+// by the lower global triangle id.
+//
+// Parity status per function is stated in DESIGN.md's oracle table; the functions
+// whose arithmetic lives outside /root/reference (the `partition` 0.1.2 crate order,
+// Vulkan barycentrics, GLSL built-ins) are "parity unpinned" against the reference.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+extern "C" {
+
+// ---------- RNG (raytrace.rs:108-149) ----------
+uint32_t or_murmur3_combine(uint32_t h, uint32_t k);
+uint32_t or_murmur3_finalize(uint32_t h);
+float or_float_construct(uint32_t m);
+// deterministic sin / cos used by both sides (own polynomial, see oracle.cpp)
+void or_sincos(float x, float *s, float *c);
+
+// ---------- BVH (build.rs) ----------
+// nodes are 40-byte BvhNode records (bvh/mod.rs:6-18)
+int or_build_bl_bvh(uint64_t n, const float *tri_xyz, const float *lum, const uint32_t *ids,
+                    void *nodes_out, uint32_t *prim_order_out, float *root_box6,
+                    float *root_power, uint32_t *depth_out);
+int or_build_tl_bvh(uint64_t m, const float *xforms12, const float *aabbs6, const float *powers,
+                    const uint32_t *ids, void *nodes_out);
+// brute-force check of find_best_plane (build.rs:45-128) on the range of the root
+int or_find_best_plane(uint64_t n, const float *tri_xyz, uint32_t *dim_out, float *split_out);
+
+// ---------- mesher (chunk.rs:158-292) ----------
+int or_gen_mesh_chunk(const uint16_t *blocks, const uint32_t dims[3], uint32_t n_types,
+                      const uint8_t *translucent, uint32_t cx, uint32_t cy, uint32_t cz,
+                      void *out_verts, uint64_t cap, uint64_t *n_out);
+
+// ---------- scene + renderer ----------
+typedef struct or_scene or_scene;
+or_scene *or_scene_create(const void *materials /*vrgi_material[n]*/, const float *lum,
+                          uint32_t n_slots);
+void or_scene_destroy(or_scene *s);
+int or_scene_add(or_scene *s, uint32_t key, const void *verts /*vrgi_vertex3d*/, uint64_t n,
+                 const float xform[12]);
+int or_scene_remove(or_scene *s, uint32_t key);
+int or_scene_build(or_scene *s); // flatten + geometry BVH + light BVHs
+uint64_t or_scene_count(or_scene *s, int what); // 0 tris, 1 light tris, 2 instances, 3 tl nodes, 4 lgeom nodes
+int or_scene_read(or_scene *s, int what, void *dst, uint64_t bytes);
+
+int or_scene_light_bvh(or_scene *s, uint32_t key, void *dst, uint64_t cap, uint64_t *n);
+int or_trace_closest(or_scene *s, uint64_t n, const float *o, const float *d, float tmin,
+                     float tmax, int brute, uint32_t *hit, float *t, float *uv,
+                     uint64_t *visited_nodes);
+int or_trace_lights(or_scene *s, const float *o, const float *d, float tmin, float tmax,
+                    uint32_t *hits, uint32_t cap, uint32_t *n_hits, float *uvs);
+int or_traverse_light_bvh(or_scene *s, const float *point, const float *normal, uint32_t seed,
+                          uint32_t *inst, uint32_t *prim, float *prob, float *importance);
+float or_reverse_traverse_light_bvh(or_scene *s, const float *point, const float *normal,
+                                    uint32_t inst, uint32_t bl_node);
+
+typedef struct or_config {
+    uint32_t width, height, num_bounces, rr_start_bounce;
+    float rr_min_survival;
+    uint32_t restir_mode;
+    float restir_spatial_radius;
+    uint32_t di_candidates, seed, flags;
+} or_config;
+typedef struct or_camera { float eye[3], front[3], right[3], up[3]; } or_camera;
+typedef struct or_prefs { uint32_t nee_type, debug_view, restir_spatial_iterations; } or_prefs;
+
+// Dense reference-layout outputs of one frame. Any pointer may be NULL.
+typedef struct or_frame {
+    float *origin;      // [B+1][P][3]
+    float *direction;   // [B+1][P][3]
+    float *normal;      // [B][P][3]
+    float *emissivity;  // [B][P][3]
+    float *reflectivity;// [B][P][3]
+    float *mis_weight;  // [B][P]
+    float *bsdf_pdf;    // [B][P]
+    float *nee_pdf;     // [B][P]
+    float *l_out;       // [B][P][3]
+    float *q_omega;     // [B][P]
+    uint32_t *hit_prim; // [B][P]
+    float *hit_t;       // [B][P]
+    uint32_t *path_len; // [P]
+    float *temporal;    // [P][20] (slot frame % 3 after the update)
+    float *spatial;     // [P][20]
+    float *restir;      // [P][3]
+    float *debug;       // [P][3]
+    uint8_t *bgra8;     // [P][4]
+    uint32_t *di_light; // [P]
+    float *di_weights;  // [P][4]
+} or_frame;
+
+typedef struct or_renderer or_renderer;
+or_renderer *or_renderer_create(const or_config *cfg);
+void or_renderer_destroy(or_renderer *r);
+// renders frame number `frame` (ReSTIR slot frame % 3); threads: OpenMP threads (0 = default)
+int or_render(or_renderer *r, or_scene *s, const or_camera *cam, const or_prefs *prefs,
+              uint64_t frame, or_frame *out, int threads);
+// rows [y0, y1) only (for the bounded CPU baseline sample); no ReSTIR
+int or_render_rows(or_renderer *r, or_scene *s, const or_camera *cam, const or_prefs *prefs,
+                   uint64_t frame, uint32_t y0, uint32_t y1, float *radiance, int threads);
+}

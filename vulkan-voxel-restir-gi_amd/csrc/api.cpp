@@ -1,0 +1,936 @@
+// api.cpp — host side of the library: the C ABI of include/vrgi.h over HIP.
+//
+// Mirrors the reference's render_system interface: SceneUploader::upload_object
+// (scene.rs:547-725), Scene::{add,update,remove}_object / dispose_old_objects
+// (scene.rs:150-231), Scene::get_tlas's light part (scene.rs:234-358) and
+// Renderer::{rebuild, render} (interactive_rendering.rs:818-1669). All device work
+// of a context is queued on one HIP stream; upload_object is thread-safe and uses
+// its own stream.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "devcommon.h"
+#include "vrgi_internal.h"
+#include "vrgi_kernels.h"
+
+using namespace vrgi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define API_BEGIN try {
+#define API_END                                                  \
+    }                                                            \
+    catch (const HipError &e) { return fail(VRGI_E_DEVICE, e.what()); } \
+    catch (const std::bad_alloc &) { return fail(VRGI_E_DEVICE, "host allocation failed"); } \
+    catch (const std::exception &e) { return fail(VRGI_E_INVALID, e.what()); }
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    void reserve(size_t n) {
+        if (n <= cap && p) return;
+        release();
+        dev_alloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T));
+        cap = std::max<size_t>(n, 1);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+struct Counters {
+    unsigned long long rays_closest, rays_light, frames, pad;
+};
+
+}  // namespace
+
+struct vrgi_object {
+    std::atomic<int> refs{1};
+    vrgi_ctx *ctx = nullptr;
+    uint32_t n_tris = 0;
+    vrgi_vertex3d *d_verts = nullptr;
+    uint32_t n_light = 0;
+    vrgi_bvh_node *d_light_bvh = nullptr;
+    vrgi_light_vertex3d *d_light_verts = nullptr;
+    std::vector<vrgi_bvh_node> h_light_bvh;
+    bool light_empty = true;
+    float light_aabb[6] = {0, 0, 0, 0, 0, 0};
+    float power = 0.0f;
+};
+
+struct vrgi_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr, up_stream = nullptr;
+    std::mutex up_mu;
+    vrgi_config cfg{};
+    uint32_t W = 0, H = 0, P = 0, B = 0;
+    std::vector<vrgi_material> mats;
+    std::vector<float> lum;
+    DevBuf<vrgi_material> d_mats;
+
+    std::map<uint32_t, std::pair<vrgi_object *, std::vector<float>>> objects;
+    std::vector<vrgi_object *> old;
+    bool inst_dirty = true, geo_dirty = true;
+
+    Builder geo_builder, up_builder, tl_builder, api_builder;
+
+    // instances
+    std::vector<vrgi_object *> inst_objs;
+    uint32_t n_inst = 0, n_tris = 0, n_ltri = 0;
+    DevBuf<DevInst> d_inst;
+    DevBuf<const vrgi_vertex3d *> d_vtxptr;
+    DevBuf<const vrgi_light_vertex3d *> d_lvtxptr;
+    DevBuf<uint32_t> d_light_off;
+    DevBuf<vrgi_bvh_node> d_tl;
+    uint32_t n_tl = 1;
+    DevBuf<float> d_tl_prim9, d_tl_aabb, d_tl_xf, d_tl_power;
+    DevBuf<uint32_t> d_tl_ids, d_tl_trees, d_tl_depth;
+
+    // geometry + light-geometry BVHs (one batched build)
+    DevBuf<Tri> d_tri, d_ltri;
+    DevBuf<uint32_t> d_tri_inst, d_ltri_inst, d_ltri_node;
+    DevBuf<float> d_prim9, d_power;
+    DevBuf<uint32_t> d_ids, d_trees, d_depth;
+    DevBuf<vrgi_bvh_node> d_nodes;
+    DevBuf<WideNode> d_gwide, d_lwide;
+    uint32_t geo_depth = 0, lgeo_depth = 0;
+
+    // frame buffers
+    DevBuf<float> f_origin, f_dir, f_normal, f_emis, f_refl, f_mis, f_pdf, f_nee, f_hit_t;
+    DevBuf<uint32_t> f_hit, f_plen, f_queue, f_qcount;
+    DevBuf<float> f_radiance, f_restir, f_debug, f_temporal, f_spatial;
+    DevBuf<uint8_t> f_bgra;
+    DevBuf<Counters> d_counters;
+    uint64_t frame_count = 0;
+    bool have_frame = false;
+
+    // host-side scratch of the standalone operators
+    DevBuf<float> a_f0, a_f1, a_f2;
+    DevBuf<uint32_t> a_u0, a_u1, a_u2;
+    DevBuf<vrgi_bvh_node> a_nodes;
+
+    ~vrgi_ctx();
+};
+
+namespace {
+
+void set_dev(vrgi_ctx *c) { HIP_CHECK(hipSetDevice(c->device)); }
+
+SceneView scene_view(vrgi_ctx *c) {
+    SceneView sv{};
+    sv.gwide = c->d_gwide.p;
+    sv.tri = c->d_tri.p;
+    sv.n_tris = c->n_tris;
+    sv.root_prim = 0;
+    sv.depth = c->geo_depth;
+    sv.tri_inst = c->d_tri_inst.p;
+    sv.inst = c->d_inst.p;
+    sv.n_inst = c->n_inst;
+    sv.tl = c->d_tl.p;
+    sv.lwide = c->d_lwide.p;
+    sv.ltri = c->d_ltri.p;
+    sv.n_ltri = c->n_ltri;
+    sv.lroot_prim = 0;
+    sv.ldepth = c->lgeo_depth;
+    sv.ltri_inst = c->d_ltri_inst.p;
+    sv.ltri_node = c->d_ltri_node.p;
+    sv.mats = c->d_mats.p;
+    sv.n_mats = (uint32_t)c->mats.size();
+    return sv;
+}
+
+void alloc_frame(vrgi_ctx *c) {
+    size_t P = c->P, B = c->B;
+    c->f_origin.reserve(B * P * 3);
+    c->f_dir.reserve(B * P * 3);
+    c->f_normal.reserve(B * P * 3);
+    c->f_emis.reserve(B * P * 3);
+    c->f_refl.reserve(B * P * 3);
+    c->f_mis.reserve(B * P);
+    c->f_pdf.reserve(B * P);
+    c->f_nee.reserve(B * P);
+    c->f_hit_t.reserve(B * P);
+    c->f_hit.reserve(B * P);
+    c->f_plen.reserve(P);
+    c->f_queue.reserve((B + 1) * P);
+    c->f_qcount.reserve(B + 2);
+    c->f_radiance.reserve(P * 3);
+    c->f_restir.reserve(P * 3);
+    c->f_debug.reserve(P * 3);
+    c->f_temporal.reserve(3 * P * 20);
+    c->f_spatial.reserve(P * 20);
+    c->f_bgra.reserve(P * 4);
+    HIP_CHECK(hipMemsetAsync(c->f_temporal.p, 0, 3 * P * 20 * 4, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->f_spatial.p, 0, P * 20 * 4, c->stream));
+    c->have_frame = false;
+}
+
+// instance table, per-instance pointers, light TL-BVH (scene.rs:243-338)
+void rebuild_instances(vrgi_ctx *c) {
+    c->inst_objs.clear();
+    std::vector<DevInst> hi;
+    std::vector<const vrgi_vertex3d *> vp;
+    std::vector<const vrgi_light_vertex3d *> lp;
+    std::vector<uint32_t> loff;
+    std::vector<float> tl_aabb, tl_xf, tl_pw;
+    std::vector<uint32_t> tl_ids;
+    uint32_t tri_off = 0, light_off = 0;
+    for (auto &kv : c->objects) {
+        vrgi_object *o = kv.second.first;
+        if (!o) continue;  // the Empty handle (scene.rs:156-158)
+        uint32_t i = (uint32_t)c->inst_objs.size();
+        c->inst_objs.push_back(o);
+        DevInst d{};
+        std::memcpy(d.xf, kv.second.second.data(), 48);
+        d.tri_off = tri_off;
+        d.tl_idx = 0;  // prim2node.get(i).unwrap_or(0)
+        d.light_bvh = o->d_light_bvh;
+        hi.push_back(d);
+        vp.push_back(o->d_verts);
+        lp.push_back(o->d_light_verts);
+        loff.push_back(light_off);
+        tri_off += o->n_tris;
+        light_off += o->n_light;
+        if (!o->light_empty) {
+            tl_aabb.insert(tl_aabb.end(), o->light_aabb, o->light_aabb + 6);
+            tl_xf.insert(tl_xf.end(), kv.second.second.begin(), kv.second.second.end());
+            tl_pw.push_back(o->power);
+            tl_ids.push_back(i);
+        }
+    }
+    c->n_inst = (uint32_t)hi.size();
+    c->n_tris = tri_off;
+    c->n_ltri = light_off;
+    hipStream_t s = c->stream;
+    c->d_inst.reserve(hi.size());
+    c->d_vtxptr.reserve(vp.size());
+    c->d_lvtxptr.reserve(lp.size());
+    c->d_light_off.reserve(loff.size());
+    if (!hi.empty()) {
+        HIP_CHECK(hipMemcpyAsync(c->d_inst.p, hi.data(), hi.size() * sizeof(DevInst), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(c->d_vtxptr.p, vp.data(), vp.size() * sizeof(void *), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(c->d_lvtxptr.p, lp.data(), lp.size() * sizeof(void *), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(c->d_light_off.p, loff.data(), loff.size() * 4, hipMemcpyHostToDevice, s));
+    }
+    uint32_t m = (uint32_t)tl_ids.size();
+    c->d_tl.reserve(m ? 2 * m - 1 : 1);
+    if (m == 0) {
+        vrgi_bvh_node dummy;  // BvhNode::default() (bvh/mod.rs:27-38, scene.rs:275-277)
+        dummy.left = dummy.right_or_prim = dummy.parent = VRGI_INVALID;
+        std::memset(dummy.min, 0, 12);
+        std::memset(dummy.max, 0, 12);
+        dummy.power = 0.0f;
+        HIP_CHECK(hipMemcpyAsync(c->d_tl.p, &dummy, sizeof(dummy), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        c->n_tl = 1;
+        return;
+    }
+    c->d_tl_aabb.reserve(6 * m);
+    c->d_tl_xf.reserve(12 * m);
+    c->d_tl_power.reserve(m);
+    c->d_tl_ids.reserve(m);
+    c->d_tl_prim9.reserve(9 * m);
+    c->d_tl_trees.reserve(3);
+    c->d_tl_depth.reserve(1);
+    HIP_CHECK(hipMemcpyAsync(c->d_tl_aabb.p, tl_aabb.data(), tl_aabb.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->d_tl_xf.p, tl_xf.data(), tl_xf.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->d_tl_power.p, tl_pw.data(), m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->d_tl_ids.p, tl_ids.data(), m * 4, hipMemcpyHostToDevice, s));
+    uint32_t trees[3] = {0, m, 0};
+    HIP_CHECK(hipMemcpyAsync(c->d_tl_trees.p, trees, 12, hipMemcpyHostToDevice, s));
+    launch_tl_inputs(s, c->d_tl_aabb.p, c->d_tl_xf.p, m, c->d_tl_prim9.p);
+    if (c->tl_builder.build(s, c->d_tl_prim9.p, c->d_tl_power.p, c->d_tl_ids.p, m, 1, c->d_tl_trees.p,
+                            c->d_tl_trees.p + 1, c->d_tl_trees.p + 2, trees + 2, c->d_tl.p,
+                            c->d_tl_depth.p, nullptr) != 0)
+        throw HipError("light top-level build exceeded its workspace");
+    c->n_tl = 2 * m - 1;
+    launch_tl_idx(s, c->d_tl.p, c->n_tl, c->d_inst.p);
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// world triangles + world light triangles, geometry and light-geometry BVHs in one
+// batched build, traversal copies
+void rebuild_geometry(vrgi_ctx *c) {
+    hipStream_t s = c->stream;
+    uint32_t nt = c->n_tris, nl = c->n_ltri, N = nt + nl;
+    c->d_tri.reserve(nt);
+    c->d_tri_inst.reserve(nt);
+    c->d_ltri.reserve(nl);
+    c->d_ltri_inst.reserve(nl);
+    c->d_ltri_node.reserve(nl);
+    c->d_prim9.reserve((size_t)9 * N);
+    c->d_power.reserve(N);
+    c->d_ids.reserve(N);
+    uint32_t gnodes = nt ? 2 * nt - 1 : 0, lnodes = nl ? 2 * nl - 1 : 0;
+    c->d_nodes.reserve(gnodes + lnodes + 1);
+    c->d_gwide.reserve(nt > 1 ? nt - 1 : 1);
+    c->d_lwide.reserve(nl > 1 ? nl - 1 : 1);
+    c->d_trees.reserve(6);
+    c->d_depth.reserve(2);
+    c->geo_depth = c->lgeo_depth = 0;
+    if (N == 0) return;
+    FlattenArgs f{};
+    f.inst = c->d_inst.p;
+    f.n_inst = c->n_inst;
+    f.vtx = c->d_vtxptr.p;
+    f.lvtx = c->d_lvtxptr.p;
+    f.light_off = c->d_light_off.p;
+    f.prim_stride = N;
+    f.n_tris = nt;
+    f.tri = c->d_tri.p;
+    f.tri_inst = c->d_tri_inst.p;
+    f.prim9 = c->d_prim9.p;
+    f.power = c->d_power.p;
+    f.ids = c->d_ids.p;
+    f.prim_off = 0;
+    launch_flatten(s, f);
+    FlattenArgs fl = f;
+    fl.n_tris = nl;
+    fl.tri = c->d_ltri.p;
+    fl.tri_inst = c->d_ltri_inst.p;
+    fl.light_node = c->d_ltri_node.p;
+    fl.prim_off = nt;
+    launch_flatten_lights(s, fl);
+    uint32_t trees[6];
+    uint32_t nt_trees = 0;
+    uint32_t beg[2], cnt[2], base[2];
+    if (nt) { beg[nt_trees] = 0; cnt[nt_trees] = nt; base[nt_trees] = 0; ++nt_trees; }
+    if (nl) { beg[nt_trees] = nt; cnt[nt_trees] = nl; base[nt_trees] = gnodes; ++nt_trees; }
+    for (uint32_t t = 0; t < nt_trees; ++t) { trees[t] = beg[t]; trees[2 + t] = cnt[t]; trees[4 + t] = base[t]; }
+    HIP_CHECK(hipMemcpyAsync(c->d_trees.p, trees, 24, hipMemcpyHostToDevice, s));
+    if (c->geo_builder.build(s, c->d_prim9.p, c->d_power.p, c->d_ids.p, N, nt_trees, c->d_trees.p,
+                             c->d_trees.p + 2, c->d_trees.p + 4, base, c->d_nodes.p, c->d_depth.p,
+                             nullptr) != 0)
+        throw HipError("geometry build exceeded its workspace");
+    uint32_t dep[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(dep, c->d_depth.p, nt_trees * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    uint32_t t = 0;
+    if (nt) c->geo_depth = dep[t++];
+    if (nl) c->lgeo_depth = dep[t++];
+    if (c->geo_depth + 2 > VRGI_MAX_STACK || c->lgeo_depth + 2 > VRGI_MAX_STACK)
+        throw std::runtime_error("BVH deeper than the traversal stack");
+    launch_wide(s, c->d_nodes.p, gnodes, c->d_gwide.p);
+    launch_wide(s, c->d_nodes.p + gnodes, lnodes, c->d_lwide.p);
+}
+
+void free_object(vrgi_object *o) {
+    if (o->ctx) {
+        (void)hipSetDevice(o->ctx->device);
+        (void)hipStreamSynchronize(o->ctx->stream);
+    }
+    if (o->d_verts) (void)hipFree(o->d_verts);
+    if (o->d_light_bvh) (void)hipFree(o->d_light_bvh);
+    if (o->d_light_verts) (void)hipFree(o->d_light_verts);
+    delete o;
+}
+
+void release(vrgi_object *o) {
+    if (o && o->refs.fetch_sub(1) == 1) free_object(o);
+}
+
+bool valid_xf(const float *xf) {
+    if (!xf) return false;
+    for (int k = 0; k < 12; ++k)
+        if (!(xf[k] == xf[k])) return false;
+    return true;
+}
+
+}  // namespace
+
+vrgi_ctx::~vrgi_ctx() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &kv : objects) release(kv.second.first);
+    for (vrgi_object *o : old) release(o);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (up_stream) (void)hipStreamDestroy(up_stream);
+}
+
+extern "C" {
+
+const char *vrgi_last_error(void) { return g_err.c_str(); }
+uint32_t vrgi_abi_version(void) { return VRGI_ABI_VERSION; }
+
+void vrgi_config_reference(vrgi_config *cfg, uint32_t width, uint32_t height) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->width = width;
+    cfg->height = height;
+    cfg->num_bounces = 3;                // interactive_rendering.rs:767
+    cfg->rr_start_bounce = 0xFFFFFFFFu;  // no Russian roulette in the reference
+    cfg->rr_min_survival = 0.1f;
+    cfg->restir_mode = 0;                // ReSTIR GI
+    cfg->restir_spatial_radius = 15.0f;  // restir_spatial_resampling.rs:314
+    cfg->di_candidates = 32;
+    cfg->seed = 0;
+    cfg->flags = 0;
+}
+
+int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const float *lum,
+                uint32_t n_slots, int device, vrgi_ctx **out) {
+    API_BEGIN
+    if (!cfg || !out || (n_slots && (!materials || !lum))) return fail(VRGI_E_INVALID, "null argument");
+    if (cfg->width == 0 || cfg->height == 0 || cfg->num_bounces < 2 || cfg->num_bounces > 16)
+        return fail(VRGI_E_INVALID, "extent must be non-zero and num_bounces in [2, 16]");
+    if ((uint64_t)cfg->width * cfg->height > (1ull << 26)) return fail(VRGI_E_LIMIT, "extent above 2^26 pixels");
+    if (cfg->restir_mode != 0) return fail(VRGI_E_INVALID, "restir_mode: only 0 (GI) is available");
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(VRGI_E_INVALID, "no such HIP device");
+    vrgi_ctx *c = new vrgi_ctx();
+    try {
+        c->device = device;
+        HIP_CHECK(hipSetDevice(device));
+        HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking));
+        c->cfg = *cfg;
+        c->W = cfg->width;
+        c->H = cfg->height;
+        c->P = c->W * c->H;
+        c->B = cfg->num_bounces;
+        c->mats.assign(materials, materials + n_slots);
+        c->lum.assign(lum, lum + n_slots);
+        c->d_mats.reserve(n_slots);
+        if (n_slots)
+            HIP_CHECK(hipMemcpy(c->d_mats.p, materials, n_slots * sizeof(vrgi_material), hipMemcpyHostToDevice));
+        c->d_counters.reserve(1);
+        HIP_CHECK(hipMemset(c->d_counters.p, 0, sizeof(Counters)));
+        alloc_frame(c);
+        rebuild_instances(c);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        delete c;
+        throw;
+    }
+    *out = c;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_destroy(vrgi_ctx *c) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    delete c;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_upload_object(vrgi_ctx *c, const vrgi_vertex3d *v, size_t n, vrgi_object **out) {
+    API_BEGIN
+    if (!c || !out || (n && !v)) return fail(VRGI_E_INVALID, "null argument");
+    *out = nullptr;
+    if (n == 0) return VRGI_OK;  // SceneUploadedObjectHandle::Empty
+    if (n % 3 != 0) return fail(VRGI_E_INVALID, "vertex count is not a multiple of 3 (scene.rs:553)");
+    if (n / 3 > (1u << 30)) return fail(VRGI_E_LIMIT, "more than 2^30 triangles in one object");
+    for (size_t i = 0; i < n; i += 3)
+        if (v[i].t >= c->lum.size()) return fail(VRGI_E_INVALID, "material slot out of range");
+    std::lock_guard<std::mutex> lk(c->up_mu);
+    set_dev(c);
+    hipStream_t s = c->up_stream;
+    vrgi_object *o = new vrgi_object();
+    o->ctx = c;
+    try {
+        o->n_tris = (uint32_t)(n / 3);
+        dev_alloc((void **)&o->d_verts, n * sizeof(vrgi_vertex3d));
+        HIP_CHECK(hipMemcpyAsync(o->d_verts, v, n * sizeof(vrgi_vertex3d), hipMemcpyHostToDevice, s));
+        // emissive triangles (scene.rs:555-571): the caller-side gather of the reference
+        std::vector<float> tri9, pw;
+        std::vector<uint32_t> ids;
+        for (uint32_t i = 0; i < o->n_tris; ++i) {
+            float l = c->lum[v[3 * i].t];
+            if (!(l > 0.0f)) continue;
+            float a[3], b[3], cc[3];
+            for (int k = 0; k < 3; ++k) {
+                a[k] = v[3 * i].position[k] + 0.0f;
+                b[k] = v[3 * i + 1].position[k] + 0.0f;
+                cc[k] = v[3 * i + 2].position[k] + 0.0f;
+            }
+            float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, e2[3] = {cc[0] - a[0], cc[1] - a[1], cc[2] - a[2]};
+            float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            float area = sqrtf(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) / 2.0f;
+            tri9.insert(tri9.end(), a, a + 3);
+            tri9.insert(tri9.end(), b, b + 3);
+            tri9.insert(tri9.end(), cc, cc + 3);
+            pw.push_back(l * area);
+            ids.push_back(i);
+        }
+        uint32_t m = (uint32_t)ids.size();
+        o->n_light = m;
+        std::vector<vrgi_light_vertex3d> lv;  // outlives its asynchronous upload
+        if (m) {
+            // light BL-BVH on the GPU (build_bl_bvh, build.rs:249-372)
+            DevBuf<float> dtri, dprim9, dpw;
+            DevBuf<uint32_t> dids, dtrees, ddepth;
+            dtri.reserve(9 * (size_t)m);
+            dprim9.reserve(9 * (size_t)m);
+            dpw.reserve(m);
+            dids.reserve(m);
+            dtrees.reserve(3);
+            ddepth.reserve(1);
+            dev_alloc((void **)&o->d_light_bvh, (2 * (size_t)m - 1) * sizeof(vrgi_bvh_node));
+            uint32_t trees[3] = {0, m, 0};
+            HIP_CHECK(hipMemcpyAsync(dtri.p, tri9.data(), tri9.size() * 4, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(dpw.p, pw.data(), m * 4, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(dids.p, ids.data(), m * 4, hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(dtrees.p, trees, 12, hipMemcpyHostToDevice, s));
+            launch_prims_from_tris(s, dtri.p, m, dprim9.p);
+            if (c->up_builder.build(s, dprim9.p, dpw.p, dids.p, m, 1, dtrees.p, dtrees.p + 1, dtrees.p + 2,
+                                    trees + 2, o->d_light_bvh, ddepth.p, nullptr) != 0)
+                throw HipError("light build exceeded its workspace");
+            o->h_light_bvh.resize(2 * (size_t)m - 1);
+            HIP_CHECK(hipMemcpyAsync(o->h_light_bvh.data(), o->d_light_bvh, o->h_light_bvh.size() * sizeof(vrgi_bvh_node),
+                                     hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            // light vertices in leaf order (scene.rs:509-524)
+            lv.reserve(3 * (size_t)m);
+            for (size_t i = 0; i < o->h_light_bvh.size(); ++i) {
+                const vrgi_bvh_node &nd = o->h_light_bvh[i];
+                if (nd.left != VRGI_INVALID) continue;
+                for (int k = 0; k < 3; ++k) {
+                    vrgi_light_vertex3d x;
+                    std::memcpy(x.position, v[(size_t)nd.right_or_prim * 3 + k].position, 12);
+                    x.bvh_node_idx = (uint32_t)i;
+                    lv.push_back(x);
+                }
+            }
+            dev_alloc((void **)&o->d_light_verts, lv.size() * sizeof(vrgi_light_vertex3d));
+            HIP_CHECK(hipMemcpyAsync(o->d_light_verts, lv.data(), lv.size() * sizeof(vrgi_light_vertex3d),
+                                     hipMemcpyHostToDevice, s));
+            // returned (Aabb, power) of build_bl_bvh (build.rs:359-371)
+            const vrgi_bvh_node &r = o->h_light_bvh[0];
+            o->light_empty = false;
+            if (r.left == VRGI_INVALID) {
+                for (int k = 0; k < 3; ++k) {
+                    o->light_aabb[k] = std::min(std::min(tri9[k], tri9[3 + k]), tri9[6 + k]);
+                    o->light_aabb[3 + k] = std::max(std::max(tri9[k], tri9[3 + k]), tri9[6 + k]);
+                }
+                o->power = pw[0];
+            } else {
+                for (int k = 0; k < 3; ++k) { o->light_aabb[k] = r.min[k]; o->light_aabb[3 + k] = r.max[k]; }
+                o->power = r.power;
+            }
+        }
+        HIP_CHECK(hipStreamSynchronize(s));
+    } catch (...) {
+        free_object(o);
+        throw;
+    }
+    *out = o;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_object_retain(vrgi_object *o) {
+    if (!o) return fail(VRGI_E_INVALID, "null object");
+    o->refs.fetch_add(1);
+    return VRGI_OK;
+}
+int vrgi_object_release(vrgi_object *o) {
+    API_BEGIN
+    if (!o) return fail(VRGI_E_INVALID, "null object");
+    release(o);
+    return VRGI_OK;
+    API_END
+}
+int vrgi_object_light_bvh(vrgi_object *o, vrgi_bvh_node *dst, size_t cap, size_t *n_nodes) {
+    if (!o || !n_nodes) return fail(VRGI_E_INVALID, "null argument");
+    *n_nodes = o->h_light_bvh.size();
+    if (dst) {
+        if (cap < o->h_light_bvh.size()) return fail(VRGI_E_LIMIT, "destination too small");
+        std::memcpy(dst, o->h_light_bvh.data(), o->h_light_bvh.size() * sizeof(vrgi_bvh_node));
+    }
+    return VRGI_OK;
+}
+
+int vrgi_scene_add(vrgi_ctx *c, uint32_t key, vrgi_object *o, const float xf[12]) {
+    API_BEGIN
+    if (!c || !valid_xf(xf)) return fail(VRGI_E_INVALID, "null context or non-finite transform");
+    auto it = c->objects.find(key);
+    if (it != c->objects.end()) {
+        if (it->second.first) c->old.push_back(it->second.first);
+        c->objects.erase(it);
+    }
+    if (o) o->refs.fetch_add(1);
+    c->objects[key] = {o, std::vector<float>(xf, xf + 12)};
+    c->inst_dirty = c->geo_dirty = true;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_scene_update(vrgi_ctx *c, uint32_t key, const float xf[12]) {
+    API_BEGIN
+    if (!c || !valid_xf(xf)) return fail(VRGI_E_INVALID, "null context or non-finite transform");
+    auto it = c->objects.find(key);
+    if (it == c->objects.end()) return fail(VRGI_E_INVALID, "object with key does not exist (scene.rs:215)");
+    if (!it->second.first) return VRGI_OK;  // Some(None): no-op
+    it->second.second.assign(xf, xf + 12);
+    c->inst_dirty = c->geo_dirty = true;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_scene_remove(vrgi_ctx *c, uint32_t key) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    auto it = c->objects.find(key);
+    if (it == c->objects.end()) return VRGI_OK;
+    if (it->second.first) {
+        c->old.push_back(it->second.first);
+        c->inst_dirty = c->geo_dirty = true;
+    }
+    c->objects.erase(it);
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_dispose_old(vrgi_ctx *c) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    set_dev(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (vrgi_object *o : c->old) release(o);
+    c->old.clear();
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_resize(vrgi_ctx *c, uint32_t w, uint32_t h) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    if (w == 0 || h == 0) return VRGI_OK;  // zero extent: frames are skipped (interactive_rendering.rs:978-981)
+    if ((uint64_t)w * h > (1ull << 26)) return fail(VRGI_E_LIMIT, "extent above 2^26 pixels");
+    set_dev(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->W = w;
+    c->H = h;
+    c->P = w * h;
+    c->cfg.width = w;
+    c->cfg.height = h;
+    alloc_frame(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vrgi_frame_out *out) {
+    API_BEGIN
+    if (!c || !cam || !prefs) return fail(VRGI_E_INVALID, "null argument");
+    if (prefs->nee_type > 2) return fail(VRGI_E_INVALID, "nee_type must be 0, 1 or 2");
+    if (prefs->restir_spatial_iterations > 20) return fail(VRGI_E_INVALID, "restir_spatial_iterations above 20");
+    set_dev(c);
+    hipStream_t s = c->stream;
+    if (c->inst_dirty) {
+        rebuild_instances(c);
+        c->inst_dirty = false;
+        c->geo_dirty = true;
+    }
+    if (c->geo_dirty || (c->cfg.flags & VRGI_FLAG_REBUILD_EVERY_FRAME)) {
+        rebuild_geometry(c);
+        c->geo_dirty = false;
+    }
+    const uint32_t P = c->P, B = c->B;
+    const uint32_t frame = (uint32_t)c->frame_count;
+    HIP_CHECK(hipMemsetAsync(c->f_qcount.p, 0, (B + 2) * 4, s));
+    if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) {
+        size_t n = (size_t)B * P;
+        HIP_CHECK(hipMemsetAsync(c->f_origin.p, 0, n * 12, s));
+        HIP_CHECK(hipMemsetAsync(c->f_dir.p, 0, n * 12, s));
+        HIP_CHECK(hipMemsetAsync(c->f_normal.p, 0, n * 12, s));
+        HIP_CHECK(hipMemsetAsync(c->f_emis.p, 0, n * 12, s));
+        HIP_CHECK(hipMemsetAsync(c->f_refl.p, 0, n * 12, s));
+        HIP_CHECK(hipMemsetAsync(c->f_mis.p, 0, n * 4, s));
+        HIP_CHECK(hipMemsetAsync(c->f_pdf.p, 0, n * 4, s));
+        HIP_CHECK(hipMemsetAsync(c->f_nee.p, 0, n * 4, s));
+        HIP_CHECK(hipMemsetAsync(c->f_hit_t.p, 0, n * 4, s));
+        HIP_CHECK(hipMemsetAsync(c->f_hit.p, 0xFF, n * 4, s));
+    }
+    SceneView sv = scene_view(c);
+    BounceArgs ba{};
+    ba.B = B; ba.W = c->W; ba.H = c->H; ba.P = P;
+    ba.nee_type = prefs->nee_type;
+    ba.rr_start = c->cfg.rr_start_bounce;
+    ba.rr_min = c->cfg.rr_min_survival;
+    ba.cam = *cam;
+    ba.qcount = c->f_qcount.p;
+    ba.origin = c->f_origin.p; ba.dir = c->f_dir.p; ba.normal = c->f_normal.p;
+    ba.emis = c->f_emis.p; ba.refl = c->f_refl.p; ba.mis = c->f_mis.p; ba.pdf = c->f_pdf.p;
+    ba.nee = c->f_nee.p; ba.hit_t = c->f_hit_t.p; ba.hit = c->f_hit.p; ba.path_len = c->f_plen.p;
+    ba.counters = (unsigned long long *)c->d_counters.p;
+    for (uint32_t b = 0; b < B; ++b) {
+        ba.bounce = b;
+        ba.inv_seed = frame * B + b;  // interactive_rendering.rs:1189
+        ba.queue_in = c->f_queue.p + (size_t)b * P;
+        ba.queue_out = c->f_queue.p + (size_t)(b + 1) * P;
+        launch_trace_shade(s, sv, ba, P);
+    }
+    if (prefs->nee_type != 0) {
+        for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
+            ba.bounce = b;
+            ba.queue_in = c->f_queue.p + (size_t)(b + 1) * P;
+            launch_nee_pdf(s, sv, ba, P);
+        }
+    } else {
+        // nee_pdf is 0 wherever mis_weight is 0; every live ray has mis_weight 0 here
+    }
+    FrameArgs fa{};
+    fa.W = c->W; fa.H = c->H; fa.P = P; fa.B = B; fa.frame = frame;
+    fa.slot = (uint32_t)(c->frame_count % 3);
+    fa.tseed = frame_seed_host(c->cfg.seed, frame, 2);
+    fa.sseed = frame_seed_host(c->cfg.seed, frame, 3);
+    fa.spatial_iters = prefs->restir_spatial_iterations;
+    fa.radius = c->cfg.restir_spatial_radius;
+    fa.debug_view = prefs->debug_view;
+    fa.origin = c->f_origin.p; fa.normal = c->f_normal.p; fa.emis = c->f_emis.p; fa.refl = c->f_refl.p;
+    fa.mis = c->f_mis.p; fa.pdf = c->f_pdf.p; fa.nee = prefs->nee_type ? c->f_nee.p : nullptr;
+    fa.path_len = c->f_plen.p;
+    fa.radiance = c->f_radiance.p;
+    fa.temporal = c->f_temporal.p + (size_t)fa.slot * P * 20;
+    fa.spatial = c->f_spatial.p;
+    fa.restir = c->f_restir.p;
+    fa.debug = c->f_debug.p;
+    fa.bgra8 = c->f_bgra.p;
+    launch_accum_temporal(s, fa);
+    launch_spatial_finalize(s, fa);
+    launch_postprocess(s, fa);
+    launch_frame_stats(s, c->f_qcount.p, B, P, (unsigned long long *)c->d_counters.p);
+    c->have_frame = true;
+    if (out) {
+        out->radiance = c->f_radiance.p;
+        out->restir = c->f_restir.p;
+        out->bgra8 = c->f_bgra.p;
+        out->frame_index = c->frame_count;
+    }
+    ++c->frame_count;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_sync(vrgi_ctx *c) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    set_dev(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_stats(vrgi_ctx *c, vrgi_frame_stats *st, int reset) {
+    API_BEGIN
+    if (!c || !st) return fail(VRGI_E_INVALID, "null argument");
+    set_dev(c);
+    Counters h{};
+    HIP_CHECK(hipMemcpyAsync(&h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    st->frames = h.frames;
+    st->rays_closest = h.rays_closest;
+    st->rays_light = h.rays_light;
+    st->bvh_nodes = c->n_tris ? 2ull * c->n_tris - 1 : 0;
+    st->bvh_depth = c->geo_depth;
+    st->pad = 0;
+    if (reset) {
+        HIP_CHECK(hipMemsetAsync(c->d_counters.p, 0, sizeof(Counters), c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_buffer_size(vrgi_ctx *c, int which, size_t *bytes) {
+    if (!c || !bytes) return fail(VRGI_E_INVALID, "null argument");
+    size_t P = c->P, B = c->B, nt = c->n_tris, nl = c->n_ltri;
+    size_t r = 0;
+    switch (which) {
+        case VRGI_BUF_GEOM_BVH: r = nt ? (2 * nt - 1) * 40 : 0; break;
+        case VRGI_BUF_GEOM_PRIM_ORDER: r = nt * 4; break;
+        case VRGI_BUF_LIGHT_TL_BVH: r = (size_t)c->n_tl * 40; break;
+        case VRGI_BUF_LIGHT_GEOM_BVH: r = nl ? (2 * nl - 1) * 40 : 0; break;
+        case VRGI_BUF_PATH_LEN: r = P * 4; break;
+        case VRGI_BUF_BOUNCE_ORIGIN: case VRGI_BUF_BOUNCE_DIRECTION: case VRGI_BUF_BOUNCE_NORMAL:
+        case VRGI_BUF_BOUNCE_EMISSIVITY: case VRGI_BUF_BOUNCE_REFLECTIVITY: r = B * P * 12; break;
+        case VRGI_BUF_BOUNCE_MIS_WEIGHT: case VRGI_BUF_BOUNCE_BSDF_PDF: case VRGI_BUF_BOUNCE_NEE_PDF:
+        case VRGI_BUF_HIT_PRIM: case VRGI_BUF_HIT_T: r = B * P * 4; break;
+        case VRGI_BUF_RADIANCE: case VRGI_BUF_RESTIR: r = P * 12; break;
+        case VRGI_BUF_TEMPORAL_RESERVOIR: case VRGI_BUF_SPATIAL_RESERVOIR: r = P * 80; break;
+        case VRGI_BUF_BGRA8: r = P * 4; break;
+        default: r = 0;
+    }
+    *bytes = r;
+    return VRGI_OK;
+}
+
+int vrgi_read_buffer(vrgi_ctx *c, int which, void *dst, size_t bytes) {
+    API_BEGIN
+    if (!c || !dst) return fail(VRGI_E_INVALID, "null argument");
+    size_t need = 0;
+    vrgi_buffer_size(c, which, &need);
+    if (need == 0) return fail(VRGI_E_STATE, "buffer not available");
+    if (bytes < need) return fail(VRGI_E_LIMIT, "destination too small");
+    set_dev(c);
+    const void *src = nullptr;
+    size_t P = c->P, nt = c->n_tris;
+    switch (which) {
+        case VRGI_BUF_GEOM_BVH: src = c->d_nodes.p; break;
+        case VRGI_BUF_GEOM_PRIM_ORDER: src = c->geo_builder.prim_order(); break;
+        case VRGI_BUF_LIGHT_TL_BVH: src = c->d_tl.p; break;
+        case VRGI_BUF_LIGHT_GEOM_BVH: src = c->d_nodes.p + (nt ? 2 * nt - 1 : 0); break;
+        case VRGI_BUF_PATH_LEN: src = c->f_plen.p; break;
+        case VRGI_BUF_BOUNCE_ORIGIN: src = c->f_origin.p; break;
+        case VRGI_BUF_BOUNCE_DIRECTION: src = c->f_dir.p; break;
+        case VRGI_BUF_BOUNCE_NORMAL: src = c->f_normal.p; break;
+        case VRGI_BUF_BOUNCE_EMISSIVITY: src = c->f_emis.p; break;
+        case VRGI_BUF_BOUNCE_REFLECTIVITY: src = c->f_refl.p; break;
+        case VRGI_BUF_BOUNCE_MIS_WEIGHT: src = c->f_mis.p; break;
+        case VRGI_BUF_BOUNCE_BSDF_PDF: src = c->f_pdf.p; break;
+        case VRGI_BUF_BOUNCE_NEE_PDF: src = c->f_nee.p; break;
+        case VRGI_BUF_HIT_PRIM: src = c->f_hit.p; break;
+        case VRGI_BUF_HIT_T: src = c->f_hit_t.p; break;
+        case VRGI_BUF_RADIANCE: src = c->f_radiance.p; break;
+        case VRGI_BUF_RESTIR: src = c->f_restir.p; break;
+        case VRGI_BUF_TEMPORAL_RESERVOIR: {
+            uint64_t last = c->frame_count ? c->frame_count - 1 : 0;
+            src = c->f_temporal.p + (size_t)(last % 3) * P * 20;
+            break;
+        }
+        case VRGI_BUF_SPATIAL_RESERVOIR: src = c->f_spatial.p; break;
+        case VRGI_BUF_BGRA8: src = c->f_bgra.p; break;
+        default: return fail(VRGI_E_INVALID, "unknown buffer id");
+    }
+    HIP_CHECK(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_build_bl_bvh(vrgi_ctx *c, const float *tri, const float *lum, const uint32_t *ids, uint64_t n,
+                      vrgi_bvh_node *nodes, uint32_t *order, float root_box[6], float *root_power) {
+    API_BEGIN
+    if (!c || !tri || !lum || !ids || !nodes) return fail(VRGI_E_INVALID, "null argument");
+    if (n == 0) return fail(VRGI_E_INVALID, "build_bl_bvh needs at least one primitive");
+    if (n > (1ull << 30)) return fail(VRGI_E_LIMIT, "more than 2^30 primitives");
+    set_dev(c);
+    hipStream_t s = c->stream;
+    uint32_t m = (uint32_t)n;
+    c->a_f0.reserve(9 * (size_t)m);
+    c->a_f1.reserve(9 * (size_t)m);
+    c->a_f2.reserve(m);
+    c->a_u0.reserve(m);
+    c->a_u1.reserve(4);
+    c->a_u2.reserve(m);
+    c->a_nodes.reserve(2 * (size_t)m - 1);
+    uint32_t trees[4] = {0, m, 0, 0};
+    HIP_CHECK(hipMemcpyAsync(c->a_f0.p, tri, 9 * (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->a_f2.p, lum, (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->a_u0.p, ids, (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->a_u1.p, trees, 12, hipMemcpyHostToDevice, s));
+    launch_prims_from_tris(s, c->a_f0.p, m, c->a_f1.p);
+    if (c->api_builder.build(s, c->a_f1.p, c->a_f2.p, c->a_u0.p, m, 1, c->a_u1.p, c->a_u1.p + 1, c->a_u1.p + 2,
+                             trees + 2, c->a_nodes.p, c->a_u1.p + 3, order ? c->a_u2.p : nullptr) != 0)
+        return fail(VRGI_E_LIMIT, "build exceeded its workspace");
+    HIP_CHECK(hipMemcpyAsync(nodes, c->a_nodes.p, (2 * (size_t)m - 1) * 40, hipMemcpyDeviceToHost, s));
+    if (order) HIP_CHECK(hipMemcpyAsync(order, c->a_u2.p, (size_t)m * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (root_box || root_power) {  // build.rs:359-371
+        float rb[6], rp;
+        if (nodes[0].left == VRGI_INVALID) {
+            for (int k = 0; k < 3; ++k) {
+                float a = tri[k] + 0.0f, b = tri[3 + k] + 0.0f, d = tri[6 + k] + 0.0f;
+                rb[k] = std::min(std::min(a, b), d);
+                rb[3 + k] = std::max(std::max(a, b), d);
+            }
+            rp = lum[0];
+        } else {
+            for (int k = 0; k < 3; ++k) { rb[k] = nodes[0].min[k]; rb[3 + k] = nodes[0].max[k]; }
+            rp = nodes[0].power;
+        }
+        if (root_box) std::memcpy(root_box, rb, 24);
+        if (root_power) *root_power = rp;
+    }
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_build_tl_bvh(vrgi_ctx *c, const float *xf, const float *aabbs, const float *pw, const uint32_t *ids,
+                      uint64_t n, vrgi_bvh_node *nodes) {
+    API_BEGIN
+    if (!c || !xf || !aabbs || !pw || !ids || !nodes) return fail(VRGI_E_INVALID, "null argument");
+    if (n == 0) return fail(VRGI_E_INVALID, "build_tl_bvh needs at least one instance");
+    if (n > (1ull << 30)) return fail(VRGI_E_LIMIT, "more than 2^30 instances");
+    set_dev(c);
+    hipStream_t s = c->stream;
+    uint32_t m = (uint32_t)n;
+    c->a_f0.reserve(12 * (size_t)m);
+    c->a_f1.reserve(9 * (size_t)m);
+    c->a_f2.reserve(7 * (size_t)m);
+    c->a_u0.reserve(m);
+    c->a_u1.reserve(4);
+    c->a_nodes.reserve(2 * (size_t)m - 1);
+    uint32_t trees[4] = {0, m, 0, 0};
+    HIP_CHECK(hipMemcpyAsync(c->a_f0.p, xf, 12 * (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->a_f2.p, aabbs, 6 * (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->a_f2.p + 6 * (size_t)m, pw, (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->a_u0.p, ids, (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->a_u1.p, trees, 12, hipMemcpyHostToDevice, s));
+    launch_tl_inputs(s, c->a_f2.p, c->a_f0.p, m, c->a_f1.p);
+    if (c->api_builder.build(s, c->a_f1.p, c->a_f2.p + 6 * (size_t)m, c->a_u0.p, m, 1, c->a_u1.p, c->a_u1.p + 1,
+                             c->a_u1.p + 2, trees + 2, c->a_nodes.p, c->a_u1.p + 3, nullptr) != 0)
+        return fail(VRGI_E_LIMIT, "build exceeded its workspace");
+    HIP_CHECK(hipMemcpyAsync(nodes, c->a_nodes.p, (2 * (size_t)m - 1) * 40, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_trace_closest(vrgi_ctx *c, const float *o, const float *d, uint64_t n, float tmin, float tmax,
+                       uint32_t *hit, float *t, float *uv) {
+    API_BEGIN
+    if (!c || !o || !d || !hit || !t || !uv) return fail(VRGI_E_INVALID, "null argument");
+    if (n == 0) return VRGI_OK;
+    if (n > (1ull << 30)) return fail(VRGI_E_LIMIT, "more than 2^30 rays");
+    set_dev(c);
+    hipStream_t s = c->stream;
+    if (c->inst_dirty) {
+        rebuild_instances(c);
+        c->inst_dirty = false;
+        c->geo_dirty = true;
+    }
+    if (c->geo_dirty) {
+        rebuild_geometry(c);
+        c->geo_dirty = false;
+    }
+    uint32_t m = (uint32_t)n;
+    c->a_f0.reserve(3 * (size_t)m);
+    c->a_f1.reserve(3 * (size_t)m);
+    c->a_f2.reserve(3 * (size_t)m);
+    c->a_u0.reserve(m);
+    HIP_CHECK(hipMemcpyAsync(c->a_f0.p, o, 12 * (size_t)m, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->a_f1.p, d, 12 * (size_t)m, hipMemcpyHostToDevice, s));
+    launch_trace_closest(s, scene_view(c), c->a_f0.p, c->a_f1.p, m, tmin, tmax, c->a_u0.p, c->a_f2.p,
+                         c->a_f2.p + m);
+    HIP_CHECK(hipMemcpyAsync(hit, c->a_u0.p, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(t, c->a_f2.p, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(uv, c->a_f2.p + m, 8 * (size_t)m, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return VRGI_OK;
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,812 @@
+// bvh_build.hip — bit-exact binned-SAH BVH builder for gfx950, batched over many trees.
+//
+// Semantics: build_bl_bvh / build_tl_bvh of the reference (bvh/build.rs:36-480):
+// 32 bins x 3 axes, strict-< first minimum of area(L)*nL + area(R)*nR
+// (build.rs:45-128), in-place `partition` (build.rs:145-149; the i-th false of
+// [0,T) trades places with the i-th true of [T,n) counted from the right), the
+// half split fallback (build.rs:152-157), the un-binned 2-primitive case
+// (build.rs:203-230), one primitive per leaf, 40-byte BvhNode output with 1e-4
+// padding on internal boxes (build.rs:300-339) and power = (0 + left) + right
+// (build.rs:345-357). Node numbering: the reference allocates children in DFS
+// preorder of splits, so the split node of preorder rank k owns children 2k+1 and
+// 2k+2; a left child has rank k+1, a right child rank k + (left count). Nodes are
+// therefore numbered independently of the order in which this builder visits them.
+//
+// Execution: level-synchronous "top" phase for nodes larger than SUB_MAX (one
+// workgroup per CHUNK of a node's range; bins reduced in LDS then global
+// atomics), then one 64-lane wave per remaining subtree ("jobs", <= SUB_MAX
+// primitives) with the subtree's primitives resident in LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "devcommon.h"
+#include "vrgi_internal.h"
+
+namespace vrgi {
+
+static constexpr int BINS = 32;
+static constexpr int CHUNK = 4096;      // primitives per top-phase work item
+static constexpr int TOP_THREADS = 256;
+static constexpr int PER_THREAD = CHUNK / TOP_THREADS;  // 16
+static constexpr int BIN_WORDS = 3 * BINS * 7;          // count + 6 box keys per bin
+
+struct OpenNode {
+    uint32_t first, count, rank, node, parent, tree, depth, pad;
+};
+struct Item {
+    uint32_t open, beg, len, pad;
+};
+struct SplitInfo {
+    uint32_t dim;
+    float split;
+    uint32_t T;
+    uint32_t fallback;
+    int box[6];  // node box keys (union of the range's primitive boxes)
+};
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ uint32_t bin_index(float c, float bmin, float scale) {
+    // build.rs:79-82: usize::min(31, ((c - min) * scale) as usize); `as` saturates, NaN -> 0
+    float f = (c - bmin) * scale;
+    if (!(f == f)) return 0;
+    if (f >= 31.0f) return 31;
+    if (f <= 0.0f) return 0;
+    return (uint32_t)f;
+}
+__device__ __forceinline__ float area_keys(int mnx, int mny, int mnz, int mxx, int mxy, int mxz) {
+    // aabb.rs:44-52
+    float dx = funkey(mxx) - funkey(mnx), dy = funkey(mxy) - funkey(mny), dz = funkey(mxz) - funkey(mnz);
+    return 2.0f * (dx * dy + dx * dz + dy * dz);
+}
+
+__device__ __forceinline__ void write_node(vrgi_bvh_node *dst, uint32_t left, uint32_t right,
+                                           const float mn[3], float power, const float mx[3],
+                                           uint32_t parent) {
+    vrgi_bvh_node n;
+    n.left = left;
+    n.right_or_prim = right;
+    n.min[0] = mn[0]; n.min[1] = mn[1]; n.min[2] = mn[2];
+    n.power = power;
+    n.max[0] = mx[0]; n.max[1] = mx[1]; n.max[2] = mx[2];
+    n.parent = parent;
+    *dst = n;
+}
+
+// leaf (build.rs:311-326) for primitive p
+__device__ __forceinline__ void emit_leaf(const BuildArgs &a, uint32_t gnode, uint32_t parent, uint32_t p) {
+    float mn[3] = {a.prim9[0 * a.n + p], a.prim9[1 * a.n + p], a.prim9[2 * a.n + p]};
+    float mx[3] = {a.prim9[3 * a.n + p], a.prim9[4 * a.n + p], a.prim9[5 * a.n + p]};
+    write_node(a.nodes + gnode, VRGI_INVALID, a.ids[p], mn, a.power[p], mx, parent);
+}
+
+// 2-primitive node (build.rs:203-230) of tree-local rank r / index idx, leaves at p0, p1
+__device__ __forceinline__ void emit_pair(const BuildArgs &a, uint32_t base, uint32_t idx, uint32_t r,
+                                          uint32_t parent, uint32_t p0, uint32_t p1) {
+    const float pad = 0.0001f;
+    float mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = gmin(a.prim9[k * a.n + p0], a.prim9[k * a.n + p1]) - pad;
+        mx[k] = gmax(a.prim9[(3 + k) * a.n + p0], a.prim9[(3 + k) * a.n + p1]) + pad;
+    }
+    float pw = (0.0f + a.power[p0]) + a.power[p1];
+    write_node(a.nodes + base + idx, 2 * r + 1, 2 * r + 2, mn, pw, mx, parent);
+    emit_leaf(a, base + 2 * r + 1, idx, p0);
+    emit_leaf(a, base + 2 * r + 2, idx, p1);
+}
+
+// ---------------------------------------------------------------- init
+__global__ void k_init(BuildArgs a) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    a.perm[i] = i;
+    a.pc[0 * a.n + i] = a.prim9[6 * a.n + i];
+    a.pc[1 * a.n + i] = a.prim9[7 * a.n + i];
+    a.pc[2 * a.n + i] = a.prim9[8 * a.n + i];
+}
+
+// one thread per tree: place the root (leaf / pair / job / open node)
+__global__ void k_roots(BuildArgs a, BuildState s) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_trees) return;
+    uint32_t beg = a.tree_beg[t], n = a.tree_n[t], base = a.tree_base[t];
+    a.depth[t] = 0;
+    if (n == 1) {
+        emit_leaf(a, base, VRGI_INVALID, beg);
+    } else if (n == 2) {
+        emit_pair(a, base, 0, 0, VRGI_INVALID, beg, beg + 1);
+        a.depth[t] = 1;
+    } else {
+        OpenNode o{beg, n, 0, 0, VRGI_INVALID, t, 0, 0};
+        if (n <= (uint32_t)a.sub_max) {
+            uint32_t j = atomicAdd(s.n_jobs, 1u);
+            ((OpenNode *)s.jobs)[j] = o;
+        } else {
+            uint32_t j = atomicAdd(s.n_open_next, 1u);
+            ((OpenNode *)s.open_next)[j] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- top phase
+// items for the open nodes of this level + reset of their bounds / bins
+__global__ void k_items(BuildState s) {
+    // single block: scan over open nodes of ceil(count / CHUNK)
+    __shared__ uint32_t s_sum[1024];
+    const OpenNode *open = (const OpenNode *)s.open;
+    uint32_t n_open = *s.n_open;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n_open; base += 1024) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t c = i < n_open ? (open[i].count + CHUNK - 1) / CHUNK : 0;
+        s_sum[threadIdx.x] = c;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint32_t v = threadIdx.x >= (uint32_t)off ? s_sum[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_sum[threadIdx.x] += v;
+            __syncthreads();
+        }
+        uint32_t excl = carry + s_sum[threadIdx.x] - c;
+        if (i < n_open) {
+            s.item_beg[i] = excl;
+            for (uint32_t k = 0; k < c; ++k) {
+                Item it;
+                it.open = i;
+                it.beg = open[i].first + k * CHUNK;
+                it.len = min((uint32_t)CHUNK, open[i].count - k * CHUNK);
+                it.pad = 0;
+                ((Item *)s.items)[excl + k] = it;
+            }
+            s.item_end[i] = excl + c;
+        }
+        carry += s_sum[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *s.n_items = carry;
+}
+
+__global__ void k_clear_node_data(BuildState s) {
+    uint32_t o = blockIdx.x;
+    if (o >= *s.n_open) return;
+    for (uint32_t k = threadIdx.x; k < BIN_WORDS; k += blockDim.x) {
+        uint32_t w = k % 7;
+        s.bins[(size_t)o * BIN_WORDS + k] = w == 0 ? 0u : (w <= 3 ? (uint32_t)KEY_MIN_ID : (uint32_t)KEY_MAX_ID);
+    }
+    if (threadIdx.x < 6) s.bounds[o * 6 + threadIdx.x] = threadIdx.x < 3 ? KEY_MIN_ID : KEY_MAX_ID;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// centroid bounds of each open node (find_best_plane's first loop, build.rs:60-66)
+__global__ __launch_bounds__(TOP_THREADS) void k_bounds(BuildArgs a, BuildState s) {
+    if (blockIdx.x >= *s.n_items) return;
+    Item it = ((const Item *)s.items)[blockIdx.x];
+    int mn[3] = {KEY_MIN_ID, KEY_MIN_ID, KEY_MIN_ID}, mx[3] = {KEY_MAX_ID, KEY_MAX_ID, KEY_MAX_ID};
+    for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
+        uint32_t i = it.beg + k;
+        for (int d = 0; d < 3; ++d) {
+            int key = fkey(a.pc[d * a.n + i]);
+            mn[d] = min(mn[d], key);
+            mx[d] = max(mx[d], key);
+        }
+    }
+    __shared__ int red[TOP_THREADS / 64][6];
+    int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    for (int d = 0; d < 3; ++d) {
+        int a0 = wave_min(mn[d]), a1 = wave_max(mx[d]);
+        if (l == 0) { red[w][d] = a0; red[w][3 + d] = a1; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int v = red[0][threadIdx.x];
+        for (int k = 1; k < TOP_THREADS / 64; ++k)
+            v = threadIdx.x < 3 ? min(v, red[k][threadIdx.x]) : max(v, red[k][threadIdx.x]);
+        int *dst = s.bounds + it.open * 6 + threadIdx.x;
+        if (threadIdx.x < 3) atomicMin(dst, v);
+        else atomicMax(dst, v);
+    }
+}
+
+// bin counts and bin boxes (build.rs:68-85)
+__global__ __launch_bounds__(TOP_THREADS) void k_bin(BuildArgs a, BuildState s) {
+    if (blockIdx.x >= *s.n_items) return;
+    Item it = ((const Item *)s.items)[blockIdx.x];
+    __shared__ int sb[BIN_WORDS];
+    for (int k = threadIdx.x; k < BIN_WORDS; k += TOP_THREADS) {
+        int w = k % 7;
+        sb[k] = w == 0 ? 0 : (w <= 3 ? KEY_MIN_ID : KEY_MAX_ID);
+    }
+    float bmin[3], scale[3];
+    for (int d = 0; d < 3; ++d) {
+        float lo = funkey(s.bounds[it.open * 6 + d]), hi = funkey(s.bounds[it.open * 6 + 3 + d]);
+        bmin[d] = lo;
+        scale[d] = (float)BINS / (hi - lo);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
+        uint32_t i = it.beg + k;
+        uint32_t p = a.perm[i];
+        int bk[6];
+        for (int q = 0; q < 6; ++q) bk[q] = fkey(a.prim9[q * a.n + p]);
+        for (int d = 0; d < 3; ++d) {
+            uint32_t b = bin_index(a.pc[d * a.n + i], bmin[d], scale[d]);
+            int *bin = sb + (d * BINS + b) * 7;
+            atomicAdd(bin, 1);
+            atomicMin(bin + 1, bk[0]); atomicMin(bin + 2, bk[1]); atomicMin(bin + 3, bk[2]);
+            atomicMax(bin + 4, bk[3]); atomicMax(bin + 5, bk[4]); atomicMax(bin + 6, bk[5]);
+        }
+    }
+    __syncthreads();
+    int *g = (int *)s.bins + (size_t)it.open * BIN_WORDS;
+    for (int k = threadIdx.x; k < BIN_WORDS; k += TOP_THREADS) {
+        int bin = k / 7, w = k % 7;
+        if (sb[bin * 7] == 0) continue;
+        if (w == 0) atomicAdd(g + k, sb[k]);
+        else if (w <= 3) atomicMin(g + k, sb[k]);
+        else atomicMax(g + k, sb[k]);
+    }
+}
+
+// SAH sweep over the 3 x 31 planes (build.rs:87-127) for one node held as bins in
+// `bins` (3*32*7 ints, LDS or global). Executed by one full wave; returns the
+// chosen (dim, split) and the node box keys on every lane.
+__device__ void sah_sweep_wave(const int *bins, const int *bounds, uint32_t *dim_out,
+                               float *split_out, int box_out[6]) {
+    const int lane = threadIdx.x % 64;
+    float best_c = __int_as_float(0x7f800000);  // +inf marks "no candidate"
+    uint32_t best_i = 0xFFFFu;
+    for (int pass = 0; pass < 2; ++pass) {
+        int d = pass == 0 ? lane / 32 : 2;
+        int b = lane % 32;
+        bool active = pass == 0 || lane < 32;
+        const int *bn = bins + (d * BINS + b) * 7;
+        uint32_t cnt = active ? (uint32_t)bn[0] : 0u;
+        int k[6];
+        for (int q = 0; q < 6; ++q) k[q] = active ? bn[1 + q] : (q < 3 ? KEY_MIN_ID : KEY_MAX_ID);
+        // inclusive prefix (bins 0..b) and suffix (bins b..31) within 32-lane segments
+        uint32_t lc = cnt, rc = cnt;
+        int lk[6], rk[6];
+        for (int q = 0; q < 6; ++q) { lk[q] = k[q]; rk[q] = k[q]; }
+        for (int off = 1; off < 32; off <<= 1) {
+            uint32_t uc = __shfl_up(lc, off, 32);
+            uint32_t dc = __shfl_down(rc, off, 32);
+            int uk[6], dk[6];
+            for (int q = 0; q < 6; ++q) { uk[q] = __shfl_up(lk[q], off, 32); dk[q] = __shfl_down(rk[q], off, 32); }
+            if (b >= off) {
+                lc += uc;
+                for (int q = 0; q < 6; ++q) lk[q] = q < 3 ? min(lk[q], uk[q]) : max(lk[q], uk[q]);
+            }
+            if (b + off < 32) {
+                rc += dc;
+                for (int q = 0; q < 6; ++q) rk[q] = q < 3 ? min(rk[q], dk[q]) : max(rk[q], dk[q]);
+            }
+        }
+        // plane b: left = prefix(b), right = suffix(b + 1)
+        uint32_t nr = __shfl_down(rc, 1, 32);
+        int rr[6];
+        for (int q = 0; q < 6; ++q) rr[q] = __shfl_down(rk[q], 1, 32);
+        if (active && b < 31) {
+            float al = lc ? area_keys(lk[0], lk[1], lk[2], lk[3], lk[4], lk[5]) : 0.0f;
+            float ar = nr ? area_keys(rr[0], rr[1], rr[2], rr[3], rr[4], rr[5]) : 0.0f;
+            float cost = al * (float)lc + ar * (float)nr;  // cost_function, build.rs:287-289
+            uint32_t idx = (uint32_t)(d * 31 + b);
+            if (cost < 3.40282347e+38f) {
+                if (cost < best_c || (cost == best_c && idx < best_i)) { best_c = cost; best_i = idx; }
+            }
+        }
+        if (pass == 0 && lane == 0) {
+            // node box: union over all bins of axis 0 = lane 31's... prefix at b = 31 of segment 0
+        }
+        if (pass == 0) {
+            // the segment-0 prefix at lane 31 is the union of all primitive boxes
+            for (int q = 0; q < 6; ++q) box_out[q] = __shfl(lk[q], 31, 64);
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        float oc = __shfl_xor(best_c, m, 64);
+        uint32_t oi = __shfl_xor(best_i, m, 64);
+        if (oc < best_c || (oc == best_c && oi < best_i)) { best_c = oc; best_i = oi; }
+    }
+    if (best_i == 0xFFFFu) {  // no cost below f32::MAX: initial values (0, 0.0)
+        *dim_out = 0;
+        *split_out = 0.0f;
+    } else {
+        uint32_t d = best_i / 31, pl = best_i % 31;
+        float lo = funkey(bounds[d]), hi = funkey(bounds[3 + d]);
+        float sc = (hi - lo) / (float)BINS;
+        *dim_out = d;
+        *split_out = lo + ((float)pl + 1.0f) * sc;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_split(BuildState s) {
+    uint32_t o = blockIdx.x;
+    if (o >= *s.n_open) return;
+    uint32_t dim;
+    float split;
+    int box[6];
+    sah_sweep_wave((const int *)s.bins + (size_t)o * BIN_WORDS, s.bounds + o * 6, &dim, &split, box);
+    if (threadIdx.x == 0) {
+        SplitInfo si;
+        si.dim = dim;
+        si.split = split;
+        si.T = 0;
+        si.fallback = 0;
+        for (int q = 0; q < 6; ++q) si.box[q] = box[q];
+        ((SplitInfo *)s.split)[o] = si;
+    }
+}
+
+// per-item count of `centroid[dim] < split` (the partition predicate, build.rs:148)
+__global__ __launch_bounds__(TOP_THREADS) void k_count(BuildArgs a, BuildState s) {
+    if (blockIdx.x >= *s.n_items) return;
+    Item it = ((const Item *)s.items)[blockIdx.x];
+    SplitInfo si = ((const SplitInfo *)s.split)[it.open];
+    uint32_t c = 0;
+    for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS)
+        c += a.pc[si.dim * a.n + it.beg + k] < si.split ? 1u : 0u;
+    c = wave_sum(c);
+    __shared__ uint32_t red[TOP_THREADS / 64];
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < TOP_THREADS / 64; ++k) t += red[k];
+        s.item_true[blockIdx.x] = t;
+    }
+}
+
+// per node (one wave): exclusive scan of its items' true counts, T, fallback
+__global__ __launch_bounds__(64) void k_prep(BuildState s) {
+    uint32_t o = blockIdx.x;
+    if (o >= *s.n_open) return;
+    uint32_t ib = s.item_beg[o], ie = s.item_end[o];
+    uint32_t carry = 0;
+    int lane = threadIdx.x;
+    for (uint32_t base = ib; base < ie; base += 64) {
+        uint32_t i = base + lane;
+        uint32_t v = i < ie ? s.item_true[i] : 0u;
+        uint32_t inc = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t u = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += u;
+        }
+        if (i < ie) s.item_off[i] = carry + inc - v;
+        carry += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) {
+        const OpenNode &on = ((const OpenNode *)s.open)[o];
+        SplitInfo *si = (SplitInfo *)s.split + o;
+        si->T = carry;
+        si->fallback = (carry == 0 || carry == on.count) ? 1u : 0u;  // build.rs:152
+    }
+}
+
+// block-wide exclusive scan of one value per thread
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tmp) {
+    int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    uint32_t inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t u = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += u;
+    }
+    if (lane == 63) tmp[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int k = 0; k < w; ++k) pre += tmp[k];
+    __syncthreads();
+    return pre + inc - v;
+}
+
+// partition, pass A: every element learns its exclusive count of trues c_j within the
+// node; right-region trues publish their position at slot T-1-c_j
+__global__ __launch_bounds__(TOP_THREADS) void k_part_a(BuildArgs a, BuildState s) {
+    if (blockIdx.x >= *s.n_items) return;
+    Item it = ((const Item *)s.items)[blockIdx.x];
+    SplitInfo si = ((const SplitInfo *)s.split)[it.open];
+    if (si.fallback) return;
+    const OpenNode on = ((const OpenNode *)s.open)[it.open];
+    __shared__ uint32_t tmp[TOP_THREADS / 64];
+    uint32_t k0 = threadIdx.x * PER_THREAD;
+    uint32_t flags = 0, cnt = 0;
+    for (int q = 0; q < PER_THREAD; ++q) {
+        uint32_t k = k0 + q;
+        bool p = k < it.len && a.pc[si.dim * a.n + it.beg + k] < si.split;
+        flags |= (p ? 1u : 0u) << q;
+        cnt += p ? 1u : 0u;
+    }
+    uint32_t c = s.item_off[blockIdx.x] + block_excl_scan(cnt, tmp);
+    for (int q = 0; q < PER_THREAD; ++q) {
+        uint32_t k = k0 + q;
+        if (k >= it.len) break;
+        uint32_t i = it.beg + k;
+        uint32_t j = i - on.first;
+        bool p = (flags >> q) & 1u;
+        s.rank_tmp[i] = c | (p ? 0x80000000u : 0u);
+        if (p && j >= si.T) s.pair[on.first + (si.T - 1u - c)] = i;
+        c += p ? 1u : 0u;
+    }
+}
+
+// partition, pass B: the r-th left-region false swaps with the r-th true from the right
+__global__ __launch_bounds__(TOP_THREADS) void k_part_b(BuildArgs a, BuildState s) {
+    if (blockIdx.x >= *s.n_items) return;
+    Item it = ((const Item *)s.items)[blockIdx.x];
+    SplitInfo si = ((const SplitInfo *)s.split)[it.open];
+    if (si.fallback) return;
+    const OpenNode on = ((const OpenNode *)s.open)[it.open];
+    for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
+        uint32_t i = it.beg + k;
+        uint32_t j = i - on.first;
+        if (j >= si.T) continue;
+        uint32_t rt = s.rank_tmp[i];
+        if (rt & 0x80000000u) continue;
+        uint32_t r = j - rt;
+        uint32_t p = s.pair[on.first + r];
+        uint32_t t0 = a.perm[i];
+        a.perm[i] = a.perm[p];
+        a.perm[p] = t0;
+        for (int d = 0; d < 3; ++d) {
+            float f = a.pc[d * a.n + i];
+            a.pc[d * a.n + i] = a.pc[d * a.n + p];
+            a.pc[d * a.n + p] = f;
+        }
+    }
+}
+
+// children of each open node: node records, leaves, pairs, jobs, next open nodes
+__global__ void k_children(BuildArgs a, BuildState s) {
+    uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= *s.n_open) return;
+    const OpenNode on = ((const OpenNode *)s.open)[o];
+    const SplitInfo si = ((const SplitInfo *)s.split)[o];
+    uint32_t nl = si.fallback ? on.count / 2 : si.T;
+    uint32_t nr = on.count - nl;
+    uint32_t base = a.tree_base[on.tree];
+    const float pad = 0.0001f;
+    float mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) { mn[k] = funkey(si.box[k]) - pad; mx[k] = funkey(si.box[3 + k]) + pad; }
+    uint32_t li = 2 * on.rank + 1, ri = 2 * on.rank + 2;
+    write_node(a.nodes + base + on.node, li, ri, mn, 0.0f, mx, on.parent);
+    uint32_t ln = atomicAdd(s.n_level_nodes, 1u);
+    s.level_nodes[2 * ln] = base + on.node;
+    s.level_nodes[2 * ln + 1] = base;
+    atomicMax(a.depth + on.tree, on.depth + 1);
+    for (int side = 0; side < 2; ++side) {
+        uint32_t cf = side ? on.first + nl : on.first;
+        uint32_t cc = side ? nr : nl;
+        uint32_t cr = side ? on.rank + nl : on.rank + 1;
+        uint32_t ci = side ? ri : li;
+        if (cc == 1) {
+            emit_leaf(a, base + ci, on.node, a.perm[cf]);
+        } else if (cc == 2) {
+            emit_pair(a, base, ci, cr, on.node, a.perm[cf], a.perm[cf + 1]);
+            atomicMax(a.depth + on.tree, on.depth + 2);
+        } else {
+            OpenNode ch{cf, cc, cr, ci, on.node, on.tree, on.depth + 1, 0};
+            if (cc <= (uint32_t)a.sub_max) ((OpenNode *)s.jobs)[atomicAdd(s.n_jobs, 1u)] = ch;
+            else ((OpenNode *)s.open_next)[atomicAdd(s.n_open_next, 1u)] = ch;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- bottom phase
+// One 64-lane wave builds a whole subtree of <= SUB primitives held in LDS. Nodes
+// are split one at a time (an LDS stack of pending ranges, right pushed first);
+// ranges of 1 and 2 primitives are emitted by their parent directly.
+template <int SUB>
+__global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
+    constexpr int STK = SUB / 3 + 2;  // pending ranges are disjoint and hold >= 3 primitives
+    __shared__ uint32_t s_perm[SUB];
+    __shared__ float s_box[9][SUB];  // min xyz, max xyz, centroid xyz
+    __shared__ uint32_t s_pair[SUB];
+    __shared__ uint32_t s_cj[SUB];
+    __shared__ int s_bins[BIN_WORDS];
+    __shared__ float s_pow[2 * SUB];
+    __shared__ uint32_t s_split_local[SUB];
+    __shared__ uint32_t s_split_g[SUB];
+    __shared__ uint32_t s_stack[STK][7];  // first, count, rank, node, local id, depth, parent
+
+    const uint32_t job = blockIdx.x;
+    if (job >= *s.n_jobs) return;
+    const OpenNode jn = ((const OpenNode *)s.jobs)[job];
+    const int lane = threadIdx.x;
+    const uint32_t c = jn.count;
+    const uint32_t base = a.tree_base[jn.tree];
+    for (uint32_t k = lane; k < c; k += 64) {
+        uint32_t p = a.perm[jn.first + k];
+        s_perm[k] = p;
+        for (int q = 0; q < 6; ++q) s_box[q][k] = a.prim9[q * a.n + p];
+        for (int q = 0; q < 3; ++q) s_box[6 + q][k] = a.pc[q * a.n + jn.first + k];
+    }
+    if (lane == 0) {
+        s_stack[0][0] = 0; s_stack[0][1] = c; s_stack[0][2] = jn.rank; s_stack[0][3] = jn.node;
+        s_stack[0][4] = 0; s_stack[0][5] = jn.depth; s_stack[0][6] = jn.parent;
+    }
+    uint32_t sp = 1, nsplit = 0, maxdepth = jn.depth;
+    __syncthreads();
+    const float pad = 0.0001f;
+    while (sp > 0) {
+        --sp;
+        const uint32_t nf = s_stack[sp][0], nc = s_stack[sp][1], rank = s_stack[sp][2];
+        const uint32_t gidx = s_stack[sp][3], loc = s_stack[sp][4], dep = s_stack[sp][5];
+        const uint32_t parent = s_stack[sp][6];
+        // ---- centroid bounds (build.rs:60-66)
+        int bmn[3] = {KEY_MIN_ID, KEY_MIN_ID, KEY_MIN_ID}, bmx[3] = {KEY_MAX_ID, KEY_MAX_ID, KEY_MAX_ID};
+        for (uint32_t k = lane; k < nc; k += 64)
+            for (int d = 0; d < 3; ++d) {
+                int key = fkey(s_box[6 + d][nf + k]);
+                bmn[d] = min(bmn[d], key);
+                bmx[d] = max(bmx[d], key);
+            }
+        int bounds[6];
+        for (int d = 0; d < 3; ++d) { bounds[d] = wave_min(bmn[d]); bounds[3 + d] = wave_max(bmx[d]); }
+        // ---- bins (build.rs:68-85)
+        for (int k = lane; k < BIN_WORDS; k += 64) {
+            int w = k % 7;
+            s_bins[k] = w == 0 ? 0 : (w <= 3 ? KEY_MIN_ID : KEY_MAX_ID);
+        }
+        float lo[3], sc[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = funkey(bounds[d]);
+            sc[d] = (float)BINS / (funkey(bounds[3 + d]) - lo[d]);
+        }
+        __syncthreads();
+        for (uint32_t k = lane; k < nc; k += 64) {
+            uint32_t e = nf + k;
+            int bk[6];
+            for (int q = 0; q < 6; ++q) bk[q] = fkey(s_box[q][e]);
+            for (int d = 0; d < 3; ++d) {
+                uint32_t b = bin_index(s_box[6 + d][e], lo[d], sc[d]);
+                int *bin = s_bins + (d * BINS + b) * 7;
+                atomicAdd(bin, 1);
+                atomicMin(bin + 1, bk[0]); atomicMin(bin + 2, bk[1]); atomicMin(bin + 3, bk[2]);
+                atomicMax(bin + 4, bk[3]); atomicMax(bin + 5, bk[4]); atomicMax(bin + 6, bk[5]);
+            }
+        }
+        __syncthreads();
+        uint32_t dim;
+        float split;
+        int box[6];
+        sah_sweep_wave(s_bins, bounds, &dim, &split, box);
+        // ---- partition (build.rs:145-157)
+        const uint32_t iters = (nc + 63) / 64;
+        uint32_t T = 0;
+        for (uint32_t itn = 0; itn < iters; ++itn) {
+            uint32_t k = itn * 64 + lane;
+            bool p = k < nc && s_box[6 + dim][nf + k] < split;
+            T += (uint32_t)__popcll(__ballot(p));
+        }
+        const bool fb = (T == 0 || T == nc);
+        if (!fb) {
+            uint32_t run = 0;
+            for (uint32_t itn = 0; itn < iters; ++itn) {
+                uint32_t k = itn * 64 + lane;
+                bool p = k < nc && s_box[6 + dim][nf + k] < split;
+                uint64_t bal = __ballot(p);
+                uint32_t cj = run + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                if (k < nc) {
+                    s_cj[k] = cj | (p ? 0x80000000u : 0u);
+                    if (p && k >= T) s_pair[T - 1 - cj] = k;
+                }
+                run += (uint32_t)__popcll(bal);
+            }
+            __syncthreads();
+            for (uint32_t k = lane; k < T; k += 64) {
+                uint32_t cj = s_cj[k];
+                if (cj & 0x80000000u) continue;
+                uint32_t e0 = nf + k, e1 = nf + s_pair[k - cj];
+                uint32_t tp = s_perm[e0]; s_perm[e0] = s_perm[e1]; s_perm[e1] = tp;
+                for (int q = 0; q < 9; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
+            }
+            __syncthreads();
+        }
+        const uint32_t nl = fb ? nc / 2 : T, nr = nc - nl;
+        // ---- this node's record (power filled in by the reverse pass)
+        const uint32_t li = 2 * rank + 1, ri = 2 * rank + 2;
+        const uint32_t my_split = nsplit;
+        if (lane == 0) {
+            float mn[3], mx[3];
+            for (int k = 0; k < 3; ++k) { mn[k] = funkey(box[k]) - pad; mx[k] = funkey(box[3 + k]) + pad; }
+            write_node(a.nodes + base + gidx, li, ri, mn, 0.0f, mx, parent);
+            s_split_local[my_split] = loc;
+            s_split_g[my_split] = base + gidx;
+        }
+        ++nsplit;
+        maxdepth = max(maxdepth, dep + 1);
+        // ---- children; the right one is pushed first so the left subtree is split first
+        for (int side = 1; side >= 0; --side) {
+            uint32_t cf = side ? nf + nl : nf;
+            uint32_t cc = side ? nr : nl;
+            uint32_t cr = side ? rank + nl : rank + 1;
+            uint32_t ci = side ? ri : li;
+            uint32_t cl = 2 * my_split + 1 + side;  // local id of the child
+            if (cc == 1) {
+                if (lane == 0) {
+                    uint32_t p = s_perm[cf];
+                    emit_leaf(a, base + ci, gidx, p);
+                    s_pow[cl] = a.power[p];
+                }
+            } else if (cc == 2) {
+                // the pair is a split of its own: its leaves get local ids 2k+1, 2k+2
+                if (lane == 0) {
+                    uint32_t p0 = s_perm[cf], p1 = s_perm[cf + 1];
+                    emit_pair(a, base, ci, cr, gidx, p0, p1);
+                    s_split_local[nsplit] = cl;
+                    s_split_g[nsplit] = base + ci;
+                    s_pow[2 * nsplit + 1] = a.power[p0];
+                    s_pow[2 * nsplit + 2] = a.power[p1];
+                }
+                ++nsplit;
+                maxdepth = max(maxdepth, dep + 2);
+            } else {
+                if (lane == 0) {
+                    s_stack[sp][0] = cf; s_stack[sp][1] = cc; s_stack[sp][2] = cr;
+                    s_stack[sp][3] = ci; s_stack[sp][4] = cl; s_stack[sp][5] = dep + 1;
+                    s_stack[sp][6] = gidx;
+                }
+                ++sp;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- power, children before parents: reverse split order (build.rs:345-357)
+    if (lane == 0) {
+        for (int k = (int)nsplit - 1; k >= 0; --k) {
+            float p = 0.0f;
+            p += s_pow[2 * k + 1];
+            p += s_pow[2 * k + 2];
+            s_pow[s_split_local[k]] = p;
+            a.nodes[s_split_g[k]].power = p;
+        }
+        atomicMax(a.depth + jn.tree, maxdepth);
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < c; k += 64) a.perm[jn.first + k] = s_perm[k];
+}
+
+// ---------------------------------------------------------------- host driver
+void Builder::reserve(uint32_t n, uint32_t n_trees) {
+    if (n <= cap_n && n_trees <= cap_t) return;
+    free_all();
+    cap_n = std::max(n, 16u);
+    cap_t = std::max(n_trees, 1u);
+    uint32_t open_cap = cap_n / SUB_MAX_DEFAULT + cap_t + 16;
+    uint32_t item_cap = cap_n / CHUNK + open_cap + 16;
+    auto A = [&](void **p, size_t bytes) { dev_alloc(p, bytes); };
+    A((void **)&perm, (size_t)cap_n * 4);
+    A((void **)&pc, (size_t)cap_n * 12);
+    A((void **)&st.rank_tmp, (size_t)cap_n * 4);
+    A((void **)&st.pair, (size_t)cap_n * 4);
+    A((void **)&st.open, (size_t)open_cap * sizeof(OpenNode));
+    A((void **)&st.open_next, (size_t)open_cap * sizeof(OpenNode));
+    A((void **)&st.jobs, (size_t)(cap_n / 3 + cap_t + 16) * sizeof(OpenNode));
+    A((void **)&st.items, (size_t)item_cap * sizeof(Item));
+    A((void **)&st.item_true, (size_t)item_cap * 4);
+    A((void **)&st.item_off, (size_t)item_cap * 4);
+    A((void **)&st.item_beg, (size_t)open_cap * 4);
+    A((void **)&st.item_end, (size_t)open_cap * 4);
+    A((void **)&st.bins, (size_t)open_cap * BIN_WORDS * 4);
+    A((void **)&st.bounds, (size_t)open_cap * 6 * 4);
+    A((void **)&st.split, (size_t)open_cap * sizeof(SplitInfo));
+    A((void **)&st.level_nodes, (size_t)cap_n * 8 + 64);
+    A((void **)&counters, 64 * 4);
+    st.n_open = counters + 0;
+    st.n_open_next = counters + 1;
+    st.n_jobs = counters + 2;
+    st.n_items = counters + 3;
+    st.n_level_nodes = counters + 4;
+    open_capacity = open_cap;
+}
+
+void Builder::free_all() {
+    void *ptrs[] = {perm, pc, st.rank_tmp, st.pair, st.open, st.open_next, st.jobs, st.items,
+                    st.item_true, st.item_off, st.item_beg, st.item_end, st.bins, st.bounds,
+                    st.split, st.level_nodes, counters};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    perm = nullptr; pc = nullptr; counters = nullptr;
+    st = BuildState{};
+    cap_n = cap_t = 0;
+}
+
+Builder::~Builder() { free_all(); }
+
+// power pass kernel over (global node, tree base) pairs
+__global__ void k_power_pairs(vrgi_bvh_node *nodes, const uint32_t *pairs, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t gi = pairs[2 * i], b = pairs[2 * i + 1];
+    vrgi_bvh_node nd = nodes[gi];
+    float p = 0.0f;
+    p += nodes[b + nd.left].power;
+    p += nodes[b + nd.right_or_prim].power;
+    nodes[gi].power = p;
+}
+
+int Builder::build(hipStream_t stream, const float *prim9, const float *power, const uint32_t *ids,
+                   uint32_t n, uint32_t n_trees, const uint32_t *d_tree_beg,
+                   const uint32_t *d_tree_n, const uint32_t *d_tree_base,
+                   const uint32_t *h_tree_base, vrgi_bvh_node *nodes, uint32_t *d_depth,
+                   uint32_t *prim_order_out) {
+    if (n == 0 || n_trees == 0) return 0;
+    reserve(n, n_trees);
+    BuildArgs a{};
+    a.prim9 = prim9;
+    a.power = power;
+    a.ids = ids;
+    a.n = n;
+    a.n_trees = n_trees;
+    a.tree_beg = d_tree_beg;
+    a.tree_n = d_tree_n;
+    a.tree_base = d_tree_base;
+    a.nodes = nodes;
+    a.depth = d_depth;
+    a.perm = perm;
+    a.pc = pc;
+    a.sub_max = SUB_MAX_DEFAULT;
+    (void)h_tree_base;
+    HIP_CHECK(hipMemsetAsync(counters, 0, 64 * 4, stream));
+    k_init<<<(n + 255) / 256, 256, 0, stream>>>(a);
+    k_roots<<<(n_trees + 255) / 256, 256, 0, stream>>>(a, st);
+    // level loop: open_next -> open
+    uint32_t h_cnt[8];
+    uint32_t level_off = 0;
+    std::vector<uint32_t> lvl_sizes;
+    for (int lvl = 0;; ++lvl) {
+        // move open_next -> open
+        std::swap(st.open, st.open_next);
+        HIP_CHECK(hipMemcpyAsync(st.n_open, st.n_open_next, 4, hipMemcpyDeviceToDevice, stream));
+        HIP_CHECK(hipMemsetAsync(st.n_open_next, 0, 4, stream));
+        HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 5 * 4, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        uint32_t n_open = h_cnt[0];
+        if (n_open == 0) break;
+        if (n_open > open_capacity) return -4;
+        k_items<<<1, 1024, 0, stream>>>(st);
+        k_clear_node_data<<<n_open, 256, 0, stream>>>(st);
+        uint32_t max_items = n / CHUNK + n_open + 1;
+        k_bounds<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+        k_bin<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+        k_split<<<n_open, 64, 0, stream>>>(st);
+        k_count<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+        k_prep<<<n_open, 64, 0, stream>>>(st);
+        k_part_a<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+        k_part_b<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+        // level node list goes to st.level_nodes[level_off ...]: k_children appends
+        k_children<<<(n_open + 127) / 128, 128, 0, stream>>>(a, st);
+        lvl_sizes.push_back(n_open);
+        level_off += n_open;
+    }
+    // jobs
+    HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 5 * 4, hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    uint32_t n_jobs = h_cnt[2];
+    if (n_jobs) k_subtree<SUB_MAX_DEFAULT><<<n_jobs, 64, 0, stream>>>(a, st);
+    // power of the top-phase nodes, deepest level first; st.level_nodes holds
+    // (global index, tree base) pairs, one contiguous run per level
+    uint32_t off = level_off;
+    for (int l = (int)lvl_sizes.size() - 1; l >= 0; --l) {
+        off -= lvl_sizes[l];
+        uint32_t cnt = lvl_sizes[l];
+        k_power_pairs<<<(cnt + 255) / 256, 256, 0, stream>>>(nodes, st.level_nodes + 2 * off, cnt);
+    }
+    if (prim_order_out && prim_order_out != perm)
+        HIP_CHECK(hipMemcpyAsync(prim_order_out, perm, (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
+    return 0;
+}
+
+}  // namespace vrgi

@@ -1,0 +1,166 @@
+// restir.hip — per-pixel resolve kernels for gfx950: outgoing radiance + ReSTIR GI
+// temporal update (fused), spatial resampling + finalize (fused), postprocess.
+//
+// Replaces outgoing_radiance.rs:58-94, the six initial-sample copies of
+// interactive_rendering.rs:1359-1414, restir_temporal_resampling.rs:235-301,
+// restir_spatial_resampling.rs:253-406, restir_finalize.rs:121-141 and
+// postprocess.rs:39-88. Per-bounce data is read only for the bounces a path reached
+// (path_len); the reference's dead-ray entries are the constants of
+// raytrace.rs:406-416 (origin passed through, everything else 0, bsdf_pdf 1), which
+// these kernels substitute without reading them.
+#include <hip/hip_runtime.h>
+
+#include "devcommon.h"
+#include "vrgi_kernels.h"
+
+namespace vrgi {
+
+// reservoir words: x_v 0-2, n_v 3-5, x_s 6-8, n_s 9-11, l_o_hat 12-14, p_omega 15,
+// seed 16 (u32), ucw 17, m 18 (u32), w_sum 19 (restir_temporal_resampling.rs:14-30)
+struct Sample { f3 xv, nv, xs, ns, lo; float pw; uint32_t seed; };
+struct Reservoir { Sample z; float ucw; uint32_t m; float wsum; };
+
+__device__ __forceinline__ Reservoir res_load(const float *p) {
+    const float4 *q = (const float4 *)p;
+    float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+    Reservoir r;
+    r.z.xv = mk(a.x, a.y, a.z); r.z.nv = mk(a.w, b.x, b.y); r.z.xs = mk(b.z, b.w, c.x);
+    r.z.ns = mk(c.y, c.z, c.w); r.z.lo = mk(d.x, d.y, d.z); r.z.pw = d.w;
+    r.z.seed = __float_as_uint(e.x); r.ucw = e.y; r.m = __float_as_uint(e.z); r.wsum = e.w;
+    return r;
+}
+__device__ __forceinline__ void res_store(float *p, const Reservoir &r) {
+    float4 *q = (float4 *)p;
+    q[0] = make_float4(r.z.xv.x, r.z.xv.y, r.z.xv.z, r.z.nv.x);
+    q[1] = make_float4(r.z.nv.y, r.z.nv.z, r.z.xs.x, r.z.xs.y);
+    q[2] = make_float4(r.z.xs.z, r.z.ns.x, r.z.ns.y, r.z.ns.z);
+    q[3] = make_float4(r.z.lo.x, r.z.lo.y, r.z.lo.z, r.z.pw);
+    q[4] = make_float4(__uint_as_float(r.z.seed), r.ucw, __uint_as_float(r.m), r.wsum);
+}
+__device__ __forceinline__ float luminance(f3 v) { return 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z; }
+// updateReservoir, restir_temporal_resampling.rs:235-246
+__device__ __forceinline__ void update_res(uint32_t seed, Reservoir &r, const Sample &z, float w) {
+    r.wsum += w;
+    r.m += 1;
+    if (float_construct(seed) < w / r.wsum) r.z = z;
+}
+
+// outgoing_radiance (backward recurrence) + initial sample + temporal update
+__global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= fa.P) return;
+    const uint32_t plen = fa.path_len[id];
+    const float bsp = 1.0f / (2.0f * VRGI_PI_F);
+    f3 L = mk(0, 0, 0), L1 = mk(0, 0, 0);
+    float q0 = bsp;
+    for (int b = (int)plen - 1; b >= 0; --b) {
+        const size_t bi = (size_t)b * fa.P + id;
+        float bp = fa.pdf[bi], w = fa.mis[bi];
+        float np = (b + 1 < (int)plen) ? fa.nee[bi] : 0.0f;  // written only for continued rays
+        float q = np * w + (1.0f - w) * bsp;
+        float rf = bp / q;
+        L = ld3(fa.emis + bi * 3) + ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;
+        if (b == 1) L1 = L;
+        if (b == 0) q0 = q;
+    }
+    st3(fa.radiance + (size_t)id * 3, L);
+    // initial sample (interactive_rendering.rs:1359-1414)
+    Sample S;
+    S.xv = ld3(fa.origin + (size_t)id * 3);
+    S.nv = ld3(fa.normal + (size_t)id * 3);
+    S.xs = plen >= 2 ? ld3(fa.origin + ((size_t)fa.P + id) * 3) : S.xv;
+    S.ns = plen >= 2 ? ld3(fa.normal + ((size_t)fa.P + id) * 3) : mk(0, 0, 0);
+    S.lo = L1;
+    S.pw = q0;
+    S.seed = 0;
+    // restir_temporal_resampling.rs:256-301
+    const uint32_t ps = mm_combine(fa.tseed, id);
+    float *slot = fa.temporal + (size_t)id * 20;
+    Reservoir R = res_load(slot);
+    if (mm_finalizef(mm_combine(ps, 0)) < 0.4f) { R.wsum = 0.0f; R.m = 0; }
+    float w = luminance(S.lo) / S.pw;
+    update_res(mm_finalize(mm_combine(ps, 1)), R, S, w);
+    float ph = luminance(R.z.lo);
+    R.ucw = ph > 0.0f ? R.wsum / ((float)R.m * ph) : 0.0f;
+    res_store(slot, R);
+}
+
+// restir_spatial_resampling.rs:316-406 + restir_finalize.rs:121-141
+__global__ __launch_bounds__(256) void k_spatial_finalize(const FrameArgs fa) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= fa.P) return;
+    const uint32_t qx = id % fa.W, qy = id / fa.W;
+    const uint32_t ps = mm_combine(fa.sseed, id);
+    Reservoir Rs = res_load(fa.temporal + (size_t)id * 20);
+    const Sample S = Rs.z;
+    const float cos25 = 0.906307787036650f;  // cos(radians(25.0))
+    uint32_t Z = luminance(Rs.z.lo) > 0.0f ? Rs.m : 0u;  // Q[0] = the unmerged reservoir
+    for (uint32_t it = 0; it < fa.spatial_iters; ++it) {
+        uint32_t is = mm_combine(ps, it);
+        float jx = fa.radius * (mm_finalizef(mm_combine(is, 0)) - 0.5f);
+        float jy = fa.radius * (mm_finalizef(mm_combine(is, 1)) - 0.5f);
+        int nx = (int)(((float)qx + 0.5f) + jx), ny = (int)(((float)qy + 0.5f) + jy);
+        nx = min(max(nx, 0), (int)fa.W - 1);
+        ny = min(max(ny, 0), (int)fa.H - 1);
+        Reservoir Rn = res_load(fa.temporal + ((size_t)ny * fa.W + nx) * 20);
+        // geometricallySimilar, :277-288
+        if (zero3(S.nv) || zero3(Rn.z.nv)) continue;
+        if (dot3(S.nv, Rn.z.nv) < cos25) continue;
+        // computeJacobianQToR, :299-311
+        float J;
+        if (zero3(Rn.z.ns)) {
+            J = 1.0f;
+        } else {
+            f3 qv = Rn.z.xv - Rn.z.xs, rv = S.xv - Rn.z.xs;
+            float cq = fabsf(dot3(norm3(qv), Rn.z.ns));
+            float cr = fabsf(dot3(norm3(rv), Rn.z.ns));
+            J = (cr / cq) * (len3(qv) / len3(rv));
+        }
+        J = gmin(gmax(J, 0.1f), 10.0f);
+        float padj = luminance(Rn.z.lo) / J;
+        uint32_t m0 = Rs.m;  // mergeReservoir, :266-275
+        update_res(mm_finalize(mm_combine(is, 2)), Rs, Rn.z, (padj * Rn.ucw) * (float)Rn.m);
+        Rs.m = m0 + Rn.m;
+        if (luminance(Rn.z.lo) > 0.0f) Z += Rn.m;
+    }
+    float ph = luminance(Rs.z.lo);
+    Rs.ucw = ph == 0.0f ? 0.0f : Rs.wsum / ((float)Z * ph);
+    res_store(fa.spatial + (size_t)id * 20, Rs);
+    // finalize
+    f3 x1 = ld3(fa.origin + (size_t)id * 3);
+    f3 dir = norm3(Rs.z.xs - x1);
+    float bp = dot3(dir, Rs.z.nv) / VRGI_PI_F;
+    f3 Lf = ld3(fa.emis + (size_t)id * 3) + ((ld3(fa.refl + (size_t)id * 3) * Rs.z.lo) * Rs.ucw) * bp;
+    st3(fa.restir + (size_t)id * 3, Lf);
+    float dv = Rs.ucw / 10.0f;
+    st3(fa.debug + (size_t)id * 3, mk(dv, dv, dv));
+}
+
+// postprocess.rs:39-88 with scale 1: view select, BGRA8 (clamped; the reference leaves
+// out-of-range conversion undefined)
+__global__ __launch_bounds__(256) void k_postprocess(const FrameArgs fa) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= fa.P) return;
+    const float *src = fa.debug_view == 0 ? fa.radiance : (fa.debug_view == 1 ? fa.restir : fa.debug);
+    f3 c = ld3(src + (size_t)id * 3) / (float)(1 * 1);
+    auto cv = [](float v) -> uint32_t {
+        float x = v * 255.0f;
+        if (!(x > 0.0f)) return 0u;
+        if (x >= 255.0f) return 255u;
+        return (uint32_t)x;
+    };
+    uint32_t px = cv(c.z) | (cv(c.y) << 8) | (cv(c.x) << 16) | (255u << 24);
+    ((uint32_t *)fa.bgra8)[id] = px;
+}
+
+void launch_accum_temporal(hipStream_t s, const FrameArgs &fa) {
+    k_accum_temporal<<<(fa.P + 255) / 256, 256, 0, s>>>(fa);
+}
+void launch_spatial_finalize(hipStream_t s, const FrameArgs &fa) {
+    k_spatial_finalize<<<(fa.P + 255) / 256, 256, 0, s>>>(fa);
+}
+void launch_postprocess(hipStream_t s, const FrameArgs &fa) {
+    k_postprocess<<<(fa.P + 255) / 256, 256, 0, s>>>(fa);
+}
+
+}  // namespace vrgi

@@ -1,0 +1,97 @@
+// vrgi_internal.h — host/device interfaces shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/vrgi.h"
+
+namespace vrgi {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(expr)                                                                     \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::vrgi::HipError(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+inline void dev_alloc(void **p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    HIP_CHECK(hipMalloc(p, bytes));
+}
+
+// largest subtree built by one wave with its primitives in LDS
+static constexpr int SUB_MAX_DEFAULT = 256;
+
+// inputs of one batched build: n primitives, prim9 = 9 planes of n floats
+// (box min xyz, box max xyz, centroid xyz), per-primitive power and id; tree t
+// owns primitives [tree_beg[t], tree_beg[t] + tree_n[t]) and writes its
+// 2*tree_n[t]-1 nodes at nodes + tree_base[t].
+struct BuildArgs {
+    const float *prim9;
+    const float *power;
+    const uint32_t *ids;
+    uint32_t n;
+    uint32_t n_trees;
+    const uint32_t *tree_beg;
+    const uint32_t *tree_n;
+    const uint32_t *tree_base;
+    vrgi_bvh_node *nodes;
+    uint32_t *depth;
+    uint32_t *perm;  // working permutation -> final primitive order per leaf position
+    float *pc;       // centroids permuted along with perm (3 planes)
+    int sub_max;
+};
+
+struct BuildState {
+    uint32_t *rank_tmp = nullptr;
+    uint32_t *pair = nullptr;
+    void *open = nullptr;
+    void *open_next = nullptr;
+    void *jobs = nullptr;
+    void *items = nullptr;
+    uint32_t *item_true = nullptr;
+    uint32_t *item_off = nullptr;
+    uint32_t *item_beg = nullptr;
+    uint32_t *item_end = nullptr;
+    uint32_t *bins = nullptr;
+    int *bounds = nullptr;
+    void *split = nullptr;
+    uint32_t *level_nodes = nullptr;
+    uint32_t *n_open = nullptr;
+    uint32_t *n_open_next = nullptr;
+    uint32_t *n_jobs = nullptr;
+    uint32_t *n_items = nullptr;
+    uint32_t *n_level_nodes = nullptr;
+};
+
+// Owns the builder's workspace (grown on demand, reused across builds).
+class Builder {
+  public:
+    ~Builder();
+    // Queues a batched build on `stream`; synchronises the stream once per top-phase
+    // level to size the next level's grids. prim_order_out (may be null) receives the
+    // final primitive index per leaf position (global over all trees).
+    int build(hipStream_t stream, const float *prim9, const float *power, const uint32_t *ids,
+              uint32_t n, uint32_t n_trees, const uint32_t *d_tree_beg, const uint32_t *d_tree_n,
+              const uint32_t *d_tree_base, const uint32_t *h_tree_base, vrgi_bvh_node *nodes,
+              uint32_t *d_depth, uint32_t *prim_order_out);
+    const uint32_t *prim_order() const { return perm; }
+
+  private:
+    void reserve(uint32_t n, uint32_t n_trees);
+    void free_all();
+    uint32_t cap_n = 0, cap_t = 0, open_capacity = 0;
+    uint32_t *perm = nullptr;
+    float *pc = nullptr;
+    uint32_t *counters = nullptr;
+    BuildState st;
+};
+
+}  // namespace vrgi

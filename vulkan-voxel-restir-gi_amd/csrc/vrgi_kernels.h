@@ -1,0 +1,245 @@
+// vrgi_kernels.h — kernel argument blocks, traversal primitives and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "devcommon.h"
+
+namespace vrgi {
+
+#define VRGI_MAX_STACK 160  // traversal stack entries per lane (LDS); deeper trees are rejected
+#define VRGI_BOX_PAD 0.001f  // slack added to every box of the traversal copies (see box_hit)
+
+// ---------------------------------------------------------------- argument blocks
+struct SceneView {
+    const WideNode *gwide;     // geometry BVH, traversal copy (one entry per internal node)
+    const Tri *tri;            // world triangles by global id
+    uint32_t n_tris;
+    uint32_t root_prim;        // primitive of the root when the tree is a single leaf
+    uint32_t depth;
+    const uint32_t *tri_inst;  // instance of each global triangle
+    const DevInst *inst;
+    uint32_t n_inst;
+    const vrgi_bvh_node *tl;   // light top-level BVH
+    const WideNode *lwide;     // light-triangle BVH, traversal copy
+    const Tri *ltri;           // world light triangles by global light id
+    uint32_t n_ltri;
+    uint32_t lroot_prim;
+    uint32_t ldepth;
+    const uint32_t *ltri_inst;  // instance of each light triangle
+    const uint32_t *ltri_node;  // LightVertex3D.bvh_node_idx of each light triangle
+    const vrgi_material *mats;
+    uint32_t n_mats;
+};
+
+struct BounceArgs {
+    uint32_t bounce, B, W, H, P;
+    uint32_t inv_seed;  // frame * B + bounce (interactive_rendering.rs:1189)
+    uint32_t nee_type;
+    uint32_t rr_start;
+    float rr_min;
+    vrgi_camera cam;
+    const uint32_t *queue_in;
+    uint32_t *queue_out;
+    uint32_t *qcount;  // live rays per bounce
+    // per bounce b, [B][P][3] / [B][P]: outputs of bounce b (origin/dir = ray b + 1)
+    float *origin, *dir, *normal, *emis, *refl, *mis, *pdf, *nee, *hit_t;
+    uint32_t *hit, *path_len;
+    unsigned long long *counters;  // [0] closest-hit rays, [1] light rays, [2] frames
+};
+
+struct FlattenArgs {
+    const DevInst *inst;
+    uint32_t n_inst;
+    const vrgi_vertex3d *const *vtx;        // per instance: object vertices
+    const vrgi_light_vertex3d *const *lvtx;  // per instance: object light vertices
+    const uint32_t *light_off;              // per instance: first global light id
+    uint32_t n_tris;
+    Tri *tri;
+    uint32_t *tri_inst;
+    uint32_t *light_node;
+    float *prim9;         // builder planes of prim_stride floats; this list starts at prim_off
+    float *power;
+    uint32_t *ids;
+    uint32_t prim_stride, prim_off;
+};
+
+struct FrameArgs {
+    uint32_t W, H, P, B, frame, slot;
+    uint32_t tseed, sseed;
+    uint32_t spatial_iters;
+    float radius;
+    uint32_t debug_view;
+    const float *origin, *normal, *emis, *refl, *mis, *pdf, *nee;
+    const uint32_t *path_len;
+    float *radiance;   // [P][3]
+    float *temporal;   // [P][20], slot frame % 3
+    float *spatial;    // [P][20]
+    float *restir;     // [P][3]
+    float *debug;      // [P][3]
+    uint8_t *bgra8;    // [P][4]
+};
+
+// ---------------------------------------------------------------- primitives
+// builder inputs of one triangle: Aabb::from_points (aabb.rs:10-22) and the centroid
+// ((v0 + v1) + v2) / 3 (build.rs:267-270); planes of n floats
+__device__ __forceinline__ void write_prim9(float *prim9, size_t n, uint32_t g, const f3 w[3]) {
+    float mn[3] = {w[0].x, w[0].y, w[0].z}, mx[3] = {w[0].x, w[0].y, w[0].z};
+    for (int k = 0; k < 3; ++k) {
+        mn[0] = gmin(mn[0], w[k].x); mn[1] = gmin(mn[1], w[k].y); mn[2] = gmin(mn[2], w[k].z);
+        mx[0] = gmax(mx[0], w[k].x); mx[1] = gmax(mx[1], w[k].y); mx[2] = gmax(mx[2], w[k].z);
+    }
+    f3 c = ((w[0] + w[1]) + w[2]) / 3.0f;
+    prim9[0 * (size_t)n + g] = mn[0]; prim9[1 * (size_t)n + g] = mn[1]; prim9[2 * (size_t)n + g] = mn[2];
+    prim9[3 * (size_t)n + g] = mx[0]; prim9[4 * (size_t)n + g] = mx[1]; prim9[5 * (size_t)n + g] = mx[2];
+    prim9[6 * (size_t)n + g] = c.x; prim9[7 * (size_t)n + g] = c.y; prim9[8 * (size_t)n + g] = c.z;
+}
+
+// Woop, Benthin, Wald: per-ray shear constants
+struct RayPre {
+    f3 o, d, inv;
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+__device__ __forceinline__ RayPre ray_pre(f3 o, f3 d) {
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    r.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    r.kx = r.kz == 2 ? 0 : r.kz + 1;
+    r.ky = r.kx == 2 ? 0 : r.kx + 1;
+    float dz = comp3(d, r.kz);
+    if (dz < 0.0f) { int t = r.kx; r.kx = r.ky; r.ky = t; }
+    r.sx = comp3(d, r.kx) / dz;
+    r.sy = comp3(d, r.ky) / dz;
+    r.sz = 1.0f / dz;
+    r.inv = mk(d.x != 0.0f ? 1.0f / d.x : 1e30f, d.y != 0.0f ? 1.0f / d.y : 1e30f,
+               d.z != 0.0f ? 1.0f / d.z : 1e30f);
+    return r;
+}
+
+// watertight ray/triangle test; on a hit in [tmin, tmax] returns t and the weights
+// (u, v) of vertices 1 and 2
+__device__ __forceinline__ bool tri_hit(const RayPre &r, f3 v0, f3 v1, f3 v2, float tmin, float tmax,
+                                        float *t_out, float *u_out, float *v_out) {
+    f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    float Akz = comp3(A, r.kz), Bkz = comp3(B, r.kz), Ckz = comp3(C, r.kz);
+    float Ax = comp3(A, r.kx) - r.sx * Akz, Ay = comp3(A, r.ky) - r.sy * Akz;
+    float Bx = comp3(B, r.kx) - r.sx * Bkz, By = comp3(B, r.ky) - r.sy * Bkz;
+    float Cx = comp3(C, r.kx) - r.sx * Ckz, Cy = comp3(C, r.ky) - r.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {  // edge case: exact products in binary64
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.sz * Akz, Bz = r.sz * Bkz, Cz = r.sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= tmin && t <= tmax)) return false;
+    *t_out = t;
+    *u_out = V * rcp;
+    *v_out = W * rcp;
+    return true;
+}
+
+// conservative slab test (Ize 2013: far distance scaled by 1 + 2 gamma(3)); boxes of
+// the traversal copies carry VRGI_BOX_PAD of slack so that a triangle hit reported at
+// rounding distance outside its exact box is never culled
+__device__ __forceinline__ bool box_hit_t(const RayPre &r, float mnx, float mny, float mnz, float mxx,
+                                          float mxy, float mxz, float tmin, float tmax, float *tn) {
+    float tx0 = (mnx - r.o.x) * r.inv.x, tx1 = (mxx - r.o.x) * r.inv.x;
+    float ty0 = (mny - r.o.y) * r.inv.y, ty1 = (mxy - r.o.y) * r.inv.y;
+    float tz0 = (mnz - r.o.z) * r.inv.z, tz1 = (mxz - r.o.z) * r.inv.z;
+    float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    *tn = t0;
+    return t0 <= t1 * 1.00000024f;
+}
+__device__ __forceinline__ bool box_hit(const RayPre &r, float mnx, float mny, float mnz, float mxx,
+                                        float mxy, float mxz, float tmin, float tmax) {
+    float tn;
+    return box_hit_t(r, mnx, mny, mnz, mxx, mxy, mxz, tmin, tmax, &tn);
+}
+
+struct Hit {
+    uint32_t prim;
+    float t, u, v;
+};
+
+// closest hit over the geometry BVH; equal t resolves to the lower primitive id.
+// stack: STACK * 64 LDS words of this wave, entry k of lane l at [k * 64 + l].
+template <int STACK>
+__device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r, float tmin,
+                                           float tmax, uint32_t *stack) {
+    Hit h;
+    h.prim = VRGI_INVALID;
+    h.t = 0.0f; h.u = 0.0f; h.v = 0.0f;
+    if (sv.n_tris == 0) return h;
+    float best = tmax;
+    uint32_t *stk = stack + (threadIdx.x & 63);
+    auto test = [&](uint32_t prim) {
+        Tri t = sv.tri[prim];
+        float tt, u, v;
+        if (tri_hit(r, mk(t.v0.x, t.v0.y, t.v0.z), mk(t.v1.x, t.v1.y, t.v1.z), mk(t.v2.x, t.v2.y, t.v2.z),
+                    tmin, best, &tt, &u, &v)) {
+            if (tt < best || (tt == best && prim < h.prim)) {
+                best = tt;
+                h.prim = prim; h.t = tt; h.u = u; h.v = v;
+            }
+        }
+    };
+    if (sv.n_tris == 1) {
+        test(sv.root_prim);
+        return h;
+    }
+    uint32_t ref = 0;
+    int sp = 0;
+    for (;;) {
+        WideNode wn = sv.gwide[ref];
+        float tn0, tn1;
+        bool h0 = box_hit_t(r, wn.a.x, wn.a.y, wn.a.z, wn.a.w, wn.b.x, wn.b.y, tmin, best, &tn0);
+        bool h1 = box_hit_t(r, wn.b.z, wn.b.w, wn.c.x, wn.c.y, wn.c.z, wn.c.w, tmin, best, &tn1);
+        uint32_t r0 = wn.r.x, r1 = wn.r.y;
+        if (h0 && (r0 & 0x80000000u)) { test(r0 & 0x7FFFFFFFu); h0 = false; }
+        if (h1 && (r1 & 0x80000000u)) { test(r1 & 0x7FFFFFFFu); h1 = false; }
+        if (h0 && h1) {
+            bool first0 = tn0 <= tn1;
+            stk[(sp++) * 64] = first0 ? r1 : r0;
+            ref = first0 ? r0 : r1;
+        } else if (h0) {
+            ref = r0;
+        } else if (h1) {
+            ref = r1;
+        } else {
+            if (sp == 0) break;
+            ref = stk[(--sp) * 64];
+        }
+    }
+    return h;
+}
+
+// ---------------------------------------------------------------- launchers
+void launch_flatten(hipStream_t s, const FlattenArgs &f);
+void launch_flatten_lights(hipStream_t s, const FlattenArgs &f);
+void launch_wide(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, WideNode *wide);
+void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, const float *d,
+                          uint32_t n, float tmin, float tmax, uint32_t *hit, float *t, float *uv);
+void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays);
+void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays);
+void launch_accum_temporal(hipStream_t s, const FrameArgs &fa);
+void launch_spatial_finalize(hipStream_t s, const FrameArgs &fa);
+void launch_postprocess(hipStream_t s, const FrameArgs &fa);
+void launch_prims_from_tris(hipStream_t s, const float *tri9, uint32_t n, float *prim9);
+void launch_tl_inputs(hipStream_t s, const float *aabb6, const float *xf12, uint32_t n, float *prim9);
+void launch_tl_idx(hipStream_t s, const vrgi_bvh_node *tl, uint32_t n_nodes, DevInst *inst);
+void launch_frame_stats(hipStream_t s, const uint32_t *qcount, uint32_t B, uint32_t P,
+                        unsigned long long *counters);
+
+}  // namespace vrgi

@@ -1,0 +1,305 @@
+"""ctypes harness over libvrgi.so (include/vrgi.h) for tests and benchmarks.
+
+The product is the C ABI; this module only loads it, converts numpy arrays to plain
+pointers and raises on a non-zero status. It never falls back to anything: when
+libvrgi.so is missing it raises ImportError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvrgi.so")
+
+VERTEX_DTYPE = np.dtype([("position", "<f4", 3), ("t", "<u4"), ("uv", "<f4", 2)])
+NODE_DTYPE = np.dtype([("left", "<u4"), ("right", "<u4"), ("min", "<f4", 3), ("power", "<f4"),
+                       ("max", "<f4", 3), ("parent", "<u4")])
+MATERIAL_DTYPE = np.dtype([("reflectivity", "<f4", 4), ("emissivity", "<f4", 4), ("metallicity", "<f4", 4)])
+assert VERTEX_DTYPE.itemsize == 24 and NODE_DTYPE.itemsize == 40 and MATERIAL_DTYPE.itemsize == 48
+
+(BUF_GEOM_BVH, BUF_GEOM_PRIM_ORDER, BUF_LIGHT_TL_BVH, BUF_LIGHT_GEOM_BVH, BUF_PATH_LEN,
+ BUF_BOUNCE_ORIGIN, BUF_BOUNCE_DIRECTION, BUF_BOUNCE_NORMAL, BUF_BOUNCE_EMISSIVITY,
+ BUF_BOUNCE_REFLECTIVITY, BUF_BOUNCE_MIS_WEIGHT, BUF_BOUNCE_BSDF_PDF, BUF_BOUNCE_NEE_PDF,
+ BUF_RADIANCE, BUF_RESTIR, BUF_TEMPORAL_RESERVOIR, BUF_SPATIAL_RESERVOIR, BUF_BGRA8,
+ BUF_HIT_PRIM, BUF_HIT_T, BUF_DI_LIGHT, BUF_DI_WEIGHTS) = range(22)
+
+FLAG_REBUILD_EVERY_FRAME = 1
+FLAG_KEEP_BOUNCES = 2
+
+
+class Config(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("num_bounces", C.c_uint32),
+                ("rr_start_bounce", C.c_uint32), ("rr_min_survival", C.c_float),
+                ("restir_mode", C.c_uint32), ("restir_spatial_radius", C.c_float),
+                ("di_candidates", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("front", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3)]
+
+
+class Prefs(C.Structure):
+    _fields_ = [("nee_type", C.c_uint32), ("debug_view", C.c_uint32),
+                ("restir_spatial_iterations", C.c_uint32)]
+
+
+class FrameOut(C.Structure):
+    _fields_ = [("radiance", C.c_void_p), ("restir", C.c_void_p), ("bgra8", C.c_void_p),
+                ("frame_index", C.c_uint64)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("rays_closest", C.c_uint64), ("rays_light", C.c_uint64),
+                ("bvh_nodes", C.c_uint64), ("bvh_depth", C.c_uint32), ("pad", C.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libvrgi.so once; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.vrgi_last_error.restype = C.c_char_p
+        L.vrgi_abi_version.restype = C.c_uint32
+        _lib = L
+    return _lib
+
+
+class VrgiError(RuntimeError):
+    pass
+
+
+def check(code):
+    if code != 0:
+        raise VrgiError(f"vrgi status {code}: {lib().vrgi_last_error().decode()}")
+
+
+def ptr(a):
+    """Pointer argument that keeps `a` alive until the call returns (data_as holds a
+    reference to the array, so temporaries built in the call expression stay valid)."""
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# ------------------------------------------------------------------ caller-side scenes
+def world(kind, seed=0):
+    """Seeded voxel world of a benchmark workload: (blocks u16 [dx,dy,dz], n_types)."""
+    L = lib()
+    dims = (C.c_uint32 * 3)()
+    nt = C.c_uint32()
+    check(L.vrgi_world_dims(kind, dims, C.byref(nt)))
+    blocks = np.zeros(tuple(dims), np.uint16)
+    check(L.vrgi_world_generate(kind, C.c_uint64(seed), ptr(blocks)))
+    return blocks, nt.value
+
+
+def world_materials(kind, seed=0):
+    L = lib()
+    ns = C.c_uint32()
+    check(L.vrgi_world_materials(kind, C.c_uint64(seed), None, None, 0, C.byref(ns)))
+    mats = np.zeros(ns.value, MATERIAL_DTYPE)
+    lum = np.zeros(ns.value, np.float32)
+    check(L.vrgi_world_materials(kind, C.c_uint64(seed), ptr(mats), ptr(lum), ns.value, C.byref(ns)))
+    return mats, lum
+
+
+def world_translucent(kind, n_types):
+    tr = np.zeros(n_types, np.uint8)
+    check(lib().vrgi_world_translucent(kind, ptr(tr), n_types))
+    return tr
+
+
+def gen_mesh_chunk(blocks, n_types, translucent, cx, cy, cz):
+    """chunk::gen_mesh of the 32^3 chunk (cx, cy, cz): structured vertex array."""
+    dims = np.array(blocks.shape, np.uint32)
+    blocks = np.ascontiguousarray(blocks, np.uint16)
+    n = C.c_size_t()
+    check_code = lib().vrgi_gen_mesh_chunk(ptr(blocks), ptr(dims), n_types, ptr(translucent), cx, cy, cz,
+                                           None, C.c_size_t(0), C.byref(n))
+    if check_code not in (0, -4):
+        check(check_code)
+    out = np.zeros(n.value, VERTEX_DTYPE)
+    check(lib().vrgi_gen_mesh_chunk(ptr(blocks), ptr(dims), n_types, ptr(translucent), cx, cy, cz,
+                                    ptr(out), C.c_size_t(n.value), C.byref(n)))
+    return out
+
+
+def chunk_objects(blocks, n_types, translucent):
+    """Meshes every 32^3 chunk: list of (key, transform[12], vertices); empty chunks
+    are kept (they become Empty handles, as in the reference's chunk manager)."""
+    nx, ny, nz = (s // 32 for s in blocks.shape)
+    objs = []
+    for cx in range(nx):
+        for cy in range(ny):
+            for cz in range(nz):
+                v = gen_mesh_chunk(blocks, n_types, translucent, cx, cy, cz)
+                xf = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 32 * cx, 32 * cy, 32 * cz], np.float32)
+                objs.append(((cx * ny + cy) * nz + cz, xf, v))
+    return objs
+
+
+def camera_look(eye, target):
+    """Camera vectors as the reference's SphericalCamera builds them (camera.rs:19-33):
+    world up (0, -1, 0), right = front x worldup, up = right x front."""
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    r = np.cross(f, [0.0, -1.0, 0.0])
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    u /= np.linalg.norm(u)
+    cam = Camera()
+    cam.eye[:] = eye.astype(np.float32)
+    cam.front[:] = f.astype(np.float32)
+    cam.right[:] = r.astype(np.float32)
+    cam.up[:] = u.astype(np.float32)
+    return cam
+
+
+def camera_spline(kind, frame, n_frames=16, seed=0):
+    """Seeded Catmull-Rom camera path over the world of `kind` (frame in [0, n_frames))."""
+    rng = np.random.default_rng(1000 + seed)
+    if kind == 0:
+        lo, hi, ylo, yhi, span = 8.0, 56.0, 60.0, 72.0, 64.0
+    else:
+        lo, hi, ylo, yhi, span = 32.0, 224.0, 128.0, 150.0, 256.0
+    pts = np.stack([rng.uniform(lo, hi, 6), rng.uniform(ylo, yhi, 6), rng.uniform(lo, hi, 6)], 1)
+
+    def cr(t):
+        seg = min(int(t * 3), 2)
+        u = t * 3 - seg
+        p0, p1, p2, p3 = pts[seg], pts[seg + 1], pts[seg + 2], pts[seg + 3]
+        return 0.5 * ((2 * p1) + (-p0 + p2) * u + (2 * p0 - 5 * p1 + 4 * p2 - p3) * u * u
+                      + (-p0 + 3 * p1 - 3 * p2 + p3) * u ** 3)
+
+    t = frame / max(n_frames - 1, 1) * 0.98
+    eye = cr(t)
+    ahead = cr(min(t + 0.02, 1.0))
+    d = ahead - eye
+    d[1] = 0.0
+    if np.linalg.norm(d) < 1e-6:
+        d = np.array([1.0, 0.0, 0.0])
+    d /= np.linalg.norm(d)
+    target = eye + d * span * 0.25 + np.array([0.0, -0.45 * span * 0.25, 0.0])
+    return camera_look(eye, target)
+
+
+# ------------------------------------------------------------------ library objects
+class Context:
+    """One vrgi_ctx: Scene + SceneUploader + Renderer of the reference in one handle."""
+
+    def __init__(self, cfg, mats, lum, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        self.cfg = cfg
+        check(self.L.vrgi_create(C.byref(cfg), ptr(mats), ptr(lum), len(lum), device, C.byref(self.h)))
+        self.objects = {}
+
+    def close(self):
+        if self.h:
+            for o in self.objects.values():
+                if o:
+                    self.L.vrgi_object_release(o)
+            self.objects = {}
+            check(self.L.vrgi_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, verts):
+        o = C.c_void_p()
+        v = np.ascontiguousarray(verts, VERTEX_DTYPE)
+        check(self.L.vrgi_upload_object(self.h, ptr(v), C.c_size_t(len(v)), C.byref(o)))
+        return o
+
+    def add(self, key, obj, xf):
+        check(self.L.vrgi_scene_add(self.h, C.c_uint32(key), obj, ptr(np.ascontiguousarray(xf, np.float32))))
+
+    def add_objects(self, objs):
+        for key, xf, v in objs:
+            o = self.upload(v)
+            self.add(key, o, xf)
+            if o:
+                self.L.vrgi_object_release(o)  # the scene holds its own reference
+
+    def update(self, key, xf):
+        check(self.L.vrgi_scene_update(self.h, C.c_uint32(key), ptr(np.ascontiguousarray(xf, np.float32))))
+
+    def remove(self, key):
+        check(self.L.vrgi_scene_remove(self.h, C.c_uint32(key)))
+
+    def render(self, cam, nee_type=0, debug_view=0, spatial=0):
+        p = Prefs(nee_type, debug_view, spatial)
+        out = FrameOut()
+        check(self.L.vrgi_render(self.h, C.byref(cam), C.byref(p), C.byref(out)))
+        return out
+
+    def sync(self):
+        check(self.L.vrgi_sync(self.h))
+
+    def stats(self, reset=False):
+        st = FrameStats()
+        check(self.L.vrgi_stats(self.h, C.byref(st), int(reset)))
+        return st
+
+    def read(self, which, dtype=np.float32):
+        n = C.c_size_t()
+        check(self.L.vrgi_buffer_size(self.h, which, C.byref(n)))
+        if n.value == 0:
+            return np.zeros(0, dtype)
+        out = np.zeros(n.value // np.dtype(dtype).itemsize, dtype)
+        check(self.L.vrgi_read_buffer(self.h, which, ptr(out), C.c_size_t(out.nbytes)))
+        return out
+
+    def trace_closest(self, o, d, tmin=1e-3, tmax=1000.0):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        n = len(o)
+        hit = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        uv = np.zeros((n, 2), np.float32)
+        check(self.L.vrgi_trace_closest(self.h, ptr(o), ptr(d), C.c_uint64(n), C.c_float(tmin),
+                                        C.c_float(tmax), ptr(hit), ptr(t), ptr(uv)))
+        return hit, t, uv
+
+    def build_bl_bvh(self, tri, lum, ids):
+        tri = np.ascontiguousarray(tri, np.float32).reshape(-1, 9)
+        n = len(tri)
+        nodes = np.zeros(2 * n - 1, NODE_DTYPE)
+        order = np.zeros(n, np.uint32)
+        box = np.zeros(6, np.float32)
+        pw = C.c_float()
+        check(self.L.vrgi_build_bl_bvh(self.h, ptr(tri), ptr(np.ascontiguousarray(lum, np.float32)),
+                                       ptr(np.ascontiguousarray(ids, np.uint32)), C.c_uint64(n),
+                                       ptr(nodes), ptr(order), ptr(box), C.byref(pw)))
+        return nodes, order, box, pw.value
+
+    def build_tl_bvh(self, xf, aabbs, powers, ids):
+        m = len(powers)
+        nodes = np.zeros(2 * m - 1, NODE_DTYPE)
+        check(self.L.vrgi_build_tl_bvh(self.h, ptr(np.ascontiguousarray(xf, np.float32)),
+                                       ptr(np.ascontiguousarray(aabbs, np.float32)),
+                                       ptr(np.ascontiguousarray(powers, np.float32)),
+                                       ptr(np.ascontiguousarray(ids, np.uint32)), C.c_uint64(m), ptr(nodes)))
+        return nodes
+
+
+def config(width, height, bounces=3, rr_start=0xFFFFFFFF, rr_min=0.1, seed=0, flags=0, radius=15.0):
+    cfg = Config()
+    lib().vrgi_config_reference(C.byref(cfg), width, height)
+    cfg.num_bounces = bounces
+    cfg.rr_start_bounce = rr_start
+    cfg.rr_min_survival = rr_min
+    cfg.seed = seed
+    cfg.flags = flags
+    cfg.restir_spatial_radius = radius
+    return cfg
