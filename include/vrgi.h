@@ -154,6 +154,30 @@ int vrgi_render(vrgi_ctx *ctx, const vrgi_camera *cam, const vrgi_prefs *prefs,
 int vrgi_sync(vrgi_ctx *ctx);
 int vrgi_stats(vrgi_ctx *ctx, vrgi_frame_stats *st, int reset);
 
+/* ---- measurement (no counterpart in the reference: it only prints fps, main.rs:354-361) ----
+ * VRGI_PROFILE_STAGES: every render() records HIP events around its stages on the
+ * context's stream and waits for the frame to accumulate their durations.
+ * VRGI_PROFILE_TOUCH: the next render() counts the distinct BVH nodes and triangles
+ * each bounce touches (a counting instantiation of the trace kernel; that frame's
+ * timings are not representative). */
+#define VRGI_PROFILE_STAGES 1u
+#define VRGI_PROFILE_TOUCH 2u
+typedef struct vrgi_profile_out {
+    uint64_t frames;             /* frames accumulated */
+    double build_ms;             /* flatten + batched BVH builds + traversal copies */
+    double trace_ms[16];         /* trace + shade kernel, per bounce */
+    double nee_ms[16];           /* NEE pdf kernel, per bounce */
+    double resolve_ms;           /* outgoing radiance + ReSTIR + postprocess */
+    double frame_ms;             /* first to last event of the frame */
+    uint64_t rays[16];           /* rays traced per bounce (summed over frames) */
+    uint64_t touched_bytes[16];  /* unique node (64 B) + triangle (48 B) bytes per bounce, last TOUCH frame */
+    uint64_t build_prims;        /* primitives of the last geometry build (both trees) */
+    uint64_t build_leaf_depth_sum; /* sum over leaves of their depth, last build */
+    uint64_t build_nodes;        /* nodes written by the last build */
+} vrgi_profile_out;
+int vrgi_profile(vrgi_ctx *ctx, uint32_t flags);
+int vrgi_profile_read(vrgi_ctx *ctx, vrgi_profile_out *out, int reset);
+
 /* ---- readback / test hooks (host copies; each synchronises the stream) ---- */
 enum vrgi_buffer_id {
     VRGI_BUF_GEOM_BVH = 0,       /* vrgi_bvh_node[2N-1], geometry BVH (build_bl_bvh semantics) */

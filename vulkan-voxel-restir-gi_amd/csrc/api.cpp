@@ -122,6 +122,14 @@ struct vrgi_ctx {
     uint64_t frame_count = 0;
     bool have_frame = false;
 
+    // measurement
+    uint32_t profile_flags = 0;
+    vrgi_profile_out prof{};
+    std::vector<hipEvent_t> events;
+    DevBuf<uint32_t> touch_bits;
+    DevBuf<unsigned long long> touch_count;
+    DevBuf<unsigned long long> depth_sum_copy;
+
     // host-side scratch of the standalone operators
     DevBuf<float> a_f0, a_f1, a_f2;
     DevBuf<uint32_t> a_u0, a_u1, a_u2;
@@ -359,6 +367,7 @@ bool valid_xf(const float *xf) {
 vrgi_ctx::~vrgi_ctx() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (auto &kv : objects) release(kv.second.first);
     for (vrgi_object *o : old) release(o);
     if (stream) (void)hipStreamDestroy(stream);
@@ -638,6 +647,22 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     if (prefs->restir_spatial_iterations > 20) return fail(VRGI_E_INVALID, "restir_spatial_iterations above 20");
     set_dev(c);
     hipStream_t s = c->stream;
+    const uint32_t P = c->P, B = c->B;
+    const bool stages = (c->profile_flags & VRGI_PROFILE_STAGES) != 0;
+    const bool touch = (c->profile_flags & VRGI_PROFILE_TOUCH) != 0;
+    // events: 0 start, 1 after the builds, 2+b after trace b, 2+B+b after NEE b, 2+2B end
+    const size_t n_ev = 3 + 2 * (size_t)B;
+    if (stages)
+        while (c->events.size() < n_ev) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            c->events.push_back(e);
+        }
+    auto rec = [&](size_t k) {
+        if (stages) HIP_CHECK(hipEventRecord(c->events[k], s));
+    };
+    rec(0);
+    bool built = false;
     if (c->inst_dirty) {
         rebuild_instances(c);
         c->inst_dirty = false;
@@ -646,8 +671,9 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     if (c->geo_dirty || (c->cfg.flags & VRGI_FLAG_REBUILD_EVERY_FRAME)) {
         rebuild_geometry(c);
         c->geo_dirty = false;
+        built = true;
     }
-    const uint32_t P = c->P, B = c->B;
+    rec(1);
     const uint32_t frame = (uint32_t)c->frame_count;
     HIP_CHECK(hipMemsetAsync(c->f_qcount.p, 0, (B + 2) * 4, s));
     if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) {
@@ -675,22 +701,39 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     ba.emis = c->f_emis.p; ba.refl = c->f_refl.p; ba.mis = c->f_mis.p; ba.pdf = c->f_pdf.p;
     ba.nee = c->f_nee.p; ba.hit_t = c->f_hit_t.p; ba.hit = c->f_hit.p; ba.path_len = c->f_plen.p;
     ba.counters = (unsigned long long *)c->d_counters.p;
+    size_t touch_words = ((size_t)c->n_tris + 31) / 32;
+    if (touch) {
+        c->touch_bits.reserve(2 * touch_words);
+        c->touch_count.reserve(2 * (size_t)B);
+        HIP_CHECK(hipMemsetAsync(c->touch_count.p, 0, 16 * (size_t)B, s));
+    }
     for (uint32_t b = 0; b < B; ++b) {
         ba.bounce = b;
         ba.inv_seed = frame * B + b;  // interactive_rendering.rs:1189
         ba.queue_in = c->f_queue.p + (size_t)b * P;
         ba.queue_out = c->f_queue.p + (size_t)(b + 1) * P;
+        if (touch) {
+            HIP_CHECK(hipMemsetAsync(c->touch_bits.p, 0, 8 * touch_words, s));
+            ba.touch_nodes = c->touch_bits.p;
+            ba.touch_tris = c->touch_bits.p + touch_words;
+        }
         launch_trace_shade(s, sv, ba, P);
+        if (touch) {
+            launch_popcount(s, ba.touch_nodes, touch_words, c->touch_count.p + 2 * b);
+            launch_popcount(s, ba.touch_tris, touch_words, c->touch_count.p + 2 * b + 1);
+        }
+        rec(2 + b);
     }
-    if (prefs->nee_type != 0) {
-        for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
+    ba.touch_nodes = ba.touch_tris = nullptr;
+    for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
+        if (prefs->nee_type != 0) {
             ba.bounce = b;
             ba.queue_in = c->f_queue.p + (size_t)(b + 1) * P;
             launch_nee_pdf(s, sv, ba, P);
         }
-    } else {
-        // nee_pdf is 0 wherever mis_weight is 0; every live ray has mis_weight 0 here
+        rec(2 + B + b);
     }
+    rec(2 + B + (B - 1));
     FrameArgs fa{};
     fa.W = c->W; fa.H = c->H; fa.P = P; fa.B = B; fa.frame = frame;
     fa.slot = (uint32_t)(c->frame_count % 3);
@@ -712,6 +755,42 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     launch_spatial_finalize(s, fa);
     launch_postprocess(s, fa);
     launch_frame_stats(s, c->f_qcount.p, B, P, (unsigned long long *)c->d_counters.p);
+    rec(2 + 2 * (size_t)B);
+    if (stages || touch) {
+        HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<uint32_t> q(B + 2);
+        HIP_CHECK(hipMemcpy(q.data(), c->f_qcount.p, (B + 2) * 4, hipMemcpyDeviceToHost));
+        vrgi_profile_out &p = c->prof;
+        if (stages) {
+            auto el = [&](size_t a, size_t b) {
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, c->events[a], c->events[b]));
+                return (double)ms;
+            };
+            if (built) p.build_ms += el(0, 1);
+            for (uint32_t b = 0; b < B && b < 16; ++b) {
+                p.trace_ms[b] += el(b == 0 ? 1 : 1 + b, 2 + b);
+                if (b + 1 < B) p.nee_ms[b] += el(b == 0 ? 1 + B : 1 + B + b, 2 + B + b);
+                p.rays[b] += b == 0 ? P : q[b];
+            }
+            p.resolve_ms += el(2 + B + (B - 1), 2 + 2 * (size_t)B);
+            p.frame_ms += el(0, 2 + 2 * (size_t)B);
+            p.frames += 1;
+        }
+        if (touch) {
+            std::vector<unsigned long long> tc(2 * (size_t)B);
+            HIP_CHECK(hipMemcpy(tc.data(), c->touch_count.p, 16 * (size_t)B, hipMemcpyDeviceToHost));
+            for (uint32_t b = 0; b < B && b < 16; ++b) p.touched_bytes[b] = tc[2 * b] * 64ull + tc[2 * b + 1] * 48ull;
+            c->profile_flags &= ~VRGI_PROFILE_TOUCH;
+        }
+        if (built) {
+            unsigned long long S = 0;
+            HIP_CHECK(hipMemcpy(&S, c->geo_builder.leaf_depth_sum(), 8, hipMemcpyDeviceToHost));
+            p.build_leaf_depth_sum = S;
+            p.build_prims = (uint64_t)c->n_tris + c->n_ltri;
+            p.build_nodes = (c->n_tris ? 2ull * c->n_tris - 1 : 0) + (c->n_ltri ? 2ull * c->n_ltri - 1 : 0);
+        }
+    }
     c->have_frame = true;
     if (out) {
         out->radiance = c->f_radiance.p;
@@ -752,6 +831,28 @@ int vrgi_stats(vrgi_ctx *c, vrgi_frame_stats *st, int reset) {
     }
     return VRGI_OK;
     API_END
+}
+
+int vrgi_profile(vrgi_ctx *c, uint32_t flags) {
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    if (flags & ~(VRGI_PROFILE_STAGES | VRGI_PROFILE_TOUCH)) return fail(VRGI_E_INVALID, "unknown profile flag");
+    c->profile_flags = flags;
+    return VRGI_OK;
+}
+
+int vrgi_profile_read(vrgi_ctx *c, vrgi_profile_out *out, int reset) {
+    if (!c || !out) return fail(VRGI_E_INVALID, "null argument");
+    *out = c->prof;
+    if (reset) {
+        vrgi_profile_out keep = c->prof;
+        c->prof = vrgi_profile_out{};
+        // structural figures of the last build and touch frame survive a reset
+        c->prof.build_prims = keep.build_prims;
+        c->prof.build_leaf_depth_sum = keep.build_leaf_depth_sum;
+        c->prof.build_nodes = keep.build_nodes;
+        std::memcpy(c->prof.touched_bytes, keep.touched_bytes, sizeof(keep.touched_bytes));
+    }
+    return VRGI_OK;
 }
 
 int vrgi_buffer_size(vrgi_ctx *c, int which, size_t *bytes) {

@@ -117,6 +117,7 @@ __global__ void k_roots(BuildArgs a, BuildState s) {
     } else if (n == 2) {
         emit_pair(a, base, 0, 0, VRGI_INVALID, beg, beg + 1);
         a.depth[t] = 1;
+        atomicAdd(a.split_count_sum, 2ull);
     } else {
         OpenNode o{beg, n, 0, 0, VRGI_INVALID, t, 0, 0};
         if (n <= (uint32_t)a.sub_max) {
@@ -485,6 +486,7 @@ __global__ void k_children(BuildArgs a, BuildState s) {
     s.level_nodes[2 * ln] = base + on.node;
     s.level_nodes[2 * ln + 1] = base;
     atomicMax(a.depth + on.tree, on.depth + 1);
+    unsigned long long splits = on.count;
     for (int side = 0; side < 2; ++side) {
         uint32_t cf = side ? on.first + nl : on.first;
         uint32_t cc = side ? nr : nl;
@@ -495,12 +497,14 @@ __global__ void k_children(BuildArgs a, BuildState s) {
         } else if (cc == 2) {
             emit_pair(a, base, ci, cr, on.node, a.perm[cf], a.perm[cf + 1]);
             atomicMax(a.depth + on.tree, on.depth + 2);
+            splits += 2;
         } else {
             OpenNode ch{cf, cc, cr, ci, on.node, on.tree, on.depth + 1, 0};
             if (cc <= (uint32_t)a.sub_max) ((OpenNode *)s.jobs)[atomicAdd(s.n_jobs, 1u)] = ch;
             else ((OpenNode *)s.open_next)[atomicAdd(s.n_open_next, 1u)] = ch;
         }
     }
+    atomicAdd(a.split_count_sum, splits);
 }
 
 // ---------------------------------------------------------------- bottom phase
@@ -537,6 +541,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         s_stack[0][4] = 0; s_stack[0][5] = jn.depth; s_stack[0][6] = jn.parent;
     }
     uint32_t sp = 1, nsplit = 0, maxdepth = jn.depth;
+    unsigned long long splits = 0;  // sum of split node sizes of this subtree
     __syncthreads();
     const float pad = 0.0001f;
     while (sp > 0) {
@@ -626,6 +631,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             s_split_g[my_split] = base + gidx;
         }
         ++nsplit;
+        splits += nc;
         maxdepth = max(maxdepth, dep + 1);
         // ---- children; the right one is pushed first so the left subtree is split first
         for (int side = 1; side >= 0; --side) {
@@ -651,6 +657,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                     s_pow[2 * nsplit + 2] = a.power[p1];
                 }
                 ++nsplit;
+                splits += 2;
                 maxdepth = max(maxdepth, dep + 2);
             } else {
                 if (lane == 0) {
@@ -673,6 +680,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             a.nodes[s_split_g[k]].power = p;
         }
         atomicMax(a.depth + jn.tree, maxdepth);
+        atomicAdd(a.split_count_sum, splits);
     }
     __syncthreads();
     for (uint32_t k = lane; k < c; k += 64) a.perm[jn.first + k] = s_perm[k];
@@ -758,6 +766,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     a.perm = perm;
     a.pc = pc;
     a.sub_max = SUB_MAX_DEFAULT;
+    a.split_count_sum = (unsigned long long *)(counters + 8);
     (void)h_tree_base;
     HIP_CHECK(hipMemsetAsync(counters, 0, 64 * 4, stream));
     k_init<<<(n + 255) / 256, 256, 0, stream>>>(a);

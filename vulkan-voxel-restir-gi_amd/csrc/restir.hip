@@ -56,7 +56,9 @@ __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
     for (int b = (int)plen - 1; b >= 0; --b) {
         const size_t bi = (size_t)b * fa.P + id;
         float bp = fa.pdf[bi], w = fa.mis[bi];
-        float np = (b + 1 < (int)plen) ? fa.nee[bi] : 0.0f;  // written only for continued rays
+        // written only for continued rays, and only when NEE is on (nee == nullptr otherwise:
+        // mis_weight is then 0 everywhere and the reference's nee_pdf is 0)
+        float np = (fa.nee != nullptr && b + 1 < (int)plen) ? fa.nee[bi] : 0.0f;
         float q = np * w + (1.0f - w) * bsp;
         float rf = bp / q;
         L = ld3(fa.emis + bi * 3) + ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;

@@ -231,7 +231,7 @@ __device__ __forceinline__ IcS local_cs(f3 a, f3 b, f3 c) {  // raytrace.rs:319-
     return r;
 }
 
-template <int STACK>
+template <int STACK, bool COUNT>
 __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const BounceArgs ba) {
     __shared__ uint32_t stack[STACK * 64];
     const uint32_t b = ba.bounce;
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
     h.prim = VRGI_INVALID;
     if (active) {
         RayPre r = ray_pre(o, d);
-        h = closest_hit<STACK>(sv, r, VRGI_EPS_RT, 1000.0f, stack);
+        h = closest_hit<STACK, COUNT>(sv, r, VRGI_EPS_RT, 1000.0f, stack, ba.touch_nodes, ba.touch_tris);
     }
     f3 out_o, out_d = mk(0, 0, 0), nrm = mk(0, 0, 0), emis = mk(0, 0, 0), refl = mk(0, 0, 0);
     float mis = 0.0f, pdf = 1.0f;
@@ -438,8 +438,25 @@ void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, co
 }
 void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
     if (!max_rays) return;
-    if (sv.depth + 2 <= 64) k_trace_shade<64><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
-    else k_trace_shade<VRGI_MAX_STACK><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+    bool cnt = ba.touch_nodes != nullptr;
+    if (sv.depth + 2 <= 64) {
+        if (cnt) k_trace_shade<64, true><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+        else k_trace_shade<64, false><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+    } else {
+        if (cnt) k_trace_shade<VRGI_MAX_STACK, true><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+        else k_trace_shade<VRGI_MAX_STACK, false><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+    }
+}
+
+// number of set bits of a bitmap, added to *out
+__global__ void k_popcount(const uint32_t *bits, size_t n_words, unsigned long long *out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long c = i < n_words ? (unsigned long long)__popc(bits[i]) : 0ull;
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsigned long long *out) {
+    if (n_words) k_popcount<<<(unsigned)((n_words + 255) / 256), 256, 0, s>>>(bits, n_words, out);
 }
 void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
     if (!max_rays) return;

@@ -44,6 +44,8 @@ struct BuildArgs {
     const uint32_t *tree_base;
     vrgi_bvh_node *nodes;
     uint32_t *depth;
+    // sum over split nodes of their primitive count (= sum over leaves of their depth)
+    unsigned long long *split_count_sum;
     uint32_t *perm;  // working permutation -> final primitive order per leaf position
     float *pc;       // centroids permuted along with perm (3 planes)
     int sub_max;
@@ -83,6 +85,8 @@ class Builder {
               const uint32_t *d_tree_base, const uint32_t *h_tree_base, vrgi_bvh_node *nodes,
               uint32_t *d_depth, uint32_t *prim_order_out);
     const uint32_t *prim_order() const { return perm; }
+    // device address of the last build's sum over leaves of their depth
+    const unsigned long long *leaf_depth_sum() const { return (const unsigned long long *)(counters + 8); }
 
   private:
     void reserve(uint32_t n, uint32_t n_trees);

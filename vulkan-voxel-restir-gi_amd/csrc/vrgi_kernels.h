@@ -45,6 +45,7 @@ struct BounceArgs {
     float *origin, *dir, *normal, *emis, *refl, *mis, *pdf, *nee, *hit_t;
     uint32_t *hit, *path_len;
     unsigned long long *counters;  // [0] closest-hit rays, [1] light rays, [2] frames
+    uint32_t *touch_nodes, *touch_tris;  // non-null: counting launch (measurement only)
 };
 
 struct FlattenArgs {
@@ -175,9 +176,13 @@ struct Hit {
 
 // closest hit over the geometry BVH; equal t resolves to the lower primitive id.
 // stack: STACK * 64 LDS words of this wave, entry k of lane l at [k * 64 + l].
-template <int STACK>
+// COUNT (measurement only): marks every visited internal node / tested triangle in the
+// bitmaps touch_nodes / touch_tris.
+template <int STACK, bool COUNT = false>
 __device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r, float tmin,
-                                           float tmax, uint32_t *stack) {
+                                           float tmax, uint32_t *stack,
+                                           uint32_t *touch_nodes = nullptr,
+                                           uint32_t *touch_tris = nullptr) {
     Hit h;
     h.prim = VRGI_INVALID;
     h.t = 0.0f; h.u = 0.0f; h.v = 0.0f;
@@ -185,6 +190,7 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r,
     float best = tmax;
     uint32_t *stk = stack + (threadIdx.x & 63);
     auto test = [&](uint32_t prim) {
+        if (COUNT) atomicOr(touch_tris + (prim >> 5), 1u << (prim & 31));
         Tri t = sv.tri[prim];
         float tt, u, v;
         if (tri_hit(r, mk(t.v0.x, t.v0.y, t.v0.z), mk(t.v1.x, t.v1.y, t.v1.z), mk(t.v2.x, t.v2.y, t.v2.z),
@@ -202,6 +208,7 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r,
     uint32_t ref = 0;
     int sp = 0;
     for (;;) {
+        if (COUNT) atomicOr(touch_nodes + (ref >> 5), 1u << (ref & 31));
         WideNode wn = sv.gwide[ref];
         float tn0, tn1;
         bool h0 = box_hit_t(r, wn.a.x, wn.a.y, wn.a.z, wn.a.w, wn.b.x, wn.b.y, tmin, best, &tn0);
@@ -241,5 +248,6 @@ void launch_tl_inputs(hipStream_t s, const float *aabb6, const float *xf12, uint
 void launch_tl_idx(hipStream_t s, const vrgi_bvh_node *tl, uint32_t n_nodes, DevInst *inst);
 void launch_frame_stats(hipStream_t s, const uint32_t *qcount, uint32_t B, uint32_t P,
                         unsigned long long *counters);
+void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsigned long long *out);
 
 }  // namespace vrgi

@@ -185,7 +185,9 @@ def camera_spline(kind, frame, n_frames=16, seed=0):
     if np.linalg.norm(d) < 1e-6:
         d = np.array([1.0, 0.0, 0.0])
     d /= np.linalg.norm(d)
-    target = eye + d * span * 0.25 + np.array([0.0, -0.45 * span * 0.25, 0.0])
+    # look at the world's interior, swinging with the path's heading
+    centre = np.array([span * 0.5, 0.15 * span if kind == 1 else 16.0, span * 0.5])
+    target = centre + d * span * 0.15
     return camera_look(eye, target)
 
 
