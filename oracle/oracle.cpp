@@ -978,7 +978,158 @@ inline void update_res(uint32_t seed, Reservoir &r, const Sample &z, float w) {
 struct or_renderer {
     Cfg c;
     std::vector<float> temporal;  // [3][P][20]
+    std::vector<float> di_hist;   // [2][P][8] ReSTIR DI history: light, y xyz, W, M, -, -
 };
+
+namespace {
+
+// ---------------------------------------------------------------- ReSTIR DI (benchmark mode)
+// Not in the reference (its ReSTIR is GI). Assembled from the reference's primitives:
+// candidates from traverseBvh (raytrace.rs:218-281) + triangleSample (:305-311),
+// updateReservoir / mergeReservoir (restir_*_resampling.rs), geometricallySimilar
+// (restir_spatial_resampling.rs:277-288). Target p_hat = luminance of the unshadowed
+// contribution f * Le * cos_x * cos_y / d^2 with f = albedo / pi and
+// Le = 1000 * emissivity (raytrace.rs:477); source pdf = pick probability / area.
+struct DiRes {
+    uint32_t light = 0xFFFFFFFFu;
+    v3 y{0, 0, 0};
+    float wsum = 0.0f, W = 0.0f, ph = 0.0f;
+    uint32_t M = 0;
+};
+
+inline v3 di_contrib(const or_scene *s, v3 x, v3 n, v3 fr, uint32_t lg, v3 y) {
+    v3 l0 = tri_v(s->tri, lg, 0), l1 = tri_v(s->tri, lg, 1), l2 = tri_v(s->tri, lg, 2);
+    v3 nl = normalize(cross(l1 - l0, l2 - l0));
+    v3 w = y - x;
+    float d2 = dot(w, w);
+    float dist = std::sqrt(d2);
+    v3 wn = w / dist;
+    float cx = std::max(0.0f, dot(n, wn));
+    float cy = std::max(0.0f, -dot(nl, wn));
+    const Material &m = s->mats[s->tri_t[lg]];
+    v3 le = 1000.0f * V(m.emis[0], m.emis[1], m.emis[2]);
+    return (fr * le) * ((cx * cy) / d2);
+}
+
+inline void di_update(uint32_t seed, DiRes &r, uint32_t light, v3 y, float w, float ph) {
+    r.wsum += w;
+    r.M += 1;
+    if (fconstruct(seed) < w / r.wsum) { r.light = light; r.y = y; r.ph = ph; }
+}
+
+struct DiInputs {  // bounce-0 buffers of the frame ([B][P] layouts, bounce 0 first)
+    const std::vector<uint32_t> &hit;
+    const std::vector<float> &origin, &normal, &emis;
+};
+
+void restir_di(or_renderer *rr, const or_scene *s, const or_prefs *prefs, uint32_t frame,
+               const DiInputs &f, std::vector<float> &fin, std::vector<uint32_t> &di_light,
+               std::vector<float> &di_w) {
+    const Cfg &c = rr->c;
+    const size_t P = (size_t)c.W * c.H;
+    if (rr->di_hist.size() != 2 * P * 8) rr->di_hist.assign(2 * P * 8, 0.0f);
+    float *hist_prev = &rr->di_hist[(size_t)((frame + 1) % 2) * P * 8];
+    float *hist_cur = &rr->di_hist[(size_t)(frame % 2) * P * 8];
+    const uint32_t dseed = frame_seed(c.seed, frame, 4);
+    const float cos25 = 0.906307787036650f;
+    std::vector<DiRes> T(P);
+    std::vector<v3> X(P), N(P), FR(P);
+    std::vector<uint8_t> ok(P, 0);
+    // initial candidates + temporal merge
+    for (size_t id = 0; id < P; ++id) {
+        uint32_t hp = f.hit[id];
+        if (hp == 0xFFFFFFFFu) continue;
+        ok[id] = 1;
+        const Material &mt = s->mats[s->tri_t[hp]];
+        v3 x = ld3(&f.origin[(1 * P + id) * 3]), n = ld3(&f.normal[id * 3]);
+        v3 fr = V(mt.refl[0], mt.refl[1], mt.refl[2]) / PI_F;
+        X[id] = x; N[id] = n; FR[id] = fr;
+        const uint32_t pd = combine(dseed, (uint32_t)id);
+        DiRes r;
+        for (uint32_t k = 0; k < c.di_cand; ++k) {
+            uint32_t sk = combine(pd, k);
+            Pick pk = traverse_light(s, x, n, combine(sk, 0));
+            float w = 0.0f, ph = 0.0f;
+            uint32_t lg = 0xFFFFFFFFu;
+            v3 y = V(0, 0, 0);
+            if (pk.ok) {
+                lg = s->inst_tri_off[pk.inst] + pk.prim;
+                v3 l0 = tri_v(s->tri, lg, 0), l1 = tri_v(s->tri, lg, 1), l2 = tri_v(s->tri, lg, 2);
+                float ux = finalizef(combine(sk, 1)), uy = finalizef(combine(sk, 2));
+                if (ux + uy > 1.0f) { ux = 1.0f - ux; uy = 1.0f - uy; }
+                v3 bb = V((1.0f - ux) - uy, ux, uy);
+                y = (bb.x * l0 + bb.y * l1) + bb.z * l2;
+                float area = 0.5f * length(cross(l1 - l0, l2 - l0));
+                float psrc = pk.prob / area;
+                ph = luminance(di_contrib(s, x, n, fr, lg, y));
+                w = psrc > 0.0f ? ph / psrc : 0.0f;
+            }
+            di_update(finalize(combine(sk, 3)), r, lg, y, w, ph);
+        }
+        // temporal: the previous frame's final reservoir of this pixel, M capped at 20x
+        const float *h = hist_prev + id * 8;
+        uint32_t hl = fbits(h[0]);
+        if (hl != 0xFFFFFFFFu && hl < s->tri_inst.size()) {
+            uint32_t hm = std::min(fbits(h[5]), 20u * c.di_cand);
+            v3 hy = V(h[1], h[2], h[3]);
+            float ph = luminance(di_contrib(s, x, n, fr, hl, hy));
+            uint32_t m0 = r.M;
+            di_update(finalize(combine(pd, 0x10000u)), r, hl, hy, (ph * h[4]) * (float)hm, ph);
+            r.M = m0 + hm;
+        }
+        r.W = r.ph > 0.0f ? r.wsum / ((float)r.M * r.ph) : 0.0f;
+        T[id] = r;
+    }
+    // spatial merge, visibility, final radiance, history
+    for (size_t id = 0; id < P; ++id) {
+        float *hc = hist_cur + id * 8;
+        v3 E0 = ld3(&f.emis[id * 3]);
+        if (!ok[id]) {
+            hc[0] = bitsf(0xFFFFFFFFu); hc[1] = hc[2] = hc[3] = hc[4] = 0.0f; hc[5] = bitsf(0u);
+            st3(&fin[id * 3], E0);
+            continue;
+        }
+        const uint32_t pd = combine(dseed, (uint32_t)id);
+        uint32_t qx = (uint32_t)(id % c.W), qy = (uint32_t)(id / c.W);
+        DiRes r = T[id];
+        v3 x = X[id], n = N[id], fr = FR[id];
+        for (uint32_t it = 0; it < prefs->restir_spatial_iterations; ++it) {
+            uint32_t is = combine(combine(pd, 0x20000u), it);
+            float jx = c.radius * (finalizef(combine(is, 0)) - 0.5f);
+            float jy = c.radius * (finalizef(combine(is, 1)) - 0.5f);
+            int nx = (int)(((float)qx + 0.5f) + jx), ny = (int)(((float)qy + 0.5f) + jy);
+            nx = std::min(std::max(nx, 0), (int)c.W - 1);
+            ny = std::min(std::max(ny, 0), (int)c.H - 1);
+            size_t q = (size_t)ny * c.W + nx;
+            if (!ok[q]) continue;
+            if (is_zero(n) || is_zero(N[q]) || dot(n, N[q]) < cos25) continue;
+            const DiRes &rq = T[q];
+            if (rq.light == 0xFFFFFFFFu) continue;
+            float ph = luminance(di_contrib(s, x, n, fr, rq.light, rq.y));
+            uint32_t m0 = r.M;
+            di_update(finalize(combine(is, 2)), r, rq.light, rq.y, (ph * rq.W) * (float)rq.M, ph);
+            r.M = m0 + rq.M;
+        }
+        r.W = r.ph > 0.0f ? r.wsum / ((float)r.M * r.ph) : 0.0f;
+        v3 L = E0;
+        if (r.light != 0xFFFFFFFFu && r.ph > 0.0f) {
+            v3 w = r.y - x;
+            float dist = std::sqrt(dot(w, w));
+            Hit sh = closest_bvh(s, x, w / dist, EPS_RT, dist - 0.002f, nullptr);
+            if (sh.prim == 0xFFFFFFFFu) L = E0 + di_contrib(s, x, n, fr, r.light, r.y) * r.W;
+        }
+        st3(&fin[id * 3], L);
+        di_light[id] = r.light;
+        di_w[id * 4 + 0] = r.wsum;
+        di_w[id * 4 + 1] = r.W;
+        di_w[id * 4 + 2] = (float)r.M;
+        di_w[id * 4 + 3] = r.ph;
+        hc[0] = bitsf(r.light); hc[1] = r.y.x; hc[2] = r.y.y; hc[3] = r.y.z; hc[4] = r.W;
+        hc[5] = bitsf(r.M);
+    }
+}
+
+}  // namespace
 
 namespace {
 
@@ -1407,6 +1558,9 @@ int or_render(or_renderer *rr, or_scene *s, const or_camera *cam, const or_prefs
         float dv = R.ucw / 10.0f;
         st3(&dbg[id * 3], V(dv, dv, dv));
     }
+    std::vector<uint32_t> di_light(P, 0xFFFFFFFFu);
+    std::vector<float> di_w(P * 4, 0.0f);
+    if (c.mode == 1) restir_di(rr, s, prefs, frame, DiInputs{f.hit, f.origin, f.normal, f.emis}, fin, di_light, di_w);
     if (out->bgra8) {
         for (size_t id = 0; id < P; ++id) {
             v3 col = prefs->debug_view == 0 ? ld3(&f.lout[id * 3])
@@ -1442,6 +1596,8 @@ int or_render(or_renderer *rr, or_scene *s, const or_camera *cam, const or_prefs
     cp(out->spatial, sp.data(), sp.size() * 4);
     cp(out->restir, fin.data(), fin.size() * 4);
     cp(out->debug, dbg.data(), dbg.size() * 4);
+    cp(out->di_light, di_light.data(), P * 4);
+    cp(out->di_weights, di_w.data(), P * 16);
     return 0;
 }
 

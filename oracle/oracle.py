@@ -231,9 +231,10 @@ class Scene:
 
 
 class Renderer:
-    def __init__(self, width, height, bounces=3, rr_start=0xFFFFFFFF, rr_min=0.1, seed=0, radius=15.0):
+    def __init__(self, width, height, bounces=3, rr_start=0xFFFFFFFF, rr_min=0.1, seed=0, radius=15.0,
+                 mode=0, di_candidates=32):
         self.L = lib()
-        self.cfg = Config(width, height, bounces, rr_start, rr_min, 0, radius, 32, seed, 0)
+        self.cfg = Config(width, height, bounces, rr_start, rr_min, mode, radius, di_candidates, seed, 0)
         self.h = C.c_void_p(self.L.or_renderer_create(C.byref(self.cfg)))
         self.W, self.H, self.B = width, height, bounces
 
@@ -260,6 +261,7 @@ class Renderer:
             "path_len": np.zeros(P, np.uint32), "temporal": np.zeros((P, 20), np.float32),
             "spatial": np.zeros((P, 20), np.float32), "restir": np.zeros((P, 3), np.float32),
             "debug": np.zeros((P, 3), np.float32), "bgra8": np.zeros((P, 4), np.uint8),
+            "di_light": np.zeros(P, np.uint32), "di_weights": np.zeros((P, 4), np.float32),
         }
         fr = Frame(**{k: C.c_void_p(v.ctypes.data) for k, v in out.items()})
         p = Prefs(nee_type, debug_view, spatial)
