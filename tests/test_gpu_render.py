@@ -121,3 +121,30 @@ def test_frames_parity(world0, scenes, nee_type, spatial):
             assert c.read(vrgi.BUF_SPATIAL_RESERVOIR).tobytes() == ref["spatial"].tobytes()
             assert c.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
             assert c.read(vrgi.BUF_BGRA8, np.uint8).tobytes() == ref["bgra8"].tobytes()
+
+
+def test_restir_di_parity(world0, scenes):
+    """ReSTIR DI mode (32 candidates, 1 temporal, 5 spatial neighbours within 30 px):
+    reservoir light ids exact and weights (w_sum, W, M, p_hat) within 1e-5 relative
+    (the test requires bit equality), final DI radiance bit-exact, over 3 frames."""
+    import oracle
+    import vrgi
+    s, objs = scenes
+    W, H, B = 96, 64, 3
+    cam = _camera0()
+    r = oracle.Renderer(W, H, B, mode=1, radius=30.0)
+    cfg = vrgi.config(W, H, B, mode=1, radius=30.0)
+    with vrgi.Context(cfg, world0["mats"], world0["lum"]) as c:
+        c.add_objects(objs)
+        for f in range(3):
+            say(f"DI frame {f}")
+            c.render(cam, nee_type=0, spatial=5)
+            c.sync()
+            ref = r.render(s, cam, f, nee_type=0, spatial=5)
+            gl = c.read(vrgi.BUF_DI_LIGHT, np.uint32)
+            assert np.array_equal(gl, ref["di_light"])
+            gw = c.read(vrgi.BUF_DI_WEIGHTS).reshape(-1, 4)
+            np.testing.assert_allclose(gw, ref["di_weights"], rtol=1e-5, atol=0)
+            assert gw.tobytes() == ref["di_weights"].tobytes()
+            assert c.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
+            assert (gl != 0xFFFFFFFF).mean() > 0.1

@@ -118,6 +118,8 @@ struct vrgi_ctx {
     DevBuf<uint32_t> f_hit, f_plen, f_queue, f_qcount;
     DevBuf<float> f_radiance, f_restir, f_debug, f_temporal, f_spatial;
     DevBuf<uint8_t> f_bgra;
+    DevBuf<float> di_T, di_hist, di_w;
+    DevBuf<uint32_t> di_light;
     DevBuf<Counters> d_counters;
     uint64_t frame_count = 0;
     bool have_frame = false;
@@ -188,6 +190,16 @@ void alloc_frame(vrgi_ctx *c) {
     c->f_bgra.reserve(P * 4);
     HIP_CHECK(hipMemsetAsync(c->f_temporal.p, 0, 3 * P * 20 * 4, c->stream));
     HIP_CHECK(hipMemsetAsync(c->f_spatial.p, 0, P * 20 * 4, c->stream));
+    if (c->cfg.restir_mode == 1) {
+        c->di_T.reserve(P * 8);
+        c->di_hist.reserve(2 * P * 8);
+        c->di_w.reserve(P * 4);
+        c->di_light.reserve(P);
+        // an empty history: light = 0xFFFFFFFF in word 0 of every pixel
+        HIP_CHECK(hipMemsetAsync(c->di_hist.p, 0xFF, 2 * P * 8 * 4, c->stream));
+        HIP_CHECK(hipMemsetAsync(c->di_w.p, 0, P * 16, c->stream));
+        HIP_CHECK(hipMemsetAsync(c->di_light.p, 0xFF, P * 4, c->stream));
+    }
     c->have_frame = false;
 }
 
@@ -401,7 +413,9 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
     if (cfg->width == 0 || cfg->height == 0 || cfg->num_bounces < 2 || cfg->num_bounces > 16)
         return fail(VRGI_E_INVALID, "extent must be non-zero and num_bounces in [2, 16]");
     if ((uint64_t)cfg->width * cfg->height > (1ull << 26)) return fail(VRGI_E_LIMIT, "extent above 2^26 pixels");
-    if (cfg->restir_mode != 0) return fail(VRGI_E_INVALID, "restir_mode: only 0 (GI) is available");
+    if (cfg->restir_mode > 1) return fail(VRGI_E_INVALID, "restir_mode must be 0 (GI) or 1 (DI)");
+    if (cfg->restir_mode == 1 && (cfg->di_candidates == 0 || cfg->di_candidates > 1024))
+        return fail(VRGI_E_INVALID, "di_candidates must be in [1, 1024]");
     int ndev = 0;
     HIP_CHECK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(VRGI_E_INVALID, "no such HIP device");
@@ -752,7 +766,24 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     fa.debug = c->f_debug.p;
     fa.bgra8 = c->f_bgra.p;
     launch_accum_temporal(s, fa);
-    launch_spatial_finalize(s, fa);
+    if (c->cfg.restir_mode == 1) {
+        DiArgs da{};
+        da.W = c->W; da.H = c->H; da.P = P; da.frame = frame;
+        da.candidates = c->cfg.di_candidates;
+        da.spatial_iters = prefs->restir_spatial_iterations;
+        da.radius = c->cfg.restir_spatial_radius;
+        da.dseed = frame_seed_host(c->cfg.seed, frame, 4);
+        da.origin = c->f_origin.p; da.normal = c->f_normal.p; da.emis = c->f_emis.p; da.hit = c->f_hit.p;
+        da.T = c->di_T.p;
+        da.hist_prev = c->di_hist.p + (size_t)((frame + 1) % 2) * P * 8;
+        da.hist_cur = c->di_hist.p + (size_t)(frame % 2) * P * 8;
+        da.restir = c->f_restir.p;
+        da.di_light = c->di_light.p;
+        da.di_weights = c->di_w.p;
+        launch_di(s, sv, da);
+    } else {
+        launch_spatial_finalize(s, fa);
+    }
     launch_postprocess(s, fa);
     launch_frame_stats(s, c->f_qcount.p, B, P, (unsigned long long *)c->d_counters.p);
     rec(2 + 2 * (size_t)B);
@@ -872,6 +903,8 @@ int vrgi_buffer_size(vrgi_ctx *c, int which, size_t *bytes) {
         case VRGI_BUF_RADIANCE: case VRGI_BUF_RESTIR: r = P * 12; break;
         case VRGI_BUF_TEMPORAL_RESERVOIR: case VRGI_BUF_SPATIAL_RESERVOIR: r = P * 80; break;
         case VRGI_BUF_BGRA8: r = P * 4; break;
+        case VRGI_BUF_DI_LIGHT: r = c->cfg.restir_mode == 1 ? P * 4 : 0; break;
+        case VRGI_BUF_DI_WEIGHTS: r = c->cfg.restir_mode == 1 ? P * 16 : 0; break;
         default: r = 0;
     }
     *bytes = r;
@@ -913,6 +946,8 @@ int vrgi_read_buffer(vrgi_ctx *c, int which, void *dst, size_t bytes) {
         }
         case VRGI_BUF_SPATIAL_RESERVOIR: src = c->f_spatial.p; break;
         case VRGI_BUF_BGRA8: src = c->f_bgra.p; break;
+        case VRGI_BUF_DI_LIGHT: src = c->di_light.p; break;
+        case VRGI_BUF_DI_WEIGHTS: src = c->di_w.p; break;
         default: return fail(VRGI_E_INVALID, "unknown buffer id");
     }
     HIP_CHECK(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, c->stream));

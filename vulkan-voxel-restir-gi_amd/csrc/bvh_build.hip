@@ -61,6 +61,12 @@ __device__ __forceinline__ float area_keys(int mnx, int mny, int mnz, int mxx, i
     return 2.0f * (dx * dy + dx * dz + dy * dz);
 }
 
+__device__ __forceinline__ float area6(const float b[6]) {
+    // aabb.rs:44-52 on a box held as min xyz, max xyz
+    float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return 2.0f * (dx * dy + dx * dz + dy * dz);
+}
+
 __device__ __forceinline__ void write_node(vrgi_bvh_node *dst, uint32_t left, uint32_t right,
                                            const float mn[3], float power, const float mx[3],
                                            uint32_t parent) {
@@ -508,21 +514,100 @@ __global__ void k_children(BuildArgs a, BuildState s) {
 }
 
 // ---------------------------------------------------------------- bottom phase
-// One 64-lane wave builds a whole subtree of <= SUB primitives held in LDS. Nodes
-// are split one at a time (an LDS stack of pending ranges, right pushed first);
-// ranges of 1 and 2 primitives are emitted by their parent directly.
+// One 64-lane wave builds a whole subtree of <= SUB primitives held in LDS.
+// Phase A: ranges larger than SMALL are split one at a time by the whole wave (an LDS
+// stack of pending ranges). Phase B: ranges of 3..SMALL primitives are split by one
+// lane each, sequentially, 64 subtrees at a time. Phase C: power of every internal
+// node of the subtree, children before parents: internal nodes of a subtree have
+// consecutive preorder ranks and descendants have larger ranks, so one pass over
+// decreasing ranks suffices (build.rs:345-357).
+struct RankInfo {  // per internal node of the subtree, indexed by rank - job rank
+    uint16_t first, nl;
+    uint16_t count, pad;
+    uint32_t gidx;
+};
+
+// SAH plane search of one range by a single lane (build.rs:45-128). Every plane of a run
+// of planes with the same (left set, right set) has the same cost, so only the plane at
+// each occupied bin (and plane 0 when bin 0 is empty) is evaluated; ascending order
+// with a strict < keeps the reference's first minimum. The lane writes the bin of each
+// of its primitives into bin8[f..f+c), a range no other lane touches.
 template <int SUB>
+__device__ void lane_best_plane(const float (*box)[SUB], uint8_t *bin8, uint32_t f, uint32_t c,
+                                uint32_t *dim_out, float *split_out, float nb[6]) {
+    float best_c = 3.40282347e+38f;
+    uint32_t best_d = 0;
+    float best_s = 0.0f;
+    for (int q = 0; q < 3; ++q) { nb[q] = box[q][f]; nb[3 + q] = box[3 + q][f]; }
+    for (uint32_t k = 1; k < c; ++k)
+        for (int q = 0; q < 3; ++q) { nb[q] = gmin(nb[q], box[q][f + k]); nb[3 + q] = gmax(nb[3 + q], box[3 + q][f + k]); }
+    const float all_cost = area6(nb) * (float)c;
+    for (uint32_t d = 0; d < 3; ++d) {
+        float lo = box[6 + d][f], hi = lo;
+        for (uint32_t k = 1; k < c; ++k) { lo = gmin(lo, box[6 + d][f + k]); hi = gmax(hi, box[6 + d][f + k]); }
+        const float scale = (float)BINS / (hi - lo);
+        uint32_t occ = 0;
+        for (uint32_t k = 0; k < c; ++k) {
+            uint32_t b = bin_index(box[6 + d][f + k], lo, scale);
+            bin8[f + k] = (uint8_t)b;
+            occ |= 1u << b;
+        }
+        float dbest = 3.40282347e+38f;
+        uint32_t dplane = 0;
+        if (!(occ & 1u)) { dbest = 0.0f * 0.0f + all_cost; dplane = 0; }  // left side empty
+        uint32_t rest = occ & 0x7fffffffu;                                  // no plane after bin 31
+        while (rest) {
+            const uint32_t b = (uint32_t)__builtin_ctz(rest);
+            rest &= rest - 1u;
+            float l[6], r[6];
+            uint32_t nl = 0;
+            for (uint32_t k = 0; k < c; ++k) {
+                const uint32_t e = f + k;
+                const bool left = bin8[e] <= b;
+                const bool lf = nl == 0, rf = (k - nl) == 0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const float vn = box[q][e], vx = box[3 + q][e];
+                    if (left) {
+                        l[q] = lf ? vn : gmin(l[q], vn);
+                        l[3 + q] = lf ? vx : gmax(l[3 + q], vx);
+                    } else {
+                        r[q] = rf ? vn : gmin(r[q], vn);
+                        r[3 + q] = rf ? vx : gmax(r[3 + q], vx);
+                    }
+                }
+                nl += left ? 1u : 0u;
+            }
+            const float al = area6(l) * (float)nl;
+            const float ar = nl < c ? area6(r) * (float)(c - nl) : 0.0f * 0.0f;
+            const float cost = al + ar;
+            if (cost < dbest) { dbest = cost; dplane = b; }
+        }
+        if (dbest < best_c) {
+            best_c = dbest;
+            best_d = d;
+            best_s = lo + ((float)dplane + 1.0f) * ((hi - lo) / (float)BINS);
+        }
+    }
+    *dim_out = best_d;
+    *split_out = best_s;
+}
+
+template <int SUB, int SMALL>
 __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
-    constexpr int STK = SUB / 3 + 2;  // pending ranges are disjoint and hold >= 3 primitives
+    constexpr int STK = SUB / (SMALL + 1) + 2;  // pending ranges are disjoint and hold > SMALL
+    constexpr int SMQ = SUB / 3 + 2;            // small ranges are disjoint and hold >= 3
     __shared__ uint32_t s_perm[SUB];
     __shared__ float s_box[9][SUB];  // min xyz, max xyz, centroid xyz
     __shared__ uint32_t s_pair[SUB];
     __shared__ uint32_t s_cj[SUB];
     __shared__ int s_bins[BIN_WORDS];
-    __shared__ float s_pow[2 * SUB];
-    __shared__ uint32_t s_split_local[SUB];
-    __shared__ uint32_t s_split_g[SUB];
-    __shared__ uint32_t s_stack[STK][7];  // first, count, rank, node, local id, depth, parent
+    __shared__ RankInfo s_rank[SUB];
+    __shared__ float s_ipow[SUB];
+    __shared__ uint32_t s_stack[STK][6];  // first, count, rank, node, depth, parent
+    __shared__ uint4 s_q[2][SMQ];         // small ranges: first|count<<8|(rank-job rank)<<16, node, depth, parent
+    __shared__ uint8_t s_bin8[SUB];
+    __shared__ uint32_t s_nsmall;
 
     const uint32_t job = blockIdx.x;
     if (job >= *s.n_jobs) return;
@@ -536,20 +621,24 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         for (int q = 0; q < 6; ++q) s_box[q][k] = a.prim9[q * a.n + p];
         for (int q = 0; q < 3; ++q) s_box[6 + q][k] = a.pc[q * a.n + jn.first + k];
     }
+    uint32_t sp = 0, maxdepth = jn.depth;
+    uint32_t splitsA = 0, splitsB = 0;  // split sizes (sum = sum of leaf depths): wave-wide, per lane
     if (lane == 0) {
-        s_stack[0][0] = 0; s_stack[0][1] = c; s_stack[0][2] = jn.rank; s_stack[0][3] = jn.node;
-        s_stack[0][4] = 0; s_stack[0][5] = jn.depth; s_stack[0][6] = jn.parent;
+        uint32_t e[6] = {0, c, jn.rank, jn.node, jn.depth, jn.parent};
+        if (c > (uint32_t)SMALL)
+            for (int q = 0; q < 6; ++q) s_stack[0][q] = e[q];
+        else
+            s_q[0][0] = make_uint4(c << 8, jn.node, jn.depth, jn.parent);
+        s_nsmall = c > (uint32_t)SMALL ? 0u : 1u;
     }
-    uint32_t sp = 1, nsplit = 0, maxdepth = jn.depth;
-    unsigned long long splits = 0;  // sum of split node sizes of this subtree
+    if (c > (uint32_t)SMALL) sp = 1;
     __syncthreads();
     const float pad = 0.0001f;
+    // ---------------- phase A: ranges larger than SMALL, one wave per range
     while (sp > 0) {
         --sp;
         const uint32_t nf = s_stack[sp][0], nc = s_stack[sp][1], rank = s_stack[sp][2];
-        const uint32_t gidx = s_stack[sp][3], loc = s_stack[sp][4], dep = s_stack[sp][5];
-        const uint32_t parent = s_stack[sp][6];
-        // ---- centroid bounds (build.rs:60-66)
+        const uint32_t gidx = s_stack[sp][3], dep = s_stack[sp][4], parent = s_stack[sp][5];
         int bmn[3] = {KEY_MIN_ID, KEY_MIN_ID, KEY_MIN_ID}, bmx[3] = {KEY_MAX_ID, KEY_MAX_ID, KEY_MAX_ID};
         for (uint32_t k = lane; k < nc; k += 64)
             for (int d = 0; d < 3; ++d) {
@@ -559,7 +648,6 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             }
         int bounds[6];
         for (int d = 0; d < 3; ++d) { bounds[d] = wave_min(bmn[d]); bounds[3 + d] = wave_max(bmx[d]); }
-        // ---- bins (build.rs:68-85)
         for (int k = lane; k < BIN_WORDS; k += 64) {
             int w = k % 7;
             s_bins[k] = w == 0 ? 0 : (w <= 3 ? KEY_MIN_ID : KEY_MAX_ID);
@@ -587,7 +675,6 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         float split;
         int box[6];
         sah_sweep_wave(s_bins, bounds, &dim, &split, box);
-        // ---- partition (build.rs:145-157)
         const uint32_t iters = (nc + 63) / 64;
         uint32_t T = 0;
         for (uint32_t itn = 0; itn < iters; ++itn) {
@@ -620,65 +707,116 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             __syncthreads();
         }
         const uint32_t nl = fb ? nc / 2 : T, nr = nc - nl;
-        // ---- this node's record (power filled in by the reverse pass)
         const uint32_t li = 2 * rank + 1, ri = 2 * rank + 2;
-        const uint32_t my_split = nsplit;
         if (lane == 0) {
             float mn[3], mx[3];
             for (int k = 0; k < 3; ++k) { mn[k] = funkey(box[k]) - pad; mx[k] = funkey(box[3 + k]) + pad; }
             write_node(a.nodes + base + gidx, li, ri, mn, 0.0f, mx, parent);
-            s_split_local[my_split] = loc;
-            s_split_g[my_split] = base + gidx;
+            s_rank[rank - jn.rank] = RankInfo{(uint16_t)nf, (uint16_t)nl, (uint16_t)nc, 0, gidx};
         }
-        ++nsplit;
-        splits += nc;
+        splitsA += nc;
         maxdepth = max(maxdepth, dep + 1);
-        // ---- children; the right one is pushed first so the left subtree is split first
         for (int side = 1; side >= 0; --side) {
             uint32_t cf = side ? nf + nl : nf;
             uint32_t cc = side ? nr : nl;
             uint32_t cr = side ? rank + nl : rank + 1;
             uint32_t ci = side ? ri : li;
-            uint32_t cl = 2 * my_split + 1 + side;  // local id of the child
             if (cc == 1) {
-                if (lane == 0) {
-                    uint32_t p = s_perm[cf];
-                    emit_leaf(a, base + ci, gidx, p);
-                    s_pow[cl] = a.power[p];
-                }
+                if (lane == 0) emit_leaf(a, base + ci, gidx, s_perm[cf]);
             } else if (cc == 2) {
-                // the pair is a split of its own: its leaves get local ids 2k+1, 2k+2
                 if (lane == 0) {
-                    uint32_t p0 = s_perm[cf], p1 = s_perm[cf + 1];
-                    emit_pair(a, base, ci, cr, gidx, p0, p1);
-                    s_split_local[nsplit] = cl;
-                    s_split_g[nsplit] = base + ci;
-                    s_pow[2 * nsplit + 1] = a.power[p0];
-                    s_pow[2 * nsplit + 2] = a.power[p1];
+                    emit_pair(a, base, ci, cr, gidx, s_perm[cf], s_perm[cf + 1]);
+                    s_rank[cr - jn.rank] = RankInfo{(uint16_t)cf, 1, 2, 0, ci};
                 }
-                ++nsplit;
-                splits += 2;
+                splitsA += 2;
                 maxdepth = max(maxdepth, dep + 2);
             } else {
-                if (lane == 0) {
-                    s_stack[sp][0] = cf; s_stack[sp][1] = cc; s_stack[sp][2] = cr;
-                    s_stack[sp][3] = ci; s_stack[sp][4] = cl; s_stack[sp][5] = dep + 1;
-                    s_stack[sp][6] = gidx;
+                uint32_t e[6] = {cf, cc, cr, ci, dep + 1, gidx};
+                if (cc > (uint32_t)SMALL) {
+                    if (lane == 0)
+                        for (int q = 0; q < 6; ++q) s_stack[sp][q] = e[q];
+                    ++sp;
+                } else if (lane == 0) {
+                    s_q[0][s_nsmall++] = make_uint4(cf | (cc << 8) | ((cr - jn.rank) << 16), ci, dep + 1, gidx);
                 }
-                ++sp;
             }
         }
         __syncthreads();
     }
-    // ---- power, children before parents: reverse split order (build.rs:345-357)
-    if (lane == 0) {
-        for (int k = (int)nsplit - 1; k >= 0; --k) {
-            float p = 0.0f;
-            p += s_pow[2 * k + 1];
-            p += s_pow[2 * k + 2];
-            s_pow[s_split_local[k]] = p;
-            a.nodes[s_split_g[k]].power = p;
+    // ---------------- phase B: ranges of 3..SMALL primitives, one lane each, one level
+    // of the subtree per pass; the ranges of a pass are disjoint, so at most SUB/3
+    uint32_t cur = 0, ncur = s_nsmall;
+    while (ncur > 0) {
+        __syncthreads();
+        if (lane == 0) s_nsmall = 0;
+        __syncthreads();
+        for (uint32_t w = lane; w < ncur; w += 64) {
+            const uint4 e = s_q[cur][w];
+            const uint32_t nf = e.x & 0xffu, nc = (e.x >> 8) & 0xffu, rank = jn.rank + (e.x >> 16);
+            const uint32_t gidx = e.y, dep = e.z, parent = e.w;
+            uint32_t dim;
+            float split, nb[6];
+            lane_best_plane<SUB>(s_box, s_bin8, nf, nc, &dim, &split, nb);
+            // partition (build.rs:145): the crate's l/r scan, in place
+            uint32_t l = 0, r = nc - 1, T;
+            for (;;) {
+                while (l < nc && s_box[6 + dim][nf + l] < split) ++l;
+                while (r > 0 && !(s_box[6 + dim][nf + r] < split)) --r;
+                if (l >= r) { T = l; break; }
+                const uint32_t e0 = nf + l, e1 = nf + r;
+                const uint32_t tp = s_perm[e0]; s_perm[e0] = s_perm[e1]; s_perm[e1] = tp;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
+            }
+            const uint32_t nl = (T == 0 || T == nc) ? nc / 2 : T, nr = nc - nl;
+            const uint32_t li = 2 * rank + 1, ri = 2 * rank + 2;
+            float mn[3], mx[3];
+            for (int k = 0; k < 3; ++k) { mn[k] = nb[k] - pad; mx[k] = nb[3 + k] + pad; }
+            write_node(a.nodes + base + gidx, li, ri, mn, 0.0f, mx, parent);
+            s_rank[rank - jn.rank] = RankInfo{(uint16_t)nf, (uint16_t)nl, (uint16_t)nc, 0, gidx};
+            splitsB += nc;
+            maxdepth = max(maxdepth, dep + 1);
+            for (int side = 1; side >= 0; --side) {
+                const uint32_t cf = side ? nf + nl : nf;
+                const uint32_t cc = side ? nr : nl;
+                const uint32_t cr = side ? rank + nl : rank + 1;
+                const uint32_t ci = side ? ri : li;
+                if (cc == 1) {
+                    emit_leaf(a, base + ci, gidx, s_perm[cf]);
+                } else if (cc == 2) {
+                    emit_pair(a, base, ci, cr, gidx, s_perm[cf], s_perm[cf + 1]);
+                    s_rank[cr - jn.rank] = RankInfo{(uint16_t)cf, 1, 2, 0, ci};
+                    splitsB += 2;
+                    maxdepth = max(maxdepth, dep + 2);
+                } else {
+                    const uint32_t slot = atomicAdd(&s_nsmall, 1u);
+                    s_q[cur ^ 1][slot] = make_uint4(cf | (cc << 8) | ((cr - jn.rank) << 16), ci, dep + 1, gidx);
+                }
+            }
         }
+        __syncthreads();
+        ncur = s_nsmall;
+        cur ^= 1;
+    }
+    __syncthreads();
+    // ---------------- phase C: power, decreasing rank (children first)
+    if (lane == 0) {
+        for (int k = (int)c - 2; k >= 0; --k) {
+            RankInfo ri = s_rank[k];
+            uint32_t nr = (uint32_t)ri.count - ri.nl;
+            float pl = ri.nl == 1 ? a.power[s_perm[ri.first]] : s_ipow[k + 1];
+            float pr = nr == 1 ? a.power[s_perm[ri.first + ri.nl]] : s_ipow[k + ri.nl];
+            float p = 0.0f;
+            p += pl;
+            p += pr;
+            s_ipow[k] = p;
+            a.nodes[base + ri.gidx].power = p;
+        }
+    }
+    // depth and leaf-depth sum: reduce over lanes
+    maxdepth = (uint32_t)wave_max((int)maxdepth);
+    const unsigned long long splits = (unsigned long long)splitsA + wave_sum(splitsB);
+    if (lane == 0) {
         atomicMax(a.depth + jn.tree, maxdepth);
         atomicAdd(a.split_count_sum, splits);
     }
@@ -804,7 +942,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 5 * 4, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     uint32_t n_jobs = h_cnt[2];
-    if (n_jobs) k_subtree<SUB_MAX_DEFAULT><<<n_jobs, 64, 0, stream>>>(a, st);
+    if (n_jobs) k_subtree<SUB_MAX_DEFAULT, 32><<<n_jobs, 64, 0, stream>>>(a, st);
     // power of the top-phase nodes, deepest level first; st.level_nodes holds
     // (global index, tree base) pairs, one contiguous run per level
     uint32_t off = level_off;

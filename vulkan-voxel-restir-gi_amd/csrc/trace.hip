@@ -9,6 +9,8 @@
 // primitive id, so the closest hit does not depend on the traversal order.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "devcommon.h"
 #include "vrgi_kernels.h"
 
@@ -407,7 +409,179 @@ __global__ __launch_bounds__(64) void k_nee_pdf(const SceneView sv, const Bounce
     ba.nee[bi] = np;
 }
 
+// ---------------------------------------------------------------- ReSTIR DI (benchmark mode)
+// Not in the reference (its ReSTIR is GI); assembled from its primitives: candidates
+// from traverseBvh (raytrace.rs:218-281) and triangleSample (:305-311), reservoirs
+// merged as updateReservoir / mergeReservoir (restir_*_resampling.rs), neighbours
+// filtered by geometricallySimilar (restir_spatial_resampling.rs:277-288).
+// Target p_hat = luminance(f * Le * cos_x * cos_y / d^2), f = albedo / pi,
+// Le = 1000 * emissivity (raytrace.rs:477); source pdf = pick probability / area.
+struct DiRes {
+    uint32_t light;
+    f3 y;
+    float wsum, W, ph;
+    uint32_t M;
+};
+__device__ __forceinline__ float di_lum(f3 v) { return 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z; }
+
+__device__ __forceinline__ f3 di_contrib(const SceneView &sv, f3 x, f3 n, f3 fr, uint32_t lg, f3 y) {
+    Tri lt = sv.tri[lg];
+    f3 l0 = mk(lt.v0.x, lt.v0.y, lt.v0.z), l1 = mk(lt.v1.x, lt.v1.y, lt.v1.z), l2 = mk(lt.v2.x, lt.v2.y, lt.v2.z);
+    f3 nl = norm3(cross3(l1 - l0, l2 - l0));
+    f3 w = y - x;
+    float d2 = dot3(w, w);
+    float dist = sqrtf(d2);
+    f3 wn = w / dist;
+    float cx = gmax(0.0f, dot3(n, wn));
+    float cy = gmax(0.0f, -dot3(nl, wn));
+    const vrgi_material &m = sv.mats[__float_as_uint(lt.v0.w)];
+    f3 le = 1000.0f * mk(m.emissivity[0], m.emissivity[1], m.emissivity[2]);
+    return (fr * le) * ((cx * cy) / d2);
+}
+__device__ __forceinline__ void di_update(uint32_t seed, DiRes &r, uint32_t light, f3 y, float w, float ph) {
+    r.wsum += w;
+    r.M += 1;
+    if (float_construct(seed) < w / r.wsum) { r.light = light; r.y = y; r.ph = ph; }
+}
+__device__ __forceinline__ void di_store(float *p, const DiRes &r) {
+    float4 *q = (float4 *)p;
+    q[0] = make_float4(__uint_as_float(r.light), r.y.x, r.y.y, r.y.z);
+    q[1] = make_float4(r.wsum, r.W, r.ph, __uint_as_float(r.M));
+}
+__device__ __forceinline__ DiRes di_load(const float *p) {
+    const float4 *q = (const float4 *)p;
+    float4 a = q[0], b = q[1];
+    DiRes r;
+    r.light = __float_as_uint(a.x); r.y = mk(a.y, a.z, a.w);
+    r.wsum = b.x; r.W = b.y; r.ph = b.z; r.M = __float_as_uint(b.w);
+    return r;
+}
+__device__ __forceinline__ f3 di_fr(const SceneView &sv, uint32_t hit) {
+    const vrgi_material &m = sv.mats[__float_as_uint(sv.tri[hit].v0.w)];
+    return mk(m.reflectivity[0], m.reflectivity[1], m.reflectivity[2]) / VRGI_PI_F;
+}
+
+// initial candidates + temporal merge with the previous frame (M capped at 20x)
+__global__ __launch_bounds__(64) void k_di_initial(const SceneView sv, const DiArgs da) {
+    const uint32_t id = blockIdx.x * 64 + threadIdx.x;
+    if (id >= da.P) return;
+    DiRes r;
+    r.light = VRGI_INVALID; r.y = mk(0, 0, 0); r.wsum = 0.0f; r.W = 0.0f; r.ph = 0.0f; r.M = 0;
+    const uint32_t hp = da.hit[id];
+    if (hp != VRGI_INVALID) {
+        f3 x = ld3(da.origin + (size_t)id * 3), n = ld3(da.normal + (size_t)id * 3);
+        f3 fr = di_fr(sv, hp);
+        const uint32_t pd = mm_combine(da.dseed, id);
+        for (uint32_t k = 0; k < da.candidates; ++k) {
+            uint32_t sk = mm_combine(pd, k);
+            Pick pk = traverse_light(sv, x, n, mm_combine(sk, 0));
+            float w = 0.0f, ph = 0.0f;
+            uint32_t lg = VRGI_INVALID;
+            f3 y = mk(0, 0, 0);
+            if (pk.ok) {
+                lg = sv.inst[pk.inst].tri_off + pk.prim;
+                Tri lt = sv.tri[lg];
+                f3 l0 = mk(lt.v0.x, lt.v0.y, lt.v0.z), l1 = mk(lt.v1.x, lt.v1.y, lt.v1.z), l2 = mk(lt.v2.x, lt.v2.y, lt.v2.z);
+                float ux = mm_finalizef(mm_combine(sk, 1)), uy = mm_finalizef(mm_combine(sk, 2));
+                if (ux + uy > 1.0f) { ux = 1.0f - ux; uy = 1.0f - uy; }
+                f3 bb = mk((1.0f - ux) - uy, ux, uy);
+                y = (bb.x * l0 + bb.y * l1) + bb.z * l2;
+                float area = 0.5f * len3(cross3(l1 - l0, l2 - l0));
+                float psrc = pk.prob / area;
+                ph = di_lum(di_contrib(sv, x, n, fr, lg, y));
+                w = psrc > 0.0f ? ph / psrc : 0.0f;
+            }
+            di_update(mm_finalize(mm_combine(sk, 3)), r, lg, y, w, ph);
+        }
+        const float *h = da.hist_prev + (size_t)id * 8;
+        uint32_t hl = __float_as_uint(h[0]);
+        if (hl != VRGI_INVALID && hl < sv.n_tris) {
+            uint32_t hm = min(__float_as_uint(h[5]), 20u * da.candidates);
+            f3 hy = mk(h[1], h[2], h[3]);
+            float ph = di_lum(di_contrib(sv, x, n, fr, hl, hy));
+            uint32_t m0 = r.M;
+            di_update(mm_finalize(mm_combine(pd, 0x10000u)), r, hl, hy, (ph * h[4]) * (float)hm, ph);
+            r.M = m0 + hm;
+        }
+        r.W = r.ph > 0.0f ? r.wsum / ((float)r.M * r.ph) : 0.0f;
+    }
+    di_store(da.T + (size_t)id * 8, r);
+}
+
+// spatial merge, shadow ray, final radiance, history for the next frame
+template <int STACK>
+__global__ __launch_bounds__(64) void k_di_final(const SceneView sv, const DiArgs da) {
+    __shared__ uint32_t stack[STACK * 64];
+    const uint32_t id = blockIdx.x * 64 + threadIdx.x;
+    if (id >= da.P) return;
+    float *hc = da.hist_cur + (size_t)id * 8;
+    const f3 E0 = ld3(da.emis + (size_t)id * 3);
+    const uint32_t hp = da.hit[id];
+    if (hp == VRGI_INVALID) {
+        float4 *q = (float4 *)hc;
+        q[0] = make_float4(__uint_as_float(VRGI_INVALID), 0.0f, 0.0f, 0.0f);
+        q[1] = make_float4(0.0f, __uint_as_float(0u), 0.0f, 0.0f);
+        st3(da.restir + (size_t)id * 3, E0);
+        da.di_light[id] = VRGI_INVALID;
+        ((float4 *)da.di_weights)[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const uint32_t pd = mm_combine(da.dseed, id);
+    const uint32_t qx = id % da.W, qy = id / da.W;
+    const float cos25 = 0.906307787036650f;
+    DiRes r = di_load(da.T + (size_t)id * 8);
+    f3 x = ld3(da.origin + (size_t)id * 3), n = ld3(da.normal + (size_t)id * 3);
+    f3 fr = di_fr(sv, hp);
+    for (uint32_t it = 0; it < da.spatial_iters; ++it) {
+        uint32_t is = mm_combine(mm_combine(pd, 0x20000u), it);
+        float jx = da.radius * (mm_finalizef(mm_combine(is, 0)) - 0.5f);
+        float jy = da.radius * (mm_finalizef(mm_combine(is, 1)) - 0.5f);
+        int nx = (int)(((float)qx + 0.5f) + jx), ny = (int)(((float)qy + 0.5f) + jy);
+        nx = min(max(nx, 0), (int)da.W - 1);
+        ny = min(max(ny, 0), (int)da.H - 1);
+        size_t q = (size_t)ny * da.W + nx;
+        if (da.hit[q] == VRGI_INVALID) continue;
+        f3 nq = ld3(da.normal + q * 3);
+        if (zero3(n) || zero3(nq) || dot3(n, nq) < cos25) continue;
+        DiRes rq = di_load(da.T + q * 8);
+        if (rq.light == VRGI_INVALID) continue;
+        float ph = di_lum(di_contrib(sv, x, n, fr, rq.light, rq.y));
+        uint32_t m0 = r.M;
+        di_update(mm_finalize(mm_combine(is, 2)), r, rq.light, rq.y, (ph * rq.W) * (float)rq.M, ph);
+        r.M = m0 + rq.M;
+    }
+    r.W = r.ph > 0.0f ? r.wsum / ((float)r.M * r.ph) : 0.0f;
+    f3 L = E0;
+    if (r.light != VRGI_INVALID && r.ph > 0.0f) {
+        f3 w = r.y - x;
+        float dist = sqrtf(dot3(w, w));
+        RayPre rp = ray_pre(x, w / dist);
+        Hit sh = closest_hit<STACK>(sv, rp, VRGI_EPS_RT, dist - 0.002f, stack);
+        if (sh.prim == VRGI_INVALID) L = E0 + di_contrib(sv, x, n, fr, r.light, r.y) * r.W;
+    }
+    st3(da.restir + (size_t)id * 3, L);
+    da.di_light[id] = r.light;
+    ((float4 *)da.di_weights)[id] = make_float4(r.wsum, r.W, (float)r.M, r.ph);
+    float4 *q = (float4 *)hc;
+    q[0] = make_float4(__uint_as_float(r.light), r.y.x, r.y.y, r.y.z);
+    q[1] = make_float4(r.W, __uint_as_float(r.M), 0.0f, 0.0f);
+}
+
 // ---------------------------------------------------------------- host launchers
+// Traversal stack size per lane (LDS): a lane never holds more than depth - 1 pending
+// nodes, so the smallest size above the tree depth is chosen; smaller stacks raise
+// occupancy (64-lane blocks of 8 / 16 / 40 KB).
+template <class F>
+static void with_stack(uint32_t depth, F f) {
+    if (depth + 2 <= 32) f(std::integral_constant<int, 32>());
+    else if (depth + 2 <= 64) f(std::integral_constant<int, 64>());
+    else f(std::integral_constant<int, VRGI_MAX_STACK>());
+}
+
+void launch_di(hipStream_t s, const SceneView &sv, const DiArgs &da) {
+    k_di_initial<<<(da.P + 63) / 64, 64, 0, s>>>(sv, da);
+    with_stack(sv.depth, [&](auto S) { k_di_final<decltype(S)::value><<<(da.P + 63) / 64, 64, 0, s>>>(sv, da); });
+}
 // frame counters: closest-hit rays = P + live rays of bounces 1..B-1
 __global__ void k_frame_stats(const uint32_t *qcount, uint32_t B, uint32_t P, unsigned long long *c) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -433,19 +607,18 @@ void launch_wide(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, Wi
 void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, const float *d,
                           uint32_t n, float tmin, float tmax, uint32_t *hit, float *t, float *uv) {
     if (!n) return;
-    if (sv.depth + 2 <= 64) k_trace_closest<64><<<(n + 63) / 64, 64, 0, s>>>(sv, o, d, n, tmin, tmax, hit, t, uv);
-    else k_trace_closest<VRGI_MAX_STACK><<<(n + 63) / 64, 64, 0, s>>>(sv, o, d, n, tmin, tmax, hit, t, uv);
+    with_stack(sv.depth, [&](auto S) {
+        k_trace_closest<decltype(S)::value><<<(n + 63) / 64, 64, 0, s>>>(sv, o, d, n, tmin, tmax, hit, t, uv);
+    });
 }
 void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
     if (!max_rays) return;
     bool cnt = ba.touch_nodes != nullptr;
-    if (sv.depth + 2 <= 64) {
-        if (cnt) k_trace_shade<64, true><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
-        else k_trace_shade<64, false><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
-    } else {
-        if (cnt) k_trace_shade<VRGI_MAX_STACK, true><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
-        else k_trace_shade<VRGI_MAX_STACK, false><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
-    }
+    with_stack(sv.depth, [&](auto S) {
+        constexpr int K = decltype(S)::value;
+        if (cnt) k_trace_shade<K, true><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+        else k_trace_shade<K, false><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+    });
 }
 
 // number of set bits of a bitmap, added to *out
@@ -460,8 +633,9 @@ void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsign
 }
 void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
     if (!max_rays) return;
-    if (sv.ldepth + 2 <= 64) k_nee_pdf<64><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
-    else k_nee_pdf<VRGI_MAX_STACK><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+    with_stack(sv.ldepth, [&](auto S) {
+        k_nee_pdf<decltype(S)::value><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+    });
 }
 
 }  // namespace vrgi

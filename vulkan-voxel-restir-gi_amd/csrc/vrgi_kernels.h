@@ -80,6 +80,21 @@ struct FrameArgs {
     uint8_t *bgra8;    // [P][4]
 };
 
+// ReSTIR DI (benchmark mode, restir_mode = 1)
+struct DiArgs {
+    uint32_t W, H, P, frame, candidates, spatial_iters;
+    float radius;
+    uint32_t dseed;
+    const float *origin, *normal, *emis;  // bounce-0 slices
+    const uint32_t *hit;                  // bounce-0 hit ids
+    float *T;                             // [P][8] reservoirs after the temporal merge
+    const float *hist_prev;               // [P][8] previous frame's final reservoirs
+    float *hist_cur;                      // [P][8]
+    float *restir;                        // [P][3] final radiance
+    uint32_t *di_light;                   // [P]
+    float *di_weights;                    // [P][4] w_sum, W, M, p_hat
+};
+
 // ---------------------------------------------------------------- primitives
 // builder inputs of one triangle: Aabb::from_points (aabb.rs:10-22) and the centroid
 // ((v0 + v1) + v2) / 3 (build.rs:267-270); planes of n floats
@@ -249,5 +264,6 @@ void launch_tl_idx(hipStream_t s, const vrgi_bvh_node *tl, uint32_t n_nodes, Dev
 void launch_frame_stats(hipStream_t s, const uint32_t *qcount, uint32_t B, uint32_t P,
                         unsigned long long *counters);
 void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsigned long long *out);
+void launch_di(hipStream_t s, const SceneView &sv, const DiArgs &da);
 
 }  // namespace vrgi

@@ -295,9 +295,12 @@ class Context:
         return nodes
 
 
-def config(width, height, bounces=3, rr_start=0xFFFFFFFF, rr_min=0.1, seed=0, flags=0, radius=15.0):
+def config(width, height, bounces=3, rr_start=0xFFFFFFFF, rr_min=0.1, seed=0, flags=0, radius=15.0,
+           mode=0, di_candidates=32):
     cfg = Config()
     lib().vrgi_config_reference(C.byref(cfg), width, height)
+    cfg.restir_mode = mode
+    cfg.di_candidates = di_candidates
     cfg.num_bounces = bounces
     cfg.rr_start_bounce = rr_start
     cfg.rr_min_survival = rr_min
