@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=24)
+    ap.add_argument("--sort-bits", type=int, default=0,
+                    help="Morton ray-queue ordering: origin bits per axis (0 = compaction order)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the frame would shard by "
@@ -95,7 +97,7 @@ def main():
     objs = vrgi.chunk_objects(blocks, nt, tr)
     cams = [vrgi.camera_spline(1, k, 16) for k in range(16)]
     cfg = vrgi.config(W_PX, H_PX, BOUNCES, rr_start=RR_START, rr_min=0.1,
-                      flags=vrgi.FLAG_REBUILD_EVERY_FRAME)
+                      flags=vrgi.FLAG_REBUILD_EVERY_FRAME, sort_bits=a.sort_bits)
     L = vrgi.lib()
     with vrgi.Context(cfg, mats, lum) as ctx:
         log("uploading objects (per-object light BVHs)")

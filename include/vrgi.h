@@ -77,6 +77,9 @@ typedef struct vrgi_config {
     uint32_t di_candidates;       /* DI mode: light candidates per pixel */
     uint32_t seed;                /* salt of the counter-based per-frame seeds */
     uint32_t flags;               /* VRGI_FLAG_* */
+    uint32_t ray_sort_bits;       /* 0: queue in compaction order (reference: no reordering);
+                                     1..5: reorder each bounce's ray queue by direction octant
+                                     and the Morton code of the origin, this many bits per axis */
 } vrgi_config;
 
 #define VRGI_FLAG_REBUILD_EVERY_FRAME 1u /* rebuild the geometry BVH on every render() */

@@ -80,18 +80,25 @@ def _camera0():
     return vrgi.camera_look([6.0, 62.0, 8.0], [40.0, 0.0, 44.0])
 
 
-@pytest.mark.parametrize("nee_type,spatial", [(0, 0), (1, 5), (2, 3)])
-def test_frames_parity(world0, scenes, nee_type, spatial):
+NO_RR = 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("nee_type,spatial,rr_start,sort_bits", [
+    (0, 0, NO_RR, 0), (1, 5, NO_RR, 0), (2, 3, NO_RR, 0),
+    # Morton-ordered ray queues (with and without Russian roulette) give the same bits
+    (1, 5, NO_RR, 4), (1, 2, 1, 3), (0, 1, 1, 5)])
+def test_frames_parity(world0, scenes, nee_type, spatial, rr_start, sort_bits):
     import oracle
     import vrgi
     s, objs = scenes
     W, H, B = 96, 64, 3
     cam = _camera0()
-    r = oracle.Renderer(W, H, B)
-    with vrgi.Context(vrgi.config(W, H, B, flags=vrgi.FLAG_KEEP_BOUNCES), world0["mats"], world0["lum"]) as c:
+    r = oracle.Renderer(W, H, B, rr_start=rr_start)
+    cfg = vrgi.config(W, H, B, rr_start=rr_start, flags=vrgi.FLAG_KEEP_BOUNCES, sort_bits=sort_bits)
+    with vrgi.Context(cfg, world0["mats"], world0["lum"]) as c:
         c.add_objects(objs)
         for f in range(4):
-            say(f"frame {f} nee={nee_type} spatial={spatial}")
+            say(f"frame {f} nee={nee_type} spatial={spatial} rr={rr_start} sort={sort_bits}")
             c.render(cam, nee_type=nee_type, spatial=spatial)
             c.sync()
             ref = r.render(s, cam, f, nee_type=nee_type, spatial=spatial)

@@ -69,6 +69,7 @@ struct vrgi_object {
     vrgi_vertex3d *d_verts = nullptr;
     uint32_t n_light = 0;
     vrgi_bvh_node *d_light_bvh = nullptr;
+    LPair *d_lpairs = nullptr;  // child records of the light BL-BVH
     vrgi_light_vertex3d *d_light_verts = nullptr;
     std::vector<vrgi_bvh_node> h_light_bvh;
     bool light_empty = true;
@@ -100,6 +101,8 @@ struct vrgi_ctx {
     DevBuf<const vrgi_light_vertex3d *> d_lvtxptr;
     DevBuf<uint32_t> d_light_off;
     DevBuf<vrgi_bvh_node> d_tl;
+    DevBuf<LPair> d_tlpairs;
+    DevBuf<uint32_t> d_lanc, d_lanc_len;  // per light triangle: ancestors for the NEE pdf
     uint32_t n_tl = 1;
     DevBuf<float> d_tl_prim9, d_tl_aabb, d_tl_xf, d_tl_power;
     DevBuf<uint32_t> d_tl_ids, d_tl_trees, d_tl_depth;
@@ -116,6 +119,7 @@ struct vrgi_ctx {
     // frame buffers
     DevBuf<float> f_origin, f_dir, f_normal, f_emis, f_refl, f_mis, f_pdf, f_nee, f_hit_t;
     DevBuf<uint32_t> f_hit, f_plen, f_queue, f_qcount;
+    DevBuf<uint32_t> f_squeue, f_skeys, f_shist;  // Morton-ordered queues, keys, buckets
     DevBuf<float> f_radiance, f_restir, f_debug, f_temporal, f_spatial;
     DevBuf<uint8_t> f_bgra;
     DevBuf<float> di_T, di_hist, di_w;
@@ -162,6 +166,9 @@ SceneView scene_view(vrgi_ctx *c) {
     sv.ldepth = c->lgeo_depth;
     sv.ltri_inst = c->d_ltri_inst.p;
     sv.ltri_node = c->d_ltri_node.p;
+    sv.tlpairs = c->d_tlpairs.p;
+    sv.lanc = c->d_lanc.p;
+    sv.lanc_len = c->d_lanc_len.p;
     sv.mats = c->d_mats.p;
     sv.n_mats = (uint32_t)c->mats.size();
     return sv;
@@ -181,6 +188,11 @@ void alloc_frame(vrgi_ctx *c) {
     c->f_hit.reserve(B * P);
     c->f_plen.reserve(P);
     c->f_queue.reserve((B + 1) * P);
+    if (c->cfg.ray_sort_bits) {
+        c->f_squeue.reserve((B + 1) * P);
+        c->f_skeys.reserve(P);
+        c->f_shist.reserve((size_t)8 << (3 * c->cfg.ray_sort_bits));
+    }
     c->f_qcount.reserve(B + 2);
     c->f_radiance.reserve(P * 3);
     c->f_restir.reserve(P * 3);
@@ -223,6 +235,7 @@ void rebuild_instances(vrgi_ctx *c) {
         d.tri_off = tri_off;
         d.tl_idx = 0;  // prim2node.get(i).unwrap_or(0)
         d.light_bvh = o->d_light_bvh;
+        d.lpairs = o->d_lpairs;
         hi.push_back(d);
         vp.push_back(o->d_verts);
         lp.push_back(o->d_light_verts);
@@ -252,6 +265,7 @@ void rebuild_instances(vrgi_ctx *c) {
     }
     uint32_t m = (uint32_t)tl_ids.size();
     c->d_tl.reserve(m ? 2 * m - 1 : 1);
+    c->d_tlpairs.reserve(m ? 2 * m - 1 : 1);
     if (m == 0) {
         vrgi_bvh_node dummy;  // BvhNode::default() (bvh/mod.rs:27-38, scene.rs:275-277)
         dummy.left = dummy.right_or_prim = dummy.parent = VRGI_INVALID;
@@ -283,6 +297,7 @@ void rebuild_instances(vrgi_ctx *c) {
         throw HipError("light top-level build exceeded its workspace");
     c->n_tl = 2 * m - 1;
     launch_tl_idx(s, c->d_tl.p, c->n_tl, c->d_inst.p);
+    launch_light_pairs(s, c->d_tl.p, c->n_tl, c->d_tlpairs.p);
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -329,6 +344,10 @@ void rebuild_geometry(vrgi_ctx *c) {
     fl.light_node = c->d_ltri_node.p;
     fl.prim_off = nt;
     launch_flatten_lights(s, fl);
+    c->d_lanc.reserve((size_t)nl * VRGI_LANC_MAX);
+    c->d_lanc_len.reserve(nl);
+    launch_light_ancestry(s, c->d_ltri_inst.p, c->d_ltri_node.p, c->d_inst.p, c->d_tl.p, nl, c->d_lanc.p,
+                          c->d_lanc_len.p);
     uint32_t trees[6];
     uint32_t nt_trees = 0;
     uint32_t beg[2], cnt[2], base[2];
@@ -359,6 +378,7 @@ void free_object(vrgi_object *o) {
     }
     if (o->d_verts) (void)hipFree(o->d_verts);
     if (o->d_light_bvh) (void)hipFree(o->d_light_bvh);
+    if (o->d_lpairs) (void)hipFree(o->d_lpairs);
     if (o->d_light_verts) (void)hipFree(o->d_light_verts);
     delete o;
 }
@@ -414,6 +434,7 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
         return fail(VRGI_E_INVALID, "extent must be non-zero and num_bounces in [2, 16]");
     if ((uint64_t)cfg->width * cfg->height > (1ull << 26)) return fail(VRGI_E_LIMIT, "extent above 2^26 pixels");
     if (cfg->restir_mode > 1) return fail(VRGI_E_INVALID, "restir_mode must be 0 (GI) or 1 (DI)");
+    if (cfg->ray_sort_bits > 5) return fail(VRGI_E_INVALID, "ray_sort_bits must be in [0, 5]");
     if (cfg->restir_mode == 1 && (cfg->di_candidates == 0 || cfg->di_candidates > 1024))
         return fail(VRGI_E_INVALID, "di_candidates must be in [1, 1024]");
     int ndev = 0;
@@ -519,6 +540,8 @@ int vrgi_upload_object(vrgi_ctx *c, const vrgi_vertex3d *v, size_t n, vrgi_objec
             if (c->up_builder.build(s, dprim9.p, dpw.p, dids.p, m, 1, dtrees.p, dtrees.p + 1, dtrees.p + 2,
                                     trees + 2, o->d_light_bvh, ddepth.p, nullptr) != 0)
                 throw HipError("light build exceeded its workspace");
+            dev_alloc((void **)&o->d_lpairs, (2 * (size_t)m - 1) * sizeof(LPair));
+            launch_light_pairs(s, o->d_light_bvh, 2 * m - 1, o->d_lpairs);
             o->h_light_bvh.resize(2 * (size_t)m - 1);
             HIP_CHECK(hipMemcpyAsync(o->h_light_bvh.data(), o->d_light_bvh, o->h_light_bvh.size() * sizeof(vrgi_bvh_node),
                                      hipMemcpyDeviceToHost, s));
@@ -721,10 +744,13 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         c->touch_count.reserve(2 * (size_t)B);
         HIP_CHECK(hipMemsetAsync(c->touch_count.p, 0, 16 * (size_t)B, s));
     }
+    // queue of the rays of each bounce: compaction order, or Morton order when enabled
+    std::vector<const uint32_t *> qin(B + 1, nullptr);
+    const bool sort_rays = c->cfg.ray_sort_bits != 0 && c->n_tris >= 2;
     for (uint32_t b = 0; b < B; ++b) {
         ba.bounce = b;
         ba.inv_seed = frame * B + b;  // interactive_rendering.rs:1189
-        ba.queue_in = c->f_queue.p + (size_t)b * P;
+        ba.queue_in = qin[b];
         ba.queue_out = c->f_queue.p + (size_t)(b + 1) * P;
         if (touch) {
             HIP_CHECK(hipMemsetAsync(c->touch_bits.p, 0, 8 * touch_words, s));
@@ -732,6 +758,19 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             ba.touch_tris = c->touch_bits.p + touch_words;
         }
         launch_trace_shade(s, sv, ba, P);
+        qin[b + 1] = ba.queue_out;
+        if (sort_rays && b + 1 < B) {
+            SortArgs sa{};
+            sa.bounce = b; sa.P = P; sa.bits = c->cfg.ray_sort_bits;
+            sa.root = c->d_gwide.p;
+            sa.origin = c->f_origin.p; sa.dir = c->f_dir.p;
+            sa.count = c->f_qcount.p + b + 1;
+            sa.queue_in = ba.queue_out;
+            sa.queue_out = c->f_squeue.p + (size_t)(b + 1) * P;
+            sa.keys = c->f_skeys.p; sa.hist = c->f_shist.p;
+            launch_sort_queue(s, sa, P);
+            qin[b + 1] = sa.queue_out;
+        }
         if (touch) {
             launch_popcount(s, ba.touch_nodes, touch_words, c->touch_count.p + 2 * b);
             launch_popcount(s, ba.touch_tris, touch_words, c->touch_count.p + 2 * b + 1);
@@ -742,7 +781,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
         if (prefs->nee_type != 0) {
             ba.bounce = b;
-            ba.queue_in = c->f_queue.p + (size_t)(b + 1) * P;
+            ba.queue_in = qin[b + 1];
             launch_nee_pdf(s, sv, ba, P);
         }
         rec(2 + B + b);

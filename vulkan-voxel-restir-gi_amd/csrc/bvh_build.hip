@@ -527,6 +527,16 @@ struct RankInfo {  // per internal node of the subtree, indexed by rank - job ra
     uint32_t gidx;
 };
 
+// Tuning of the bottom phase: ranges of at most VRGI_SUB_SMALL primitives are split by
+// one lane each; VRGI_PLANE_UNROLL is the unroll factor of the per-plane union loop.
+#ifndef VRGI_SUB_SMALL
+#define VRGI_SUB_SMALL 12
+#endif
+#ifndef VRGI_PLANE_UNROLL
+#define VRGI_PLANE_UNROLL 1
+#endif
+#define VRGI_PRAGMA(x) _Pragma(#x)
+
 // SAH plane search of one range by a single lane (build.rs:45-128). Every plane of a run
 // of planes with the same (left set, right set) has the same cost, so only the plane at
 // each occupied bin (and plane 0 when bin 0 is empty) is evaluated; ascending order
@@ -561,6 +571,7 @@ __device__ void lane_best_plane(const float (*box)[SUB], uint8_t *bin8, uint32_t
             rest &= rest - 1u;
             float l[6], r[6];
             uint32_t nl = 0;
+            VRGI_PRAGMA(unroll VRGI_PLANE_UNROLL)
             for (uint32_t k = 0; k < c; ++k) {
                 const uint32_t e = f + k;
                 const bool left = bin8[e] <= b;
@@ -597,6 +608,7 @@ template <int SUB, int SMALL>
 __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     constexpr int STK = SUB / (SMALL + 1) + 2;  // pending ranges are disjoint and hold > SMALL
     constexpr int SMQ = SUB / 3 + 2;            // small ranges are disjoint and hold >= 3
+    static_assert(SUB <= 1024 && SMALL <= 127, "small-range queue packs 10 + 7 + 10 bits");
     __shared__ uint32_t s_perm[SUB];
     __shared__ float s_box[9][SUB];  // min xyz, max xyz, centroid xyz
     __shared__ uint32_t s_pair[SUB];
@@ -605,7 +617,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     __shared__ RankInfo s_rank[SUB];
     __shared__ float s_ipow[SUB];
     __shared__ uint32_t s_stack[STK][6];  // first, count, rank, node, depth, parent
-    __shared__ uint4 s_q[2][SMQ];         // small ranges: first|count<<8|(rank-job rank)<<16, node, depth, parent
+    __shared__ uint4 s_q[2][SMQ];         // small ranges: first|count<<10|(rank-job rank)<<17, node, depth, parent
     __shared__ uint8_t s_bin8[SUB];
     __shared__ uint32_t s_nsmall;
 
@@ -628,7 +640,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         if (c > (uint32_t)SMALL)
             for (int q = 0; q < 6; ++q) s_stack[0][q] = e[q];
         else
-            s_q[0][0] = make_uint4(c << 8, jn.node, jn.depth, jn.parent);
+            s_q[0][0] = make_uint4(c << 10, jn.node, jn.depth, jn.parent);
         s_nsmall = c > (uint32_t)SMALL ? 0u : 1u;
     }
     if (c > (uint32_t)SMALL) sp = 1;
@@ -737,7 +749,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                         for (int q = 0; q < 6; ++q) s_stack[sp][q] = e[q];
                     ++sp;
                 } else if (lane == 0) {
-                    s_q[0][s_nsmall++] = make_uint4(cf | (cc << 8) | ((cr - jn.rank) << 16), ci, dep + 1, gidx);
+                    s_q[0][s_nsmall++] = make_uint4(cf | (cc << 10) | ((cr - jn.rank) << 17), ci, dep + 1, gidx);
                 }
             }
         }
@@ -752,7 +764,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         __syncthreads();
         for (uint32_t w = lane; w < ncur; w += 64) {
             const uint4 e = s_q[cur][w];
-            const uint32_t nf = e.x & 0xffu, nc = (e.x >> 8) & 0xffu, rank = jn.rank + (e.x >> 16);
+            const uint32_t nf = e.x & 0x3ffu, nc = (e.x >> 10) & 0x7fu, rank = jn.rank + (e.x >> 17);
             const uint32_t gidx = e.y, dep = e.z, parent = e.w;
             uint32_t dim;
             float split, nb[6];
@@ -790,7 +802,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                     maxdepth = max(maxdepth, dep + 2);
                 } else {
                     const uint32_t slot = atomicAdd(&s_nsmall, 1u);
-                    s_q[cur ^ 1][slot] = make_uint4(cf | (cc << 8) | ((cr - jn.rank) << 16), ci, dep + 1, gidx);
+                    s_q[cur ^ 1][slot] = make_uint4(cf | (cc << 10) | ((cr - jn.rank) << 17), ci, dep + 1, gidx);
                 }
             }
         }
@@ -799,17 +811,23 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         cur ^= 1;
     }
     __syncthreads();
-    // ---------------- phase C: power, decreasing rank (children first)
+    // ---------------- phase C: power, decreasing rank (children first). Leaf powers are
+    // gathered into LDS by all lanes first (s_cj is free after phase A); the left child of
+    // rank k, when internal, is rank k + 1, the value computed one step earlier.
+    for (uint32_t k = lane; k < c; k += 64) s_cj[k] = __float_as_uint(a.power[s_perm[k]]);
+    __syncthreads();
     if (lane == 0) {
+        float prev = 0.0f;
         for (int k = (int)c - 2; k >= 0; --k) {
-            RankInfo ri = s_rank[k];
-            uint32_t nr = (uint32_t)ri.count - ri.nl;
-            float pl = ri.nl == 1 ? a.power[s_perm[ri.first]] : s_ipow[k + 1];
-            float pr = nr == 1 ? a.power[s_perm[ri.first + ri.nl]] : s_ipow[k + ri.nl];
+            const RankInfo ri = s_rank[k];
+            const uint32_t nr = (uint32_t)ri.count - ri.nl;
+            const float pl = ri.nl == 1 ? __uint_as_float(s_cj[ri.first]) : prev;
+            const float pr = nr == 1 ? __uint_as_float(s_cj[ri.first + ri.nl]) : s_ipow[k + ri.nl];
             float p = 0.0f;
             p += pl;
             p += pr;
             s_ipow[k] = p;
+            prev = p;
             a.nodes[base + ri.gidx].power = p;
         }
     }
@@ -942,7 +960,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 5 * 4, hipMemcpyDeviceToHost, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     uint32_t n_jobs = h_cnt[2];
-    if (n_jobs) k_subtree<SUB_MAX_DEFAULT, 32><<<n_jobs, 64, 0, stream>>>(a, st);
+    if (n_jobs) k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL><<<n_jobs, 64, 0, stream>>>(a, st);
     // power of the top-phase nodes, deepest level first; st.level_nodes holds
     // (global index, tree base) pairs, one contiguous run per level
     uint32_t off = level_off;

@@ -136,9 +136,21 @@ struct __align__(16) Tri { float4 v0, v1, v2; };
 
 // ---------------------------------------------------------------- device scene view
 // InstanceData (vertex.rs:31-38) of one instance, as the kernels read it
+// Both children of one internal light-BVH node in one 64-byte record (four 16-byte
+// loads), indexed like the node: boxes and powers exactly as in the children's
+// BvhNode, plus a reference per child (leaf: 0x80000000 | right_or_prim, else the
+// child's node index).
+struct __align__(16) LPair {
+    float4 a;  // left min xyz, left power
+    float4 b;  // left max xyz, right power
+    float4 c;  // right min xyz, left ref
+    float4 d;  // right max xyz, right ref
+};
+
 struct DevInst {
     float xf[12];                     // transform (column-major mat4x3)
     uint32_t tri_off;                 // first global triangle id of the instance
     uint32_t tl_idx;                  // light_bvh_tl_idx
     const vrgi_bvh_node *light_bvh;   // bvh_node_buffer_addr: the object's light BL-BVH
+    const LPair *lpairs;              // child records of that BL-BVH
 };

@@ -114,10 +114,10 @@ __global__ __launch_bounds__(64) void k_trace_closest(const SceneView sv, const 
 
 // ---------------------------------------------------------------- light BVH walks
 // raytrace.rs:181-208 and nee_pdf.rs:125-152: power / dist_sq * (visible corners / 8)
-__device__ __forceinline__ float node_importance(f3 p, f3 n, const float *m,
-                                                 const vrgi_bvh_node &nd, float eps) {
-    f3 a = xform_pt(m, mk(nd.min[0], nd.min[1], nd.min[2]));
-    f3 b = xform_pt(m, mk(nd.max[0], nd.max[1], nd.max[2]));
+__device__ __forceinline__ float box_importance(f3 p, f3 n, const float *m, f3 mn, f3 mx, float power,
+                                                float eps) {
+    f3 a = xform_pt(m, mn);
+    f3 b = xform_pt(m, mx);
     uint32_t vis = 0;
     vis += dot3(a - p, n) >= eps;
     vis += dot3(mk(a.x, a.y, b.z) - p, n) >= eps;
@@ -130,43 +130,72 @@ __device__ __forceinline__ float node_importance(f3 p, f3 n, const float *m,
     f3 diag = b - a;
     f3 mid = 0.5f * (a + b) - p;
     float dsq = gmax(dot3(diag, diag), dot3(mid, mid));
-    return nd.power / dsq * ((float)vis / 8.0f);
+    return power / dsq * ((float)vis / 8.0f);
+}
+__device__ __forceinline__ float node_importance(f3 p, f3 n, const float *m,
+                                                 const vrgi_bvh_node &nd, float eps) {
+    return box_importance(p, n, m, mk(nd.min[0], nd.min[1], nd.min[2]), mk(nd.max[0], nd.max[1], nd.max[2]),
+                          nd.power, eps);
+}
+// normalised importances of both children of one record (raytrace.rs:266-271,
+// nee_pdf.rs:201-206)
+struct PairImp { float li, ri, ln, rn; };
+__device__ __forceinline__ PairImp pair_importance(f3 p, f3 n, const float *m, const LPair &r, float eps) {
+    PairImp q;
+    q.li = box_importance(p, n, m, mk(r.a.x, r.a.y, r.a.z), mk(r.b.x, r.b.y, r.b.z), r.a.w, eps);
+    q.ri = box_importance(p, n, m, mk(r.c.x, r.c.y, r.c.z), mk(r.d.x, r.d.y, r.d.z), r.b.w, eps);
+    float tot = q.li + q.ri;
+    q.ln = q.li / tot;
+    q.rn = q.ri / tot;
+    return q;
+}
+__device__ __forceinline__ LPair ld_pair(const LPair *p) {
+    const float4 *q = (const float4 *)p;
+    LPair r;
+    r.a = q[0]; r.b = q[1]; r.c = q[2]; r.d = q[3];
+    return r;
 }
 
 __device__ const float IDENTITY12[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
 
 struct Pick { bool ok; uint32_t inst, prim; float prob, importance; };
 
-// traverseBvh, raytrace.rs:218-281
+// traverseBvh, raytrace.rs:218-281. Each level reads one child record: the same
+// boxes and powers as the two child nodes, so the same arithmetic.
 __device__ Pick traverse_light(const SceneView &sv, f3 p, f3 n, uint32_t seed) {
-    const vrgi_bvh_node *root = sv.tl;
-    vrgi_bvh_node node = root[0];
+    const vrgi_bvh_node tl0 = sv.tl[0];
     Pick res{false, 0, 0, 1.0f, 0.0f};
-    if (node.left == VRGI_INVALID && node.right_or_prim == VRGI_INVALID) return res;
+    if (tl0.left == VRGI_INVALID && tl0.right_or_prim == VRGI_INVALID) return res;  // dummy root
     float prob = 1.0f, imp = 0.0f;
     const float *m = IDENTITY12;
+    const LPair *pairs = sv.tlpairs;
     uint32_t inst = VRGI_INVALID;
     bool top = true;
+    bool leaf = tl0.left == VRGI_INVALID;
+    uint32_t idx = leaf ? tl0.right_or_prim : 0u;  // leaf: primitive / instance, else node index
     for (int guard = 0; guard < 4096; ++guard) {
-        if (top && node.left == VRGI_INVALID) {
-            inst = node.right_or_prim;
-            m = sv.inst[inst].xf;
+        if (top && leaf) {  // TL leaf: continue in the instance's BL-BVH from its root
+            inst = idx;
+            const DevInst &I = sv.inst[inst];
+            m = I.xf;
+            pairs = I.lpairs;
             top = false;
-            root = sv.inst[inst].light_bvh;
-            node = root[0];
-            if (imp == 0.0f) imp = node_importance(p, n, m, node, VRGI_EPS_RT);
+            const vrgi_bvh_node bl0 = I.light_bvh[0];
+            if (imp == 0.0f) imp = node_importance(p, n, m, bl0, VRGI_EPS_RT);
+            leaf = bl0.left == VRGI_INVALID;
+            idx = leaf ? bl0.right_or_prim : 0u;
         }
-        if (!top && node.left == VRGI_INVALID) {
-            res.ok = true; res.inst = inst; res.prim = node.right_or_prim; res.prob = prob; res.importance = imp;
+        if (!top && leaf) {
+            res.ok = true; res.inst = inst; res.prim = idx; res.prob = prob; res.importance = imp;
             return res;
         }
-        vrgi_bvh_node l = root[node.left], r = root[node.right_or_prim];
-        float li = node_importance(p, n, m, l, VRGI_EPS_RT);
-        float ri = node_importance(p, n, m, r, VRGI_EPS_RT);
-        float tot = li + ri;
-        float ln = li / tot, rn = ri / tot;
-        if (mm_finalizef(seed) < ln) { node = l; prob *= ln; imp = li; }
-        else { node = r; prob *= rn; imp = ri; }
+        const LPair r = ld_pair(pairs + idx);
+        const PairImp q = pair_importance(p, n, m, r, VRGI_EPS_RT);
+        uint32_t ref;
+        if (mm_finalizef(seed) < q.ln) { ref = __float_as_uint(r.c.w); prob *= q.ln; imp = q.li; }
+        else { ref = __float_as_uint(r.d.w); prob *= q.rn; imp = q.ri; }
+        leaf = (ref & 0x80000000u) != 0u;
+        idx = ref & 0x7FFFFFFFu;
         seed = mm_combine(seed, 0);
     }
     return res;
@@ -206,6 +235,33 @@ __device__ float reverse_traverse(const SceneView &sv, f3 p, f3 n, uint32_t inst
     return prob;
 }
 
+// reverseTraverseBvh over the precomputed ancestor list of light lid: the records of
+// all levels are independent loads; the product is formed leaf to root as in the walk
+// above
+__device__ float reverse_traverse_anc(const SceneView &sv, f3 p, f3 n, uint32_t lid) {
+    const uint32_t len = sv.lanc_len[lid];
+    const uint32_t in = sv.ltri_inst[lid];
+    if (len == VRGI_INVALID) return reverse_traverse(sv, p, n, in, sv.ltri_node[lid]);
+    const DevInst &I = sv.inst[in];
+    float xf[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) xf[k] = I.xf[k];
+    const float idm[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    const LPair *blp = I.lpairs;
+    const uint32_t *anc = sv.lanc + (size_t)lid * VRGI_LANC_MAX;
+    float prob = 1.0f;
+#pragma unroll 2
+    for (uint32_t k = 0; k < len; ++k) {
+        const uint32_t e = anc[k];
+        const bool tl = (e & 0x40000000u) != 0u;
+        const uint32_t ni = e & 0x3FFFFFFFu;
+        const LPair r = ld_pair((tl ? sv.tlpairs : blp) + ni);
+        const PairImp q = tl ? pair_importance(p, n, idm, r, VRGI_EPS_NEE) : pair_importance(p, n, xf, r, VRGI_EPS_NEE);
+        prob *= (e & 0x80000000u) ? q.rn : q.ln;
+    }
+    return prob;
+}
+
 // computeNeePdf, nee_pdf.rs:240-279, for world light triangle lid; barycentric
 // convention (x, y, 1-x-y) of pointOnTriangle (nee_pdf.rs:230-232)
 __device__ float nee_pdf_one(const SceneView &sv, f3 p, f3 n, f3 w, uint32_t lid, float bx, float by) {
@@ -216,7 +272,7 @@ __device__ float nee_pdf_one(const SceneView &sv, f3 p, f3 n, f3 w, uint32_t lid
     float area = 0.5f * len3(cross3(t0 - t1, t2 - t1));
     float cs = dot3(n, w);
     float ppp = dist * dist / (cs * area);
-    float prim_p = reverse_traverse(sv, p, n, sv.ltri_inst[lid], sv.ltri_node[lid]);
+    float prim_p = reverse_traverse_anc(sv, p, n, lid);
     return prim_p * ppp;
 }
 

@@ -27,7 +27,11 @@ inline void dev_alloc(void **p, size_t bytes) {
 }
 
 // largest subtree built by one wave with its primitives in LDS
-static constexpr int SUB_MAX_DEFAULT = 256;
+#ifndef VRGI_SUB_MAX
+#define VRGI_SUB_MAX 128
+#endif
+// largest range built by one wave entirely in LDS (bottom phase of the builder)
+static constexpr int SUB_MAX_DEFAULT = VRGI_SUB_MAX;
 
 // inputs of one batched build: n primitives, prim9 = 9 planes of n floats
 // (box min xyz, box max xyz, centroid xyz), per-primitive power and id; tree t

@@ -27,6 +27,13 @@ struct SceneView {
     uint32_t ldepth;
     const uint32_t *ltri_inst;  // instance of each light triangle
     const uint32_t *ltri_node;  // LightVertex3D.bvh_node_idx of each light triangle
+    const LPair *tlpairs;       // child records of the light top-level BVH
+    // ancestors of each light triangle, leaf to root (BL levels, then TL levels):
+    // parent node index | 0x40000000 for a TL level | 0x80000000 when the child is the
+    // right one; VRGI_LANC_MAX entries per light, lanc_len[lid] used (0xFFFFFFFF: more
+    // than VRGI_LANC_MAX, walk the parent links instead)
+    const uint32_t *lanc;
+    const uint32_t *lanc_len;
     const vrgi_material *mats;
     uint32_t n_mats;
 };
@@ -261,8 +268,23 @@ void launch_postprocess(hipStream_t s, const FrameArgs &fa);
 void launch_prims_from_tris(hipStream_t s, const float *tri9, uint32_t n, float *prim9);
 void launch_tl_inputs(hipStream_t s, const float *aabb6, const float *xf12, uint32_t n, float *prim9);
 void launch_tl_idx(hipStream_t s, const vrgi_bvh_node *tl, uint32_t n_nodes, DevInst *inst);
+#define VRGI_LANC_MAX 64
+void launch_light_pairs(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, LPair *out);
+void launch_light_ancestry(hipStream_t s, const uint32_t *ltri_inst, const uint32_t *ltri_node,
+                           const DevInst *inst, const vrgi_bvh_node *tl, uint32_t n_ltri,
+                           uint32_t *lanc, uint32_t *lanc_len);
 void launch_frame_stats(hipStream_t s, const uint32_t *qcount, uint32_t B, uint32_t P,
                         unsigned long long *counters);
+// queue reordering by (direction octant, origin Morton code) between bounces
+struct SortArgs {
+    uint32_t bounce, P, bits;         // rays of bounce + 1 read their origin / dir from bounce
+    const WideNode *root;             // root node of the geometry tree: scene box
+    const float *origin, *dir;
+    const uint32_t *count;            // device: rays in the queue
+    const uint32_t *queue_in;
+    uint32_t *queue_out, *keys, *hist;  // hist: 8 << (3 bits) counters
+};
+void launch_sort_queue(hipStream_t s, const SortArgs &a, uint32_t max_rays);
 void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsigned long long *out);
 void launch_di(hipStream_t s, const SceneView &sv, const DiArgs &da);
 

@@ -32,7 +32,8 @@ class Config(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("num_bounces", C.c_uint32),
                 ("rr_start_bounce", C.c_uint32), ("rr_min_survival", C.c_float),
                 ("restir_mode", C.c_uint32), ("restir_spatial_radius", C.c_float),
-                ("di_candidates", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32)]
+                ("di_candidates", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32),
+                ("ray_sort_bits", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -296,7 +297,7 @@ class Context:
 
 
 def config(width, height, bounces=3, rr_start=0xFFFFFFFF, rr_min=0.1, seed=0, flags=0, radius=15.0,
-           mode=0, di_candidates=32):
+           mode=0, di_candidates=32, sort_bits=0):
     cfg = Config()
     lib().vrgi_config_reference(C.byref(cfg), width, height)
     cfg.restir_mode = mode
@@ -307,4 +308,5 @@ def config(width, height, bounces=3, rr_start=0xFFFFFFFF, rr_min=0.1, seed=0, fl
     cfg.seed = seed
     cfg.flags = flags
     cfg.restir_spatial_radius = radius
+    cfg.ray_sort_bits = sort_bits
     return cfg
