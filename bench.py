@@ -37,25 +37,60 @@ class ProfileOut(C.Structure):
                 ("build_leaf_depth_sum", C.c_uint64), ("build_nodes", C.c_uint64)]
 
 
-def alg_bytes(prof, P, B, spatial):
-    """Algorithmic (compulsory) bytes per frame of each stage, as defined for this path:
-    build 72 N + 36 S + 40 (2N-1) (N primitives, S = sum of leaf depths);
-    trace+shade per bounce (48 + 150) R_b + U_b (R_b rays, U_b unique node / triangle
-    bytes touched, counted on the GPU); resolve 64 P B + 228 P + (k+2) 80 P + 128 P."""
+def light_structure_bytes(n_ltri):
+    """Bytes of every structure the NEE pdf walk can read, per light triangle: its
+    traversal-copy node (64), triangle (48), instance + leaf ids (8), ancestor list
+    (<= 64 levels x 4, about 20 used; 80 counted) and the child records of the light
+    BVHs (2 x 64). An upper bound of the unique structure bytes of one launch."""
+    return n_ltri * (64 + 48 + 8 + 80 + 128)
+
+
+def kernel_rooflines(prof, touch_bytes, n_ltri, B):
+    """Per-launch algorithmic bytes and average launch time of the per-ray kernels.
+    k_trace_shade (one launch per bounce): (48 + 150) B per ray -- ray in 24-32 B and hit
+    out 16 B, shading outputs and material / triangle reads 150 B -- plus the unique node
+    and triangle bytes the launch touches (counted on the GPU). k_nee_pdf (one launch per
+    bounce but the last): per ray 48 B (queue id, origin, direction, normal, MIS weight,
+    pdf out) plus the light structures."""
+    f = max(prof.frames, 1)
+    rays = [prof.rays[b] / f for b in range(B)]
+    trace_ms = [prof.trace_ms[b] / f for b in range(B)]
+    nee_ms = [prof.nee_ms[b] / f for b in range(B - 1)]
+    trace_bytes = [198 * rays[b] + touch_bytes[b] for b in range(B)]
+    nee_bytes = [48 * rays[b + 1] + light_structure_bytes(n_ltri) for b in range(B - 1)]
+    return {
+        "k_trace_shade": {"launches": B, "bytes": sum(trace_bytes) / B, "ms": sum(trace_ms) / B},
+        "k_nee_pdf": {"launches": B - 1, "bytes": sum(nee_bytes) / (B - 1), "ms": sum(nee_ms) / (B - 1)},
+    }
+
+
+def build_alg_bytes(prof):
+    """BVH build: 72 N (three 24-B vertices per primitive) + 36 S (S = sum of leaf depths:
+    28 B of bin input and 8 B of partition traffic per primitive per level) + 40 (2N-1)
+    node writes."""
     N, S = prof.build_prims, prof.build_leaf_depth_sum
-    build = 72 * N + 36 * S + 40 * prof.build_nodes
-    frames = max(prof.frames, 1)
-    trace = [198 * prof.rays[b] / frames + prof.touched_bytes[b] for b in range(B)]
-    resolve = 64 * P * B + 228 * P + (spatial + 2) * 80 * P + 128 * P
-    return build, trace, resolve
+    return 72 * N + 36 * S + 40 * prof.build_nodes
 
 
-def cpu_baseline(objs, mats, lum, cams, budget_rows=24):
+def calibrate_copy():
+    """A plain device-to-device copy of known size (256 MiB read, 256 MiB written per
+    launch), run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    n = 64 << 20  # float32 elements = 256 MiB
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
+
+
+def cpu_baseline(objs, mats, lum, cams):
     """The oracle (C++ restatement of the reference algorithms) on the host cores, on a
-    bounded sample of the same workload: one full geometry + light BVH build of the
-    world (single thread, as the reference's per-object builder) and `budget_rows` rows
-    of one 1920x1080 frame (all bounces, NEE pdf; OpenMP over pixels). Extrapolated to
-    a full frame: build + rows * (1080 / budget_rows)."""
+    bounded sample of the same workload: one full geometry + light BVH build of the world
+    (single thread, as the reference's per-object builder) and one complete 1920x1080
+    frame of the same configuration (4 bounces, RR from bounce 2, NEE pdf, ReSTIR GI
+    temporal + 5 spatial; OpenMP over pixels). ms/frame = build + frame."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = min(os.cpu_count() or 1, 16)
@@ -65,14 +100,13 @@ def cpu_baseline(objs, mats, lum, cams, budget_rows=24):
     s.build()
     t_build = time.perf_counter() - t0
     r = oracle.Renderer(W_PX, H_PX, BOUNCES, rr_start=RR_START)
-    y0 = H_PX // 2 - budget_rows // 2
     t0 = time.perf_counter()
-    r.render_rows(s, cams[0], 0, y0, y0 + budget_rows, nee_type=NEE, threads=threads)
-    t_rows = time.perf_counter() - t0
-    ms = 1000.0 * (t_build + t_rows * H_PX / budget_rows)
+    r.render(s, cams[0], 0, nee_type=NEE, spatial=SPATIAL, threads=threads)
+    t_frame = time.perf_counter() - t0
+    ms = 1000.0 * (t_build + t_frame)
     return {"value": round(ms, 1), "unit": "ms/frame", "cores": threads, "kind": "port",
-            "sample": f"oracle: 1 full BVH build (1 thread, {t_build:.1f} s) + {budget_rows} of 1080 rows "
-                      f"({t_rows:.1f} s, {threads} threads), extrapolated to one frame; ReSTIR passes excluded"}
+            "sample": f"oracle: 1 full geometry + light BVH build (1 thread, {t_build:.2f} s) + 1 full 1920x1080 "
+                      f"frame with NEE and ReSTIR GI ({t_frame:.2f} s, {threads} threads)"}
 
 
 def main():
@@ -81,10 +115,14 @@ def main():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=24)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy (pmc counter scaling)")
     ap.add_argument("--sort-bits", type=int, default=0,
                     help="Morton ray-queue ordering: origin bits per axis (0 = compaction order)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the frame would shard by "
               "image rows / scenes; see the design notes). Run with --gpus 1.", file=sys.stderr)
@@ -107,6 +145,9 @@ def main():
             ctx.render(cams[k % 16], nee_type=NEE, spatial=SPATIAL)
         ctx.sync()
         ctx.stats(reset=True)
+        check = vrgi.check
+        # per-kernel HIP events on the library's stream during the timed frames
+        check(L.vrgi_profile(ctx.h, 1))
         log(f"timing {a.steps} frames")
         t0 = time.perf_counter()
         for k in range(a.steps):
@@ -114,24 +155,18 @@ def main():
         ctx.sync()
         dt = time.perf_counter() - t0
         st = ctx.stats(reset=True)
+        prof = ProfileOut()
+        check(L.vrgi_profile_read(ctx.h, C.byref(prof), 1))
         ms = 1000.0 * dt / a.steps
         mrays = st.rays_closest / dt / 1e6
         log(f"{ms:.3f} ms/frame, {mrays:.1f} Mrays/s closest-hit, depth {st.bvh_depth}, nodes {st.bvh_nodes}")
-        # per-stage HIP-event timing on the library's stream, then one counting frame
-        n_prof = min(a.steps, 8)
-        check = vrgi.check
-        check(L.vrgi_profile(ctx.h, 1))
-        for k in range(n_prof):
-            ctx.render(cams[k % 16], nee_type=NEE, spatial=SPATIAL)
-        prof = ProfileOut()
-        check(L.vrgi_profile_read(ctx.h, C.byref(prof), 1))
+        # one counting frame after the timed region: unique node / triangle bytes per bounce
         check(L.vrgi_profile(ctx.h, 2))
         ctx.render(cams[0], nee_type=NEE, spatial=SPATIAL)
         touch = ProfileOut()
         check(L.vrgi_profile_read(ctx.h, C.byref(touch), 1))
         check(L.vrgi_profile(ctx.h, 0))
-        for b in range(BOUNCES):
-            prof.touched_bytes[b] = touch.touched_bytes[b]
+        n_ltri = (len(ctx.read(vrgi.BUF_LIGHT_GEOM_BVH, np.uint8)) // 40 + 1) // 2
     f = max(prof.frames, 1)
     P = W_PX * H_PX
     stages = {"build": prof.build_ms / f, "resolve": prof.resolve_ms / f, "frame": prof.frame_ms / f}
@@ -139,24 +174,29 @@ def main():
         stages[f"trace_{b}"] = prof.trace_ms[b] / f
         if b + 1 < BOUNCES:
             stages[f"nee_{b}"] = prof.nee_ms[b] / f
-    b_build, b_trace, b_resolve = alg_bytes(prof, P, BOUNCES, SPATIAL)
-    t_trace = sum(stages[f"trace_{b}"] for b in range(BOUNCES))
-    cands = {"bvh_build": (b_build, stages["build"]), "trace_shade": (sum(b_trace), t_trace),
-             "resolve_restir": (b_resolve, stages["resolve"])}
-    dom = max(cands, key=lambda k: cands[k][1])
-    alg, t_ms = cands[dom]
-    achieved = alg / (t_ms * 1e-3) / 1e9 if t_ms > 0 else 0.0
+    kr = kernel_rooflines(prof, list(touch.touched_bytes), n_ltri, BOUNCES)
+    dom = max(kr, key=lambda k: kr[k]["ms"] * kr[k]["launches"])
+    kd = kr[dom]
+    achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9 if kd["ms"] > 0 else 0.0
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(dom)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "alg_bytes_per_frame": int(alg), "ms_per_frame": round(t_ms, 4)}
+        traffic = json.load(open(tfile)).get(dom, {}).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "alg_bytes_per_launch": int(kd["bytes"]), "avg_launch_ms": round(kd["ms"], 4),
+                "launches_per_frame": kd["launches"]}
+    others = {k: {"alg_bytes_per_launch": int(v["bytes"]), "avg_launch_ms": round(v["ms"], 4),
+                  "achieved_GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 2) if v["ms"] > 0 else 0.0}
+              for k, v in kr.items()}
+    if prof.build_ms > 0:
+        bb = build_alg_bytes(prof)
+        others["bvh_build (all builder kernels)"] = {"alg_bytes_per_frame": int(bb), "ms": round(stages["build"], 4),
+                                                     "achieved_GBps": round(bb / (stages["build"] * 1e-3) / 1e9, 2)}
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline (oracle on the host cores, bounded sample)")
-        cpu = cpu_baseline(objs, mats, lum, cams, a.cpu_rows)
+        cpu = cpu_baseline(objs, mats, lum, cams)
     out = {
         "metric": METRIC, "value": round(ms, 4), "unit": "ms/frame", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": False, "scaling": "weak",
@@ -167,6 +207,7 @@ def main():
         "mrays_per_s": round(mrays, 2),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roofline,
+        "kernels": others,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
