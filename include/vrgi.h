@@ -205,11 +205,67 @@ enum vrgi_buffer_id {
     VRGI_BUF_HIT_T,              /* float[B][P] hit distance at bounce b */
     VRGI_BUF_DI_LIGHT,           /* uint32[P] DI reservoir light (global light triangle id) */
     VRGI_BUF_DI_WEIGHTS,         /* float[P][4] DI reservoir (w_sum, W, M, p_hat) */
+    VRGI_BUF_BOUNCE_ALBEDO,      /* float[B][P][3] material reflectivity at each hit (cache enabled) */
+    VRGI_BUF_CACHE_RADIANCE,     /* float[P][3] cache radiance of the queried pixels (cache enabled) */
+    VRGI_BUF_CACHE_QUERIES,      /* float[n][16] query records of the last frame (cache enabled) */
+    VRGI_BUF_CACHE_TRAIN_RECORDS,/* float[n][16] training records of the last frame (cache enabled) */
     VRGI_BUF_COUNT
 };
 /* bytes needed for buffer `which` in the current state (0 if not available). */
 int vrgi_buffer_size(vrgi_ctx *ctx, int which, size_t *bytes);
 int vrgi_read_buffer(vrgi_ctx *ctx, int which, void *dst, size_t bytes);
+
+/* ---- learned radiance cache (no counterpart in the reference, which has no learned part) ----
+ * A small MLP (64 encoded inputs -> 4 x 64 ReLU -> 3, bf16 on the matrix cores, fp32
+ * accumulation) over a 12-level x 2-feature hash grid, spherical harmonics (l < 4) of the
+ * ray direction and of the normal, and the albedo; relative L2 loss, Adam. While enabled,
+ * render() traces full training paths (train_bounces) from a random train_fraction of
+ * the pixels, ends every other path at its bounce-1 vertex with the cache's reflected
+ * radiance there, and then takes steps_per_frame Adam steps on the training paths'
+ * vertices. A record (query or training) is 16 floats: position, incoming direction,
+ * normal, albedo, target radiance (training), key = pixel * 16 + bounce (uint bits). */
+typedef struct vrgi_cache_config {
+    uint32_t log2_table;        /* entries per grid level = 2^log2_table (18) */
+    uint32_t res_min, res_max;  /* grid resolution of the coarsest and finest level (16, 2048) */
+    float box_lo[3], box_hi[3]; /* region mapped to the grid; all zero: the geometry's root box at
+                                   the first render after enabling */
+    float train_fraction;       /* share of pixels tracing training paths (0.03) */
+    uint32_t train_bounces;     /* bounces of a training path, 2..16 (8) */
+    uint32_t batch;             /* records per Adam step, multiple of 16 (16384) */
+    uint32_t steps_per_frame;   /* Adam steps per frame (4) */
+    float lr, beta1, beta2, eps;/* Adam (0.01, 0.9, 0.99, 1e-10) */
+    float loss_scale;           /* output-gradient scale, divided out before Adam (128) */
+    uint32_t seed;              /* initial weights and table */
+    uint32_t flags;             /* VRGI_CACHE_* */
+} vrgi_cache_config;
+
+#define VRGI_CACHE_FREEZE 1u  /* query only: render() takes no training steps */
+#define VRGI_CACHE_WEIGHTS (4 * 64 * 64 + 64 * 16) /* floats: layers 1-4 [k][n] 64x64, layer 5 [k][n] 64x16 */
+
+typedef struct vrgi_cache_info {
+    uint32_t levels, feats, log2_table;
+    uint32_t res[16];
+    float box_lo[3], box_hi[3];
+    uint64_t table_floats;      /* levels x 2^log2_table x feats */
+    uint64_t steps;             /* Adam steps taken */
+    float last_loss;            /* mean loss of the last frame's steps */
+} vrgi_cache_info;
+
+void vrgi_cache_config_default(vrgi_cache_config *cfg);
+int vrgi_cache_enable(vrgi_ctx *ctx, const vrgi_cache_config *cfg);
+int vrgi_cache_disable(vrgi_ctx *ctx);
+int vrgi_cache_info_get(vrgi_ctx *ctx, vrgi_cache_info *out);
+/* host copies of the fp32 parameters: weights (VRGI_CACHE_WEIGHTS), table (table_floats) */
+int vrgi_cache_get(vrgi_ctx *ctx, float *weights, float *table);
+/* replaces the parameters (either may be NULL) and clears the Adam state */
+int vrgi_cache_set(vrgi_ctx *ctx, const float *weights, const float *table);
+/* n samples of 12 floats (position, direction, normal, albedo) -> n x 3 radiance */
+int vrgi_cache_infer(vrgi_ctx *ctx, uint64_t n, const float *samples, float *out);
+/* gradients of the mean loss over n samples (targets n x 3), without an update */
+int vrgi_cache_grads(vrgi_ctx *ctx, uint64_t n, const float *samples, const float *targets, float *grad_w,
+                     float *grad_table, float *loss);
+/* one Adam step on n samples; loss before the step */
+int vrgi_cache_train(vrgi_ctx *ctx, uint64_t n, const float *samples, const float *targets, float *loss);
 
 /* ---- standalone operators (the drop-in pieces, usable without a scene) ---- */
 

@@ -852,6 +852,7 @@ inline IcS local_cs(v3 a, v3 b, v3 c) {
 
 struct BounceOut {
     v3 origin, dir, normal, emis, refl;
+    v3 alb = V(0, 0, 0);  // material reflectivity at the hit (learned-cache inputs)
     float mis, pdf;
     uint32_t hit;
     float t;
@@ -890,6 +891,7 @@ BounceOut raytrace_px(const or_scene *s, const Cfg &c, uint32_t bounce, uint32_t
     v3 new_origin = (t0 * b3.x + t1 * b3.y) + t2 * b3.z;
     v3 new_dir;
     v3 refl = V(mt.refl[0], mt.refl[1], mt.refl[2]);
+    o.alb = refl;
     float alpha = mt.refl[3];
     v3 emis = (1000.0f * V(mt.emis[0], mt.emis[1], mt.emis[2])) * -dot(dir, ics.n);
     float metal = mt.metal[0];
@@ -1138,8 +1140,11 @@ struct FrameBufs {
     uint32_t B;
     std::vector<float> origin, dir, normal, emis, refl, mis, pdf, nee, lout, q, t;
     std::vector<uint32_t> hit, plen;
+    std::vector<float> alb;                 // learned cache: [B][P][3]
+    std::vector<std::vector<float>> qrec, trec;  // learned cache: per pixel records
     void alloc(size_t p, uint32_t b) {
         P = p; B = b;
+        alb.assign(b * p * 3, 0);
         origin.assign((b + 1) * p * 3, 0); dir.assign((b + 1) * p * 3, 0);
         normal.assign(b * p * 3, 0); emis.assign(b * p * 3, 0); refl.assign(b * p * 3, 0);
         mis.assign(b * p, 0); pdf.assign(b * p, 0); nee.assign(b * p, 0);
@@ -1152,8 +1157,14 @@ struct FrameBufs {
 // the frame (raygen coordinates and RNG stream); `slot` indexes the buffers of f.
 void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t frame, size_t id,
                  size_t slot, const v3 &eye, const v3 &front, const v3 &right, const v3 &up,
-                 FrameBufs &f) {
+                 FrameBufs &f, const or_cache_frame *cf = nullptr) {
     const size_t P = f.P;
+    bool train = false;
+    uint32_t cap = c.B;  // bounces this pixel's path may take
+    if (cf) {
+        train = finalizef(combine(cf->train_seed, (uint32_t)id)) < cf->train_fraction;
+        cap = train ? c.B : 2u;
+    }
     uint32_t x = (uint32_t)(id % c.W), y = (uint32_t)(id / c.W);
     // raygen.rs:80-111 (jitter is 0.00 * (...), i.e. zero)
     float uvx = 2.0f * (float)x / (float)c.W - 1.0f;
@@ -1163,11 +1174,12 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
     st3(&f.origin[slot * 3], eye);
     st3(&f.dir[slot * 3], d0);
     uint32_t plen = 0;
-    for (uint32_t b = 0; b < c.B; ++b) {
+    for (uint32_t b = 0; b < cap; ++b) {
         size_t bi = b * P + slot, bo = (b + 1) * P + slot;
         v3 o = ld3(&f.origin[bi * 3]), d = ld3(&f.dir[bi * 3]);
         if (!is_zero(d)) plen = b + 1;
         BounceOut r = raytrace_px(s, c, b, nee_type, frame * c.B + b, (uint32_t)id, o, d);
+        if (cf) st3(&f.alb[bi * 3], r.alb);
         st3(&f.origin[bo * 3], r.origin);
         st3(&f.dir[bo * 3], r.dir);
         st3(&f.normal[bi * 3], r.normal);
@@ -1181,7 +1193,7 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
     f.plen[slot] = plen;
     // nee_pdf.rs:281-337, bounces 0..B-2 (the last bounce's nee_pdf is never written)
     std::vector<LHit> hits;
-    for (uint32_t b = 0; b + 1 < c.B; ++b) {
+    for (uint32_t b = 0; b + 1 < cap; ++b) {
         size_t bi = b * P + slot, bo = (b + 1) * P + slot;
         v3 d = ld3(&f.dir[bo * 3]);
         float w = f.mis[bi];
@@ -1198,15 +1210,39 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
     // outgoing_radiance.rs:58-94
     v3 L = V(0, 0, 0);
     const float bsp = 1.0f / (2.0f * PI_F);
+    const bool queried = cf && !train && plen == 2 && f.hit[1 * P + slot] != 0xFFFFFFFFu;
     for (int b = (int)c.B - 1; b >= 0; --b) {
         size_t bid = (size_t)b * P + slot;
         float valid = is_zero(ld3(&f.dir[bid * 3])) ? 0.0f : 1.0f;
         float bp = f.pdf[bid], np = f.nee[bid], w = f.mis[bid];
         float q = np * w + (1.0f - w) * bsp;
         float rf = bp / q;
-        L = ld3(&f.emis[bid * 3]) + ((ld3(&f.refl[bid * 3]) * L) * rf) * valid;
+        v3 term = ((ld3(&f.refl[bid * 3]) * L) * rf) * valid;
+        if (queried && b == 1) term = ld3(cf->cache_rad + id * 3);
+        L = ld3(&f.emis[bid * 3]) + term;
         st3(&f.lout[bid * 3], L);
         f.q[bid] = q;
+        // training record of vertex b (its reflected radiance is known unless the path was
+        // cut by the bounce cap right after it)
+        if (cf && train && b >= 1 && b < (int)plen && f.hit[bid] != 0xFFFFFFFFu &&
+            (b + 1 < (int)plen || is_zero(ld3(&f.dir[(bid + P) * 3])))) {
+            v3 p = ld3(&f.origin[(bid + P) * 3]), dd = ld3(&f.dir[bid * 3]);
+            v3 nn = ld3(&f.normal[bid * 3]), aa = ld3(&f.alb[bid * 3]);
+            uint32_t key = (uint32_t)id * 16u + (uint32_t)b;
+            float kf;
+            std::memcpy(&kf, &key, 4);
+            f.trec[slot].insert(f.trec[slot].begin(), {p.x, p.y, p.z, dd.x, dd.y, dd.z, nn.x, nn.y, nn.z, aa.x, aa.y,
+                                                       aa.z, term.x, term.y, term.z, kf});
+        }
+    }
+    if (cf && !train && plen == 2 && f.hit[1 * P + slot] != 0xFFFFFFFFu) {
+        size_t b1 = 1 * P + slot;
+        v3 p = ld3(&f.origin[(b1 + P) * 3]), dd = ld3(&f.dir[b1 * 3]);
+        v3 nn = ld3(&f.normal[b1 * 3]), aa = ld3(&f.alb[b1 * 3]);
+        uint32_t key = (uint32_t)id * 16u + 1u;
+        float kf;
+        std::memcpy(&kf, &key, 4);
+        f.qrec[slot] = {p.x, p.y, p.z, dd.x, dd.y, dd.z, nn.x, nn.y, nn.z, aa.x, aa.y, aa.z, 0.0f, 0.0f, 0.0f, kf};
     }
 }
 
@@ -1465,21 +1501,40 @@ or_renderer *or_renderer_create(const or_config *cfg) {
 }
 void or_renderer_destroy(or_renderer *r) { delete r; }
 
-int or_render(or_renderer *rr, or_scene *s, const or_camera *cam, const or_prefs *prefs,
-              uint64_t frame64, or_frame *out, int threads) {
+static int render_impl(or_renderer *rr, or_scene *s, const or_camera *cam, const or_prefs *prefs,
+                       uint64_t frame64, or_frame *out, int threads, or_cache_frame *cf) {
     const Cfg &c = rr->c;
     if (c.B < 2) return -1;
     size_t P = (size_t)c.W * c.H;
     uint32_t frame = (uint32_t)frame64;
     FrameBufs f;
     f.alloc(P, c.B);
+    if (cf) {
+        f.qrec.assign(P, {});
+        f.trec.assign(P, {});
+    }
     v3 eye = ld3(cam->eye), front = ld3(cam->front), right = ld3(cam->right), up = ld3(cam->up);
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #endif
 #pragma omp parallel for schedule(dynamic, 64)
     for (long id = 0; id < (long)P; ++id)
-        trace_pixel(s, c, prefs->nee_type, frame, (size_t)id, (size_t)id, eye, front, right, up, f);
+        trace_pixel(s, c, prefs->nee_type, frame, (size_t)id, (size_t)id, eye, front, right, up, f, cf);
+    if (cf) {
+        if (cf->albedo) std::memcpy(cf->albedo, f.alb.data(), f.alb.size() * 4);
+        cf->n_qrec = 0;
+        cf->n_trec = 0;
+        for (size_t id = 0; id < P; ++id) {
+            if (!f.qrec[id].empty()) {
+                if (cf->qrec) std::memcpy(cf->qrec + cf->n_qrec * 16, f.qrec[id].data(), 64);
+                cf->n_qrec += 1;
+            }
+            for (size_t k = 0; k < f.trec[id].size(); k += 16) {
+                if (cf->trec && cf->n_trec < cf->trec_cap) std::memcpy(cf->trec + cf->n_trec * 16, &f.trec[id][k], 64);
+                cf->n_trec += 1;
+            }
+        }
+    }
 
     // ReSTIR GI temporal (restir_temporal_resampling.rs:256-301) on slot frame % 3,
     // with the initial sample assembled as interactive_rendering.rs:1359-1414
@@ -1599,6 +1654,16 @@ int or_render(or_renderer *rr, or_scene *s, const or_camera *cam, const or_prefs
     cp(out->di_light, di_light.data(), P * 4);
     cp(out->di_weights, di_w.data(), P * 16);
     return 0;
+}
+
+int or_render(or_renderer *rr, or_scene *s, const or_camera *cam, const or_prefs *prefs, uint64_t frame64,
+              or_frame *out, int threads) {
+    return render_impl(rr, s, cam, prefs, frame64, out, threads, nullptr);
+}
+
+int or_render_cache(or_renderer *rr, or_scene *s, const or_camera *cam, const or_prefs *prefs, uint64_t frame64,
+                    or_frame *out, int threads, or_cache_frame *cf) {
+    return render_impl(rr, s, cam, prefs, frame64, out, threads, cf);
 }
 
 int or_render_rows(or_renderer *rr, or_scene *s, const or_camera *cam, const or_prefs *prefs,

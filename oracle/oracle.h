@@ -103,12 +103,31 @@ typedef struct or_frame {
     float *di_weights;  // [P][4]
 } or_frame;
 
+// Learned-radiance-cache frame: a pixel traces a training path of all B bounces when
+// u(finalize(combine(train_seed, pixel))) < train_fraction, otherwise it stops at bounce 1
+// and, when that bounce hit, ends with L_1 = E_1 + cache_rad[pixel] (the reflected
+// radiance the cache returned there). Records are 16 floats: position, incoming direction,
+// normal, albedo, target (training records: the reflected radiance of the vertex), key =
+// pixel * 16 + bounce; both lists come out sorted by key.
+typedef struct or_cache_frame {
+    float train_fraction;
+    uint32_t train_seed;
+    const float *cache_rad;  // [P][3] in
+    float *albedo;           // [B][P][3] out: material reflectivity at each hit
+    float *qrec;             // [P][16] out
+    uint64_t n_qrec;
+    float *trec;             // [trec_cap][16] out
+    uint64_t trec_cap, n_trec;
+} or_cache_frame;
+
 typedef struct or_renderer or_renderer;
 or_renderer *or_renderer_create(const or_config *cfg);
 void or_renderer_destroy(or_renderer *r);
 // renders frame number `frame` (ReSTIR slot frame % 3); threads: OpenMP threads (0 = default)
 int or_render(or_renderer *r, or_scene *s, const or_camera *cam, const or_prefs *prefs,
               uint64_t frame, or_frame *out, int threads);
+int or_render_cache(or_renderer *r, or_scene *s, const or_camera *cam, const or_prefs *prefs,
+                    uint64_t frame, or_frame *out, int threads, or_cache_frame *cf);
 // rows [y0, y1) only (for the bounded CPU baseline sample); no ReSTIR
 int or_render_rows(or_renderer *r, or_scene *s, const or_camera *cam, const or_prefs *prefs,
                    uint64_t frame, uint32_t y0, uint32_t y1, float *radiance, int threads);

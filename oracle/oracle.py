@@ -39,6 +39,10 @@ def lib():
         L.or_scene_count.restype = C.c_uint64
         L.or_renderer_create.restype = C.c_void_p
         L.or_reverse_traverse_light_bvh.restype = C.c_float
+        L.or_nrc_create.restype = C.c_void_p
+        L.or_nrc_hash.restype = C.c_uint32
+        L.or_nrc_grads.restype = C.c_double
+        L.or_nrc_train_step.restype = C.c_double
         _lib = L
     return _lib
 
@@ -128,6 +132,18 @@ class Prefs(C.Structure):
 FRAME_FIELDS = ["origin", "direction", "normal", "emissivity", "reflectivity", "mis_weight", "bsdf_pdf",
                 "nee_pdf", "l_out", "q_omega", "hit_prim", "hit_t", "path_len", "temporal", "spatial",
                 "restir", "debug", "bgra8", "di_light", "di_weights"]
+
+
+class CacheFrame(C.Structure):
+    _fields_ = [("train_fraction", C.c_float), ("train_seed", C.c_uint32), ("cache_rad", C.c_void_p),
+                ("albedo", C.c_void_p), ("qrec", C.c_void_p), ("n_qrec", C.c_uint64), ("trec", C.c_void_p),
+                ("trec_cap", C.c_uint64), ("n_trec", C.c_uint64)]
+
+
+def frame_seed(seed, frame, salt):
+    """Counter-based per-frame seed: finalize(combine(combine(seed, salt), frame))."""
+    L, u = lib(), C.c_uint32
+    return L.or_murmur3_finalize(u(L.or_murmur3_combine(u(L.or_murmur3_combine(u(seed), u(salt))), u(frame))))
 
 
 class Frame(C.Structure):
@@ -269,9 +285,133 @@ class Renderer:
         assert r == 0
         return out
 
+    def render_cache(self, scene, cam, frame, cache_rad, train_fraction, train_seed, nee_type=0, spatial=0,
+                     threads=0, trec_cap=1 << 20):
+        """A learned-radiance-cache frame (see or_cache_frame): cache_rad [P][3] is the
+        cache's reflected radiance at the bounce-1 vertex of every queried pixel."""
+        P, B = self.W * self.H, self.B
+        out = {
+            "origin": np.zeros((B + 1, P, 3), np.float32), "direction": np.zeros((B + 1, P, 3), np.float32),
+            "normal": np.zeros((B, P, 3), np.float32), "emissivity": np.zeros((B, P, 3), np.float32),
+            "reflectivity": np.zeros((B, P, 3), np.float32), "mis_weight": np.zeros((B, P), np.float32),
+            "bsdf_pdf": np.zeros((B, P), np.float32), "nee_pdf": np.zeros((B, P), np.float32),
+            "l_out": np.zeros((B, P, 3), np.float32), "q_omega": np.zeros((B, P), np.float32),
+            "hit_prim": np.zeros((B, P), np.uint32), "hit_t": np.zeros((B, P), np.float32),
+            "path_len": np.zeros(P, np.uint32), "temporal": np.zeros((P, 20), np.float32),
+            "spatial": np.zeros((P, 20), np.float32), "restir": np.zeros((P, 3), np.float32),
+            "debug": np.zeros((P, 3), np.float32), "bgra8": np.zeros((P, 4), np.uint8),
+            "di_light": np.zeros(P, np.uint32), "di_weights": np.zeros((P, 4), np.float32),
+        }
+        fr = Frame(**{k: C.c_void_p(v.ctypes.data) for k, v in out.items()})
+        crad = np.ascontiguousarray(cache_rad, np.float32).reshape(P, 3)
+        alb = np.zeros((B, P, 3), np.float32)
+        qrec = np.zeros((P, 16), np.float32)
+        trec = np.zeros((trec_cap, 16), np.float32)
+        cf = CacheFrame(train_fraction, train_seed, crad.ctypes.data, alb.ctypes.data, qrec.ctypes.data, 0,
+                        trec.ctypes.data, trec_cap, 0)
+        p = Prefs(nee_type, 0, spatial)
+        r = self.L.or_render_cache(self.h, scene.h, C.byref(cam), C.byref(p), C.c_uint64(frame), C.byref(fr),
+                                   threads, C.byref(cf))
+        assert r == 0
+        out["albedo"] = alb
+        out["qrec"] = qrec[:cf.n_qrec].copy()
+        out["trec"] = trec[:min(cf.n_trec, trec_cap)].copy()
+        out["n_trec"] = cf.n_trec
+        return out
+
     def render_rows(self, scene, cam, frame, y0, y1, nee_type=0, threads=0):
         rad = np.zeros(((y1 - y0) * self.W, 3), np.float32)
         r = self.L.or_render_rows(self.h, scene.h, C.byref(cam), C.byref(Prefs(nee_type, 0, 0)),
                                   C.c_uint64(frame), y0, y1, ptr(rad), threads)
         assert r == 0
         return rad
+
+
+# ---------------------------------------------------------------- learned radiance cache
+N_IN, N_WIDTH, N_HIDDEN, N_OUT_PAD = 64, 64, 4, 16
+N_WEIGHTS = N_HIDDEN * N_IN * N_WIDTH + N_WIDTH * N_OUT_PAD
+
+
+class NrcConfig(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("feats", C.c_uint32), ("log2_T", C.c_uint32),
+                ("res", C.c_uint32 * 16), ("box_lo", C.c_float * 3), ("box_hi", C.c_float * 3),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("loss_scale", C.c_float), ("loss", C.c_uint32), ("emulate_bf16", C.c_uint32)]
+
+
+def nrc_hash(x, y, z, log2_T):
+    return lib().or_nrc_hash(C.c_uint32(x), C.c_uint32(y), C.c_uint32(z), C.c_uint32(log2_T))
+
+
+def nrc_sh16(v):
+    out = np.zeros(16, np.float64)
+    lib().or_nrc_sh16(ptr(np.ascontiguousarray(v, np.float64)), ptr(out))
+    return out
+
+
+class Nrc:
+    """FP64 learned radiance cache (encoding, MLP, relative L2 loss, backprop, Adam)."""
+
+    def __init__(self, levels, feats, log2_T, res, box_lo, box_hi, weights, table, lr=1e-2, beta1=0.9,
+                 beta2=0.99, eps=1e-10, loss_scale=128.0, loss=0, emulate_bf16=False):
+        self.L = lib()
+        r = (C.c_uint32 * 16)(*list(res))
+        self.cfg = NrcConfig(levels, feats, log2_T, r, (C.c_float * 3)(*box_lo), (C.c_float * 3)(*box_hi),
+                             lr, beta1, beta2, eps, loss_scale, loss, int(emulate_bf16))
+        self.n_table = levels * (1 << log2_T) * feats
+        w = np.ascontiguousarray(weights, np.float32)
+        t = np.ascontiguousarray(table, np.float32)
+        assert w.size == N_WEIGHTS and t.size == self.n_table
+        self.h = C.c_void_p(self.L.or_nrc_create(C.byref(self.cfg), ptr(w), ptr(t)))
+
+    def close(self):
+        if self.h:
+            self.L.or_nrc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _in(pos, dir, nrm, alb):
+        f = lambda a: np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+        return f(pos), f(dir), f(nrm), f(alb)
+
+    def encode(self, pos, dir, nrm, alb):
+        p, d, n, a = self._in(pos, dir, nrm, alb)
+        out = np.zeros((len(p), N_IN), np.float64)
+        self.L.or_nrc_encode(self.h, C.c_uint64(len(p)), ptr(p), ptr(d), ptr(n), ptr(a), ptr(out))
+        return out
+
+    def forward(self, pos, dir, nrm, alb):
+        p, d, n, a = self._in(pos, dir, nrm, alb)
+        out = np.zeros((len(p), 3), np.float64)
+        self.L.or_nrc_forward(self.h, C.c_uint64(len(p)), ptr(p), ptr(d), ptr(n), ptr(a), ptr(out))
+        return out
+
+    def grads(self, pos, dir, nrm, alb, target):
+        p, d, n, a = self._in(pos, dir, nrm, alb)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        gw = np.zeros(N_WEIGHTS, np.float64)
+        gt = np.zeros(self.n_table, np.float64)
+        loss = self.L.or_nrc_grads(self.h, C.c_uint64(len(p)), ptr(p), ptr(d), ptr(n), ptr(a), ptr(t), ptr(gw), ptr(gt))
+        return loss, gw, gt
+
+    def train_step(self, pos, dir, nrm, alb, target):
+        p, d, n, a = self._in(pos, dir, nrm, alb)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        return self.L.or_nrc_train_step(self.h, C.c_uint64(len(p)), ptr(p), ptr(d), ptr(n), ptr(a), ptr(t))
+
+    def get(self):
+        w = np.zeros(N_WEIGHTS, np.float64)
+        t = np.zeros(self.n_table, np.float64)
+        self.L.or_nrc_get(self.h, ptr(w), ptr(t))
+        return w, t
+
+    def set(self, weights=None, table=None):
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        t = None if table is None else np.ascontiguousarray(table, np.float64)
+        self.L.or_nrc_set(self.h, None if w is None else ptr(w), None if t is None else ptr(t))

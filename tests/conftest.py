@@ -57,6 +57,17 @@ def world0():
     return dict(blocks=blocks, n_types=nt, mats=mats, lum=lum, tr=tr, objs=objs)
 
 
+@pytest.fixture(scope="session")
+def scenes(world0):
+    """The oracle's scene of the 64^3 world (built once) and its chunk objects."""
+    import oracle
+    objs = world0["objs"]
+    s = oracle.Scene(world0["mats"], world0["lum"])
+    s.add_objects(objs)
+    s.build()
+    return s, objs
+
+
 def random_soup(n, seed, spread=50.0, size=2.0, grid=False):
     """Random triangles; grid=True snaps vertices to integers (voxel-like, many ties)."""
     rng = np.random.default_rng(seed)

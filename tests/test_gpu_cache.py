@@ -1,0 +1,209 @@
+"""Learned radiance cache on the GPU against the CPU restatement.
+
+The network kernels (bf16 operands on the matrix cores, fp32 accumulation) are compared
+with the oracle in its bf16-emulation mode, which rounds at the same points and sums in
+double: the remaining differences come from the fp32 accumulation order, so the bounds
+are relative 1e-2 (a single activation that rounds to the other bf16 neighbour moves an
+output by ~0.4 %). The frame logic of cache mode (training-pixel draw, bounce caps, query
+and training records, radiance with the cache's value at bounce 1, ReSTIR) is compared
+bit for bit: the oracle is given the cache radiance the GPU computed."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import say
+
+pytestmark = pytest.mark.gpu
+
+LOG2_T = 14
+
+
+def _ctx(world0, W=16, H=16, **kw):
+    import vrgi
+    c = vrgi.Context(vrgi.config(W, H, 3, **kw), world0["mats"], world0["lum"])
+    return c
+
+
+def _oracle_net(ctx, w, t, emulate=True, **kw):
+    import oracle
+    info = ctx.cache_info()
+    return oracle.Nrc(info.levels, info.feats, info.log2_table, list(info.res), list(info.box_lo), list(info.box_hi),
+                      w, t, emulate_bf16=emulate, **kw)
+
+
+def _random_params(rng, scale=0.25, table_scale=0.5):
+    import vrgi
+    w = np.zeros(vrgi.CACHE_WEIGHTS, np.float32)
+    w[:4 * 4096] = rng.uniform(-scale, scale, 4 * 4096)
+    w5 = np.zeros((64, 16), np.float32)
+    w5[:, :3] = rng.uniform(-scale, scale, (64, 3))
+    w[4 * 4096:] = w5.ravel()
+    t = rng.uniform(-table_scale, table_scale, 12 * (1 << LOG2_T) * 2).astype(np.float32)
+    return w, t
+
+
+def _samples(n, rng):
+    pos = rng.uniform(0.5, 63.5, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    nn = rng.normal(size=(n, 3))
+    nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+    alb = rng.uniform(0.05, 0.95, (n, 3)).astype(np.float32)
+    return pos, d.astype(np.float32), nn.astype(np.float32), alb
+
+
+def _enable(ctx, **kw):
+    args = dict(log2_table=LOG2_T, box_lo=(0, 0, 0), box_hi=(64, 64, 64))
+    args.update(kw)
+    return ctx.cache_enable(**args)
+
+
+def test_cache_infer_parity(world0):
+    rng = np.random.default_rng(11)
+    with _ctx(world0) as c:
+        _enable(c)
+        w, t = _random_params(rng)
+        c.cache_set(w, t)
+        x = _samples(4000, rng)
+        say("cache inference, 4000 samples")
+        y = c.cache_infer(*x)
+        em = _oracle_net(c, w, t).forward(*x)
+        f64 = _oracle_net(c, w, t, emulate=False).forward(*x)
+    scale = np.abs(em).max()
+    assert scale > 0.05, "outputs too small to compare"
+    err = np.abs(y - em)
+    assert np.linalg.norm(y - em) <= 5e-3 * np.linalg.norm(em)
+    assert err.max() <= 1e-2 * scale
+    # bf16 rounding itself against the exact network: a few per cent
+    assert np.linalg.norm(y - f64) <= 3e-2 * np.linalg.norm(f64)
+
+
+def test_cache_grads_parity(world0):
+    rng = np.random.default_rng(12)
+    n = 300
+    with _ctx(world0) as c:
+        _enable(c)
+        w, t = _random_params(rng, scale=0.2)
+        c.cache_set(w, t)
+        x = _samples(n, rng)
+        tgt = rng.uniform(0, 2, (n, 3)).astype(np.float32)
+        say("cache gradients, 300 samples")
+        loss, gw, gt = c.cache_grads(*x, tgt)
+        lo, gwo, gto = _oracle_net(c, w, t).grads(*x, tgt)
+    assert loss == pytest.approx(lo, rel=2e-3)
+    for layer in range(5):
+        off, cnt = layer * 4096, (4096 if layer < 4 else 1024)
+        a, b = gw[off:off + cnt], gwo[off:off + cnt]
+        assert np.linalg.norm(a - b) <= 2e-2 * np.linalg.norm(b), f"layer {layer + 1}"
+    assert np.linalg.norm(gt - gto) <= 2e-2 * np.linalg.norm(gto)
+    assert np.all(gw[4 * 4096:].reshape(64, 16)[:, 3:] == 0)
+
+
+def test_cache_adam_steps_parity(world0):
+    """Ten Adam steps on the GPU and in the oracle from the same start: the loss
+    trajectories agree and both descend."""
+    rng = np.random.default_rng(13)
+    n = 512
+    with _ctx(world0) as c:
+        _enable(c, lr=1e-3)
+        w, t = _random_params(rng, scale=0.15, table_scale=0.05)
+        c.cache_set(w, t)
+        x = _samples(n, rng)
+        tgt = (0.5 + 0.5 * np.sin(x[0] / 9.0) * x[3]).astype(np.float32)
+        ora = _oracle_net(c, w, t, lr=1e-3)
+        gl, ol = [], []
+        say("cache Adam steps x 10")
+        for _ in range(10):
+            gl.append(c.cache_train(*x, tgt))
+            ol.append(ora.train_step(*x, tgt))
+        w1, t1 = c.cache_get()
+    wo, to = ora.get()
+    np.testing.assert_allclose(gl, ol, rtol=5e-2)
+    assert gl[-1] < 0.9 * gl[0] and ol[-1] < 0.9 * ol[0]
+    # parameters: same direction of travel
+    dw, dwo = w1 - w, wo - w
+    assert np.dot(dw, dwo) > 0.9 * np.linalg.norm(dw) * np.linalg.norm(dwo)
+
+
+def test_cache_learns_smooth_field(world0):
+    rng = np.random.default_rng(14)
+    n = 8192
+    with _ctx(world0) as c:
+        _enable(c)
+        x = _samples(n, rng)
+        p = x[0]
+        tgt = (1.0 + np.stack([np.sin(p[:, 0] / 7), np.cos(p[:, 1] / 5), np.sin(p[:, 2] / 11)], 1) *
+               x[3]).astype(np.float32)
+        say("cache training, 300 steps")
+        losses = [c.cache_train(*x, tgt) for _ in range(300)]
+        pred = c.cache_infer(*x)
+    assert losses[-1] < 0.05 * losses[0]
+    rel = np.abs(pred - tgt) / (np.abs(tgt) + 0.1)
+    assert np.median(rel) < 0.05
+
+
+def _by_key(rec):
+    keys = rec[:, 15].view(np.uint32)
+    return rec[np.argsort(keys, kind="stable")]
+
+
+@pytest.mark.parametrize("nee_type,spatial", [(1, 2), (0, 0)])
+def test_cache_frame_parity(world0, scenes, nee_type, spatial):
+    import oracle
+    import vrgi
+    s, objs = scenes
+    W, H, TB = 96, 64, 5
+    cam = vrgi.camera_look([6.0, 62.0, 8.0], [40.0, 0.0, 44.0])
+    frac, seed = 0.25, 3
+    with _ctx(world0, W, H, flags=vrgi.FLAG_KEEP_BOUNCES) as c:
+        c.add_objects(objs)
+        _enable(c, box_lo=(0, 0, 0), box_hi=(0, 0, 0), train_fraction=frac, train_bounces=TB, batch=4096,
+                steps_per_frame=2, seed=seed)
+        r = oracle.Renderer(W, H, TB)
+        P = W * H
+        for f in range(3):
+            say(f"cache frame {f} nee={nee_type} spatial={spatial}")
+            w_before, t_before = c.cache_get()  # the frame queries these; it trains after
+            c.render(cam, nee_type=nee_type, spatial=spatial)
+            c.sync()
+            crad = c.read(vrgi.BUF_CACHE_RADIANCE).reshape(P, 3)
+            ref = r.render_cache(s, cam, f, crad, frac, oracle.frame_seed(seed, f, 5), nee_type=nee_type,
+                                 spatial=spatial)
+            plen = c.read(vrgi.BUF_PATH_LEN, np.uint32)
+            assert np.array_equal(plen, ref["path_len"])
+            assert plen.max() == TB and (plen == 2).any()
+            live = np.arange(TB)[:, None] < plen[None, :]
+            hit = c.read(vrgi.BUF_HIT_PRIM, np.uint32).reshape(TB, P)
+            assert np.array_equal(hit[live], ref["hit_prim"][live])
+            for buf, key in [(vrgi.BUF_BOUNCE_ORIGIN, "origin"), (vrgi.BUF_BOUNCE_DIRECTION, "direction")]:
+                g = c.read(buf).reshape(TB, P, 3)
+                assert g[live].tobytes() == ref[key][1:][live].tobytes(), key
+            for buf, key, k in [(vrgi.BUF_BOUNCE_NORMAL, "normal", 3), (vrgi.BUF_BOUNCE_EMISSIVITY, "emissivity", 3),
+                                (vrgi.BUF_BOUNCE_REFLECTIVITY, "reflectivity", 3),
+                                (vrgi.BUF_BOUNCE_MIS_WEIGHT, "mis_weight", 1), (vrgi.BUF_BOUNCE_BSDF_PDF, "bsdf_pdf", 1),
+                                (vrgi.BUF_BOUNCE_ALBEDO, "albedo", 3)]:
+                g = c.read(buf).reshape(TB, P, k)
+                assert g[live].tobytes() == ref[key].reshape(TB, P, k)[live].tobytes(), key
+            if nee_type:
+                cont = np.arange(TB)[:, None] + 1 < plen[None, :]
+                g = c.read(vrgi.BUF_BOUNCE_NEE_PDF).reshape(TB, P)
+                assert g[cont].tobytes() == ref["nee_pdf"][cont].tobytes()
+            # records: same sets, bit for bit
+            q = _by_key(c.read(vrgi.BUF_CACHE_QUERIES).reshape(-1, 16))
+            assert q.tobytes() == ref["qrec"].tobytes()
+            tr = _by_key(c.read(vrgi.BUF_CACHE_TRAIN_RECORDS).reshape(-1, 16))
+            assert len(tr) == ref["n_trec"] and len(tr) > 100
+            assert tr.tobytes() == ref["trec"].tobytes()
+            # the frame's cache radiance is the network (weights before the frame's training
+            # steps) on its query records
+            y = _oracle_net(c, w_before, t_before).forward(q[:, 0:3], q[:, 3:6], q[:, 6:9], q[:, 9:12])
+            px = q[:, 15].view(np.uint32) >> 4
+            y = np.maximum(y, 0)
+            assert np.linalg.norm(crad[px] - y) <= 1e-2 * np.linalg.norm(y) + 1e-6
+            assert c.read(vrgi.BUF_RADIANCE).tobytes() == ref["l_out"][0].tobytes()
+            assert c.read(vrgi.BUF_TEMPORAL_RESERVOIR).tobytes() == ref["temporal"].tobytes()
+            assert c.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
+            assert c.read(vrgi.BUF_BGRA8, np.uint8).tobytes() == ref["bgra8"].tobytes()
+        info = c.cache_info()
+        assert info.steps == 6 and math.isfinite(info.last_loss)

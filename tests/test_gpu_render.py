@@ -13,17 +13,6 @@ from conftest import say
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def scenes(world0):
-    import oracle
-    import vrgi
-    objs = world0["objs"]
-    s = oracle.Scene(world0["mats"], world0["lum"])
-    s.add_objects(objs)
-    s.build()
-    return s, objs
-
-
 def _rays(n, seed, lo=-4.0, hi=68.0):
     rng = np.random.default_rng(seed)
     o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)

@@ -136,6 +136,22 @@ struct vrgi_ctx {
     DevBuf<unsigned long long> touch_count;
     DevBuf<unsigned long long> depth_sum_copy;
 
+    // learned radiance cache
+    struct Cache {
+        bool on = false;
+        vrgi_cache_config cfg{};
+        uint32_t res[16] = {0};
+        float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        bool box_ready = false;
+        size_t n_table = 0;
+        uint64_t steps = 0;
+        uint32_t plane_cap = 0, tcap = 0, hcap = 0;
+        DevBuf<float> w32, mw, vw, table, mt, vt, gw, gtable, loss;
+        DevBuf<uint16_t> wt, wb, xplane, gplane;
+        DevBuf<float> trec, qrec, crad, hrec, hout, albedo;
+        DevBuf<uint32_t> counts;  // [0] training records, [1] query records, [2] caller records
+    } cache;
+
     // host-side scratch of the standalone operators
     DevBuf<float> a_f0, a_f1, a_f2;
     DevBuf<uint32_t> a_u0, a_u1, a_u2;
@@ -174,6 +190,70 @@ SceneView scene_view(vrgi_ctx *c) {
     return sv;
 }
 
+NrcNet cache_net(const vrgi_ctx *c) {
+    NrcNet n{};
+    n.log2_T = c->cache.cfg.log2_table;
+    std::memcpy(n.res, c->cache.res, sizeof(n.res));
+    for (int k = 0; k < 3; ++k) { n.lo[k] = c->cache.lo[k]; n.hi[k] = c->cache.hi[k]; }
+    n.table = c->cache.table.p;
+    n.wt = c->cache.wt.p;
+    n.wb = c->cache.wb.p;
+    return n;
+}
+
+// the cache box defaults to the geometry's root box (union of the root's two children)
+void cache_box_from_scene(vrgi_ctx *c) {
+    auto &C = c->cache;
+    if (C.box_ready) return;
+    if (c->n_tris >= 2) {
+        WideNode w;
+        HIP_CHECK(hipMemcpyAsync(&w, c->d_gwide.p, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        const float lo[3] = {std::min(w.a.x, w.b.z), std::min(w.a.y, w.b.w), std::min(w.a.z, w.c.x)};
+        const float hi[3] = {std::max(w.a.w, w.c.y), std::max(w.b.x, w.c.z), std::max(w.b.y, w.c.w)};
+        for (int k = 0; k < 3; ++k) { C.lo[k] = lo[k]; C.hi[k] = hi[k]; }
+    } else {
+        for (int k = 0; k < 3; ++k) { C.lo[k] = 0.0f; C.hi[k] = 1.0f; }
+    }
+    C.box_ready = true;
+}
+
+// one training step over records rec[k + i * steps] (at most `batch`): gradients only, or
+// gradients + Adam + repacking of the bf16 weights
+void cache_step(vrgi_ctx *c, const float *rec, const uint32_t *count, uint32_t cap, uint32_t k, uint32_t steps,
+                uint32_t batch, bool apply) {
+    auto &C = c->cache;
+    NrcTrain tr{};
+    tr.rec = rec;
+    tr.count = count;
+    tr.rec_cap = cap;
+    tr.step_k = k;
+    tr.steps = steps;
+    tr.max_batch = batch;
+    tr.loss = 0;
+    tr.loss_scale = C.cfg.loss_scale;
+    tr.xplane = C.xplane.p;
+    tr.gplane = C.gplane.p;
+    tr.gw = C.gw.p;
+    tr.gtable = C.gtable.p;
+    tr.loss_out = C.loss.p;
+    if (!apply) {
+        launch_nrc_grads(c->stream, cache_net(c), tr);
+        return;
+    }
+    C.steps += 1;
+    NrcAdam ad{};
+    ad.lr = C.cfg.lr;
+    ad.beta1 = C.cfg.beta1;
+    ad.beta2 = C.cfg.beta2;
+    ad.eps = C.cfg.eps;
+    ad.c1 = (float)(1.0 - std::pow((double)C.cfg.beta1, (double)C.steps));
+    ad.c2 = (float)(1.0 - std::pow((double)C.cfg.beta2, (double)C.steps));
+    ad.inv_scale = 1.0f / C.cfg.loss_scale;
+    launch_nrc_train_step(c->stream, cache_net(c), tr, C.w32.p, C.mw.p, C.vw.p, C.table.p, C.mt.p, C.vt.p,
+                          C.n_table, ad);
+}
+
 void alloc_frame(vrgi_ctx *c) {
     size_t P = c->P, B = c->B;
     c->f_origin.reserve(B * P * 3);
@@ -188,6 +268,11 @@ void alloc_frame(vrgi_ctx *c) {
     c->f_hit.reserve(B * P);
     c->f_plen.reserve(P);
     c->f_queue.reserve((B + 1) * P);
+    if (c->cache.on) {
+        c->cache.albedo.reserve(B * P * 3);
+        c->cache.qrec.reserve(P * 16);
+        c->cache.crad.reserve(P * 3);
+    }
     if (c->cfg.ray_sort_bits) {
         c->f_squeue.reserve((B + 1) * P);
         c->f_skeys.reserve(P);
@@ -713,6 +798,11 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     rec(1);
     const uint32_t frame = (uint32_t)c->frame_count;
     HIP_CHECK(hipMemsetAsync(c->f_qcount.p, 0, (B + 2) * 4, s));
+    const bool cache_on = c->cache.on;
+    if (cache_on) {
+        cache_box_from_scene(c);
+        HIP_CHECK(hipMemsetAsync(c->cache.counts.p, 0, 8, s));
+    }
     if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) {
         size_t n = (size_t)B * P;
         HIP_CHECK(hipMemsetAsync(c->f_origin.p, 0, n * 12, s));
@@ -738,6 +828,15 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     ba.emis = c->f_emis.p; ba.refl = c->f_refl.p; ba.mis = c->f_mis.p; ba.pdf = c->f_pdf.p;
     ba.nee = c->f_nee.p; ba.hit_t = c->f_hit_t.p; ba.hit = c->f_hit.p; ba.path_len = c->f_plen.p;
     ba.counters = (unsigned long long *)c->d_counters.p;
+    const uint32_t train_seed = frame_seed_host(c->cache.cfg.seed, frame, 5);
+    if (cache_on) {
+        ba.cache_on = 1;
+        ba.train_seed = train_seed;
+        ba.train_frac = c->cache.cfg.train_fraction;
+        ba.albedo = c->cache.albedo.p;
+        ba.qrec = c->cache.qrec.p;
+        ba.qrec_count = c->cache.counts.p + 1;
+    }
     size_t touch_words = ((size_t)c->n_tris + 31) / 32;
     if (touch) {
         c->touch_bits.reserve(2 * touch_words);
@@ -786,6 +885,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         }
         rec(2 + B + b);
     }
+    if (cache_on)  // cache radiance of the queried bounce-1 vertices
+        launch_nrc_infer(s, cache_net(c), c->cache.qrec.p, c->cache.counts.p + 1, P, nullptr, c->cache.crad.p);
     rec(2 + B + (B - 1));
     FrameArgs fa{};
     fa.W = c->W; fa.H = c->H; fa.P = P; fa.B = B; fa.frame = frame;
@@ -804,6 +905,18 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     fa.restir = c->f_restir.p;
     fa.debug = c->f_debug.p;
     fa.bgra8 = c->f_bgra.p;
+    if (cache_on) {
+        fa.cache_on = 1;
+        fa.train_seed = train_seed;
+        fa.train_frac = c->cache.cfg.train_fraction;
+        fa.dir = c->f_dir.p;
+        fa.albedo = c->cache.albedo.p;
+        fa.crad = c->cache.crad.p;
+        fa.hit = c->f_hit.p;
+        fa.trec = c->cache.trec.p;
+        fa.trec_count = c->cache.counts.p;
+        fa.tcap = c->cache.tcap;
+    }
     launch_accum_temporal(s, fa);
     if (c->cfg.restir_mode == 1) {
         DiArgs da{};
@@ -824,6 +937,12 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         launch_spatial_finalize(s, fa);
     }
     launch_postprocess(s, fa);
+    if (cache_on && !(c->cache.cfg.flags & VRGI_CACHE_FREEZE) && c->cache.cfg.steps_per_frame) {
+        HIP_CHECK(hipMemsetAsync(c->cache.loss.p, 0, 4, s));
+        for (uint32_t k = 0; k < c->cache.cfg.steps_per_frame; ++k)
+            cache_step(c, c->cache.trec.p, c->cache.counts.p, c->cache.tcap, k, c->cache.cfg.steps_per_frame,
+                       c->cache.cfg.batch, true);
+    }
     launch_frame_stats(s, c->f_qcount.p, B, P, (unsigned long long *)c->d_counters.p);
     rec(2 + 2 * (size_t)B);
     if (stages || touch) {
@@ -926,6 +1045,7 @@ int vrgi_profile_read(vrgi_ctx *c, vrgi_profile_out *out, int reset) {
 }
 
 int vrgi_buffer_size(vrgi_ctx *c, int which, size_t *bytes) {
+    API_BEGIN
     if (!c || !bytes) return fail(VRGI_E_INVALID, "null argument");
     size_t P = c->P, B = c->B, nt = c->n_tris, nl = c->n_ltri;
     size_t r = 0;
@@ -944,10 +1064,22 @@ int vrgi_buffer_size(vrgi_ctx *c, int which, size_t *bytes) {
         case VRGI_BUF_BGRA8: r = P * 4; break;
         case VRGI_BUF_DI_LIGHT: r = c->cfg.restir_mode == 1 ? P * 4 : 0; break;
         case VRGI_BUF_DI_WEIGHTS: r = c->cfg.restir_mode == 1 ? P * 16 : 0; break;
+        case VRGI_BUF_BOUNCE_ALBEDO: r = c->cache.on ? B * P * 12 : 0; break;
+        case VRGI_BUF_CACHE_RADIANCE: r = c->cache.on ? P * 12 : 0; break;
+        case VRGI_BUF_CACHE_QUERIES:
+        case VRGI_BUF_CACHE_TRAIN_RECORDS: {
+            if (!c->cache.on) { r = 0; break; }
+            uint32_t cnt[2] = {0, 0};
+            HIP_CHECK(hipMemcpyAsync(cnt, c->cache.counts.p, 8, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            r = which == VRGI_BUF_CACHE_QUERIES ? (size_t)cnt[1] * 64 : (size_t)std::min(cnt[0], c->cache.tcap) * 64;
+            break;
+        }
         default: r = 0;
     }
     *bytes = r;
     return VRGI_OK;
+    API_END
 }
 
 int vrgi_read_buffer(vrgi_ctx *c, int which, void *dst, size_t bytes) {
@@ -987,6 +1119,10 @@ int vrgi_read_buffer(vrgi_ctx *c, int which, void *dst, size_t bytes) {
         case VRGI_BUF_BGRA8: src = c->f_bgra.p; break;
         case VRGI_BUF_DI_LIGHT: src = c->di_light.p; break;
         case VRGI_BUF_DI_WEIGHTS: src = c->di_w.p; break;
+        case VRGI_BUF_BOUNCE_ALBEDO: src = c->cache.albedo.p; break;
+        case VRGI_BUF_CACHE_RADIANCE: src = c->cache.crad.p; break;
+        case VRGI_BUF_CACHE_QUERIES: src = c->cache.qrec.p; break;
+        case VRGI_BUF_CACHE_TRAIN_RECORDS: src = c->cache.trec.p; break;
         default: return fail(VRGI_E_INVALID, "unknown buffer id");
     }
     HIP_CHECK(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, c->stream));
@@ -1104,6 +1240,271 @@ int vrgi_trace_closest(vrgi_ctx *c, const float *o, const float *d, uint64_t n, 
     HIP_CHECK(hipMemcpyAsync(t, c->a_f2.p, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(uv, c->a_f2.p + m, 8 * (size_t)m, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    return VRGI_OK;
+    API_END
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- learned radiance cache
+namespace {
+
+uint64_t splitmix64(uint64_t &x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+float uniform(uint64_t &x, float lo, float hi) {
+    return lo + (hi - lo) * (float)((double)(splitmix64(x) >> 11) * (1.0 / 9007199254740992.0));
+}
+
+void cache_clear_state(vrgi_ctx *c) {
+    auto &C = c->cache;
+    hipStream_t s = c->stream;
+    HIP_CHECK(hipMemsetAsync(C.mw.p, 0, NRC_N_WEIGHTS * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.vw.p, 0, NRC_N_WEIGHTS * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.gw.p, 0, NRC_N_WEIGHTS * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.mt.p, 0, C.n_table * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.vt.p, 0, C.n_table * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.gtable.p, 0, C.n_table * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, s));
+    C.steps = 0;
+    launch_nrc_pack(s, C.w32.p, C.wt.p, C.wb.p);
+}
+
+void cache_planes(vrgi_ctx *c, uint32_t batch) {
+    auto &C = c->cache;
+    if (batch <= C.plane_cap) return;
+    C.xplane.reserve((size_t)5 * 64 * batch);
+    C.gplane.reserve((size_t)5 * 64 * batch);
+    C.plane_cap = batch;
+}
+
+// caller samples (n x 12) and targets (n x 3, may be null) as records in hrec; count[2] = n
+void cache_upload_records(vrgi_ctx *c, uint64_t n, const float *samples, const float *targets) {
+    auto &C = c->cache;
+    std::vector<float> h((size_t)n * 16, 0.0f);
+    for (uint64_t i = 0; i < n; ++i) {
+        std::memcpy(&h[i * 16], samples + i * 12, 48);
+        if (targets) std::memcpy(&h[i * 16 + 12], targets + i * 3, 12);
+    }
+    C.hrec.reserve((size_t)n * 16);
+    uint32_t cnt = (uint32_t)n;
+    HIP_CHECK(hipMemcpyAsync(C.hrec.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(C.counts.p + 2, &cnt, 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));  // h is released on return
+}
+
+}  // namespace
+
+extern "C" {
+
+void vrgi_cache_config_default(vrgi_cache_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->log2_table = 18;
+    cfg->res_min = 16;
+    cfg->res_max = 2048;
+    cfg->train_fraction = 0.03f;
+    cfg->train_bounces = 8;
+    cfg->batch = 16384;
+    cfg->steps_per_frame = 4;
+    cfg->lr = 1e-2f;
+    cfg->beta1 = 0.9f;
+    cfg->beta2 = 0.99f;
+    cfg->eps = 1e-10f;
+    cfg->loss_scale = 128.0f;
+    cfg->seed = 1;
+    cfg->flags = 0;
+}
+
+int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
+    API_BEGIN
+    if (!c || !cfg) return fail(VRGI_E_INVALID, "null argument");
+    if (cfg->log2_table < 10 || cfg->log2_table > 24) return fail(VRGI_E_INVALID, "log2_table must be in [10, 24]");
+    if (cfg->res_min < 2 || cfg->res_max < cfg->res_min || cfg->res_max > (1u << 20))
+        return fail(VRGI_E_INVALID, "need 2 <= res_min <= res_max <= 2^20");
+    if (cfg->train_bounces < 2 || cfg->train_bounces > 16) return fail(VRGI_E_INVALID, "train_bounces must be in [2, 16]");
+    if (cfg->batch < 16 || cfg->batch % 16 || cfg->batch > (1u << 20))
+        return fail(VRGI_E_INVALID, "batch must be a multiple of 16 in [16, 2^20]");
+    if (cfg->steps_per_frame > 64) return fail(VRGI_E_INVALID, "steps_per_frame above 64");
+    if (!(cfg->train_fraction >= 0.0f && cfg->train_fraction <= 1.0f))
+        return fail(VRGI_E_INVALID, "train_fraction must be in [0, 1]");
+    if (!(cfg->lr > 0.0f) || !(cfg->loss_scale > 0.0f)) return fail(VRGI_E_INVALID, "lr and loss_scale must be positive");
+    if (c->cfg.restir_mode != 0) return fail(VRGI_E_STATE, "the learned cache runs with ReSTIR GI (restir_mode 0)");
+    bool box_zero = true;
+    for (int k = 0; k < 3; ++k) box_zero = box_zero && cfg->box_lo[k] == 0.0f && cfg->box_hi[k] == 0.0f;
+    if (!box_zero)
+        for (int k = 0; k < 3; ++k)
+            if (!(cfg->box_hi[k] > cfg->box_lo[k])) return fail(VRGI_E_INVALID, "box_hi must exceed box_lo");
+    set_dev(c);
+    auto &C = c->cache;
+    C.cfg = *cfg;
+    const double b = std::exp((std::log((double)cfg->res_max) - std::log((double)cfg->res_min)) / (NRC_LEVELS - 1));
+    for (int l = 0; l < 16; ++l) C.res[l] = l < NRC_LEVELS ? (uint32_t)std::floor(cfg->res_min * std::pow(b, l)) : 0u;
+    C.box_ready = !box_zero;
+    for (int k = 0; k < 3; ++k) { C.lo[k] = cfg->box_lo[k]; C.hi[k] = cfg->box_hi[k]; }
+    C.n_table = (size_t)NRC_LEVELS * ((size_t)1 << cfg->log2_table) * 2;
+    C.w32.reserve(NRC_N_WEIGHTS); C.mw.reserve(NRC_N_WEIGHTS); C.vw.reserve(NRC_N_WEIGHTS); C.gw.reserve(NRC_N_WEIGHTS);
+    C.table.reserve(C.n_table); C.mt.reserve(C.n_table); C.vt.reserve(C.n_table); C.gtable.reserve(C.n_table);
+    C.wt.reserve(NRC_N_WEIGHTS); C.wb.reserve(NRC_N_WEIGHTS);
+    C.loss.reserve(1);
+    C.counts.reserve(3);
+    C.plane_cap = 0;
+    cache_planes(c, cfg->batch);
+    C.tcap = cfg->batch * std::max(cfg->steps_per_frame, 1u);
+    C.trec.reserve((size_t)C.tcap * 16);
+    // initial parameters: Kaiming-uniform weights (no bias vectors), table U(-1e-4, 1e-4)
+    uint64_t st = 0x5EEDull ^ ((uint64_t)cfg->seed << 32);
+    std::vector<float> w(NRC_N_WEIGHTS, 0.0f), t(C.n_table);
+    const float bound = std::sqrt(6.0f / 64.0f);
+    for (int i = 0; i < 4 * 64 * 64; ++i) w[i] = uniform(st, -bound, bound);
+    for (int k = 0; k < 64; ++k)
+        for (int n = 0; n < 3; ++n) w[4 * 4096 + k * 16 + n] = uniform(st, -bound, bound);
+    for (size_t i = 0; i < C.n_table; ++i) t[i] = uniform(st, -1e-4f, 1e-4f);
+    HIP_CHECK(hipMemcpyAsync(C.w32.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(C.table.p, t.data(), t.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemsetAsync(C.counts.p, 0, 12, c->stream));
+    cache_clear_state(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));  // w, t are released on return
+    C.on = true;
+    c->B = cfg->train_bounces;
+    alloc_frame(c);
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_cache_disable(vrgi_ctx *c) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    set_dev(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->cache.on = false;
+    c->B = c->cfg.num_bounces;
+    alloc_frame(c);
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_cache_info_get(vrgi_ctx *c, vrgi_cache_info *out) {
+    API_BEGIN
+    if (!c || !out) return fail(VRGI_E_INVALID, "null argument");
+    if (!c->cache.on && !c->cache.w32.p) return fail(VRGI_E_STATE, "the cache was never enabled");
+    set_dev(c);
+    auto &C = c->cache;
+    std::memset(out, 0, sizeof(*out));
+    out->levels = NRC_LEVELS;
+    out->feats = 2;
+    out->log2_table = C.cfg.log2_table;
+    std::memcpy(out->res, C.res, sizeof(out->res));
+    for (int k = 0; k < 3; ++k) { out->box_lo[k] = C.lo[k]; out->box_hi[k] = C.hi[k]; }
+    out->table_floats = C.n_table;
+    out->steps = C.steps;
+    float l = 0.0f;
+    HIP_CHECK(hipMemcpyAsync(&l, C.loss.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    out->last_loss = C.cfg.steps_per_frame ? l / (float)C.cfg.steps_per_frame : l;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_cache_get(vrgi_ctx *c, float *weights, float *table) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    if (!c->cache.w32.p) return fail(VRGI_E_STATE, "the cache was never enabled");
+    set_dev(c);
+    if (weights) HIP_CHECK(hipMemcpyAsync(weights, c->cache.w32.p, NRC_N_WEIGHTS * 4, hipMemcpyDeviceToHost, c->stream));
+    if (table) HIP_CHECK(hipMemcpyAsync(table, c->cache.table.p, c->cache.n_table * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_cache_set(vrgi_ctx *c, const float *weights, const float *table) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    if (!c->cache.w32.p) return fail(VRGI_E_STATE, "the cache was never enabled");
+    set_dev(c);
+    if (weights) HIP_CHECK(hipMemcpyAsync(c->cache.w32.p, weights, NRC_N_WEIGHTS * 4, hipMemcpyHostToDevice, c->stream));
+    if (table) HIP_CHECK(hipMemcpyAsync(c->cache.table.p, table, c->cache.n_table * 4, hipMemcpyHostToDevice, c->stream));
+    cache_clear_state(c);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_cache_infer(vrgi_ctx *c, uint64_t n, const float *samples, float *out) {
+    API_BEGIN
+    if (!c || (n && (!samples || !out))) return fail(VRGI_E_INVALID, "null argument");
+    if (!c->cache.w32.p) return fail(VRGI_E_STATE, "the cache was never enabled");
+    if (n > (1u << 26)) return fail(VRGI_E_LIMIT, "more than 2^26 samples");
+    if (n == 0) return VRGI_OK;
+    set_dev(c);
+    auto &C = c->cache;
+    if (!C.box_ready) return fail(VRGI_E_STATE, "the cache box is set at the first render (or give one)");
+    cache_upload_records(c, n, samples, nullptr);
+    C.hout.reserve((size_t)n * 3);
+    launch_nrc_infer(c->stream, cache_net(c), C.hrec.p, C.counts.p + 2, (uint32_t)n, C.hout.p, nullptr);
+    HIP_CHECK(hipMemcpyAsync(out, C.hout.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_cache_grads(vrgi_ctx *c, uint64_t n, const float *samples, const float *targets, float *grad_w,
+                     float *grad_table, float *loss) {
+    API_BEGIN
+    if (!c || !samples || !targets || !n) return fail(VRGI_E_INVALID, "null argument or empty batch");
+    if (!c->cache.w32.p) return fail(VRGI_E_STATE, "the cache was never enabled");
+    if (n > (1u << 20)) return fail(VRGI_E_LIMIT, "more than 2^20 samples");
+    set_dev(c);
+    auto &C = c->cache;
+    if (!C.box_ready) return fail(VRGI_E_STATE, "the cache box is set at the first render (or give one)");
+    const uint32_t mb = (uint32_t)((n + 15) / 16 * 16);
+    cache_planes(c, mb);
+    cache_upload_records(c, n, samples, targets);
+    hipStream_t s = c->stream;
+    HIP_CHECK(hipMemsetAsync(C.gw.p, 0, NRC_N_WEIGHTS * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.gtable.p, 0, C.n_table * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, s));
+    cache_step(c, C.hrec.p, C.counts.p + 2, (uint32_t)n, 0, 1, mb, false);
+    std::vector<float> gw(NRC_N_WEIGHTS);
+    HIP_CHECK(hipMemcpyAsync(gw.data(), C.gw.p, NRC_N_WEIGHTS * 4, hipMemcpyDeviceToHost, s));
+    if (grad_table) HIP_CHECK(hipMemcpyAsync(grad_table, C.gtable.p, C.n_table * 4, hipMemcpyDeviceToHost, s));
+    float l = 0.0f;
+    HIP_CHECK(hipMemcpyAsync(&l, C.loss.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    const float inv = 1.0f / C.cfg.loss_scale;
+    if (grad_w)
+        for (int i = 0; i < NRC_N_WEIGHTS; ++i) grad_w[i] = gw[i] * inv;
+    if (grad_table)
+        for (size_t i = 0; i < C.n_table; ++i) grad_table[i] *= inv;
+    if (loss) *loss = l;
+    HIP_CHECK(hipMemsetAsync(C.gw.p, 0, NRC_N_WEIGHTS * 4, s));
+    HIP_CHECK(hipMemsetAsync(C.gtable.p, 0, C.n_table * 4, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_cache_train(vrgi_ctx *c, uint64_t n, const float *samples, const float *targets, float *loss) {
+    API_BEGIN
+    if (!c || !samples || !targets || !n) return fail(VRGI_E_INVALID, "null argument or empty batch");
+    if (!c->cache.w32.p) return fail(VRGI_E_STATE, "the cache was never enabled");
+    if (n > (1u << 20)) return fail(VRGI_E_LIMIT, "more than 2^20 samples");
+    set_dev(c);
+    auto &C = c->cache;
+    if (!C.box_ready) return fail(VRGI_E_STATE, "the cache box is set at the first render (or give one)");
+    const uint32_t mb = (uint32_t)((n + 15) / 16 * 16);
+    cache_planes(c, mb);
+    cache_upload_records(c, n, samples, targets);
+    HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, c->stream));
+    cache_step(c, C.hrec.p, C.counts.p + 2, (uint32_t)n, 0, 1, mb, true);
+    float l = 0.0f;
+    HIP_CHECK(hipMemcpyAsync(&l, C.loss.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
     return VRGI_OK;
     API_END
 }

@@ -1,0 +1,442 @@
+// Learned radiance cache: input encoding, 4 x 64 ReLU MLP on the matrix cores, relative
+// L2 loss, backpropagation and Adam. Not in the reference (it has no learned part).
+//
+// Encoding of a sample (64 inputs): 12-level hash grid x 2 features of the position in
+// the cache box (dense index while (N+1)^3 <= T, else (x ^ 2654435761 y ^ 805459861 z)
+// mod T), real spherical harmonics l < 4 of the ray direction and of the normal,
+// albedo, a constant 1 (the first layer's bias), 4 zeros.
+//
+// MLP on v_mfma_f32_16x16x16_bf16 (fp32 accumulation). A wave works on blocks of 16
+// samples in the transposed form Y^T = W^T X^T: the A operand is 16 output neurons x 16
+// inputs of the weights ([n][k] bf16, 8-byte loads), the B operand 16 inputs x 16
+// samples. The accumulator of output tile t (lane l: neurons 4(l/16)+j, sample l%16) is,
+// after ReLU and rounding to bf16, exactly the B operand of k-step t of the next layer,
+// so activations never leave the registers. Lane (s = l%16, g = l/16) therefore owns the
+// input features 16t + 4g + j, i.e. feature groups q = 4t + g of four: groups 0-5 are
+// grid levels 2q and 2q+1, 6-9 direction SH, 10-13 normal SH, 14 albedo + 1, 15 zeros.
+//
+// Backward: the same trick with the weights in [k][n] layout (gradient w.r.t. the
+// inputs of a layer, ReLU mask from the stored bf16 activations). Layer inputs and
+// output gradients are written as bf16 feature-major planes [k][sample], from which a
+// split-K MFMA kernel forms dW = X G^T. Gradients carry the loss scale until Adam.
+#include "vrgi_kernels.h"
+
+namespace vrgi {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = (short)f2bf(a); r[1] = (short)f2bf(b); r[2] = (short)f2bf(c); r[3] = (short)f2bf(d);
+    return r;
+}
+__device__ __forceinline__ bf16x4 ld_bf4(const uint16_t *p) { return *(const bf16x4 *)p; }
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// -------------------------------------------------------------------- encoding
+// real spherical harmonics l < 4 of a unit vector, coefficients 4k .. 4k + 3
+__device__ __forceinline__ f32x4 sh_group(float x, float y, float z, uint32_t k) {
+    const float c0 = 0.28209479177387814f, c1 = 0.48860251190291992f;
+    const float c2a = 1.0925484305920792f, c2b = 0.31539156525252005f, c2c = 0.54627421529603959f;
+    const float c3a = 0.59004358992664352f, c3b = 2.8906114426405538f, c3c = 0.45704579946446572f;
+    const float c3d = 0.37317633259011540f, c3e = 1.4453057213202769f;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    f32x4 o;
+    if (k == 0) {
+        o[0] = c0; o[1] = -c1 * y; o[2] = c1 * z; o[3] = -c1 * x;
+    } else if (k == 1) {
+        o[0] = c2a * x * y; o[1] = -c2a * y * z; o[2] = c2b * (3.0f * zz - 1.0f); o[3] = -c2a * x * z;
+    } else if (k == 2) {
+        o[0] = c2c * (xx - yy); o[1] = -c3a * y * (3.0f * xx - yy); o[2] = c3b * x * y * z;
+        o[3] = -c3c * y * (5.0f * zz - 1.0f);
+    } else {
+        o[0] = c3d * z * (5.0f * zz - 3.0f); o[1] = -c3c * x * (5.0f * zz - 1.0f); o[2] = c3e * z * (xx - yy);
+        o[3] = -c3a * x * (xx - 3.0f * yy);
+    }
+    return o;
+}
+
+struct GridCorners {
+    uint32_t idx[8];
+    float w[8];
+};
+__device__ __forceinline__ GridCorners grid_corners(const NrcNet &net, const float p[3], uint32_t level) {
+    GridCorners r;
+    const uint32_t N = net.res[level];
+    const uint32_t T = 1u << net.log2_T;
+    const bool dense = (uint64_t)(N + 1) * (N + 1) * (N + 1) <= (uint64_t)T;
+    uint32_t cell[3];
+    float fr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float u = (p[a] - net.lo[a]) / (net.hi[a] - net.lo[a]);
+        u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+        float s = u * (float)N;
+        float f = floorf(s);
+        if (f > (float)(N - 1)) f = (float)(N - 1);
+        cell[a] = (uint32_t)f;
+        fr[a] = s - f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint32_t g[3];
+        float w = 1.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint32_t bit = (k >> a) & 1u;
+            g[a] = cell[a] + bit;
+            w *= bit ? fr[a] : 1.0f - fr[a];
+        }
+        uint32_t idx = dense ? g[0] + g[1] * (N + 1) + g[2] * (N + 1) * (N + 1)
+                             : ((g[0] * 1u) ^ (g[1] * 2654435761u) ^ (g[2] * 805459861u)) & (T - 1u);
+        r.idx[k] = level * T + idx;
+        r.w[k] = w;
+    }
+    return r;
+}
+__device__ __forceinline__ float2 grid_level(const NrcNet &net, const float p[3], uint32_t level) {
+    GridCorners c = grid_corners(net, p, level);
+    float2 s = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float2 t = *(const float2 *)(net.table + (size_t)c.idx[k] * 2);
+        s.x += c.w[k] * t.x;
+        s.y += c.w[k] * t.y;
+    }
+    return s;
+}
+
+struct Sample {
+    float p[3], d[3], n[3], a[3];
+};
+__device__ __forceinline__ Sample load_sample(const float *rec) {
+    const float4 *q = (const float4 *)rec;
+    float4 x = q[0], y = q[1], z = q[2];
+    Sample s;
+    s.p[0] = x.x; s.p[1] = x.y; s.p[2] = x.z;
+    s.d[0] = x.w; s.d[1] = y.x; s.d[2] = y.y;
+    s.n[0] = y.z; s.n[1] = y.w; s.n[2] = z.x;
+    s.a[0] = z.y; s.a[1] = z.z; s.a[2] = z.w;
+    return s;
+}
+
+// the 4 features of group q (see the file comment)
+__device__ __forceinline__ f32x4 feature_group(const NrcNet &net, const Sample &s, uint32_t q) {
+    f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (q < 6) {
+        float2 a = grid_level(net, s.p, 2 * q), b = grid_level(net, s.p, 2 * q + 1);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    } else if (q < 10) {
+        v = sh_group(s.d[0], s.d[1], s.d[2], q - 6);
+    } else if (q < 14) {
+        v = sh_group(s.n[0], s.n[1], s.n[2], q - 10);
+    } else if (q == 14) {
+        v[0] = s.a[0]; v[1] = s.a[1]; v[2] = s.a[2]; v[3] = 1.0f;
+    }
+    return v;
+}
+
+// encoding of one sample as the four B operands of layer 1 (lane group g)
+__device__ __forceinline__ void encode(const NrcNet &net, const Sample &s, uint32_t g, bool live, bf16x4 x[4]) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 v = live ? feature_group(net, s, 4 * t + g) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        x[t] = pack4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+// one hidden layer forward: h[nt] = bf16(relu(sum_t Wt[nt-tile][t-step] x[t]))
+__device__ __forceinline__ void layer_fwd(const uint16_t *wt, const bf16x4 x[4], bf16x4 h[4], uint32_t lane) {
+    const uint32_t row = lane & 15u, g = lane >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc = mfma(ld_bf4(wt + (16 * nt + row) * 64 + 16 * t + 4 * g), x[t], acc);
+        h[nt] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+    }
+}
+__device__ __forceinline__ f32x4 layer_out(const uint16_t *wt5, const bf16x4 x[4], uint32_t lane) {
+    const uint32_t row = lane & 15u, g = lane >> 4;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc = mfma(ld_bf4(wt5 + row * 64 + 16 * t + 4 * g), x[t], acc);
+    return acc;  // lanes 0..15: outputs 0..3 of sample lane
+}
+
+// -------------------------------------------------------------------- inference
+// records: 16 floats (position, direction, normal, albedo, target, key); out: 3 floats
+// per record; when out_px is set, also out_px[3 * key + c] (negative outputs clamp to 0)
+__global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float *rec, const uint32_t *count,
+                                                   float *out, float *out_px) {
+    const uint32_t lane = threadIdx.x & 63u, s = lane & 15u, g = lane >> 4;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t n = *count;
+    const uint32_t n_blocks = (n + 15u) / 16u;
+    for (uint32_t blk = wave; blk < n_blocks; blk += n_waves) {
+        const uint32_t i = blk * 16u + s;
+        const bool live = i < n;
+        Sample sm{};
+        if (live) sm = load_sample(rec + (size_t)i * 16);
+        bf16x4 x[4], h[4];
+        encode(net, sm, g, live, x);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            layer_fwd(net.wt + l * 4096, x, h, lane);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x[t] = h[t];
+        }
+        f32x4 y = layer_out(net.wt + 4 * 4096, x, lane);
+        if (g == 0 && live) {
+            if (out) {
+                out[3 * (size_t)i] = y[0];
+                out[3 * (size_t)i + 1] = y[1];
+                out[3 * (size_t)i + 2] = y[2];
+            }
+            if (out_px) {
+                uint32_t px = __float_as_uint(rec[(size_t)i * 16 + 15]) >> 4;
+                out_px[3 * (size_t)px] = fmaxf(y[0], 0.0f);
+                out_px[3 * (size_t)px + 1] = fmaxf(y[1], 0.0f);
+                out_px[3 * (size_t)px + 2] = fmaxf(y[2], 0.0f);
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------------------- training
+// step k of `steps` uses records k, k + steps, k + 2 steps, ... (at most max_batch)
+__device__ __forceinline__ uint32_t batch_size(uint32_t count, uint32_t cap, uint32_t k, uint32_t steps,
+                                               uint32_t max_batch) {
+    count = count < cap ? count : cap;
+    uint32_t b = count > k ? (count - k + steps - 1) / steps : 0u;
+    return b < max_batch ? b : max_batch;
+}
+
+__global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
+    const uint32_t lane = threadIdx.x & 63u, s = lane & 15u, g = lane >> 4;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t bsz = batch_size(*tr.count, tr.rec_cap, tr.step_k, tr.steps, tr.max_batch);
+    const uint32_t n_blocks = (bsz + 15u) / 16u;
+    const uint32_t B = tr.max_batch;  // row stride of the planes
+    const float inv = 1.0f / (3.0f * (float)(bsz ? bsz : 1u));
+    float loss = 0.0f;
+    for (uint32_t blk = wave; blk < n_blocks; blk += n_waves) {
+        const uint32_t j = blk * 16u + s;
+        const bool live = j < bsz;
+        const uint32_t src = tr.step_k + j * tr.steps;
+        Sample sm{};
+        float tgt[3] = {0.0f, 0.0f, 0.0f};
+        if (live) {
+            const float *r = tr.rec + (size_t)src * 16;
+            sm = load_sample(r);
+            tgt[0] = r[12]; tgt[1] = r[13]; tgt[2] = r[14];
+        }
+        bf16x4 a[5][4];  // inputs of layers 1..5
+        encode(net, sm, g, live, a[0]);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) layer_fwd(net.wt + l * 4096, a[l], a[l + 1], lane);
+        f32x4 y = layer_out(net.wt + 4 * 4096, a[4], lane);
+        // output gradient (relative L2, denominator not differentiated), scaled
+        float go[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (g == 0 && live) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float e = y[c] - tgt[c];
+                float den = tr.loss == 0 ? y[c] * y[c] + 0.01f : 1.0f;
+                loss += e * e / den * inv;
+                go[c] = tr.loss_scale * 2.0f * e / den * inv;
+            }
+        }
+        bf16x4 gcur[4];
+        gcur[0] = pack4(go[0], go[1], go[2], go[3]);
+        // save layer-5 input and output gradient planes
+        {
+            uint16_t *X = tr.xplane + (size_t)4 * 64 * B;
+            uint16_t *G = tr.gplane + (size_t)4 * 64 * B;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[(size_t)(16 * t + 4 * g + r) * B + j] = (uint16_t)a[4][t][r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) G[(size_t)(4 * g + r) * B + j] = (uint16_t)gcur[0][r];
+        }
+        // backward through layer 5: d(h4) = W5 g5, masked by h4 > 0
+        bf16x4 gn[4];
+        {
+            const uint16_t *wb5 = net.wb + 4 * 4096;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                acc = mfma(ld_bf4(wb5 + (16 * mt + s) * 16 + 4 * g), gcur[0], acc);
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = bf2f((uint16_t)a[4][mt][r]) > 0.0f ? acc[r] : 0.0f;
+                gn[mt] = pack4(v[0], v[1], v[2], v[3]);
+            }
+        }
+        // hidden layers 4..1: gn = gradient w.r.t. the output of layer l + 1
+        f32x4 gin[4];
+#pragma unroll
+        for (int l = 3; l >= 0; --l) {
+            uint16_t *X = tr.xplane + (size_t)l * 64 * B;
+            uint16_t *G = tr.gplane + (size_t)l * 64 * B;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    X[(size_t)(16 * t + 4 * g + r) * B + j] = (uint16_t)a[l][t][r];
+                    G[(size_t)(16 * t + 4 * g + r) * B + j] = (uint16_t)gn[t][r];
+                }
+            const uint16_t *wb = net.wb + l * 4096;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) acc = mfma(ld_bf4(wb + (16 * mt + s) * 64 + 16 * nt + 4 * g), gn[nt], acc);
+                gin[mt] = acc;
+            }
+            if (l > 0) {
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] = bf2f((uint16_t)a[l][mt][r]) > 0.0f ? gin[mt][r] : 0.0f;
+                    gn[mt] = pack4(v[0], v[1], v[2], v[3]);
+                }
+            }
+        }
+        // grid gradient: gin[mt][r] = d(loss S)/d(input 16 mt + 4 g + r); grid inputs < 24
+        if (live) {
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const uint32_t q = 4 * mt + g;  // feature group
+                if (q < 6) {
+#pragma unroll
+                    for (int h2 = 0; h2 < 2; ++h2) {
+                        GridCorners c = grid_corners(net, sm.p, 2 * q + h2);
+                        const float g0 = gin[mt][2 * h2], g1 = gin[mt][2 * h2 + 1];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            atomicAdd(tr.gtable + (size_t)c.idx[k] * 2, c.w[k] * g0);
+                            atomicAdd(tr.gtable + (size_t)c.idx[k] * 2 + 1, c.w[k] * g1);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // batch loss (unscaled)
+    for (int off = 32; off >= 1; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    if (lane == 0 && loss != 0.0f) atomicAdd(tr.loss_out, loss);
+}
+
+// dW of layer `layer` = X G^T over the batch (split K): one wave per (tile, K chunk)
+__global__ __launch_bounds__(64) void k_nrc_dw(NrcTrain tr, uint32_t chunk) {
+    const uint32_t lane = threadIdx.x, s = lane & 15u, g = lane >> 4;
+    const uint32_t layer = blockIdx.z;
+    const uint32_t ntiles_n = layer < 4 ? 4u : 1u;
+    const uint32_t tile = blockIdx.y;  // mt * 4 + nt
+    const uint32_t mt = tile >> 2, nt = tile & 3u;
+    if (nt >= ntiles_n) return;
+    const uint32_t bsz = batch_size(*tr.count, tr.rec_cap, tr.step_k, tr.steps, tr.max_batch);
+    const uint32_t padded = (bsz + 15u) & ~15u;
+    const uint32_t k0 = blockIdx.x * chunk;
+    if (k0 >= padded) return;
+    const uint32_t k1 = min(padded, k0 + chunk);
+    const uint32_t B = tr.max_batch;
+    const uint16_t *X = tr.xplane + (size_t)layer * 64 * B + (size_t)(16 * mt + s) * B;
+    const uint16_t *G = tr.gplane + (size_t)layer * 64 * B + (size_t)(16 * nt + s) * B;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t i = k0; i < k1; i += 16) acc = mfma(ld_bf4(X + i + 4 * g), ld_bf4(G + i + 4 * g), acc);
+    // D[k = 16 mt + 4 g + r][n = 16 nt + s]
+    const uint32_t ncols = layer < 4 ? 64u : 16u;
+    float *gw = tr.gw + (size_t)layer * 4096;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t k = 16 * mt + 4 * g + r, n = 16 * nt + s;
+        if (acc[r] != 0.0f) atomicAdd(gw + (size_t)k * ncols + n, acc[r]);
+    }
+}
+
+// Adam (fp32): gradients carry the loss scale; they are cleared after use
+__global__ __launch_bounds__(256) void k_nrc_adam(float *p, float *gr, float *m, float *v, size_t n, NrcAdam a) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gg = gr[i] * a.inv_scale;
+        gr[i] = 0.0f;
+        const float mm = a.beta1 * m[i] + (1.0f - a.beta1) * gg;
+        const float vv = a.beta2 * v[i] + (1.0f - a.beta2) * gg * gg;
+        m[i] = mm;
+        v[i] = vv;
+        const float mh = mm / a.c1, vh = vv / a.c2;
+        p[i] -= a.lr * mh / (sqrtf(vh) + a.eps);
+    }
+}
+
+// fp32 master weights -> bf16 in both matrix-core layouts
+__global__ __launch_bounds__(256) void k_nrc_pack(const float *w, uint16_t *wt, uint16_t *wb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NRC_N_WEIGHTS) return;
+    const uint32_t layer = i / 4096, r = i % 4096;
+    const uint32_t ncols = layer < 4 ? 64u : 16u;
+    const uint32_t k = r / ncols, n = r % ncols;  // w is [k][n]
+    if (layer == 4 && k >= 64) return;
+    const uint16_t h = f2bf(w[i]);
+    wb[i] = h;                                  // [k][n]
+    wt[layer * 4096 + n * 64 + k] = h;          // [n][k]
+}
+
+// clear the gradient accumulators of one training step
+__global__ __launch_bounds__(256) void k_nrc_zero(float *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+void launch_nrc_infer(hipStream_t s, const NrcNet &net, const float *rec, const uint32_t *count, uint32_t max_n,
+                      float *out, float *out_px) {
+    uint32_t blocks16 = (max_n + 15) / 16;
+    uint32_t grid = std::min<uint32_t>((blocks16 + 3) / 4, 4096u);
+    if (grid) k_nrc_infer<<<grid, 256, 0, s>>>(net, rec, count, out, out_px);
+}
+
+void launch_nrc_train_step(hipStream_t s, const NrcNet &net, const NrcTrain &tr, float *w32, float *mw, float *vw,
+                           float *table, float *mt, float *vt, size_t n_table, const NrcAdam &adam) {
+    uint32_t blocks16 = (tr.max_batch + 15) / 16;
+    uint32_t grid = std::min<uint32_t>((blocks16 + 3) / 4, 2048u);
+    k_nrc_fwd_bwd<<<grid, 256, 0, s>>>(net, tr);
+    const uint32_t chunk = 1024;
+    dim3 g((tr.max_batch + 15 + chunk - 1) / chunk, 16, 5);
+    k_nrc_dw<<<g, 64, 0, s>>>(tr, chunk);
+    k_nrc_adam<<<(NRC_N_WEIGHTS + 255) / 256, 256, 0, s>>>(w32, tr.gw, mw, vw, NRC_N_WEIGHTS, adam);
+    k_nrc_adam<<<4096, 256, 0, s>>>(table, tr.gtable, mt, vt, n_table, adam);
+    k_nrc_pack<<<(NRC_N_WEIGHTS + 255) / 256, 256, 0, s>>>(w32, const_cast<uint16_t *>(net.wt),
+                                                          const_cast<uint16_t *>(net.wb));
+}
+
+void launch_nrc_grads(hipStream_t s, const NrcNet &net, const NrcTrain &tr) {
+    uint32_t blocks16 = (tr.max_batch + 15) / 16;
+    uint32_t grid = std::min<uint32_t>((blocks16 + 3) / 4, 2048u);
+    k_nrc_fwd_bwd<<<grid, 256, 0, s>>>(net, tr);
+    const uint32_t chunk = 1024;
+    dim3 g((tr.max_batch + 15 + chunk - 1) / chunk, 16, 5);
+    k_nrc_dw<<<g, 64, 0, s>>>(tr, chunk);
+}
+
+void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb) {
+    k_nrc_pack<<<(NRC_N_WEIGHTS + 255) / 256, 256, 0, s>>>(w32, wt, wb);
+}
+
+void launch_nrc_zero(hipStream_t s, float *p, size_t n) {
+    if (n) k_nrc_zero<<<(uint32_t)std::min<size_t>((n + 255) / 256, 4096), 256, 0, s>>>(p, n);
+}
+
+}  // namespace vrgi

@@ -53,6 +53,14 @@ __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
     const float bsp = 1.0f / (2.0f * VRGI_PI_F);
     f3 L = mk(0, 0, 0), L1 = mk(0, 0, 0);
     float q0 = bsp;
+    // learned radiance cache: a queried path ends at bounce 1 with the cache's reflected
+    // radiance; a training path emits one record per vertex b >= 1 whose reflected
+    // radiance is known (not the last vertex of a path cut by the bounce cap)
+    bool train = false, queried = false;
+    if (fa.cache_on) {
+        train = mm_finalizef(mm_combine(fa.train_seed, id)) < fa.train_frac;
+        queried = !train && plen == 2 && fa.hit[(size_t)fa.P + id] != VRGI_INVALID;
+    }
     for (int b = (int)plen - 1; b >= 0; --b) {
         const size_t bi = (size_t)b * fa.P + id;
         float bp = fa.pdf[bi], w = fa.mis[bi];
@@ -61,7 +69,22 @@ __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
         float np = (fa.nee != nullptr && b + 1 < (int)plen) ? fa.nee[bi] : 0.0f;
         float q = np * w + (1.0f - w) * bsp;
         float rf = bp / q;
-        L = ld3(fa.emis + bi * 3) + ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;
+        f3 term = ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;
+        if (queried && b == 1) term = ld3(fa.crad + (size_t)id * 3);
+        L = ld3(fa.emis + bi * 3) + term;
+        if (train && b >= 1 && fa.hit[bi] != VRGI_INVALID &&
+            (b + 1 < (int)plen || zero3(ld3(fa.dir + bi * 3)))) {
+            const uint32_t slot = atomicAdd(fa.trec_count, 1u);
+            if (slot < fa.tcap) {
+                const f3 p = ld3(fa.origin + bi * 3), dd = ld3(fa.dir + (bi - fa.P) * 3);
+                const f3 nn = ld3(fa.normal + bi * 3), aa = ld3(fa.albedo + bi * 3);
+                float4 *r = (float4 *)(fa.trec + (size_t)slot * 16);
+                r[0] = make_float4(p.x, p.y, p.z, dd.x);
+                r[1] = make_float4(dd.y, dd.z, nn.x, nn.y);
+                r[2] = make_float4(nn.z, aa.x, aa.y, aa.z);
+                r[3] = make_float4(term.x, term.y, term.z, __uint_as_float(id * 16u + (uint32_t)b));
+            }
+        }
         if (b == 1) L1 = L;
         if (b == 0) q0 = q;
     }

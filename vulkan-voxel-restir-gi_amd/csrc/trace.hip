@@ -318,6 +318,13 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
     }
     const size_t bi = (size_t)b * ba.P + px;
     const uint32_t seed = mm_combine(ba.inv_seed, px);  // raytrace.rs:403
+    bool train = false;
+    uint32_t cap = ba.B;  // bounces this pixel's path may take
+    if (ba.cache_on) {
+        train = mm_finalizef(mm_combine(ba.train_seed, px)) < ba.train_frac;
+        cap = train ? ba.B : 2u;
+    }
+    f3 alb = mk(0, 0, 0);
     Hit h;
     h.prim = VRGI_INVALID;
     if (active) {
@@ -339,6 +346,7 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
         const vrgi_material mt = sv.mats[slot];
         out_o = (t0 * b3.x + t1 * b3.y) + t2 * b3.z;
         refl = mk(mt.reflectivity[0], mt.reflectivity[1], mt.reflectivity[2]);
+        alb = refl;
         float alpha = mt.reflectivity[3];
         emis = (1000.0f * mk(mt.emissivity[0], mt.emissivity[1], mt.emissivity[2])) * -dot3(d, ics.n);
         float metal = mt.metallicity[0];
@@ -400,8 +408,28 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
         ba.pdf[bi] = pdf;
         ba.hit[bi] = h.prim;
         ba.hit_t[bi] = h.prim == VRGI_INVALID ? 0.0f : h.t;
-        cont = !zero3(out_d) && b + 1 < ba.B;
+        cont = !zero3(out_d) && b + 1 < cap;
         if (!cont) ba.path_len[px] = b + 1;
+        if (ba.albedo) st3(ba.albedo + bi * 3, alb);
+    }
+    // cache query of a non-training path at its bounce-1 vertex
+    if (ba.cache_on && b == 1) {
+        const bool q = active && !train && h.prim != VRGI_INVALID;
+        const uint64_t qb = __ballot(q);
+        if (qb) {
+            const uint32_t lane = threadIdx.x;
+            uint32_t base = 0;
+            if (lane == (uint32_t)(__ffsll((unsigned long long)qb) - 1))
+                base = atomicAdd(ba.qrec_count, (uint32_t)__popcll(qb));
+            base = __shfl(base, __ffsll((unsigned long long)qb) - 1, 64);
+            if (q) {
+                float4 *r = (float4 *)(ba.qrec + (size_t)(base + (uint32_t)__popcll(qb & ((1ull << lane) - 1ull))) * 16);
+                r[0] = make_float4(out_o.x, out_o.y, out_o.z, d.x);
+                r[1] = make_float4(d.y, d.z, nrm.x, nrm.y);
+                r[2] = make_float4(nrm.z, alb.x, alb.y, alb.z);
+                r[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(px * 16u + 1u));
+            }
+        }
     }
     // compaction of the rays alive at the next bounce: ballot + one atomic per wave
     uint64_t bal = __ballot(cont);

@@ -53,6 +53,14 @@ struct BounceArgs {
     uint32_t *hit, *path_len;
     unsigned long long *counters;  // [0] closest-hit rays, [1] light rays, [2] frames
     uint32_t *touch_nodes, *touch_tris;  // non-null: counting launch (measurement only)
+    // learned radiance cache (cache_on): a pixel traces a full training path when
+    // u(finalize(combine(train_seed, px))) < train_frac, else it stops at bounce 1 and
+    // queues a cache query record (16 floats) of that vertex
+    uint32_t cache_on, train_seed;
+    float train_frac;
+    float *albedo;          // [B][P][3] material reflectivity at each hit
+    float *qrec;            // [P][16] query records
+    uint32_t *qrec_count;
 };
 
 struct FlattenArgs {
@@ -85,6 +93,14 @@ struct FrameArgs {
     float *restir;     // [P][3]
     float *debug;      // [P][3]
     uint8_t *bgra8;    // [P][4]
+    // learned radiance cache (see BounceArgs)
+    uint32_t cache_on, train_seed;
+    float train_frac;
+    const float *dir, *albedo, *crad;  // crad: [P][3] cache radiance of queried pixels
+    const uint32_t *hit;
+    float *trec;                       // training records [tcap][16]
+    uint32_t *trec_count;
+    uint32_t tcap;
 };
 
 // ReSTIR DI (benchmark mode, restir_mode = 1)
@@ -275,6 +291,44 @@ void launch_light_ancestry(hipStream_t s, const uint32_t *ltri_inst, const uint3
                            uint32_t *lanc, uint32_t *lanc_len);
 void launch_frame_stats(hipStream_t s, const uint32_t *qcount, uint32_t B, uint32_t P,
                         unsigned long long *counters);
+// learned radiance cache (nrc.hip): 12 grid levels x 2 features, 64 inputs, 4 x 64
+// hidden, 3 outputs; weights [k][n] per layer, layer 5 padded to 16 columns
+#define NRC_LEVELS 12
+#define NRC_N_WEIGHTS (4 * 64 * 64 + 64 * 16)
+struct NrcNet {
+    uint32_t log2_T;
+    uint32_t res[16];
+    float lo[3], hi[3];       // box mapped to the unit cube
+    const float *table;       // levels x 2^log2_T x 2, fp32
+    const uint16_t *wt;       // bf16 [n][k] per layer (forward A operand)
+    const uint16_t *wb;       // bf16 [k][n] per layer (backward A operand)
+};
+// training records: 16 floats = position, direction, normal, albedo, target, key
+struct NrcTrain {
+    const float *rec;
+    const uint32_t *count;    // records available (clamped to rec_cap)
+    uint32_t rec_cap;
+    uint32_t step_k, steps;   // this step uses records step_k + i * steps
+    uint32_t max_batch;       // multiple of 16; row stride of the planes
+    uint32_t loss;            // 0 relative L2, 1 plain L2
+    float loss_scale;
+    uint16_t *xplane, *gplane;  // [5][64][max_batch] bf16: layer inputs, output gradients
+    float *gw, *gtable;         // gradient accumulators (scaled)
+    float *loss_out;
+};
+struct NrcAdam {
+    float lr, beta1, beta2, eps;
+    float c1, c2;             // 1 - beta^t
+    float inv_scale;          // 1 / loss scale
+};
+void launch_nrc_infer(hipStream_t s, const NrcNet &net, const float *rec, const uint32_t *count, uint32_t max_n,
+                      float *out, float *out_px);
+void launch_nrc_train_step(hipStream_t s, const NrcNet &net, const NrcTrain &tr, float *w32, float *mw, float *vw,
+                           float *table, float *mt, float *vt, size_t n_table, const NrcAdam &adam);
+void launch_nrc_grads(hipStream_t s, const NrcNet &net, const NrcTrain &tr);
+void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb);
+void launch_nrc_zero(hipStream_t s, float *p, size_t n);
+
 // queue reordering by (direction octant, origin Morton code) between bounces
 struct SortArgs {
     uint32_t bounce, P, bits;         // rays of bounce + 1 read their origin / dir from bounce

@@ -22,7 +22,10 @@ assert VERTEX_DTYPE.itemsize == 24 and NODE_DTYPE.itemsize == 40 and MATERIAL_DT
  BUF_BOUNCE_ORIGIN, BUF_BOUNCE_DIRECTION, BUF_BOUNCE_NORMAL, BUF_BOUNCE_EMISSIVITY,
  BUF_BOUNCE_REFLECTIVITY, BUF_BOUNCE_MIS_WEIGHT, BUF_BOUNCE_BSDF_PDF, BUF_BOUNCE_NEE_PDF,
  BUF_RADIANCE, BUF_RESTIR, BUF_TEMPORAL_RESERVOIR, BUF_SPATIAL_RESERVOIR, BUF_BGRA8,
- BUF_HIT_PRIM, BUF_HIT_T, BUF_DI_LIGHT, BUF_DI_WEIGHTS) = range(22)
+ BUF_HIT_PRIM, BUF_HIT_T, BUF_DI_LIGHT, BUF_DI_WEIGHTS, BUF_BOUNCE_ALBEDO, BUF_CACHE_RADIANCE,
+ BUF_CACHE_QUERIES, BUF_CACHE_TRAIN_RECORDS) = range(26)
+CACHE_FREEZE = 1
+CACHE_WEIGHTS = 4 * 64 * 64 + 64 * 16
 
 FLAG_REBUILD_EVERY_FRAME = 1
 FLAG_KEEP_BOUNCES = 2
@@ -34,6 +37,20 @@ class Config(C.Structure):
                 ("restir_mode", C.c_uint32), ("restir_spatial_radius", C.c_float),
                 ("di_candidates", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32),
                 ("ray_sort_bits", C.c_uint32)]
+
+
+class CacheConfig(C.Structure):
+    _fields_ = [("log2_table", C.c_uint32), ("res_min", C.c_uint32), ("res_max", C.c_uint32),
+                ("box_lo", C.c_float * 3), ("box_hi", C.c_float * 3), ("train_fraction", C.c_float),
+                ("train_bounces", C.c_uint32), ("batch", C.c_uint32), ("steps_per_frame", C.c_uint32),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("loss_scale", C.c_float), ("seed", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class CacheInfo(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("feats", C.c_uint32), ("log2_table", C.c_uint32),
+                ("res", C.c_uint32 * 16), ("box_lo", C.c_float * 3), ("box_hi", C.c_float * 3),
+                ("table_floats", C.c_uint64), ("steps", C.c_uint64), ("last_loss", C.c_float)]
 
 
 class Camera(C.Structure):
@@ -253,6 +270,65 @@ class Context:
         st = FrameStats()
         check(self.L.vrgi_stats(self.h, C.byref(st), int(reset)))
         return st
+
+    # ---- learned radiance cache
+    def cache_enable(self, **kw):
+        cfg = CacheConfig()
+        self.L.vrgi_cache_config_default(C.byref(cfg))
+        for k, v in kw.items():
+            if k in ("box_lo", "box_hi"):
+                setattr(cfg, k, (C.c_float * 3)(*v))
+            else:
+                setattr(cfg, k, v)
+        check(self.L.vrgi_cache_enable(self.h, C.byref(cfg)))
+        return cfg
+
+    def cache_disable(self):
+        check(self.L.vrgi_cache_disable(self.h))
+
+    def cache_info(self):
+        info = CacheInfo()
+        check(self.L.vrgi_cache_info_get(self.h, C.byref(info)))
+        return info
+
+    def cache_get(self):
+        info = self.cache_info()
+        w = np.zeros(CACHE_WEIGHTS, np.float32)
+        t = np.zeros(info.table_floats, np.float32)
+        check(self.L.vrgi_cache_get(self.h, ptr(w), ptr(t)))
+        return w, t
+
+    def cache_set(self, weights=None, table=None):
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        t = None if table is None else np.ascontiguousarray(table, np.float32)
+        check(self.L.vrgi_cache_set(self.h, None if w is None else ptr(w), None if t is None else ptr(t)))
+
+    @staticmethod
+    def _samples(pos, dir, nrm, alb):
+        return np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1, 3)
+                                                    for a in (pos, dir, nrm, alb)], axis=1))
+
+    def cache_infer(self, pos, dir, nrm, alb):
+        sm = self._samples(pos, dir, nrm, alb)
+        out = np.zeros((len(sm), 3), np.float32)
+        check(self.L.vrgi_cache_infer(self.h, C.c_uint64(len(sm)), ptr(sm), ptr(out)))
+        return out
+
+    def cache_grads(self, pos, dir, nrm, alb, target):
+        sm = self._samples(pos, dir, nrm, alb)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        gw = np.zeros(CACHE_WEIGHTS, np.float32)
+        gt = np.zeros(self.cache_info().table_floats, np.float32)
+        loss = C.c_float()
+        check(self.L.vrgi_cache_grads(self.h, C.c_uint64(len(sm)), ptr(sm), ptr(t), ptr(gw), ptr(gt), C.byref(loss)))
+        return loss.value, gw, gt
+
+    def cache_train(self, pos, dir, nrm, alb, target):
+        sm = self._samples(pos, dir, nrm, alb)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        check(self.L.vrgi_cache_train(self.h, C.c_uint64(len(sm)), ptr(sm), ptr(t), C.byref(loss)))
+        return loss.value
 
     def read(self, which, dtype=np.float32):
         n = C.c_size_t()
