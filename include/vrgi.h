@@ -222,7 +222,7 @@ int vrgi_read_buffer(vrgi_ctx *ctx, int which, void *dst, size_t bytes);
  * render() traces full training paths (train_bounces) from a random train_fraction of
  * the pixels, ends every other path at its bounce-1 vertex with the cache's reflected
  * radiance there, and then takes steps_per_frame Adam steps on the training paths'
- * vertices. A record (query or training) is 16 floats: position, incoming direction,
+ * vertices. (query_vertex selects that vertex: the camera ray's hit or the next one.) A record (query or training) is 16 floats: position, incoming direction,
  * normal, albedo, target radiance (training), key = pixel * 16 + bounce (uint bits). */
 typedef struct vrgi_cache_config {
     uint32_t log2_table;        /* entries per grid level = 2^log2_table (18) */
@@ -237,6 +237,8 @@ typedef struct vrgi_cache_config {
     float loss_scale;           /* output-gradient scale, divided out before Adam (128) */
     uint32_t seed;              /* initial weights and table */
     uint32_t flags;             /* VRGI_CACHE_* */
+    uint32_t query_vertex;      /* where non-training paths end in the cache: 0 the camera ray's
+                                   hit, 1 the next vertex (0) */
 } vrgi_cache_config;
 
 #define VRGI_CACHE_FREEZE 1u  /* query only: render() takes no training steps */
@@ -266,6 +268,14 @@ int vrgi_cache_grads(vrgi_ctx *ctx, uint64_t n, const float *samples, const floa
                      float *grad_table, float *loss);
 /* one Adam step on n samples; loss before the step */
 int vrgi_cache_train(vrgi_ctx *ctx, uint64_t n, const float *samples, const float *targets, float *loss);
+
+/* ---- convergence measurement (no counterpart in the reference) ----
+ * While enabled, every render() adds its path-traced radiance (VRGI_BUF_RADIANCE) to an
+ * FP64 running sum; vrgi_accum_read returns the mean over the frames since enabling. */
+int vrgi_accum_enable(vrgi_ctx *ctx, int on);
+/* number of the next frame (seeds: raytrace frame * B + bounce, ReSTIR frame salts) */
+int vrgi_frame_set(vrgi_ctx *ctx, uint64_t frame);
+int vrgi_accum_read(vrgi_ctx *ctx, double *mean /* P x 3 */, uint64_t *frames);
 
 /* ---- standalone operators (the drop-in pieces, usable without a scene) ---- */
 

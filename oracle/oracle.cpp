@@ -1163,7 +1163,7 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
     uint32_t cap = c.B;  // bounces this pixel's path may take
     if (cf) {
         train = finalizef(combine(cf->train_seed, (uint32_t)id)) < cf->train_fraction;
-        cap = train ? c.B : 2u;
+        cap = train ? c.B : cf->query_vertex + 1u;
     }
     uint32_t x = (uint32_t)(id % c.W), y = (uint32_t)(id / c.W);
     // raygen.rs:80-111 (jitter is 0.00 * (...), i.e. zero)
@@ -1210,7 +1210,8 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
     // outgoing_radiance.rs:58-94
     v3 L = V(0, 0, 0);
     const float bsp = 1.0f / (2.0f * PI_F);
-    const bool queried = cf && !train && plen == 2 && f.hit[1 * P + slot] != 0xFFFFFFFFu;
+    const uint32_t qv = cf ? cf->query_vertex : 0u;
+    const bool queried = cf && !train && plen == qv + 1 && f.hit[qv * P + slot] != 0xFFFFFFFFu;
     for (int b = (int)c.B - 1; b >= 0; --b) {
         size_t bid = (size_t)b * P + slot;
         float valid = is_zero(ld3(&f.dir[bid * 3])) ? 0.0f : 1.0f;
@@ -1218,14 +1219,15 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
         float q = np * w + (1.0f - w) * bsp;
         float rf = bp / q;
         v3 term = ((ld3(&f.refl[bid * 3]) * L) * rf) * valid;
-        if (queried && b == 1) term = ld3(cf->cache_rad + id * 3);
+        if (queried && b == (int)qv) term = ld3(cf->cache_rad + id * 3);
         L = ld3(&f.emis[bid * 3]) + term;
         st3(&f.lout[bid * 3], L);
         f.q[bid] = q;
         // training record of vertex b (its reflected radiance is known unless the path was
         // cut by the bounce cap right after it)
-        if (cf && train && b >= 1 && b < (int)plen && f.hit[bid] != 0xFFFFFFFFu &&
-            (b + 1 < (int)plen || is_zero(ld3(&f.dir[(bid + P) * 3])))) {
+        if (cf && train && b < (int)plen && f.hit[bid] != 0xFFFFFFFFu &&
+            (b + 1 < (int)plen || is_zero(ld3(&f.dir[(bid + P) * 3]))) &&
+            std::isfinite(term.x) && std::isfinite(term.y) && std::isfinite(term.z)) {
             v3 p = ld3(&f.origin[(bid + P) * 3]), dd = ld3(&f.dir[bid * 3]);
             v3 nn = ld3(&f.normal[bid * 3]), aa = ld3(&f.alb[bid * 3]);
             uint32_t key = (uint32_t)id * 16u + (uint32_t)b;
@@ -1235,11 +1237,11 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
                                                        aa.z, term.x, term.y, term.z, kf});
         }
     }
-    if (cf && !train && plen == 2 && f.hit[1 * P + slot] != 0xFFFFFFFFu) {
-        size_t b1 = 1 * P + slot;
+    if (queried) {
+        size_t b1 = qv * P + slot;
         v3 p = ld3(&f.origin[(b1 + P) * 3]), dd = ld3(&f.dir[b1 * 3]);
         v3 nn = ld3(&f.normal[b1 * 3]), aa = ld3(&f.alb[b1 * 3]);
-        uint32_t key = (uint32_t)id * 16u + 1u;
+        uint32_t key = (uint32_t)id * 16u + qv;
         float kf;
         std::memcpy(&kf, &key, 4);
         f.qrec[slot] = {p.x, p.y, p.z, dd.x, dd.y, dd.z, nn.x, nn.y, nn.z, aa.x, aa.y, aa.z, 0.0f, 0.0f, 0.0f, kf};

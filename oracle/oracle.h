@@ -104,14 +104,15 @@ typedef struct or_frame {
 } or_frame;
 
 // Learned-radiance-cache frame: a pixel traces a training path of all B bounces when
-// u(finalize(combine(train_seed, pixel))) < train_fraction, otherwise it stops at bounce 1
-// and, when that bounce hit, ends with L_1 = E_1 + cache_rad[pixel] (the reflected
-// radiance the cache returned there). Records are 16 floats: position, incoming direction,
+// u(finalize(combine(train_seed, pixel))) < train_fraction, otherwise it stops at bounce
+// v = query_vertex and, when that bounce hit, ends with L_v = E_v + cache_rad[pixel] (the
+// reflected radiance the cache returned there). Records are 16 floats: position, incoming direction,
 // normal, albedo, target (training records: the reflected radiance of the vertex), key =
 // pixel * 16 + bounce; both lists come out sorted by key.
 typedef struct or_cache_frame {
     float train_fraction;
     uint32_t train_seed;
+    uint32_t query_vertex;   // 0 or 1
     const float *cache_rad;  // [P][3] in
     float *albedo;           // [B][P][3] out: material reflectivity at each hit
     float *qrec;             // [P][16] out

@@ -148,9 +148,14 @@ struct vrgi_ctx {
         uint32_t plane_cap = 0, tcap = 0, hcap = 0;
         DevBuf<float> w32, mw, vw, table, mt, vt, gw, gtable, loss;
         DevBuf<uint16_t> wt, wb, xplane, gplane;
-        DevBuf<float> trec, qrec, crad, hrec, hout, albedo;
+        DevBuf<float> trec, qrec, crad, hrec, hout, albedo, d0;
         DevBuf<uint32_t> counts;  // [0] training records, [1] query records, [2] caller records
     } cache;
+
+    // FP64 running sum of the radiance output (vrgi_accum_*)
+    bool accum_on = false;
+    uint64_t accum_frames = 0;
+    DevBuf<double> accum;
 
     // host-side scratch of the standalone operators
     DevBuf<float> a_f0, a_f1, a_f2;
@@ -270,6 +275,7 @@ void alloc_frame(vrgi_ctx *c) {
     c->f_queue.reserve((B + 1) * P);
     if (c->cache.on) {
         c->cache.albedo.reserve(B * P * 3);
+        c->cache.d0.reserve(P * 3);
         c->cache.qrec.reserve(P * 16);
         c->cache.crad.reserve(P * 3);
     }
@@ -833,6 +839,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         ba.cache_on = 1;
         ba.train_seed = train_seed;
         ba.train_frac = c->cache.cfg.train_fraction;
+        ba.cache_vertex = c->cache.cfg.query_vertex;
+        ba.d0 = c->cache.d0.p;
         ba.albedo = c->cache.albedo.p;
         ba.qrec = c->cache.qrec.p;
         ba.qrec_count = c->cache.counts.p + 1;
@@ -909,7 +917,9 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         fa.cache_on = 1;
         fa.train_seed = train_seed;
         fa.train_frac = c->cache.cfg.train_fraction;
+        fa.cache_vertex = c->cache.cfg.query_vertex;
         fa.dir = c->f_dir.p;
+        fa.d0 = c->cache.d0.p;
         fa.albedo = c->cache.albedo.p;
         fa.crad = c->cache.crad.p;
         fa.hit = c->f_hit.p;
@@ -937,6 +947,11 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         launch_spatial_finalize(s, fa);
     }
     launch_postprocess(s, fa);
+    if (c->accum_on) {
+        c->accum.reserve((size_t)P * 3);
+        launch_accumulate(s, c->f_radiance.p, c->accum.p, (size_t)P * 3);
+        c->accum_frames += 1;
+    }
     if (cache_on && !(c->cache.cfg.flags & VRGI_CACHE_FREEZE) && c->cache.cfg.steps_per_frame) {
         HIP_CHECK(hipMemsetAsync(c->cache.loss.p, 0, 4, s));
         for (uint32_t k = 0; k < c->cache.cfg.steps_per_frame; ++k)
@@ -1329,6 +1344,8 @@ int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
     if (cfg->batch < 16 || cfg->batch % 16 || cfg->batch > (1u << 20))
         return fail(VRGI_E_INVALID, "batch must be a multiple of 16 in [16, 2^20]");
     if (cfg->steps_per_frame > 64) return fail(VRGI_E_INVALID, "steps_per_frame above 64");
+    if (cfg->query_vertex > 1) return fail(VRGI_E_INVALID, "query_vertex must be 0 or 1");
+    if (cfg->query_vertex + 1 >= cfg->train_bounces) return fail(VRGI_E_INVALID, "train_bounces too small");
     if (!(cfg->train_fraction >= 0.0f && cfg->train_fraction <= 1.0f))
         return fail(VRGI_E_INVALID, "train_fraction must be in [0, 1]");
     if (!(cfg->lr > 0.0f) || !(cfg->loss_scale > 0.0f)) return fail(VRGI_E_INVALID, "lr and loss_scale must be positive");
@@ -1484,6 +1501,42 @@ int vrgi_cache_grads(vrgi_ctx *c, uint64_t n, const float *samples, const float 
     HIP_CHECK(hipMemsetAsync(C.gw.p, 0, NRC_N_WEIGHTS * 4, s));
     HIP_CHECK(hipMemsetAsync(C.gtable.p, 0, C.n_table * 4, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_frame_set(vrgi_ctx *c, uint64_t frame) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    c->frame_count = frame;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_accum_enable(vrgi_ctx *c, int on) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    set_dev(c);
+    c->accum_on = on != 0;
+    c->accum_frames = 0;
+    if (c->accum_on) {
+        c->accum.reserve((size_t)c->P * 3);
+        HIP_CHECK(hipMemsetAsync(c->accum.p, 0, (size_t)c->P * 24, c->stream));
+    }
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_accum_read(vrgi_ctx *c, double *mean, uint64_t *frames) {
+    API_BEGIN
+    if (!c || !mean) return fail(VRGI_E_INVALID, "null argument");
+    if (!c->accum.p) return fail(VRGI_E_STATE, "accumulation was never enabled");
+    set_dev(c);
+    HIP_CHECK(hipMemcpyAsync(mean, c->accum.p, (size_t)c->P * 24, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const double inv = c->accum_frames ? 1.0 / (double)c->accum_frames : 0.0;
+    for (size_t i = 0; i < (size_t)c->P * 3; ++i) mean[i] *= inv;
+    if (frames) *frames = c->accum_frames;
     return VRGI_OK;
     API_END
 }

@@ -53,13 +53,14 @@ __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
     const float bsp = 1.0f / (2.0f * VRGI_PI_F);
     f3 L = mk(0, 0, 0), L1 = mk(0, 0, 0);
     float q0 = bsp;
-    // learned radiance cache: a queried path ends at bounce 1 with the cache's reflected
-    // radiance; a training path emits one record per vertex b >= 1 whose reflected
+    // learned radiance cache: a queried path ends at vertex cache_vertex with the cache's
+    // reflected radiance; a training path emits one record per vertex whose reflected
     // radiance is known (not the last vertex of a path cut by the bounce cap)
     bool train = false, queried = false;
+    const uint32_t cv = fa.cache_vertex;
     if (fa.cache_on) {
         train = mm_finalizef(mm_combine(fa.train_seed, id)) < fa.train_frac;
-        queried = !train && plen == 2 && fa.hit[(size_t)fa.P + id] != VRGI_INVALID;
+        queried = !train && plen == cv + 1 && fa.hit[(size_t)cv * fa.P + id] != VRGI_INVALID;
     }
     for (int b = (int)plen - 1; b >= 0; --b) {
         const size_t bi = (size_t)b * fa.P + id;
@@ -70,13 +71,15 @@ __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
         float q = np * w + (1.0f - w) * bsp;
         float rf = bp / q;
         f3 term = ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;
-        if (queried && b == 1) term = ld3(fa.crad + (size_t)id * 3);
+        if (queried && b == (int)cv) term = ld3(fa.crad + (size_t)id * 3);
         L = ld3(fa.emis + bi * 3) + term;
-        if (train && b >= 1 && fa.hit[bi] != VRGI_INVALID &&
-            (b + 1 < (int)plen || zero3(ld3(fa.dir + bi * 3)))) {
+        if (train && fa.hit[bi] != VRGI_INVALID &&
+            (b + 1 < (int)plen || zero3(ld3(fa.dir + bi * 3))) &&
+            isfinite(term.x) && isfinite(term.y) && isfinite(term.z)) {
             const uint32_t slot = atomicAdd(fa.trec_count, 1u);
             if (slot < fa.tcap) {
-                const f3 p = ld3(fa.origin + bi * 3), dd = ld3(fa.dir + (bi - fa.P) * 3);
+                const f3 p = ld3(fa.origin + bi * 3);
+                const f3 dd = b == 0 ? ld3(fa.d0 + (size_t)id * 3) : ld3(fa.dir + (bi - fa.P) * 3);
                 const f3 nn = ld3(fa.normal + bi * 3), aa = ld3(fa.albedo + bi * 3);
                 float4 *r = (float4 *)(fa.trec + (size_t)slot * 16);
                 r[0] = make_float4(p.x, p.y, p.z, dd.x);
@@ -186,6 +189,20 @@ void launch_spatial_finalize(hipStream_t s, const FrameArgs &fa) {
 }
 void launch_postprocess(hipStream_t s, const FrameArgs &fa) {
     k_postprocess<<<(fa.P + 255) / 256, 256, 0, s>>>(fa);
+}
+
+// running FP64 sum of the path-traced radiance (convergence measurements). A non-finite
+// sample (the reference's 0/0 of a zero total light importance or a zero cosine in the
+// NEE pdf) adds 0, as the reference's display shows it black.
+__global__ __launch_bounds__(256) void k_accumulate(const float *rad, double *acc, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float v = rad[i];
+        acc[i] += isfinite(v) ? (double)v : 0.0;
+    }
+}
+void launch_accumulate(hipStream_t s, const float *rad, double *acc, size_t n) {
+    if (n) k_accumulate<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(rad, acc, n);
 }
 
 }  // namespace vrgi

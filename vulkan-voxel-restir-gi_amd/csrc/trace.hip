@@ -322,7 +322,8 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
     uint32_t cap = ba.B;  // bounces this pixel's path may take
     if (ba.cache_on) {
         train = mm_finalizef(mm_combine(ba.train_seed, px)) < ba.train_frac;
-        cap = train ? ba.B : 2u;
+        cap = train ? ba.B : ba.cache_vertex + 1u;
+        if (active && b == 0) st3(ba.d0 + (size_t)px * 3, d);
     }
     f3 alb = mk(0, 0, 0);
     Hit h;
@@ -412,8 +413,8 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
         if (!cont) ba.path_len[px] = b + 1;
         if (ba.albedo) st3(ba.albedo + bi * 3, alb);
     }
-    // cache query of a non-training path at its bounce-1 vertex
-    if (ba.cache_on && b == 1) {
+    // cache query of a non-training path at its last vertex
+    if (ba.cache_on && b == ba.cache_vertex) {
         const bool q = active && !train && h.prim != VRGI_INVALID;
         const uint64_t qb = __ballot(q);
         if (qb) {
@@ -427,7 +428,7 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
                 r[0] = make_float4(out_o.x, out_o.y, out_o.z, d.x);
                 r[1] = make_float4(d.y, d.z, nrm.x, nrm.y);
                 r[2] = make_float4(nrm.z, alb.x, alb.y, alb.z);
-                r[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(px * 16u + 1u));
+                r[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(px * 16u + b));
             }
         }
     }

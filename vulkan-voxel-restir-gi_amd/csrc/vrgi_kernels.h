@@ -56,8 +56,9 @@ struct BounceArgs {
     // learned radiance cache (cache_on): a pixel traces a full training path when
     // u(finalize(combine(train_seed, px))) < train_frac, else it stops at bounce 1 and
     // queues a cache query record (16 floats) of that vertex
-    uint32_t cache_on, train_seed;
+    uint32_t cache_on, train_seed, cache_vertex;  // cache_vertex: bounce of the query
     float train_frac;
+    float *d0;              // [P][3] camera ray directions (incoming direction at vertex 0)
     float *albedo;          // [B][P][3] material reflectivity at each hit
     float *qrec;            // [P][16] query records
     uint32_t *qrec_count;
@@ -94,9 +95,9 @@ struct FrameArgs {
     float *debug;      // [P][3]
     uint8_t *bgra8;    // [P][4]
     // learned radiance cache (see BounceArgs)
-    uint32_t cache_on, train_seed;
+    uint32_t cache_on, train_seed, cache_vertex;
     float train_frac;
-    const float *dir, *albedo, *crad;  // crad: [P][3] cache radiance of queried pixels
+    const float *dir, *d0, *albedo, *crad;  // crad: [P][3] cache radiance of queried pixels
     const uint32_t *hit;
     float *trec;                       // training records [tcap][16]
     uint32_t *trec_count;
@@ -281,6 +282,7 @@ void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, ui
 void launch_accum_temporal(hipStream_t s, const FrameArgs &fa);
 void launch_spatial_finalize(hipStream_t s, const FrameArgs &fa);
 void launch_postprocess(hipStream_t s, const FrameArgs &fa);
+void launch_accumulate(hipStream_t s, const float *rad, double *acc, size_t n);
 void launch_prims_from_tris(hipStream_t s, const float *tri9, uint32_t n, float *prim9);
 void launch_tl_inputs(hipStream_t s, const float *aabb6, const float *xf12, uint32_t n, float *prim9);
 void launch_tl_idx(hipStream_t s, const vrgi_bvh_node *tl, uint32_t n_nodes, DevInst *inst);

@@ -39,12 +39,23 @@ class Config(C.Structure):
                 ("ray_sort_bits", C.c_uint32)]
 
 
+class ProfileOut(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("build_ms", C.c_double), ("trace_ms", C.c_double * 16),
+                ("nee_ms", C.c_double * 16), ("resolve_ms", C.c_double), ("frame_ms", C.c_double),
+                ("rays", C.c_uint64 * 16), ("touched_bytes", C.c_uint64 * 16), ("build_prims", C.c_uint64),
+                ("build_leaf_depth_sum", C.c_uint64), ("build_nodes", C.c_uint64)]
+
+
+PROFILE_STAGES, PROFILE_TOUCH = 1, 2
+
+
 class CacheConfig(C.Structure):
     _fields_ = [("log2_table", C.c_uint32), ("res_min", C.c_uint32), ("res_max", C.c_uint32),
                 ("box_lo", C.c_float * 3), ("box_hi", C.c_float * 3), ("train_fraction", C.c_float),
                 ("train_bounces", C.c_uint32), ("batch", C.c_uint32), ("steps_per_frame", C.c_uint32),
                 ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
-                ("loss_scale", C.c_float), ("seed", C.c_uint32), ("flags", C.c_uint32)]
+                ("loss_scale", C.c_float), ("seed", C.c_uint32), ("flags", C.c_uint32),
+                ("query_vertex", C.c_uint32)]
 
 
 class CacheInfo(C.Structure):
@@ -270,6 +281,29 @@ class Context:
         st = FrameStats()
         check(self.L.vrgi_stats(self.h, C.byref(st), int(reset)))
         return st
+
+    # ---- measurement
+    def profile(self, flags):
+        check(self.L.vrgi_profile(self.h, flags))
+
+    def profile_read(self, reset=True):
+        p = ProfileOut()
+        check(self.L.vrgi_profile_read(self.h, C.byref(p), int(reset)))
+        return p
+
+    def frame_set(self, frame):
+        check(self.L.vrgi_frame_set(self.h, C.c_uint64(frame)))
+
+    # ---- FP64 accumulation of the radiance (convergence studies)
+    def accum_enable(self, on=True):
+        check(self.L.vrgi_accum_enable(self.h, int(on)))
+
+    def accum_read(self):
+        P = self.cfg.width * self.cfg.height
+        out = np.zeros((P, 3), np.float64)
+        n = C.c_uint64()
+        check(self.L.vrgi_accum_read(self.h, ptr(out), C.byref(n)))
+        return out, n.value
 
     # ---- learned radiance cache
     def cache_enable(self, **kw):
