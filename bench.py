@@ -6,8 +6,12 @@ world (~1.37M exposed faces, ~2.73M triangles, ~13k emissive triangles) at 1920x
 ReSTIR GI temporal + spatial (5 neighbours), a full geometry + light-triangle BVH
 rebuild every frame, camera on a 16-frame seeded Catmull-Rom spline.
 
+Second part of the metric (field "learned_cache"): time to equal RMSE of the learned
+radiance cache against the plain path tracer on the same scene at 1920x1080, static
+camera, both against an FP64 reference of the plain 8-bounce tracer at 256 spp.
+
 Prints ONE JSON line (rank 0). Progress goes to stderr.
-  python bench.py [--gpus 1] [--steps 16] [--warmup 3] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps 16] [--warmup 3] [--no-cpu-baseline] [--no-cache-study]
 """
 import argparse
 import ctypes as C
@@ -115,6 +119,8 @@ def main():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cache-study", action="store_true",
+                    help="skip the learned-cache time-to-equal-RMSE study")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (pmc counter scaling)")
     ap.add_argument("--sort-bits", type=int, default=0,
@@ -193,6 +199,13 @@ def main():
         bb = build_alg_bytes(prof)
         others["bvh_build (all builder kernels)"] = {"alg_bytes_per_frame": int(bb), "ms": round(stages["build"], 4),
                                                      "achieved_GBps": round(bb / (stages["build"] * 1e-3) / 1e9, 2)}
+    cache = None
+    if not a.no_cache_study:
+        import convergence
+        log("learned cache vs plain: time to equal RMSE (1920x1080, FP64 reference 256 spp)")
+        cache = convergence.study(objs, mats, lum, cams[0], W_PX, H_PX, bounces=8, rr_start=RR_START, ref_spp=256,
+                                  frames=32, warmup=32, nee=NEE, spatial=SPATIAL, cache_kw=dict(query_vertex=0),
+                                  log=log)
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline (oracle on the host cores, bounded sample)")
@@ -208,6 +221,7 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roofline,
         "kernels": others,
+        "learned_cache": cache,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -80,6 +80,10 @@ struct vrgi_object {
 struct vrgi_ctx {
     int device = 0;
     hipStream_t stream = nullptr, up_stream = nullptr;
+    // learned-cache work of a frame that overlaps the training paths' bounces (default
+    // priority, like every stream here); joined into `stream` before the frame ends
+    hipStream_t side = nullptr;
+    hipEvent_t ev_query = nullptr, ev_infer = nullptr, ev_train = nullptr;
     std::mutex up_mu;
     vrgi_config cfg{};
     uint32_t W = 0, H = 0, P = 0, B = 0;
@@ -225,8 +229,8 @@ void cache_box_from_scene(vrgi_ctx *c) {
 
 // one training step over records rec[k + i * steps] (at most `batch`): gradients only, or
 // gradients + Adam + repacking of the bf16 weights
-void cache_step(vrgi_ctx *c, const float *rec, const uint32_t *count, uint32_t cap, uint32_t k, uint32_t steps,
-                uint32_t batch, bool apply) {
+void cache_step(vrgi_ctx *c, hipStream_t s, const float *rec, const uint32_t *count, uint32_t cap, uint32_t k,
+                uint32_t steps, uint32_t batch, bool apply) {
     auto &C = c->cache;
     NrcTrain tr{};
     tr.rec = rec;
@@ -243,7 +247,7 @@ void cache_step(vrgi_ctx *c, const float *rec, const uint32_t *count, uint32_t c
     tr.gtable = C.gtable.p;
     tr.loss_out = C.loss.p;
     if (!apply) {
-        launch_nrc_grads(c->stream, cache_net(c), tr);
+        launch_nrc_grads(s, cache_net(c), tr);
         return;
     }
     C.steps += 1;
@@ -255,7 +259,7 @@ void cache_step(vrgi_ctx *c, const float *rec, const uint32_t *count, uint32_t c
     ad.c1 = (float)(1.0 - std::pow((double)C.cfg.beta1, (double)C.steps));
     ad.c2 = (float)(1.0 - std::pow((double)C.cfg.beta2, (double)C.steps));
     ad.inv_scale = 1.0f / C.cfg.loss_scale;
-    launch_nrc_train_step(c->stream, cache_net(c), tr, C.w32.p, C.mw.p, C.vw.p, C.table.p, C.mt.p, C.vt.p,
+    launch_nrc_train_step(s, cache_net(c), tr, C.w32.p, C.mw.p, C.vw.p, C.table.p, C.mt.p, C.vt.p,
                           C.n_table, ad);
 }
 
@@ -495,6 +499,10 @@ vrgi_ctx::~vrgi_ctx() {
     for (vrgi_object *o : old) release(o);
     if (stream) (void)hipStreamDestroy(stream);
     if (up_stream) (void)hipStreamDestroy(up_stream);
+    if (side) (void)hipStreamSynchronize(side);
+    if (side) (void)hipStreamDestroy(side);
+    for (hipEvent_t e : {ev_query, ev_infer, ev_train})
+        if (e) (void)hipEventDestroy(e);
 }
 
 extern "C" {
@@ -537,6 +545,10 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
         HIP_CHECK(hipSetDevice(device));
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_query, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_infer, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_train, hipEventDisableTiming));
         c->cfg = *cfg;
         c->W = cfg->width;
         c->H = cfg->height;
@@ -805,9 +817,16 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     const uint32_t frame = (uint32_t)c->frame_count;
     HIP_CHECK(hipMemsetAsync(c->f_qcount.p, 0, (B + 2) * 4, s));
     const bool cache_on = c->cache.on;
+    // learned cache: training records of frame f go to half f % 2 (count word 0 or 3);
+    // the side stream trains on the other half (frame f - 1) while this frame traces
+    const uint32_t par = (uint32_t)(c->frame_count & 1);
+    float *trec_cur = c->cache.trec.p + (size_t)par * c->cache.tcap * 16;
+    float *trec_prev = c->cache.trec.p + (size_t)(par ^ 1) * c->cache.tcap * 16;
+    uint32_t *tcnt_cur = c->cache.counts.p + (par ? 3 : 0), *tcnt_prev = c->cache.counts.p + (par ? 0 : 3);
     if (cache_on) {
         cache_box_from_scene(c);
-        HIP_CHECK(hipMemsetAsync(c->cache.counts.p, 0, 8, s));
+        HIP_CHECK(hipMemsetAsync(c->cache.counts.p + 1, 0, 4, s));
+        HIP_CHECK(hipMemsetAsync(tcnt_cur, 0, 4, s));
     }
     if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) {
         size_t n = (size_t)B * P;
@@ -866,6 +885,21 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         }
         launch_trace_shade(s, sv, ba, P);
         qin[b + 1] = ba.queue_out;
+        if (cache_on && b == c->cache.cfg.query_vertex) {
+            // side stream: cache radiance of this bounce's queries, then the training
+            // steps on the previous frame's records
+            auto &C = c->cache;
+            HIP_CHECK(hipEventRecord(c->ev_query, s));
+            HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_query, 0));
+            launch_nrc_infer(c->side, cache_net(c), C.qrec.p, C.counts.p + 1, P, nullptr, C.crad.p);
+            HIP_CHECK(hipEventRecord(c->ev_infer, c->side));
+            if (!(C.cfg.flags & VRGI_CACHE_FREEZE) && C.cfg.steps_per_frame) {
+                HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, c->side));
+                for (uint32_t k = 0; k < C.cfg.steps_per_frame; ++k)
+                    cache_step(c, c->side, trec_prev, tcnt_prev, C.tcap, k, C.cfg.steps_per_frame, C.cfg.batch, true);
+            }
+            HIP_CHECK(hipEventRecord(c->ev_train, c->side));
+        }
         if (sort_rays && b + 1 < B) {
             SortArgs sa{};
             sa.bounce = b; sa.P = P; sa.bits = c->cfg.ray_sort_bits;
@@ -893,8 +927,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         }
         rec(2 + B + b);
     }
-    if (cache_on)  // cache radiance of the queried bounce-1 vertices
-        launch_nrc_infer(s, cache_net(c), c->cache.qrec.p, c->cache.counts.p + 1, P, nullptr, c->cache.crad.p);
+    if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_infer, 0));  // cache radiance is ready
     rec(2 + B + (B - 1));
     FrameArgs fa{};
     fa.W = c->W; fa.H = c->H; fa.P = P; fa.B = B; fa.frame = frame;
@@ -923,8 +956,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         fa.albedo = c->cache.albedo.p;
         fa.crad = c->cache.crad.p;
         fa.hit = c->f_hit.p;
-        fa.trec = c->cache.trec.p;
-        fa.trec_count = c->cache.counts.p;
+        fa.trec = trec_cur;
+        fa.trec_count = tcnt_cur;
         fa.tcap = c->cache.tcap;
     }
     launch_accum_temporal(s, fa);
@@ -952,12 +985,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         launch_accumulate(s, c->f_radiance.p, c->accum.p, (size_t)P * 3);
         c->accum_frames += 1;
     }
-    if (cache_on && !(c->cache.cfg.flags & VRGI_CACHE_FREEZE) && c->cache.cfg.steps_per_frame) {
-        HIP_CHECK(hipMemsetAsync(c->cache.loss.p, 0, 4, s));
-        for (uint32_t k = 0; k < c->cache.cfg.steps_per_frame; ++k)
-            cache_step(c, c->cache.trec.p, c->cache.counts.p, c->cache.tcap, k, c->cache.cfg.steps_per_frame,
-                       c->cache.cfg.batch, true);
-    }
+    if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_train, 0));  // join the side stream
     launch_frame_stats(s, c->f_qcount.p, B, P, (unsigned long long *)c->d_counters.p);
     rec(2 + 2 * (size_t)B);
     if (stages || touch) {
@@ -1084,10 +1112,11 @@ int vrgi_buffer_size(vrgi_ctx *c, int which, size_t *bytes) {
         case VRGI_BUF_CACHE_QUERIES:
         case VRGI_BUF_CACHE_TRAIN_RECORDS: {
             if (!c->cache.on) { r = 0; break; }
-            uint32_t cnt[2] = {0, 0};
-            HIP_CHECK(hipMemcpyAsync(cnt, c->cache.counts.p, 8, hipMemcpyDeviceToHost, c->stream));
+            uint32_t cnt[4] = {0, 0, 0, 0};
+            HIP_CHECK(hipMemcpyAsync(cnt, c->cache.counts.p, 16, hipMemcpyDeviceToHost, c->stream));
             HIP_CHECK(hipStreamSynchronize(c->stream));
-            r = which == VRGI_BUF_CACHE_QUERIES ? (size_t)cnt[1] * 64 : (size_t)std::min(cnt[0], c->cache.tcap) * 64;
+            const uint32_t tc = cnt[(c->frame_count & 1) ? 0 : 3];  // the last frame's half
+            r = which == VRGI_BUF_CACHE_QUERIES ? (size_t)cnt[1] * 64 : (size_t)std::min(tc, c->cache.tcap) * 64;
             break;
         }
         default: r = 0;
@@ -1137,7 +1166,9 @@ int vrgi_read_buffer(vrgi_ctx *c, int which, void *dst, size_t bytes) {
         case VRGI_BUF_BOUNCE_ALBEDO: src = c->cache.albedo.p; break;
         case VRGI_BUF_CACHE_RADIANCE: src = c->cache.crad.p; break;
         case VRGI_BUF_CACHE_QUERIES: src = c->cache.qrec.p; break;
-        case VRGI_BUF_CACHE_TRAIN_RECORDS: src = c->cache.trec.p; break;
+        case VRGI_BUF_CACHE_TRAIN_RECORDS:
+            src = c->cache.trec.p + (size_t)((c->frame_count & 1) ? 0 : 1) * c->cache.tcap * 16;
+            break;
         default: return fail(VRGI_E_INVALID, "unknown buffer id");
     }
     HIP_CHECK(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, c->stream));
@@ -1367,11 +1398,11 @@ int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
     C.table.reserve(C.n_table); C.mt.reserve(C.n_table); C.vt.reserve(C.n_table); C.gtable.reserve(C.n_table);
     C.wt.reserve(NRC_N_WEIGHTS); C.wb.reserve(NRC_N_WEIGHTS);
     C.loss.reserve(1);
-    C.counts.reserve(3);
+    C.counts.reserve(4);
     C.plane_cap = 0;
     cache_planes(c, cfg->batch);
     C.tcap = cfg->batch * std::max(cfg->steps_per_frame, 1u);
-    C.trec.reserve((size_t)C.tcap * 16);
+    C.trec.reserve((size_t)2 * C.tcap * 16);
     // initial parameters: Kaiming-uniform weights (no bias vectors), table U(-1e-4, 1e-4)
     uint64_t st = 0x5EEDull ^ ((uint64_t)cfg->seed << 32);
     std::vector<float> w(NRC_N_WEIGHTS, 0.0f), t(C.n_table);
@@ -1382,7 +1413,7 @@ int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
     for (size_t i = 0; i < C.n_table; ++i) t[i] = uniform(st, -1e-4f, 1e-4f);
     HIP_CHECK(hipMemcpyAsync(C.w32.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(C.table.p, t.data(), t.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(hipMemsetAsync(C.counts.p, 0, 12, c->stream));
+    HIP_CHECK(hipMemsetAsync(C.counts.p, 0, 16, c->stream));
     cache_clear_state(c);
     HIP_CHECK(hipStreamSynchronize(c->stream));  // w, t are released on return
     C.on = true;
@@ -1485,7 +1516,7 @@ int vrgi_cache_grads(vrgi_ctx *c, uint64_t n, const float *samples, const float 
     HIP_CHECK(hipMemsetAsync(C.gw.p, 0, NRC_N_WEIGHTS * 4, s));
     HIP_CHECK(hipMemsetAsync(C.gtable.p, 0, C.n_table * 4, s));
     HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, s));
-    cache_step(c, C.hrec.p, C.counts.p + 2, (uint32_t)n, 0, 1, mb, false);
+    cache_step(c, s, C.hrec.p, C.counts.p + 2, (uint32_t)n, 0, 1, mb, false);
     std::vector<float> gw(NRC_N_WEIGHTS);
     HIP_CHECK(hipMemcpyAsync(gw.data(), C.gw.p, NRC_N_WEIGHTS * 4, hipMemcpyDeviceToHost, s));
     if (grad_table) HIP_CHECK(hipMemcpyAsync(grad_table, C.gtable.p, C.n_table * 4, hipMemcpyDeviceToHost, s));
@@ -1553,7 +1584,7 @@ int vrgi_cache_train(vrgi_ctx *c, uint64_t n, const float *samples, const float 
     cache_planes(c, mb);
     cache_upload_records(c, n, samples, targets);
     HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, c->stream));
-    cache_step(c, C.hrec.p, C.counts.p + 2, (uint32_t)n, 0, 1, mb, true);
+    cache_step(c, c->stream, C.hrec.p, C.counts.p + 2, (uint32_t)n, 0, 1, mb, true);
     float l = 0.0f;
     HIP_CHECK(hipMemcpyAsync(&l, C.loss.p, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));

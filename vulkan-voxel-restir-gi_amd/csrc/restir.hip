@@ -73,10 +73,21 @@ __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
         f3 term = ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;
         if (queried && b == (int)cv) term = ld3(fa.crad + (size_t)id * 3);
         L = ld3(fa.emis + bi * 3) + term;
-        if (train && fa.hit[bi] != VRGI_INVALID &&
-            (b + 1 < (int)plen || zero3(ld3(fa.dir + bi * 3))) &&
-            isfinite(term.x) && isfinite(term.y) && isfinite(term.z)) {
-            const uint32_t slot = atomicAdd(fa.trec_count, 1u);
+        const bool rec = train && fa.hit[bi] != VRGI_INVALID &&
+                         (b + 1 < (int)plen || zero3(ld3(fa.dir + bi * 3))) &&
+                         isfinite(term.x) && isfinite(term.y) && isfinite(term.z);
+        // one counter update per wave and iteration (lanes still in the loop)
+        const uint64_t rb = __ballot(rec);
+        uint32_t slot = 0;
+        if (rb) {
+            const uint32_t lane = threadIdx.x & 63u;
+            const int lead = __ffsll((unsigned long long)rb) - 1;
+            uint32_t base = 0;
+            if ((int)lane == lead) base = atomicAdd(fa.trec_count, (uint32_t)__popcll(rb));
+            base = __shfl(base, lead, 64);
+            slot = base + (uint32_t)__popcll(rb & ((1ull << lane) - 1ull));
+        }
+        if (rec) {
             if (slot < fa.tcap) {
                 const f3 p = ld3(fa.origin + bi * 3);
                 const f3 dd = b == 0 ? ld3(fa.d0 + (size_t)id * 3) : ld3(fa.dir + (bi - fa.P) * 3);

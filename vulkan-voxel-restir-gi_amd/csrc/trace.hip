@@ -289,12 +289,11 @@ __device__ __forceinline__ IcS local_cs(f3 a, f3 b, f3 c) {  // raytrace.rs:319-
     return r;
 }
 
-template <int STACK, bool COUNT>
-__global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const BounceArgs ba) {
-    __shared__ uint32_t stack[STACK * 64];
+// one wave-sized slice of the bounce's rays (all 64 lanes call it: compaction ballots)
+template <int STACK, bool COUNT, bool CACHE>
+__device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const BounceArgs &ba, uint32_t *stack,
+                                                  uint32_t i, uint32_t count) {
     const uint32_t b = ba.bounce;
-    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-    const uint32_t count = b == 0 ? ba.P : ba.qcount[b];
     bool active = i < count;
     uint32_t px = 0;
     f3 o = mk(0, 0, 0), d = mk(0, 0, 0);
@@ -320,7 +319,7 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
     const uint32_t seed = mm_combine(ba.inv_seed, px);  // raytrace.rs:403
     bool train = false;
     uint32_t cap = ba.B;  // bounces this pixel's path may take
-    if (ba.cache_on) {
+    if (CACHE) {
         train = mm_finalizef(mm_combine(ba.train_seed, px)) < ba.train_frac;
         cap = train ? ba.B : ba.cache_vertex + 1u;
         if (active && b == 0) st3(ba.d0 + (size_t)px * 3, d);
@@ -411,10 +410,10 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
         ba.hit_t[bi] = h.prim == VRGI_INVALID ? 0.0f : h.t;
         cont = !zero3(out_d) && b + 1 < cap;
         if (!cont) ba.path_len[px] = b + 1;
-        if (ba.albedo) st3(ba.albedo + bi * 3, alb);
+        if (CACHE) st3(ba.albedo + bi * 3, alb);
     }
     // cache query of a non-training path at its last vertex
-    if (ba.cache_on && b == ba.cache_vertex) {
+    if (CACHE && b == ba.cache_vertex) {
         const bool q = active && !train && h.prim != VRGI_INVALID;
         const uint64_t qb = __ballot(q);
         if (qb) {
@@ -443,13 +442,21 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
     }
 }
 
+// one wave per block, one ray per lane; the queue's length is read on the device
+template <int STACK, bool COUNT, bool CACHE>
+__global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const BounceArgs ba) {
+    __shared__ uint32_t stack[STACK * 64];
+    const uint32_t count = ba.bounce == 0 ? ba.P : ba.qcount[ba.bounce];
+    if (blockIdx.x * 64u >= count) return;
+    trace_shade_slice<STACK, COUNT, CACHE>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, count);
+}
+
 // NEE pdf of bounce b (nee_pdf.rs:281-337) for the rays alive at bounce b+1
 template <int STACK>
-__global__ __launch_bounds__(64) void k_nee_pdf(const SceneView sv, const BounceArgs ba) {
-    __shared__ uint32_t stack[STACK * 64];
+__device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArgs &ba, uint32_t *stack, uint32_t i,
+                                            uint32_t count) {
     const uint32_t b = ba.bounce;  // pdf of bounce b, rays of bounce b + 1
-    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-    if (i >= ba.qcount[b + 1]) return;
+    if (i >= count) return;
     const uint32_t px = ba.queue_in[i];
     const size_t bi = (size_t)b * ba.P + px;
     float w = ba.mis[bi];
@@ -492,6 +499,11 @@ __global__ __launch_bounds__(64) void k_nee_pdf(const SceneView sv, const Bounce
         }
     }
     ba.nee[bi] = np;
+}
+template <int STACK>
+__global__ __launch_bounds__(64) void k_nee_pdf(const SceneView sv, const BounceArgs ba) {
+    __shared__ uint32_t stack[STACK * 64];
+    nee_pdf_ray<STACK>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, ba.qcount[ba.bounce + 1]);
 }
 
 // ---------------------------------------------------------------- ReSTIR DI (benchmark mode)
@@ -657,7 +669,7 @@ __global__ __launch_bounds__(64) void k_di_final(const SceneView sv, const DiArg
 // nodes, so the smallest size above the tree depth is chosen; smaller stacks raise
 // occupancy (64-lane blocks of 8 / 16 / 40 KB).
 template <class F>
-static void with_stack(uint32_t depth, F f) {
+static void with_stack(uint32_t depth, F f) {  // LDS stack entries per lane for a tree depth
     if (depth + 2 <= 32) f(std::integral_constant<int, 32>());
     else if (depth + 2 <= 64) f(std::integral_constant<int, 64>());
     else f(std::integral_constant<int, VRGI_MAX_STACK>());
@@ -701,8 +713,10 @@ void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba
     bool cnt = ba.touch_nodes != nullptr;
     with_stack(sv.depth, [&](auto S) {
         constexpr int K = decltype(S)::value;
-        if (cnt) k_trace_shade<K, true><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
-        else k_trace_shade<K, false><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+        const uint32_t grid = (max_rays + 63) / 64;
+        if (ba.cache_on) k_trace_shade<K, false, true><<<grid, 64, 0, s>>>(sv, ba);
+        else if (cnt) k_trace_shade<K, true, false><<<grid, 64, 0, s>>>(sv, ba);
+        else k_trace_shade<K, false, false><<<grid, 64, 0, s>>>(sv, ba);
     });
 }
 
