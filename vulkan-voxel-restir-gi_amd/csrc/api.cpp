@@ -118,6 +118,7 @@ struct vrgi_ctx {
     DevBuf<uint32_t> d_ids, d_trees, d_depth;
     DevBuf<vrgi_bvh_node> d_nodes;
     DevBuf<WideNode> d_gwide, d_lwide;
+    DevBuf<Wide4> d_lwide4;
     uint32_t geo_depth = 0, lgeo_depth = 0;
 
     // frame buffers
@@ -185,6 +186,7 @@ SceneView scene_view(vrgi_ctx *c) {
     sv.n_inst = c->n_inst;
     sv.tl = c->d_tl.p;
     sv.lwide = c->d_lwide.p;
+    sv.lwide4 = c->d_lwide4.p;
     sv.ltri = c->d_ltri.p;
     sv.n_ltri = c->n_ltri;
     sv.lroot_prim = 0;
@@ -413,6 +415,7 @@ void rebuild_geometry(vrgi_ctx *c) {
     c->d_nodes.reserve(gnodes + lnodes + 1);
     c->d_gwide.reserve(nt > 1 ? nt - 1 : 1);
     c->d_lwide.reserve(nl > 1 ? nl - 1 : 1);
+    c->d_lwide4.reserve(nl > 1 ? nl - 1 : 1);
     c->d_trees.reserve(6);
     c->d_depth.reserve(2);
     c->geo_depth = c->lgeo_depth = 0;
@@ -460,10 +463,11 @@ void rebuild_geometry(vrgi_ctx *c) {
     uint32_t t = 0;
     if (nt) c->geo_depth = dep[t++];
     if (nl) c->lgeo_depth = dep[t++];
-    if (c->geo_depth + 2 > VRGI_MAX_STACK || c->lgeo_depth + 2 > VRGI_MAX_STACK)
+    if (c->geo_depth + 2 > VRGI_MAX_STACK || 3 * ((c->lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
         throw std::runtime_error("BVH deeper than the traversal stack");
     launch_wide(s, c->d_nodes.p, gnodes, c->d_gwide.p);
     launch_wide(s, c->d_nodes.p + gnodes, lnodes, c->d_lwide.p);
+    launch_wide4(s, c->d_nodes.p + gnodes, lnodes, c->d_lwide4.p);
 }
 
 void free_object(vrgi_object *o) {

@@ -131,6 +131,15 @@ struct __align__(16) WideNode {
     float4 c;  // c1 min z, c1 max xyz
     uint4 r;   // c0 ref, c1 ref, -, -
 };
+// 4-ary traversal node: the binary node of rank k with two levels collapsed, its
+// (up to) four descendants in left-to-right order. Per axis the four padded bounds, then
+// the references (leaf 0x80000000 | prim, internal: rank of that binary node, empty slot
+// 0xFFFFFFFF). Indexed by rank like WideNode, so every internal node has one.
+struct __align__(16) Wide4 {
+    float4 lox, loy, loz, hix, hiy, hiz;
+    uint4 ref;
+    uint4 pad;
+};
 // world triangle, 48 bytes: v0 (xyz, w = material slot), v1, v2 (w unused)
 struct __align__(16) Tri { float4 v0, v1, v2; };
 

@@ -95,6 +95,48 @@ __global__ void k_wide(const vrgi_bvh_node *nodes, uint32_t n_nodes, WideNode *w
     wide[rank] = w;
 }
 
+// 4-ary copy: every internal node of rank k gets its children, or its grandchildren in
+// place of an internal child, in left-to-right order (boxes padded as in k_wide)
+__global__ void k_wide4(const vrgi_bvh_node *nodes, uint32_t n_nodes, Wide4 *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    const vrgi_bvh_node nd = nodes[i];
+    if (nd.left == VRGI_INVALID) return;
+    const float e = VRGI_BOX_PAD;
+    float lo[3][4], hi[3][4];
+    uint32_t ref[4];
+    int k = 0;
+    auto put = [&](const vrgi_bvh_node &c) {
+        for (int a = 0; a < 3; ++a) { lo[a][k] = c.min[a] - e; hi[a][k] = c.max[a] + e; }
+        ref[k] = c.left == VRGI_INVALID ? (0x80000000u | c.right_or_prim) : (c.left - 1) / 2;
+        ++k;
+    };
+    const uint32_t ch[2] = {nd.left, nd.right_or_prim};
+    for (int j = 0; j < 2; ++j) {
+        const vrgi_bvh_node c = nodes[ch[j]];
+        if (c.left == VRGI_INVALID) {
+            put(c);
+        } else {
+            put(nodes[c.left]);
+            put(nodes[c.right_or_prim]);
+        }
+    }
+    for (; k < 4; ++k) {
+        for (int a = 0; a < 3; ++a) { lo[a][k] = 1.0f; hi[a][k] = 0.0f; }
+        ref[k] = VRGI_INVALID;
+    }
+    Wide4 w;
+    w.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    w.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    w.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    w.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    w.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    w.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    w.ref = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+    w.pad = make_uint4(0, 0, 0, 0);
+    out[(nd.left - 1) / 2] = w;
+}
+
 // ---------------------------------------------------------------- closest hit API kernel
 template <int STACK>
 __global__ __launch_bounds__(64) void k_trace_closest(const SceneView sv, const float *orig,
@@ -487,13 +529,21 @@ __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArg
                     np += nee_pdf_one(sv, o, n, d, lid, u, v);
                 have = false;
             } else {
-                WideNode wn = sv.lwide[ref];
-                bool h0 = box_hit(r, wn.a.x, wn.a.y, wn.a.z, wn.a.w, wn.b.x, wn.b.y, VRGI_EPS_NEE, 1000.0f);
-                bool h1 = box_hit(r, wn.b.z, wn.b.w, wn.c.x, wn.c.y, wn.c.z, wn.c.w, VRGI_EPS_NEE, 1000.0f);
-                have = false;
-                if (h1 && h0) { stk[(sp++) * 64] = wn.r.y; ref = wn.r.x; have = true; }
-                else if (h0) { ref = wn.r.x; have = true; }
-                else if (h1) { ref = wn.r.y; have = true; }
+                // 4-ary node: descendants in leaf order; the first one hit is next, the
+                // others are pushed last-first, so leaves are still met left to right
+                const Wide4 *w4 = sv.lwide4 + ref;
+                const float4 lx = w4->lox, ly = w4->loy, lz = w4->loz, hx = w4->hix, hy = w4->hiy, hz = w4->hiz;
+                const uint4 rf = w4->ref;
+                uint32_t m = 0;
+                m |= (rf.x != VRGI_INVALID && box_hit(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, VRGI_EPS_NEE, 1000.0f)) ? 1u : 0u;
+                m |= (rf.y != VRGI_INVALID && box_hit(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, VRGI_EPS_NEE, 1000.0f)) ? 2u : 0u;
+                m |= (rf.z != VRGI_INVALID && box_hit(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, VRGI_EPS_NEE, 1000.0f)) ? 4u : 0u;
+                m |= (rf.w != VRGI_INVALID && box_hit(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, VRGI_EPS_NEE, 1000.0f)) ? 8u : 0u;
+                have = m != 0u;
+                if (m & 8u && m != 8u) stk[(sp++) * 64] = rf.w;
+                if (m & 4u && (m & 3u)) stk[(sp++) * 64] = rf.z;
+                if (m & 2u && (m & 1u)) stk[(sp++) * 64] = rf.y;
+                ref = (m & 1u) ? rf.x : (m & 2u) ? rf.y : (m & 4u) ? rf.z : rf.w;
             }
             if (!have && sp > 0) { ref = stk[(--sp) * 64]; have = true; }
         }
@@ -720,6 +770,10 @@ void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba
     });
 }
 
+void launch_wide4(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, Wide4 *wide4) {
+    if (n_nodes > 1) k_wide4<<<(n_nodes + 255) / 256, 256, 0, s>>>(nodes, n_nodes, wide4);
+}
+
 // number of set bits of a bitmap, added to *out
 __global__ void k_popcount(const uint32_t *bits, size_t n_words, unsigned long long *out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -732,7 +786,8 @@ void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsign
 }
 void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
     if (!max_rays) return;
-    with_stack(sv.ldepth, [&](auto S) {
+    // the 4-ary walk pushes at most 3 references per two binary levels
+    with_stack(3 * ((sv.ldepth + 1) / 2), [&](auto S) {
         k_nee_pdf<decltype(S)::value><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
     });
 }

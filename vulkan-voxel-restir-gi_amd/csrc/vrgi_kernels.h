@@ -21,6 +21,7 @@ struct SceneView {
     uint32_t n_inst;
     const vrgi_bvh_node *tl;   // light top-level BVH
     const WideNode *lwide;     // light-triangle BVH, traversal copy
+    const Wide4 *lwide4;       // the same, 4-ary
     const Tri *ltri;           // world light triangles by global light id
     uint32_t n_ltri;
     uint32_t lroot_prim;
@@ -275,6 +276,7 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r,
 void launch_flatten(hipStream_t s, const FlattenArgs &f);
 void launch_flatten_lights(hipStream_t s, const FlattenArgs &f);
 void launch_wide(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, WideNode *wide);
+void launch_wide4(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, Wide4 *wide4);
 void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, const float *d,
                           uint32_t n, float tmin, float tmax, uint32_t *hit, float *t, float *uv);
 void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays);
