@@ -118,7 +118,7 @@ struct vrgi_ctx {
     DevBuf<uint32_t> d_ids, d_trees, d_depth;
     DevBuf<vrgi_bvh_node> d_nodes;
     DevBuf<WideNode> d_gwide, d_lwide;
-    DevBuf<Wide4> d_lwide4;
+    DevBuf<Wide4> d_lwide4, d_gwide4;
     uint32_t geo_depth = 0, lgeo_depth = 0;
 
     // frame buffers
@@ -177,6 +177,7 @@ void set_dev(vrgi_ctx *c) { HIP_CHECK(hipSetDevice(c->device)); }
 SceneView scene_view(vrgi_ctx *c) {
     SceneView sv{};
     sv.gwide = c->d_gwide.p;
+    sv.gwide4 = c->d_gwide4.p;
     sv.tri = c->d_tri.p;
     sv.n_tris = c->n_tris;
     sv.root_prim = 0;
@@ -416,6 +417,7 @@ void rebuild_geometry(vrgi_ctx *c) {
     c->d_gwide.reserve(nt > 1 ? nt - 1 : 1);
     c->d_lwide.reserve(nl > 1 ? nl - 1 : 1);
     c->d_lwide4.reserve(nl > 1 ? nl - 1 : 1);
+    if (VRGI_GEOM4) c->d_gwide4.reserve(nt > 1 ? nt - 1 : 1);
     c->d_trees.reserve(6);
     c->d_depth.reserve(2);
     c->geo_depth = c->lgeo_depth = 0;
@@ -463,9 +465,10 @@ void rebuild_geometry(vrgi_ctx *c) {
     uint32_t t = 0;
     if (nt) c->geo_depth = dep[t++];
     if (nl) c->lgeo_depth = dep[t++];
-    if (c->geo_depth + 2 > VRGI_MAX_STACK || 3 * ((c->lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
+    if (geom_stack_need(c->geo_depth) + 2 > VRGI_MAX_STACK || 3 * ((c->lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
         throw std::runtime_error("BVH deeper than the traversal stack");
     launch_wide(s, c->d_nodes.p, gnodes, c->d_gwide.p);
+    if (VRGI_GEOM4) launch_wide4(s, c->d_nodes.p, gnodes, c->d_gwide4.p);
     launch_wide(s, c->d_nodes.p + gnodes, lnodes, c->d_lwide.p);
     launch_wide4(s, c->d_nodes.p + gnodes, lnodes, c->d_lwide4.p);
 }
@@ -1016,7 +1019,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         if (touch) {
             std::vector<unsigned long long> tc(2 * (size_t)B);
             HIP_CHECK(hipMemcpy(tc.data(), c->touch_count.p, 16 * (size_t)B, hipMemcpyDeviceToHost));
-            for (uint32_t b = 0; b < B && b < 16; ++b) p.touched_bytes[b] = tc[2 * b] * 64ull + tc[2 * b + 1] * 48ull;
+            for (uint32_t b = 0; b < B && b < 16; ++b)
+                p.touched_bytes[b] = tc[2 * b] * VRGI_GEOM_NODE_BYTES + tc[2 * b + 1] * 48ull;
             c->profile_flags &= ~VRGI_PROFILE_TOUCH;
         }
         if (built) {

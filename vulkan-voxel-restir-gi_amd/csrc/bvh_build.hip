@@ -868,6 +868,8 @@ void Builder::reserve(uint32_t n, uint32_t n_trees) {
     A((void **)&st.split, (size_t)open_cap * sizeof(SplitInfo));
     A((void **)&st.level_nodes, (size_t)cap_n * 8 + 64);
     A((void **)&counters, 64 * 4);
+    level_cap = 65536;
+    A((void **)&level_start, (size_t)level_cap * 4);
     st.n_open = counters + 0;
     st.n_open_next = counters + 1;
     st.n_jobs = counters + 2;
@@ -879,20 +881,30 @@ void Builder::reserve(uint32_t n, uint32_t n_trees) {
 void Builder::free_all() {
     void *ptrs[] = {perm, pc, st.rank_tmp, st.pair, st.open, st.open_next, st.jobs, st.items,
                     st.item_true, st.item_off, st.item_beg, st.item_end, st.bins, st.bounds,
-                    st.split, st.level_nodes, counters};
+                    st.split, st.level_nodes, counters, level_start};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    perm = nullptr; pc = nullptr; counters = nullptr;
+    perm = nullptr; pc = nullptr; counters = nullptr; level_start = nullptr;
     st = BuildState{};
     cap_n = cap_t = 0;
 }
 
 Builder::~Builder() { free_all(); }
 
-// power pass kernel over (global node, tree base) pairs
-__global__ void k_power_pairs(vrgi_bvh_node *nodes, const uint32_t *pairs, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// start of a top-phase level: the open list of the previous level's children becomes
+// current, and the level's first entry in the level-node list is recorded
+__global__ void k_advance(BuildState s, uint32_t *level_start, uint32_t lvl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *s.n_open = *s.n_open_next;
+    *s.n_open_next = 0;
+    level_start[lvl] = *s.n_level_nodes;
+}
+
+// power pass over the (global node, tree base) pairs of level l
+__global__ void k_power_pairs(vrgi_bvh_node *nodes, const uint32_t *pairs, const uint32_t *level_start, uint32_t l) {
+    const uint32_t beg = level_start[l], end = level_start[l + 1];
+    uint32_t i = beg + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) return;
     uint32_t gi = pairs[2 * i], b = pairs[2 * i + 1];
     vrgi_bvh_node nd = nodes[gi];
     float p = 0.0f;
@@ -927,47 +939,50 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     HIP_CHECK(hipMemsetAsync(counters, 0, 64 * 4, stream));
     k_init<<<(n + 255) / 256, 256, 0, stream>>>(a);
     k_roots<<<(n_trees + 255) / 256, 256, 0, stream>>>(a, st);
-    // level loop: open_next -> open
+    // Top phase, level-synchronous. Level l holds at most n_trees * 2^l open nodes, so the
+    // grids are sized on the host without reading the device; kernels exit on the device
+    // counts. Levels are queued in groups of 1, 2, 4, 4, ... and the host reads the
+    // counters once per group to see whether any node is still open (extra levels are
+    // empty).
+    uint32_t group = 1;
+    auto bound = [&](uint32_t lvl) -> uint32_t {
+        uint64_t b = lvl >= 32 ? ~0ull : ((uint64_t)n_trees << lvl);
+        return (uint32_t)std::min<uint64_t>(b, open_capacity);
+    };
     uint32_t h_cnt[8];
-    uint32_t level_off = 0;
-    std::vector<uint32_t> lvl_sizes;
-    for (int lvl = 0;; ++lvl) {
-        // move open_next -> open
-        std::swap(st.open, st.open_next);
-        HIP_CHECK(hipMemcpyAsync(st.n_open, st.n_open_next, 4, hipMemcpyDeviceToDevice, stream));
-        HIP_CHECK(hipMemsetAsync(st.n_open_next, 0, 4, stream));
+    uint32_t lvl = 0;
+    for (;;) {
+        for (uint32_t j = 0; j < group; ++j, ++lvl) {
+            if (lvl + 2 >= level_cap) return -5;
+            std::swap(st.open, st.open_next);
+            k_advance<<<1, 64, 0, stream>>>(st, level_start, lvl);
+            const uint32_t nb = bound(lvl);
+            k_items<<<1, 1024, 0, stream>>>(st);
+            k_clear_node_data<<<nb, 256, 0, stream>>>(st);
+            const uint32_t max_items = n / CHUNK + nb + 1;
+            k_bounds<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+            k_bin<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+            k_split<<<nb, 64, 0, stream>>>(st);
+            k_count<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+            k_prep<<<nb, 64, 0, stream>>>(st);
+            k_part_a<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+            k_part_b<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
+            k_children<<<(nb + 127) / 128, 128, 0, stream>>>(a, st);
+        }
         HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 5 * 4, hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
-        uint32_t n_open = h_cnt[0];
-        if (n_open == 0) break;
-        if (n_open > open_capacity) return -4;
-        k_items<<<1, 1024, 0, stream>>>(st);
-        k_clear_node_data<<<n_open, 256, 0, stream>>>(st);
-        uint32_t max_items = n / CHUNK + n_open + 1;
-        k_bounds<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-        k_bin<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-        k_split<<<n_open, 64, 0, stream>>>(st);
-        k_count<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-        k_prep<<<n_open, 64, 0, stream>>>(st);
-        k_part_a<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-        k_part_b<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-        // level node list goes to st.level_nodes[level_off ...]: k_children appends
-        k_children<<<(n_open + 127) / 128, 128, 0, stream>>>(a, st);
-        lvl_sizes.push_back(n_open);
-        level_off += n_open;
+        if (h_cnt[1] == 0) break;  // no node left open for the next level
+        group = std::min(group * 2, 4u);
     }
-    // jobs
-    HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 5 * 4, hipMemcpyDeviceToHost, stream));
-    HIP_CHECK(hipStreamSynchronize(stream));
-    uint32_t n_jobs = h_cnt[2];
+    std::swap(st.open, st.open_next);
+    k_advance<<<1, 64, 0, stream>>>(st, level_start, lvl);  // level_start[lvl] = end of the list
+    const uint32_t n_levels = lvl;
+    const uint32_t n_jobs = h_cnt[2];
     if (n_jobs) k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL><<<n_jobs, 64, 0, stream>>>(a, st);
-    // power of the top-phase nodes, deepest level first; st.level_nodes holds
-    // (global index, tree base) pairs, one contiguous run per level
-    uint32_t off = level_off;
-    for (int l = (int)lvl_sizes.size() - 1; l >= 0; --l) {
-        off -= lvl_sizes[l];
-        uint32_t cnt = lvl_sizes[l];
-        k_power_pairs<<<(cnt + 255) / 256, 256, 0, stream>>>(nodes, st.level_nodes + 2 * off, cnt);
+    // power of the top-phase nodes, deepest level first
+    for (int l = (int)n_levels - 1; l >= 0; --l) {
+        const uint32_t nb = bound((uint32_t)l);
+        k_power_pairs<<<(nb + 255) / 256, 256, 0, stream>>>(nodes, st.level_nodes, level_start, (uint32_t)l);
     }
     if (prim_order_out && prim_order_out != perm)
         HIP_CHECK(hipMemcpyAsync(prim_order_out, perm, (size_t)n * 4, hipMemcpyDeviceToDevice, stream));

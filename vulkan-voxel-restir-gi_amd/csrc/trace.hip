@@ -719,15 +719,16 @@ __global__ __launch_bounds__(64) void k_di_final(const SceneView sv, const DiArg
 // nodes, so the smallest size above the tree depth is chosen; smaller stacks raise
 // occupancy (64-lane blocks of 8 / 16 / 40 KB).
 template <class F>
-static void with_stack(uint32_t depth, F f) {  // LDS stack entries per lane for a tree depth
+static void with_stack(uint32_t depth, F f) {  // LDS stack entries per lane for a walk's need
     if (depth + 2 <= 32) f(std::integral_constant<int, 32>());
+    else if (depth + 2 <= 48) f(std::integral_constant<int, 48>());
     else if (depth + 2 <= 64) f(std::integral_constant<int, 64>());
     else f(std::integral_constant<int, VRGI_MAX_STACK>());
 }
 
 void launch_di(hipStream_t s, const SceneView &sv, const DiArgs &da) {
     k_di_initial<<<(da.P + 63) / 64, 64, 0, s>>>(sv, da);
-    with_stack(sv.depth, [&](auto S) { k_di_final<decltype(S)::value><<<(da.P + 63) / 64, 64, 0, s>>>(sv, da); });
+    with_stack(geom_stack_need(sv.depth), [&](auto S) { k_di_final<decltype(S)::value><<<(da.P + 63) / 64, 64, 0, s>>>(sv, da); });
 }
 // frame counters: closest-hit rays = P + live rays of bounces 1..B-1
 __global__ void k_frame_stats(const uint32_t *qcount, uint32_t B, uint32_t P, unsigned long long *c) {
@@ -754,14 +755,14 @@ void launch_wide(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, Wi
 void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, const float *d,
                           uint32_t n, float tmin, float tmax, uint32_t *hit, float *t, float *uv) {
     if (!n) return;
-    with_stack(sv.depth, [&](auto S) {
+    with_stack(geom_stack_need(sv.depth), [&](auto S) {
         k_trace_closest<decltype(S)::value><<<(n + 63) / 64, 64, 0, s>>>(sv, o, d, n, tmin, tmax, hit, t, uv);
     });
 }
 void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
     if (!max_rays) return;
     bool cnt = ba.touch_nodes != nullptr;
-    with_stack(sv.depth, [&](auto S) {
+    with_stack(geom_stack_need(sv.depth), [&](auto S) {
         constexpr int K = decltype(S)::value;
         const uint32_t grid = (max_rays + 63) / 64;
         if (ba.cache_on) k_trace_shade<K, false, true><<<grid, 64, 0, s>>>(sv, ba);

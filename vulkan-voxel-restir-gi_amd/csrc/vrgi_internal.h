@@ -81,8 +81,8 @@ struct BuildState {
 class Builder {
   public:
     ~Builder();
-    // Queues a batched build on `stream`; synchronises the stream once per top-phase
-    // level to size the next level's grids. prim_order_out (may be null) receives the
+    // Queues a batched build on `stream`; synchronises the stream after top-phase levels
+    // 1, 3, 7, 11, ... (to see whether any node is still open). prim_order_out (may be null) receives the
     // final primitive index per leaf position (global over all trees).
     int build(hipStream_t stream, const float *prim9, const float *power, const uint32_t *ids,
               uint32_t n, uint32_t n_trees, const uint32_t *d_tree_beg, const uint32_t *d_tree_n,
@@ -99,6 +99,8 @@ class Builder {
     uint32_t *perm = nullptr;
     float *pc = nullptr;
     uint32_t *counters = nullptr;
+    uint32_t *level_start = nullptr;  // first level-node entry of each top-phase level
+    uint32_t level_cap = 0;
     BuildState st;
 };
 

@@ -12,6 +12,7 @@ namespace vrgi {
 // ---------------------------------------------------------------- argument blocks
 struct SceneView {
     const WideNode *gwide;     // geometry BVH, traversal copy (one entry per internal node)
+    const Wide4 *gwide4;       // the same, 4-ary
     const Tri *tri;            // world triangles by global id
     uint32_t n_tris;
     uint32_t root_prim;        // primitive of the root when the tree is a single leaf
@@ -219,7 +220,7 @@ struct Hit {
 // COUNT (measurement only): marks every visited internal node / tested triangle in the
 // bitmaps touch_nodes / touch_tris.
 template <int STACK, bool COUNT = false>
-__device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r, float tmin,
+__device__ __forceinline__ Hit closest_hit2(const SceneView &sv, const RayPre &r, float tmin,
                                            float tmax, uint32_t *stack,
                                            uint32_t *touch_nodes = nullptr,
                                            uint32_t *touch_tris = nullptr) {
@@ -271,6 +272,92 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r,
     }
     return h;
 }
+
+// the same over the 4-ary copy: a node tests four descendants; leaves are tested at
+// once, hit internal descendants are visited nearest first (the result does not depend
+// on the order: equal t resolves to the lower primitive id)
+template <int STACK, bool COUNT = false>
+__device__ __forceinline__ Hit closest_hit4(const SceneView &sv, const RayPre &r, float tmin, float tmax,
+                                            uint32_t *stack, uint32_t *touch_nodes = nullptr,
+                                            uint32_t *touch_tris = nullptr) {
+    Hit h;
+    h.prim = VRGI_INVALID;
+    h.t = 0.0f; h.u = 0.0f; h.v = 0.0f;
+    if (sv.n_tris == 0) return h;
+    float best = tmax;
+    uint32_t *stk = stack + (threadIdx.x & 63);
+    auto test = [&](uint32_t prim) {
+        if (COUNT) atomicOr(touch_tris + (prim >> 5), 1u << (prim & 31));
+        Tri t = sv.tri[prim];
+        float tt, u, v;
+        if (tri_hit(r, mk(t.v0.x, t.v0.y, t.v0.z), mk(t.v1.x, t.v1.y, t.v1.z), mk(t.v2.x, t.v2.y, t.v2.z),
+                    tmin, best, &tt, &u, &v)) {
+            if (tt < best || (tt == best && prim < h.prim)) {
+                best = tt;
+                h.prim = prim; h.t = tt; h.u = u; h.v = v;
+            }
+        }
+    };
+    if (sv.n_tris == 1) {
+        test(sv.root_prim);
+        return h;
+    }
+    uint32_t ref = 0;
+    int sp = 0;
+    for (;;) {
+        if (COUNT) atomicOr(touch_nodes + (ref >> 5), 1u << (ref & 31));
+        const Wide4 *w = sv.gwide4 + ref;
+        const float4 lx = w->lox, ly = w->loy, lz = w->loz, hx = w->hix, hy = w->hiy, hz = w->hiz;
+        const uint4 rf = w->ref;
+        float t0, t1, t2, t3;
+        bool b0 = rf.x != VRGI_INVALID && box_hit_t(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tmin, best, &t0);
+        bool b1 = rf.y != VRGI_INVALID && box_hit_t(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tmin, best, &t1);
+        bool b2 = rf.z != VRGI_INVALID && box_hit_t(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tmin, best, &t2);
+        bool b3 = rf.w != VRGI_INVALID && box_hit_t(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tmin, best, &t3);
+        if (b0 && (rf.x & 0x80000000u)) { test(rf.x & 0x7FFFFFFFu); b0 = false; }
+        if (b1 && (rf.y & 0x80000000u)) { test(rf.y & 0x7FFFFFFFu); b1 = false; }
+        if (b2 && (rf.z & 0x80000000u)) { test(rf.z & 0x7FFFFFFFu); b2 = false; }
+        if (b3 && (rf.w & 0x80000000u)) { test(rf.w & 0x7FFFFFFFu); b3 = false; }
+        // internal hits sorted by entry distance (absent ones last)
+        const float inf = __int_as_float(0x7f800000);
+        float k0 = b0 ? t0 : inf, k1 = b1 ? t1 : inf, k2 = b2 ? t2 : inf, k3 = b3 ? t3 : inf;
+        uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+        auto cs = [](float &ka, uint32_t &ra, float &kb, uint32_t &rb) {
+            if (kb < ka) { float tk = ka; ka = kb; kb = tk; uint32_t tr = ra; ra = rb; rb = tr; }
+        };
+        cs(k0, r0, k1, r1); cs(k2, r2, k3, r3); cs(k0, r0, k2, r2); cs(k1, r1, k3, r3); cs(k1, r1, k2, r2);
+        const int n = (int)b0 + (int)b1 + (int)b2 + (int)b3;
+        if (n == 0) {
+            if (sp == 0) break;
+            ref = stk[(--sp) * 64];
+            continue;
+        }
+        if (n >= 4) stk[(sp++) * 64] = r3;
+        if (n >= 3) stk[(sp++) * 64] = r2;
+        if (n >= 2) stk[(sp++) * 64] = r1;
+        ref = r0;
+    }
+    return h;
+}
+
+// closest hit over the geometry tree's traversal copy in use
+#ifndef VRGI_GEOM4
+#define VRGI_GEOM4 0
+#endif
+template <int STACK, bool COUNT = false>
+__device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r, float tmin, float tmax,
+                                           uint32_t *stack, uint32_t *touch_nodes = nullptr,
+                                           uint32_t *touch_tris = nullptr) {
+#if VRGI_GEOM4
+    return closest_hit4<STACK, COUNT>(sv, r, tmin, tmax, stack, touch_nodes, touch_tris);
+#else
+    return closest_hit2<STACK, COUNT>(sv, r, tmin, tmax, stack, touch_nodes, touch_tris);
+#endif
+}
+// stack entries a closest-hit walk needs for a tree of binary depth d
+__host__ __device__ inline uint32_t geom_stack_need(uint32_t d) { return VRGI_GEOM4 ? 3 * ((d + 1) / 2) : d; }
+// bytes of one traversal node of the geometry tree (touch counting)
+#define VRGI_GEOM_NODE_BYTES (VRGI_GEOM4 ? 128ull : 64ull)
 
 // ---------------------------------------------------------------- launchers
 void launch_flatten(hipStream_t s, const FlattenArgs &f);
