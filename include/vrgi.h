@@ -301,6 +301,15 @@ int vrgi_trace_closest(vrgi_ctx *ctx, const float *origins, const float *dirs, u
                        float tmin, float tmax, uint32_t *hit_prim, float *hit_t,
                        float *hit_uv);
 
+/* chunk::gen_mesh (chunk.rs:158-292) on the GPU for a batch of chunks, in the same vertex
+ * order as vrgi_gen_mesh_chunk. blocks: dims[0] x dims[1] x dims[2] block ids, x-major
+ * (n_types = air); chunks: n_chunks x (cx, cy, cz). The chunks' vertices go to out one
+ * chunk after another (chunk-local positions); counts[i] = vertices of chunk i. Returns
+ * VRGI_E_LIMIT (counts filled, out untouched) when their total exceeds cap. */
+int vrgi_mesh_chunks(vrgi_ctx *ctx, const uint16_t *blocks, const uint32_t dims[3], uint32_t n_types,
+                     const uint8_t *translucent, uint32_t n_chunks, const uint32_t *chunks,
+                     vrgi_vertex3d *out, uint64_t cap, uint64_t *counts);
+
 /* ---- synthetic scenes (caller side: the game_system worldgen + chunk mesher) ---- */
 
 /* Seeded voxel world of one of the benchmark workloads: kind 0 = 64x64x64

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libvrgi.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VRGI_ARCH", "gfx950")
 
-SOURCES = ["bvh_build.hip", "trace.hip", "restir.hip", "scene.hip", "sort.hip", "nrc.hip", "api.cpp", "world.cpp"]
+SOURCES = ["bvh_build.hip", "trace.hip", "restir.hip", "scene.hip", "sort.hip", "nrc.hip", "mesher.hip", "api.cpp", "world.cpp"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -162,6 +162,13 @@ struct vrgi_ctx {
     uint64_t accum_frames = 0;
     DevBuf<double> accum;
 
+    // GPU mesher workspace
+    DevBuf<uint16_t> m_blocks;
+    DevBuf<uint8_t> m_transl, m_mask;
+    DevBuf<uint32_t> m_chunks, m_face_off, m_faces;
+    DevBuf<uint64_t> m_vert_off;
+    DevBuf<vrgi_vertex3d> m_out;
+
     // host-side scratch of the standalone operators
     DevBuf<float> a_f0, a_f1, a_f2;
     DevBuf<uint32_t> a_u0, a_u1, a_u2;
@@ -1539,6 +1546,66 @@ int vrgi_cache_grads(vrgi_ctx *c, uint64_t n, const float *samples, const float 
     if (loss) *loss = l;
     HIP_CHECK(hipMemsetAsync(C.gw.p, 0, NRC_N_WEIGHTS * 4, s));
     HIP_CHECK(hipMemsetAsync(C.gtable.p, 0, C.n_table * 4, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_mesh_chunks(vrgi_ctx *c, const uint16_t *blocks, const uint32_t dims[3], uint32_t n_types,
+                     const uint8_t *translucent, uint32_t n_chunks, const uint32_t *chunks, vrgi_vertex3d *out,
+                     uint64_t cap, uint64_t *counts) {
+    API_BEGIN
+    if (!c || !blocks || !dims || !translucent || (n_chunks && (!chunks || !counts)))
+        return fail(VRGI_E_INVALID, "null argument");
+    if (n_types == 0 || n_types > 65535) return fail(VRGI_E_INVALID, "n_types must be in [1, 65535]");
+    if (n_chunks == 0) return VRGI_OK;
+    if (n_chunks > 65535) return fail(VRGI_E_LIMIT, "more than 65535 chunks in one call");
+    const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
+    if (nvox == 0 || nvox > ((size_t)1 << 32)) return fail(VRGI_E_INVALID, "world size out of range");
+    set_dev(c);
+    hipStream_t s = c->stream;
+    c->m_blocks.reserve(nvox);
+    c->m_transl.reserve(n_types + 1);
+    c->m_chunks.reserve((size_t)3 * n_chunks);
+    c->m_mask.reserve((size_t)n_chunks * 32768);
+    c->m_face_off.reserve((size_t)n_chunks * 32768);
+    c->m_faces.reserve(n_chunks);
+    c->m_vert_off.reserve(n_chunks);
+    std::vector<uint8_t> tr(translucent, translucent + n_types);
+    tr.push_back(0);  // air (never looked up: air is tested first)
+    HIP_CHECK(hipMemcpyAsync(c->m_blocks.p, blocks, nvox * 2, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->m_transl.p, tr.data(), tr.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(c->m_chunks.p, chunks, (size_t)12 * n_chunks, hipMemcpyHostToDevice, s));
+    MeshArgsHost h{};
+    h.blocks = c->m_blocks.p;
+    for (int k = 0; k < 3; ++k) h.dims[k] = dims[k];
+    h.air = n_types;
+    h.translucent = c->m_transl.p;
+    h.chunks = c->m_chunks.p;
+    h.n_chunks = n_chunks;
+    h.mask = c->m_mask.p;
+    h.face_off = c->m_face_off.p;
+    h.faces = c->m_faces.p;
+    launch_mesh_count(s, h);
+    std::vector<uint32_t> faces(n_chunks);
+    HIP_CHECK(hipMemcpyAsync(faces.data(), c->m_faces.p, (size_t)4 * n_chunks, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint64_t> off(n_chunks);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n_chunks; ++i) {
+        off[i] = total;
+        counts[i] = (uint64_t)faces[i] * 6;
+        total += counts[i];
+    }
+    if (total > cap) return fail(VRGI_E_LIMIT, "vertex capacity too small");
+    if (total == 0) return VRGI_OK;
+    if (!out) return fail(VRGI_E_INVALID, "null output");
+    c->m_out.reserve(total);
+    HIP_CHECK(hipMemcpyAsync(c->m_vert_off.p, off.data(), (size_t)8 * n_chunks, hipMemcpyHostToDevice, s));
+    h.vert_off = c->m_vert_off.p;
+    h.out = c->m_out.p;
+    launch_mesh_emit(s, h);
+    HIP_CHECK(hipMemcpyAsync(out, c->m_out.p, total * sizeof(vrgi_vertex3d), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     return VRGI_OK;
     API_END

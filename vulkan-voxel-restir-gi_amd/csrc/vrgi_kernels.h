@@ -420,6 +420,23 @@ void launch_nrc_grads(hipStream_t s, const NrcNet &net, const NrcTrain &tr);
 void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb);
 void launch_nrc_zero(hipStream_t s, float *p, size_t n);
 
+// exposed-face meshing of 32^3 chunks (mesher.hip)
+struct MeshArgsHost {
+    const uint16_t *blocks;      // dims[0] x dims[1] x dims[2], x-major
+    uint32_t dims[3];
+    uint32_t air;
+    const uint8_t *translucent;
+    const uint32_t *chunks;      // n_chunks x (cx, cy, cz)
+    uint32_t n_chunks;
+    uint8_t *mask;               // [n_chunks][32768]
+    uint32_t *face_off;          // [n_chunks][32768]
+    uint32_t *faces;             // [n_chunks]
+    const uint64_t *vert_off;    // [n_chunks]
+    vrgi_vertex3d *out;
+};
+void launch_mesh_count(hipStream_t s, const MeshArgsHost &h);
+void launch_mesh_emit(hipStream_t s, const MeshArgsHost &h);
+
 // queue reordering by (direction octant, origin Morton code) between bounces
 struct SortArgs {
     uint32_t bounce, P, bits;         // rays of bounce + 1 read their origin / dir from bounce

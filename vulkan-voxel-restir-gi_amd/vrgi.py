@@ -282,6 +282,27 @@ class Context:
         check(self.L.vrgi_stats(self.h, C.byref(st), int(reset)))
         return st
 
+    # ---- GPU mesher
+    def mesh_chunks(self, blocks, n_types, translucent, chunks):
+        """chunk::gen_mesh on the GPU for chunks [(cx, cy, cz), ...]: list of vertex arrays."""
+        blocks = np.ascontiguousarray(blocks, np.uint16)
+        dims = np.array(blocks.shape, np.uint32)
+        ch = np.ascontiguousarray(chunks, np.uint32).reshape(-1, 3)
+        counts = np.zeros(len(ch), np.uint64)
+        tr = np.ascontiguousarray(translucent, np.uint8)
+        code = self.L.vrgi_mesh_chunks(self.h, ptr(blocks), ptr(dims), n_types, ptr(tr), len(ch), ptr(ch), None,
+                                       C.c_uint64(0), ptr(counts))
+        if code not in (0, -4):
+            check(code)
+        out = np.zeros(int(counts.sum()), VERTEX_DTYPE)
+        check(self.L.vrgi_mesh_chunks(self.h, ptr(blocks), ptr(dims), n_types, ptr(tr), len(ch), ptr(ch), ptr(out),
+                                      C.c_uint64(len(out)), ptr(counts)))
+        res, o = [], 0
+        for k in counts:
+            res.append(out[o:o + int(k)])
+            o += int(k)
+        return res
+
     # ---- measurement
     def profile(self, flags):
         check(self.L.vrgi_profile(self.h, flags))
