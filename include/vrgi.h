@@ -209,6 +209,7 @@ enum vrgi_buffer_id {
     VRGI_BUF_CACHE_RADIANCE,     /* float[P][3] cache radiance of the queried pixels (cache enabled) */
     VRGI_BUF_CACHE_QUERIES,      /* float[n][16] query records of the last frame (cache enabled) */
     VRGI_BUF_CACHE_TRAIN_RECORDS,/* float[n][16] training records of the last frame (cache enabled) */
+    VRGI_BUF_CACHE_FALLBACK,     /* uint8[tiles] per-tile fallback flags for the next frame (cache enabled) */
     VRGI_BUF_COUNT
 };
 /* bytes needed for buffer `which` in the current state (0 if not available). */
@@ -239,6 +240,11 @@ typedef struct vrgi_cache_config {
     uint32_t flags;             /* VRGI_CACHE_* */
     uint32_t query_vertex;      /* where non-training paths end in the cache: 0 the camera ray's
                                    hit, 1 the next vertex (0) */
+    float fallback_bound;       /* per-tile fallback to the plain tracer: a tile whose held-out
+                                   relative error |E[cache] - E[target]| / (E[target] + 0.01)
+                                   exceeds this traces full paths next frame; 0 = off (0) */
+    uint32_t fallback_tile;     /* tile side in pixels (16) */
+    float fallback_ema;         /* weight of a frame in the per-tile means (0.1) */
 } vrgi_cache_config;
 
 #define VRGI_CACHE_FREEZE 1u  /* query only: render() takes no training steps */

@@ -1162,7 +1162,8 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
     bool train = false;
     uint32_t cap = c.B;  // bounces this pixel's path may take
     if (cf) {
-        train = finalizef(combine(cf->train_seed, (uint32_t)id)) < cf->train_fraction;
+        train = finalizef(combine(cf->train_seed, (uint32_t)id)) < cf->train_fraction ||
+                (cf->fallback && cf->fallback[id]);
         cap = train ? c.B : cf->query_vertex + 1u;
     }
     uint32_t x = (uint32_t)(id % c.W), y = (uint32_t)(id / c.W);

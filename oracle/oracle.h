@@ -113,6 +113,7 @@ typedef struct or_cache_frame {
     float train_fraction;
     uint32_t train_seed;
     uint32_t query_vertex;   // 0 or 1
+    const uint8_t *fallback; // [P] in, may be null: pixels that trace full paths like training ones
     const float *cache_rad;  // [P][3] in
     float *albedo;           // [B][P][3] out: material reflectivity at each hit
     float *qrec;             // [P][16] out

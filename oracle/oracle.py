@@ -136,7 +136,7 @@ FRAME_FIELDS = ["origin", "direction", "normal", "emissivity", "reflectivity", "
 
 class CacheFrame(C.Structure):
     _fields_ = [("train_fraction", C.c_float), ("train_seed", C.c_uint32), ("query_vertex", C.c_uint32),
-                ("cache_rad", C.c_void_p),
+                ("fallback", C.c_void_p), ("cache_rad", C.c_void_p),
                 ("albedo", C.c_void_p), ("qrec", C.c_void_p), ("n_qrec", C.c_uint64), ("trec", C.c_void_p),
                 ("trec_cap", C.c_uint64), ("n_trec", C.c_uint64)]
 
@@ -287,7 +287,7 @@ class Renderer:
         return out
 
     def render_cache(self, scene, cam, frame, cache_rad, train_fraction, train_seed, nee_type=0, spatial=0,
-                     threads=0, trec_cap=1 << 20, query_vertex=0):
+                     threads=0, trec_cap=1 << 20, query_vertex=0, fallback=None):
         """A learned-radiance-cache frame (see or_cache_frame): cache_rad [P][3] is the
         cache's reflected radiance at the bounce-1 vertex of every queried pixel."""
         P, B = self.W * self.H, self.B
@@ -308,7 +308,8 @@ class Renderer:
         alb = np.zeros((B, P, 3), np.float32)
         qrec = np.zeros((P, 16), np.float32)
         trec = np.zeros((trec_cap, 16), np.float32)
-        cf = CacheFrame(train_fraction, train_seed, query_vertex, crad.ctypes.data, alb.ctypes.data, qrec.ctypes.data, 0,
+        fb = None if fallback is None else np.ascontiguousarray(fallback, np.uint8).reshape(P)
+        cf = CacheFrame(train_fraction, train_seed, query_vertex, None if fb is None else fb.ctypes.data, crad.ctypes.data, alb.ctypes.data, qrec.ctypes.data, 0,
                         trec.ctypes.data, trec_cap, 0)
         p = Prefs(nee_type, 0, spatial)
         r = self.L.or_render_cache(self.h, scene.h, C.byref(cam), C.byref(p), C.c_uint64(frame), C.byref(fr),

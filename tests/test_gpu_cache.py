@@ -148,8 +148,11 @@ def _by_key(rec):
     return rec[np.argsort(keys, kind="stable")]
 
 
-@pytest.mark.parametrize("nee_type,spatial,qv", [(1, 2, 0), (1, 2, 1), (0, 0, 1)])
-def test_cache_frame_parity(world0, scenes, nee_type, spatial, qv):
+@pytest.mark.parametrize("nee_type,spatial,qv,fb", [(1, 2, 0, 0.0), (1, 2, 1, 0.0), (0, 0, 1, 0.0),
+                                                     (1, 2, 0, 1e-6)])
+def test_cache_frame_parity(world0, scenes, nee_type, spatial, qv, fb):
+    """Cache-mode frames against the oracle given the GPU's cache radiance; with fb > 0
+    the per-tile fallback flags of the previous frame select full paths as well."""
     import oracle
     import vrgi
     s, objs = scenes
@@ -159,17 +162,23 @@ def test_cache_frame_parity(world0, scenes, nee_type, spatial, qv):
     with _ctx(world0, W, H, flags=vrgi.FLAG_KEEP_BOUNCES) as c:
         c.add_objects(objs)
         _enable(c, box_lo=(0, 0, 0), box_hi=(0, 0, 0), train_fraction=frac, train_bounces=TB, batch=4096,
-                steps_per_frame=2, seed=seed, query_vertex=qv)
+                steps_per_frame=2, seed=seed, query_vertex=qv, fallback_bound=fb, fallback_tile=16)
         r = oracle.Renderer(W, H, TB)
         P = W * H
         for f in range(3):
             say(f"cache frame {f} nee={nee_type} spatial={spatial} query vertex {qv}")
             w_before, t_before = c.cache_get()  # the frame queries these; it trains after
+            flags = c.read(vrgi.BUF_CACHE_FALLBACK, np.uint8)  # set by the previous frame
+            tx = (W + 15) // 16
+            yy, xx = np.divmod(np.arange(W * H), W)
+            px_fb = flags[(yy // 16) * tx + xx // 16] if fb > 0 else None
+            if fb > 0 and f > 0:
+                assert flags.any()
             c.render(cam, nee_type=nee_type, spatial=spatial)
             c.sync()
             crad = c.read(vrgi.BUF_CACHE_RADIANCE).reshape(P, 3)
             ref = r.render_cache(s, cam, f, crad, frac, oracle.frame_seed(seed, f, 5), nee_type=nee_type,
-                                 spatial=spatial, query_vertex=qv)
+                                 spatial=spatial, query_vertex=qv, fallback=px_fb)
             plen = c.read(vrgi.BUF_PATH_LEN, np.uint32)
             assert np.array_equal(plen, ref["path_len"])
             assert plen.max() == TB and (plen == qv + 1).any()

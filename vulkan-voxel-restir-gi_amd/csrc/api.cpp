@@ -83,7 +83,7 @@ struct vrgi_ctx {
     // learned-cache work of a frame that overlaps the training paths' bounces (default
     // priority, like every stream here); joined into `stream` before the frame ends
     hipStream_t side = nullptr;
-    hipEvent_t ev_query = nullptr, ev_infer = nullptr, ev_train = nullptr;
+    hipEvent_t ev_query = nullptr, ev_infer = nullptr, ev_train = nullptr, ev_accum = nullptr;
     std::mutex up_mu;
     vrgi_config cfg{};
     uint32_t W = 0, H = 0, P = 0, B = 0;
@@ -154,6 +154,10 @@ struct vrgi_ctx {
         DevBuf<float> w32, mw, vw, table, mt, vt, gw, gtable, loss;
         DevBuf<uint16_t> wt, wb, xplane, gplane;
         DevBuf<float> trec, qrec, crad, hrec, hout, albedo, d0;
+        // per-tile held-out error and fallback flags
+        uint32_t tiles_x = 0, tiles_y = 0;
+        DevBuf<float> t_acc, t_ema, hpred;
+        DevBuf<uint8_t> t_fb;
         DevBuf<uint32_t> counts;  // [0] training records, [1] query records, [2] caller records
     } cache;
 
@@ -290,6 +294,17 @@ void alloc_frame(vrgi_ctx *c) {
     if (c->cache.on) {
         c->cache.albedo.reserve(B * P * 3);
         c->cache.d0.reserve(P * 3);
+        auto &C = c->cache;
+        const uint32_t T = C.cfg.fallback_tile;
+        C.tiles_x = (c->W + T - 1) / T;
+        C.tiles_y = (c->H + T - 1) / T;
+        const size_t nt = (size_t)C.tiles_x * C.tiles_y;
+        C.t_acc.reserve(3 * nt);
+        C.t_ema.reserve(2 * nt);
+        C.t_fb.reserve(nt);
+        HIP_CHECK(hipMemsetAsync(C.t_acc.p, 0, 12 * nt, c->stream));
+        HIP_CHECK(hipMemsetAsync(C.t_ema.p, 0, 8 * nt, c->stream));
+        HIP_CHECK(hipMemsetAsync(C.t_fb.p, 0, nt, c->stream));
         c->cache.qrec.reserve(P * 16);
         c->cache.crad.reserve(P * 3);
     }
@@ -515,7 +530,7 @@ vrgi_ctx::~vrgi_ctx() {
     if (up_stream) (void)hipStreamDestroy(up_stream);
     if (side) (void)hipStreamSynchronize(side);
     if (side) (void)hipStreamDestroy(side);
-    for (hipEvent_t e : {ev_query, ev_infer, ev_train})
+    for (hipEvent_t e : {ev_query, ev_infer, ev_train, ev_accum})
         if (e) (void)hipEventDestroy(e);
 }
 
@@ -563,6 +578,7 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_query, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_infer, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_train, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_accum, hipEventDisableTiming));
         c->cfg = *cfg;
         c->W = cfg->width;
         c->H = cfg->height;
@@ -874,6 +890,9 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         ba.train_frac = c->cache.cfg.train_fraction;
         ba.cache_vertex = c->cache.cfg.query_vertex;
         ba.d0 = c->cache.d0.p;
+        ba.fallback = c->cache.cfg.fallback_bound > 0.0f ? c->cache.t_fb.p : nullptr;
+        ba.tile = c->cache.cfg.fallback_tile;
+        ba.tiles_x = c->cache.tiles_x;
         ba.albedo = c->cache.albedo.p;
         ba.qrec = c->cache.qrec.p;
         ba.qrec_count = c->cache.counts.p + 1;
@@ -965,6 +984,9 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         fa.train_seed = train_seed;
         fa.train_frac = c->cache.cfg.train_fraction;
         fa.cache_vertex = c->cache.cfg.query_vertex;
+        fa.fallback = c->cache.cfg.fallback_bound > 0.0f ? c->cache.t_fb.p : nullptr;
+        fa.tile = c->cache.cfg.fallback_tile;
+        fa.tiles_x = c->cache.tiles_x;
         fa.dir = c->f_dir.p;
         fa.d0 = c->cache.d0.p;
         fa.albedo = c->cache.albedo.p;
@@ -975,6 +997,20 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         fa.tcap = c->cache.tcap;
     }
     launch_accum_temporal(s, fa);
+    if (cache_on && c->cache.cfg.fallback_bound > 0.0f) {
+        // held-out error of this frame's records at the query vertex (the side stream has
+        // finished its training steps, none of them on these records), per screen tile
+        auto &C = c->cache;
+        HIP_CHECK(hipEventRecord(c->ev_accum, s));
+        HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_accum, 0));
+        C.hpred.reserve((size_t)C.tcap * 3);
+        launch_nrc_infer(c->side, cache_net(c), trec_cur, tcnt_cur, C.tcap, C.hpred.p, nullptr);
+        launch_tile_stats(c->side, trec_cur, tcnt_cur, C.tcap, C.hpred.p, C.cfg.query_vertex, c->W,
+                          C.cfg.fallback_tile, C.tiles_x, C.t_acc.p);
+        launch_tile_update(c->side, C.t_acc.p, C.t_ema.p, C.t_fb.p, C.tiles_x * C.tiles_y, C.cfg.fallback_ema,
+                           C.cfg.fallback_bound);
+        HIP_CHECK(hipEventRecord(c->ev_train, c->side));
+    }
     if (c->cfg.restir_mode == 1) {
         DiArgs da{};
         da.W = c->W; da.H = c->H; da.P = P; da.frame = frame;
@@ -1124,6 +1160,7 @@ int vrgi_buffer_size(vrgi_ctx *c, int which, size_t *bytes) {
         case VRGI_BUF_DI_WEIGHTS: r = c->cfg.restir_mode == 1 ? P * 16 : 0; break;
         case VRGI_BUF_BOUNCE_ALBEDO: r = c->cache.on ? B * P * 12 : 0; break;
         case VRGI_BUF_CACHE_RADIANCE: r = c->cache.on ? P * 12 : 0; break;
+        case VRGI_BUF_CACHE_FALLBACK: r = c->cache.on ? (size_t)c->cache.tiles_x * c->cache.tiles_y : 0; break;
         case VRGI_BUF_CACHE_QUERIES:
         case VRGI_BUF_CACHE_TRAIN_RECORDS: {
             if (!c->cache.on) { r = 0; break; }
@@ -1180,6 +1217,7 @@ int vrgi_read_buffer(vrgi_ctx *c, int which, void *dst, size_t bytes) {
         case VRGI_BUF_DI_WEIGHTS: src = c->di_w.p; break;
         case VRGI_BUF_BOUNCE_ALBEDO: src = c->cache.albedo.p; break;
         case VRGI_BUF_CACHE_RADIANCE: src = c->cache.crad.p; break;
+        case VRGI_BUF_CACHE_FALLBACK: src = c->cache.t_fb.p; break;
         case VRGI_BUF_CACHE_QUERIES: src = c->cache.qrec.p; break;
         case VRGI_BUF_CACHE_TRAIN_RECORDS:
             src = c->cache.trec.p + (size_t)((c->frame_count & 1) ? 0 : 1) * c->cache.tcap * 16;
@@ -1378,6 +1416,10 @@ void vrgi_cache_config_default(vrgi_cache_config *cfg) {
     cfg->loss_scale = 128.0f;
     cfg->seed = 1;
     cfg->flags = 0;
+    cfg->query_vertex = 0;
+    cfg->fallback_bound = 0.0f;
+    cfg->fallback_tile = 16;
+    cfg->fallback_ema = 0.1f;
 }
 
 int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
@@ -1391,6 +1433,9 @@ int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
         return fail(VRGI_E_INVALID, "batch must be a multiple of 16 in [16, 2^20]");
     if (cfg->steps_per_frame > 64) return fail(VRGI_E_INVALID, "steps_per_frame above 64");
     if (cfg->query_vertex > 1) return fail(VRGI_E_INVALID, "query_vertex must be 0 or 1");
+    if (!(cfg->fallback_bound >= 0.0f) || cfg->fallback_tile == 0 || cfg->fallback_tile > 4096 ||
+        !(cfg->fallback_ema > 0.0f && cfg->fallback_ema <= 1.0f))
+        return fail(VRGI_E_INVALID, "fallback_bound >= 0, fallback_tile in [1, 4096], fallback_ema in (0, 1]");
     if (cfg->query_vertex + 1 >= cfg->train_bounces) return fail(VRGI_E_INVALID, "train_bounces too small");
     if (!(cfg->train_fraction >= 0.0f && cfg->train_fraction <= 1.0f))
         return fail(VRGI_E_INVALID, "train_fraction must be in [0, 1]");

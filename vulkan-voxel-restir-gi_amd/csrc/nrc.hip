@@ -401,6 +401,49 @@ __global__ __launch_bounds__(256) void k_nrc_zero(float *p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
 
+// per-tile sums (cache luminance, target luminance, count) of the records at `vertex`
+__global__ __launch_bounds__(256) void k_tile_stats(const float *rec, const uint32_t *count, uint32_t cap,
+                                                    const float *pred, uint32_t vertex, uint32_t W, uint32_t tile,
+                                                    uint32_t tiles_x, float *acc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = min(*count, cap);
+    if (i >= n) return;
+    const uint32_t key = __float_as_uint(rec[(size_t)i * 16 + 15]);
+    if ((key & 15u) != vertex) return;
+    const uint32_t px = key >> 4;
+    const uint32_t t = ((px / W) / tile) * tiles_x + (px % W) / tile;
+    const float *y = pred + (size_t)i * 3, *g = rec + (size_t)i * 16 + 12;
+    const float ly = 0.2126f * fmaxf(y[0], 0.0f) + 0.7152f * fmaxf(y[1], 0.0f) + 0.0722f * fmaxf(y[2], 0.0f);
+    const float lg = 0.2126f * g[0] + 0.7152f * g[1] + 0.0722f * g[2];
+    atomicAdd(acc + 3 * (size_t)t, ly);
+    atomicAdd(acc + 3 * (size_t)t + 1, lg);
+    atomicAdd(acc + 3 * (size_t)t + 2, 1.0f);
+}
+// running means per tile and the fallback flags they imply; clears the sums
+__global__ __launch_bounds__(256) void k_tile_update(float *acc, float *ema, uint8_t *fallback, uint32_t n_tiles,
+                                                     float alpha, float bound) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    const float c = acc[3 * (size_t)t + 2];
+    if (c > 0.0f) {
+        ema[2 * (size_t)t] = (1.0f - alpha) * ema[2 * (size_t)t] + alpha * (acc[3 * (size_t)t] / c);
+        ema[2 * (size_t)t + 1] = (1.0f - alpha) * ema[2 * (size_t)t + 1] + alpha * (acc[3 * (size_t)t + 1] / c);
+        const float e = fabsf(ema[2 * (size_t)t] - ema[2 * (size_t)t + 1]) / (ema[2 * (size_t)t + 1] + 0.01f);
+        fallback[t] = e > bound ? 1 : 0;
+    }
+    acc[3 * (size_t)t] = 0.0f;
+    acc[3 * (size_t)t + 1] = 0.0f;
+    acc[3 * (size_t)t + 2] = 0.0f;
+}
+void launch_tile_stats(hipStream_t s, const float *rec, const uint32_t *count, uint32_t cap, const float *pred,
+                       uint32_t vertex, uint32_t W, uint32_t tile, uint32_t tiles_x, float *acc) {
+    if (cap) k_tile_stats<<<(cap + 255) / 256, 256, 0, s>>>(rec, count, cap, pred, vertex, W, tile, tiles_x, acc);
+}
+void launch_tile_update(hipStream_t s, float *acc, float *ema, uint8_t *fallback, uint32_t n_tiles, float alpha,
+                        float bound) {
+    if (n_tiles) k_tile_update<<<(n_tiles + 255) / 256, 256, 0, s>>>(acc, ema, fallback, n_tiles, alpha, bound);
+}
+
 void launch_nrc_infer(hipStream_t s, const NrcNet &net, const float *rec, const uint32_t *count, uint32_t max_n,
                       float *out, float *out_px) {
     uint32_t blocks16 = (max_n + 15) / 16;

@@ -59,7 +59,8 @@ __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
     bool train = false, queried = false;
     const uint32_t cv = fa.cache_vertex;
     if (fa.cache_on) {
-        train = mm_finalizef(mm_combine(fa.train_seed, id)) < fa.train_frac;
+        train = mm_finalizef(mm_combine(fa.train_seed, id)) < fa.train_frac ||
+                in_fallback(fa.fallback, id, fa.W, fa.tile, fa.tiles_x);
         queried = !train && plen == cv + 1 && fa.hit[(size_t)cv * fa.P + id] != VRGI_INVALID;
     }
     for (int b = (int)plen - 1; b >= 0; --b) {

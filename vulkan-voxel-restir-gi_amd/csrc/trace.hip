@@ -362,7 +362,8 @@ __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const Bou
     bool train = false;
     uint32_t cap = ba.B;  // bounces this pixel's path may take
     if (CACHE) {
-        train = mm_finalizef(mm_combine(ba.train_seed, px)) < ba.train_frac;
+        train = mm_finalizef(mm_combine(ba.train_seed, px)) < ba.train_frac ||
+                in_fallback(ba.fallback, px, ba.W, ba.tile, ba.tiles_x);
         cap = train ? ba.B : ba.cache_vertex + 1u;
         if (active && b == 0) st3(ba.d0 + (size_t)px * 3, d);
     }

@@ -61,6 +61,8 @@ struct BounceArgs {
     uint32_t cache_on, train_seed, cache_vertex;  // cache_vertex: bounce of the query
     float train_frac;
     float *d0;              // [P][3] camera ray directions (incoming direction at vertex 0)
+    const uint8_t *fallback;  // per tile: trace full paths (null: none)
+    uint32_t tile, tiles_x;
     float *albedo;          // [B][P][3] material reflectivity at each hit
     float *qrec;            // [P][16] query records
     uint32_t *qrec_count;
@@ -99,6 +101,8 @@ struct FrameArgs {
     // learned radiance cache (see BounceArgs)
     uint32_t cache_on, train_seed, cache_vertex;
     float train_frac;
+    const uint8_t *fallback;
+    uint32_t tile, tiles_x;
     const float *dir, *d0, *albedo, *crad;  // crad: [P][3] cache radiance of queried pixels
     const uint32_t *hit;
     float *trec;                       // training records [tcap][16]
@@ -419,6 +423,15 @@ void launch_nrc_train_step(hipStream_t s, const NrcNet &net, const NrcTrain &tr,
 void launch_nrc_grads(hipStream_t s, const NrcNet &net, const NrcTrain &tr);
 void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb);
 void launch_nrc_zero(hipStream_t s, float *p, size_t n);
+// held-out error per screen tile: records of vertex `vertex` against the cache's output
+void launch_tile_stats(hipStream_t s, const float *rec, const uint32_t *count, uint32_t cap, const float *pred,
+                       uint32_t vertex, uint32_t W, uint32_t tile, uint32_t tiles_x, float *acc);
+void launch_tile_update(hipStream_t s, float *acc, float *ema, uint8_t *fallback, uint32_t n_tiles, float alpha,
+                        float bound);
+// a pixel of a fallback tile traces a full path, like a training pixel
+__device__ __forceinline__ bool in_fallback(const uint8_t *fb, uint32_t px, uint32_t W, uint32_t tile, uint32_t tx) {
+    return fb && fb[((px / W) / tile) * tx + (px % W) / tile];
+}
 
 // exposed-face meshing of 32^3 chunks (mesher.hip)
 struct MeshArgsHost {

@@ -23,7 +23,7 @@ assert VERTEX_DTYPE.itemsize == 24 and NODE_DTYPE.itemsize == 40 and MATERIAL_DT
  BUF_BOUNCE_REFLECTIVITY, BUF_BOUNCE_MIS_WEIGHT, BUF_BOUNCE_BSDF_PDF, BUF_BOUNCE_NEE_PDF,
  BUF_RADIANCE, BUF_RESTIR, BUF_TEMPORAL_RESERVOIR, BUF_SPATIAL_RESERVOIR, BUF_BGRA8,
  BUF_HIT_PRIM, BUF_HIT_T, BUF_DI_LIGHT, BUF_DI_WEIGHTS, BUF_BOUNCE_ALBEDO, BUF_CACHE_RADIANCE,
- BUF_CACHE_QUERIES, BUF_CACHE_TRAIN_RECORDS) = range(26)
+ BUF_CACHE_QUERIES, BUF_CACHE_TRAIN_RECORDS, BUF_CACHE_FALLBACK) = range(27)
 CACHE_FREEZE = 1
 CACHE_WEIGHTS = 4 * 64 * 64 + 64 * 16
 
@@ -55,7 +55,8 @@ class CacheConfig(C.Structure):
                 ("train_bounces", C.c_uint32), ("batch", C.c_uint32), ("steps_per_frame", C.c_uint32),
                 ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("loss_scale", C.c_float), ("seed", C.c_uint32), ("flags", C.c_uint32),
-                ("query_vertex", C.c_uint32)]
+                ("query_vertex", C.c_uint32), ("fallback_bound", C.c_float), ("fallback_tile", C.c_uint32),
+                ("fallback_ema", C.c_float)]
 
 
 class CacheInfo(C.Structure):
