@@ -218,6 +218,56 @@ def test_cache_frame_parity(world0, scenes, nee_type, spatial, qv, fb):
         assert info.steps == 6 and math.isfinite(info.last_loss)
 
 
+def test_cache_fallback_record_overflow(world0, scenes):
+    """More training records than the buffer holds (the producers keep counting the ones
+    they drop): the held-out pass reads only the records kept, and the per-tile flags
+    follow from them. The flags are recomputed here from the kept records, the weights
+    after the frame's training steps and a float32 running mean per tile."""
+    import vrgi
+    s, objs = scenes
+    W, H, TB, T, qv, bound, alpha = 96, 64, 5, 8, 0, 0.3, 0.1
+    cam = vrgi.camera_look([6.0, 62.0, 8.0], [40.0, 0.0, 44.0])
+    tx, ty = (W + T - 1) // T, (H + T - 1) // T
+    ema_p = np.zeros(tx * ty, np.float32)
+    ema_t = np.zeros(tx * ty, np.float32)
+    flags = np.zeros(tx * ty, np.uint8)
+    lum = np.array([0.2126, 0.7152, 0.0722], np.float32)
+    with _ctx(world0, W, H) as c:
+        c.add_objects(objs)
+        _enable(c, box_lo=(0, 0, 0), box_hi=(0, 0, 0), train_fraction=0.5, train_bounces=TB, batch=256,
+                steps_per_frame=1, seed=5, query_vertex=qv, fallback_bound=bound, fallback_tile=T,
+                fallback_ema=alpha)
+        compared = 0
+        for f in range(4):
+            say(f"fallback with record overflow, frame {f}")
+            c.render(cam, nee_type=1, spatial=2)
+            c.sync()
+            w, t = c.cache_get()
+            rec = c.read(vrgi.BUF_CACHE_TRAIN_RECORDS).reshape(-1, 16)
+            assert len(rec) == 256  # the capacity: far fewer than the frame's records
+            keys = rec[:, 15].view(np.uint32)
+            rec = rec[(keys & 15) == qv]
+            px = rec[:, 15].view(np.uint32) >> 4
+            assert px.max() < W * H
+            tile = (px // W // T) * tx + (px % W) // T
+            y = _oracle_net(c, w, t).forward(rec[:, 0:3], rec[:, 3:6], rec[:, 6:9], rec[:, 9:12])
+            ly = np.maximum(y, 0).astype(np.float32) @ lum
+            lg = rec[:, 12:15] @ lum
+            n = np.bincount(tile, minlength=tx * ty).astype(np.float32)
+            sp = np.bincount(tile, ly, minlength=tx * ty).astype(np.float32)
+            st = np.bincount(tile, lg, minlength=tx * ty).astype(np.float32)
+            has = n > 0
+            ema_p[has] = (1 - alpha) * ema_p[has] + alpha * (sp[has] / n[has])
+            ema_t[has] = (1 - alpha) * ema_t[has] + alpha * (st[has] / n[has])
+            e = np.abs(ema_p - ema_t) / (ema_t + 0.01)
+            flags[has] = e[has] > bound
+            g = c.read(vrgi.BUF_CACHE_FALLBACK, np.uint8)
+            clear = np.abs(e - bound) > 0.02 + 0.02 * e  # away from the bound by more than the net's error
+            assert np.array_equal(g[clear], flags[clear]), f"frame {f}"
+            compared += int(clear.sum())
+        assert compared > 0
+
+
 def test_cache_gain_over_plain(world0):
     """What the trained cache buys: on the 64^3 world at 128 x 128 (reference: the plain
     8-bounce tracer at 256 spp, FP64), after 64 frames of online training the cache's

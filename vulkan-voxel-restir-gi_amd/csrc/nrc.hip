@@ -176,12 +176,13 @@ __device__ __forceinline__ f32x4 layer_out(const uint16_t *wt5, const bf16x4 x[4
 // -------------------------------------------------------------------- inference
 // records: 16 floats (position, direction, normal, albedo, target, key); out: 3 floats
 // per record; when out_px is set, also out_px[3 * key + c] (negative outputs clamp to 0)
+// `count` may exceed the buffer's capacity `max_n` (producers count the records they drop)
 __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float *rec, const uint32_t *count,
-                                                   float *out, float *out_px) {
+                                                   uint32_t max_n, float *out, float *out_px) {
     const uint32_t lane = threadIdx.x & 63u, s = lane & 15u, g = lane >> 4;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t n = *count;
+    const uint32_t n = min(*count, max_n);
     const uint32_t n_blocks = (n + 15u) / 16u;
     for (uint32_t blk = wave; blk < n_blocks; blk += n_waves) {
         const uint32_t i = blk * 16u + s;
@@ -448,7 +449,7 @@ void launch_nrc_infer(hipStream_t s, const NrcNet &net, const float *rec, const 
                       float *out, float *out_px) {
     uint32_t blocks16 = (max_n + 15) / 16;
     uint32_t grid = std::min<uint32_t>((blocks16 + 3) / 4, 4096u);
-    if (grid) k_nrc_infer<<<grid, 256, 0, s>>>(net, rec, count, out, out_px);
+    if (grid) k_nrc_infer<<<grid, 256, 0, s>>>(net, rec, count, max_n, out, out_px);
 }
 
 void launch_nrc_train_step(hipStream_t s, const NrcNet &net, const NrcTrain &tr, float *w32, float *mw, float *vw,
