@@ -123,6 +123,10 @@ def main():
                     help="skip the learned-cache time-to-equal-RMSE study")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (pmc counter scaling)")
+    ap.add_argument("--serial-build", action="store_true",
+                    help="one BVH buffer set built on the frame stream (default: frame f + 1 builds beside frame f)")
+    ap.add_argument("--serial-nee", action="store_true",
+                    help="NEE pdf launches after all bounces on one stream (default: beside the next trace)")
     ap.add_argument("--sort-bits", type=int, default=0,
                     help="Morton ray-queue ordering: origin bits per axis (0 = compaction order)")
     a = ap.parse_args()
@@ -141,7 +145,9 @@ def main():
     objs = vrgi.chunk_objects(blocks, nt, tr)
     cams = [vrgi.camera_spline(1, k, 16) for k in range(16)]
     cfg = vrgi.config(W_PX, H_PX, BOUNCES, rr_start=RR_START, rr_min=0.1,
-                      flags=vrgi.FLAG_REBUILD_EVERY_FRAME, sort_bits=a.sort_bits)
+                      flags=vrgi.FLAG_REBUILD_EVERY_FRAME | (vrgi.FLAG_SERIAL_NEE if a.serial_nee else 0)
+                      | (vrgi.FLAG_SERIAL_BUILD if a.serial_build else 0),
+                      sort_bits=a.sort_bits)
     L = vrgi.lib()
     with vrgi.Context(cfg, mats, lum) as ctx:
         log("uploading objects (per-object light BVHs)")

@@ -84,6 +84,11 @@ typedef struct vrgi_config {
 
 #define VRGI_FLAG_REBUILD_EVERY_FRAME 1u /* rebuild the geometry BVH on every render() */
 #define VRGI_FLAG_KEEP_BOUNCES 2u        /* keep per-bounce buffers readable after render() */
+#define VRGI_FLAG_SERIAL_NEE 4u          /* NEE pdf launches after all bounces on the frame stream (default:
+                                           each bounce's on a second stream, beside the next bounce's trace) */
+#define VRGI_FLAG_SERIAL_BUILD 8u        /* one set of BVH buffers, built on the frame stream (default: two
+                                           sets; frame f + 1 builds on a second stream while frame f's rays
+                                           still read the other) */
 
 typedef struct vrgi_camera {
     float eye[3], front[3], right[3], up[3]; /* Camera::eye_front_right_up (camera.rs:115-122) */

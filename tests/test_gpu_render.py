@@ -72,22 +72,25 @@ def _camera0():
 NO_RR = 0xFFFFFFFF
 
 
-@pytest.mark.parametrize("nee_type,spatial,rr_start,sort_bits", [
-    (0, 0, NO_RR, 0), (1, 5, NO_RR, 0), (2, 3, NO_RR, 0),
+@pytest.mark.parametrize("nee_type,spatial,rr_start,sort_bits,serial", [
+    (0, 0, NO_RR, 0, False), (1, 5, NO_RR, 0, False), (2, 3, NO_RR, 0, False),
+    # NEE pdf launches after all bounces on the frame stream
+    (1, 5, NO_RR, 0, True), (1, 2, 1, 0, True),
     # Morton-ordered ray queues (with and without Russian roulette) give the same bits
-    (1, 5, NO_RR, 4), (1, 2, 1, 3), (0, 1, 1, 5)])
-def test_frames_parity(world0, scenes, nee_type, spatial, rr_start, sort_bits):
+    (1, 5, NO_RR, 4, False), (1, 2, 1, 3, False), (0, 1, 1, 5, False)])
+def test_frames_parity(world0, scenes, nee_type, spatial, rr_start, sort_bits, serial):
     import oracle
     import vrgi
     s, objs = scenes
     W, H, B = 96, 64, 3
     cam = _camera0()
     r = oracle.Renderer(W, H, B, rr_start=rr_start)
-    cfg = vrgi.config(W, H, B, rr_start=rr_start, flags=vrgi.FLAG_KEEP_BOUNCES, sort_bits=sort_bits)
+    cfg = vrgi.config(W, H, B, rr_start=rr_start, sort_bits=sort_bits,
+                      flags=vrgi.FLAG_KEEP_BOUNCES | (vrgi.FLAG_SERIAL_NEE if serial else 0))
     with vrgi.Context(cfg, world0["mats"], world0["lum"]) as c:
         c.add_objects(objs)
         for f in range(4):
-            say(f"frame {f} nee={nee_type} spatial={spatial} rr={rr_start} sort={sort_bits}")
+            say(f"frame {f} nee={nee_type} spatial={spatial} rr={rr_start} sort={sort_bits} serial={serial}")
             c.render(cam, nee_type=nee_type, spatial=spatial)
             c.sync()
             ref = r.render(s, cam, f, nee_type=nee_type, spatial=spatial)
@@ -117,6 +120,42 @@ def test_frames_parity(world0, scenes, nee_type, spatial, rr_start, sort_bits):
             assert c.read(vrgi.BUF_SPATIAL_RESERVOIR).tobytes() == ref["spatial"].tobytes()
             assert c.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
             assert c.read(vrgi.BUF_BGRA8, np.uint8).tobytes() == ref["bgra8"].tobytes()
+
+
+@pytest.mark.parametrize("serial_build", [False, True])
+def test_back_to_back_rebuild_frames(world0, scenes, serial_build):
+    """Frames issued without waiting, each rebuilding the BVHs. By default frame f + 1
+    builds into the other buffer set on a second stream while frame f's rays still read
+    the first. The FP64 running mean of the frames equals the mean of the oracle's
+    frames bit for bit (non-finite samples count as 0), and the last frame's reservoirs
+    and image are bit-exact."""
+    import oracle
+    import vrgi
+    s, objs = scenes
+    W, H, B, n = 96, 64, 3, 5
+    cam = _camera0()
+    r = oracle.Renderer(W, H, B)
+    flags = vrgi.FLAG_REBUILD_EVERY_FRAME | (vrgi.FLAG_SERIAL_BUILD if serial_build else 0)
+    say(f"{n} back-to-back frames with a rebuild each, serial build {serial_build}")
+    with vrgi.Context(vrgi.config(W, H, B, flags=flags), world0["mats"], world0["lum"]) as c:
+        c.add_objects(objs)
+        c.accum_enable(True)
+        for f in range(n):
+            c.render(cam, nee_type=1, spatial=5)
+        c.sync()
+        mean, frames = c.accum_read()
+        restir = c.read(vrgi.BUF_RESTIR)
+        bgra = c.read(vrgi.BUF_BGRA8, np.uint8)
+    assert frames == n
+    acc = np.zeros((W * H, 3), np.float64)
+    for f in range(n):
+        ref = r.render(s, cam, f, nee_type=1, spatial=5)
+        lo = ref["l_out"][0].reshape(-1, 3)
+        acc += np.where(np.isfinite(lo), lo, 0.0).astype(np.float64)
+    want = acc * (1.0 / n)
+    assert mean.tobytes() == want.tobytes()
+    assert restir.tobytes() == ref["restir"].tobytes()
+    assert bgra.tobytes() == ref["bgra8"].tobytes()
 
 
 def test_restir_di_parity(world0, scenes):

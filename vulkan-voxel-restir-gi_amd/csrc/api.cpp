@@ -83,6 +83,9 @@ struct vrgi_ctx {
     // learned-cache work of a frame that overlaps the training paths' bounces (default
     // priority, like every stream here); joined into `stream` before the frame ends
     hipStream_t side = nullptr;
+    hipStream_t nee_s = nullptr;  // NEE pdf of bounce b beside the trace of bounce b + 1
+    hipEvent_t ev_nee_done = nullptr;
+    std::vector<hipEvent_t> ev_bounce;  // trace (and sort) of bounce b done
     hipEvent_t ev_query = nullptr, ev_infer = nullptr, ev_train = nullptr, ev_accum = nullptr;
     std::mutex up_mu;
     vrgi_config cfg{};
@@ -106,20 +109,30 @@ struct vrgi_ctx {
     DevBuf<uint32_t> d_light_off;
     DevBuf<vrgi_bvh_node> d_tl;
     DevBuf<LPair> d_tlpairs;
-    DevBuf<uint32_t> d_lanc, d_lanc_len;  // per light triangle: ancestors for the NEE pdf
     uint32_t n_tl = 1;
     DevBuf<float> d_tl_prim9, d_tl_aabb, d_tl_xf, d_tl_power;
     DevBuf<uint32_t> d_tl_ids, d_tl_trees, d_tl_depth;
 
-    // geometry + light-geometry BVHs (one batched build)
-    DevBuf<Tri> d_tri, d_ltri;
-    DevBuf<uint32_t> d_tri_inst, d_ltri_inst, d_ltri_node;
-    DevBuf<float> d_prim9, d_power;
-    DevBuf<uint32_t> d_ids, d_trees, d_depth;
-    DevBuf<vrgi_bvh_node> d_nodes;
-    DevBuf<WideNode> d_gwide, d_lwide;
-    DevBuf<Wide4> d_lwide4, d_gwide4;
-    uint32_t geo_depth = 0, lgeo_depth = 0;
+    // geometry + light-geometry BVHs (one batched build) and everything derived from
+    // them. Two sets: while frame f's rays read one, frame f + 1 builds the other on
+    // `up_stream` (unless VRGI_FLAG_SERIAL_BUILD)
+    struct GeoSet {
+        DevBuf<Tri> d_tri, d_ltri;
+        DevBuf<uint32_t> d_tri_inst, d_ltri_inst, d_ltri_node;
+        DevBuf<uint32_t> d_depth;
+        DevBuf<vrgi_bvh_node> d_nodes;
+        DevBuf<WideNode> d_gwide, d_lwide;
+        DevBuf<Wide4> d_lwide4, d_gwide4;
+        DevBuf<uint32_t> d_lanc, d_lanc_len;  // per light triangle: ancestors for the NEE pdf
+        uint32_t geo_depth = 0, lgeo_depth = 0;
+        hipEvent_t ev_free = nullptr;  // recorded after the last frame that read the set
+        bool freed_once = false;
+    } geo[2];
+    uint32_t gi = 0;  // the set the next frame reads
+    GeoSet &G() { return geo[gi]; }
+    hipEvent_t ev_geo_ready = nullptr;
+    DevBuf<float> d_prim9, d_power;  // builder inputs (one build at a time)
+    DevBuf<uint32_t> d_ids, d_trees;
 
     // frame buffers
     DevBuf<float> f_origin, f_dir, f_normal, f_emis, f_refl, f_mis, f_pdf, f_nee, f_hit_t;
@@ -136,7 +149,12 @@ struct vrgi_ctx {
     // measurement
     uint32_t profile_flags = 0;
     vrgi_profile_out prof{};
-    std::vector<hipEvent_t> events;
+    // stage events of frames f % 2 == 0 / 1; a frame's events are read at the end of the
+    // next render (or by vrgi_profile_read), so that frames still overlap while profiled
+    std::vector<hipEvent_t> events[2];
+    bool pend[2] = {false, false}, pend_built[2] = {false, false};
+    uint32_t pend_P[2] = {0, 0};
+    uint32_t *h_qcount = nullptr;  // pinned, 2 x (B + 2) queue counts
     DevBuf<uint32_t> touch_bits;
     DevBuf<unsigned long long> touch_count;
     DevBuf<unsigned long long> depth_sum_copy;
@@ -187,27 +205,27 @@ void set_dev(vrgi_ctx *c) { HIP_CHECK(hipSetDevice(c->device)); }
 
 SceneView scene_view(vrgi_ctx *c) {
     SceneView sv{};
-    sv.gwide = c->d_gwide.p;
-    sv.gwide4 = c->d_gwide4.p;
-    sv.tri = c->d_tri.p;
+    sv.gwide = c->G().d_gwide.p;
+    sv.gwide4 = c->G().d_gwide4.p;
+    sv.tri = c->G().d_tri.p;
     sv.n_tris = c->n_tris;
     sv.root_prim = 0;
-    sv.depth = c->geo_depth;
-    sv.tri_inst = c->d_tri_inst.p;
+    sv.depth = c->G().geo_depth;
+    sv.tri_inst = c->G().d_tri_inst.p;
     sv.inst = c->d_inst.p;
     sv.n_inst = c->n_inst;
     sv.tl = c->d_tl.p;
-    sv.lwide = c->d_lwide.p;
-    sv.lwide4 = c->d_lwide4.p;
-    sv.ltri = c->d_ltri.p;
+    sv.lwide = c->G().d_lwide.p;
+    sv.lwide4 = c->G().d_lwide4.p;
+    sv.ltri = c->G().d_ltri.p;
     sv.n_ltri = c->n_ltri;
     sv.lroot_prim = 0;
-    sv.ldepth = c->lgeo_depth;
-    sv.ltri_inst = c->d_ltri_inst.p;
-    sv.ltri_node = c->d_ltri_node.p;
+    sv.ldepth = c->G().lgeo_depth;
+    sv.ltri_inst = c->G().d_ltri_inst.p;
+    sv.ltri_node = c->G().d_ltri_node.p;
     sv.tlpairs = c->d_tlpairs.p;
-    sv.lanc = c->d_lanc.p;
-    sv.lanc_len = c->d_lanc_len.p;
+    sv.lanc = c->G().d_lanc.p;
+    sv.lanc_len = c->G().d_lanc_len.p;
     sv.mats = c->d_mats.p;
     sv.n_mats = (uint32_t)c->mats.size();
     return sv;
@@ -230,7 +248,7 @@ void cache_box_from_scene(vrgi_ctx *c) {
     if (C.box_ready) return;
     if (c->n_tris >= 2) {
         WideNode w;
-        HIP_CHECK(hipMemcpyAsync(&w, c->d_gwide.p, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipMemcpyAsync(&w, c->G().d_gwide.p, sizeof(w), hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
         const float lo[3] = {std::min(w.a.x, w.b.z), std::min(w.a.y, w.b.w), std::min(w.a.z, w.c.x)};
         const float hi[3] = {std::max(w.a.w, w.c.y), std::max(w.b.x, w.c.z), std::max(w.b.y, w.c.w)};
@@ -337,6 +355,9 @@ void alloc_frame(vrgi_ctx *c) {
 
 // instance table, per-instance pointers, light TL-BVH (scene.rs:243-338)
 void rebuild_instances(vrgi_ctx *c) {
+    // instance data is read by the builds and rays of frames still in flight
+    HIP_CHECK(hipStreamSynchronize(c->up_stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
     c->inst_objs.clear();
     std::vector<DevInst> hi;
     std::vector<const vrgi_vertex3d *> vp;
@@ -421,28 +442,72 @@ void rebuild_instances(vrgi_ctx *c) {
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
+void build_geometry_on(vrgi_ctx *c, hipStream_t s);
+
+// adds a rendered frame's stage times (events of parity k) to the profile output
+void harvest_frame(vrgi_ctx *c, uint32_t k) {
+    if (!c->pend[k]) return;
+    const size_t B = c->B;
+    std::vector<hipEvent_t> &ev = c->events[k];
+    HIP_CHECK(hipEventSynchronize(ev[2 + 2 * B]));
+    auto el = [&](size_t a, size_t b) {
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev[a], ev[b]));
+        return (double)ms;
+    };
+    vrgi_profile_out &p = c->prof;
+    const uint32_t *q = c->h_qcount + (size_t)k * (B + 2);
+    if (c->pend_built[k]) p.build_ms += el(3 + 3 * B, 4 + 3 * B);
+    for (size_t b = 0; b < B && b < 16; ++b) {
+        p.trace_ms[b] += el(b == 0 ? 1 : 1 + b, 2 + b);
+        if (b + 1 < B) p.nee_ms[b] += el(3 + 2 * B + b, 2 + B + b);
+        p.rays[b] += b == 0 ? c->pend_P[k] : q[b];
+    }
+    p.resolve_ms += el(2 + B + (B - 1), 2 + 2 * B);
+    p.frame_ms += el(0, 2 + 2 * B);
+    p.frames += 1;
+    c->pend[k] = false;
+}
+
 // world triangles + world light triangles, geometry and light-geometry BVHs in one
 // batched build, traversal copies
-void rebuild_geometry(vrgi_ctx *c) {
-    hipStream_t s = c->stream;
+void rebuild_geometry(vrgi_ctx *c, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const bool pipe = !(c->cfg.flags & VRGI_FLAG_SERIAL_BUILD);
+    if (pipe) {
+        // the other set; the frame that read it last must have ended (its buffers may
+        // be reallocated below)
+        c->gi ^= 1u;
+        if (c->G().freed_once) HIP_CHECK(hipEventSynchronize(c->G().ev_free));
+    }
+    hipStream_t s = pipe ? c->up_stream : c->stream;
+    if (e0) HIP_CHECK(hipEventRecord(e0, s));
+    build_geometry_on(c, s);
+    if (e1) HIP_CHECK(hipEventRecord(e1, s));
+    if (pipe) {
+        HIP_CHECK(hipEventRecord(c->ev_geo_ready, s));
+        HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_geo_ready, 0));
+    }
+}
+
+void build_geometry_on(vrgi_ctx *c, hipStream_t s) {
     uint32_t nt = c->n_tris, nl = c->n_ltri, N = nt + nl;
-    c->d_tri.reserve(nt);
-    c->d_tri_inst.reserve(nt);
-    c->d_ltri.reserve(nl);
-    c->d_ltri_inst.reserve(nl);
-    c->d_ltri_node.reserve(nl);
+    c->G().d_tri.reserve(nt);
+    c->G().d_tri_inst.reserve(nt);
+    c->G().d_ltri.reserve(nl);
+    c->G().d_ltri_inst.reserve(nl);
+    c->G().d_ltri_node.reserve(nl);
     c->d_prim9.reserve((size_t)9 * N);
     c->d_power.reserve(N);
     c->d_ids.reserve(N);
     uint32_t gnodes = nt ? 2 * nt - 1 : 0, lnodes = nl ? 2 * nl - 1 : 0;
-    c->d_nodes.reserve(gnodes + lnodes + 1);
-    c->d_gwide.reserve(nt > 1 ? nt - 1 : 1);
-    c->d_lwide.reserve(nl > 1 ? nl - 1 : 1);
-    c->d_lwide4.reserve(nl > 1 ? nl - 1 : 1);
-    if (VRGI_GEOM4) c->d_gwide4.reserve(nt > 1 ? nt - 1 : 1);
+    c->G().d_nodes.reserve(gnodes + lnodes + 1);
+    c->G().d_gwide.reserve(nt > 1 ? nt - 1 : 1);
+    c->G().d_lwide.reserve(nl > 1 ? nl - 1 : 1);
+    c->G().d_lwide4.reserve(nl > 1 ? nl - 1 : 1);
+    if (VRGI_GEOM4) c->G().d_gwide4.reserve(nt > 1 ? nt - 1 : 1);
     c->d_trees.reserve(6);
-    c->d_depth.reserve(2);
-    c->geo_depth = c->lgeo_depth = 0;
+    c->G().d_depth.reserve(2);
+    c->G().geo_depth = c->G().lgeo_depth = 0;
     if (N == 0) return;
     FlattenArgs f{};
     f.inst = c->d_inst.p;
@@ -452,8 +517,8 @@ void rebuild_geometry(vrgi_ctx *c) {
     f.light_off = c->d_light_off.p;
     f.prim_stride = N;
     f.n_tris = nt;
-    f.tri = c->d_tri.p;
-    f.tri_inst = c->d_tri_inst.p;
+    f.tri = c->G().d_tri.p;
+    f.tri_inst = c->G().d_tri_inst.p;
     f.prim9 = c->d_prim9.p;
     f.power = c->d_power.p;
     f.ids = c->d_ids.p;
@@ -461,15 +526,15 @@ void rebuild_geometry(vrgi_ctx *c) {
     launch_flatten(s, f);
     FlattenArgs fl = f;
     fl.n_tris = nl;
-    fl.tri = c->d_ltri.p;
-    fl.tri_inst = c->d_ltri_inst.p;
-    fl.light_node = c->d_ltri_node.p;
+    fl.tri = c->G().d_ltri.p;
+    fl.tri_inst = c->G().d_ltri_inst.p;
+    fl.light_node = c->G().d_ltri_node.p;
     fl.prim_off = nt;
     launch_flatten_lights(s, fl);
-    c->d_lanc.reserve((size_t)nl * VRGI_LANC_MAX);
-    c->d_lanc_len.reserve(nl);
-    launch_light_ancestry(s, c->d_ltri_inst.p, c->d_ltri_node.p, c->d_inst.p, c->d_tl.p, nl, c->d_lanc.p,
-                          c->d_lanc_len.p);
+    c->G().d_lanc.reserve((size_t)nl * VRGI_LANC_MAX);
+    c->G().d_lanc_len.reserve(nl);
+    launch_light_ancestry(s, c->G().d_ltri_inst.p, c->G().d_ltri_node.p, c->d_inst.p, c->d_tl.p, nl, c->G().d_lanc.p,
+                          c->G().d_lanc_len.p);
     uint32_t trees[6];
     uint32_t nt_trees = 0;
     uint32_t beg[2], cnt[2], base[2];
@@ -478,21 +543,27 @@ void rebuild_geometry(vrgi_ctx *c) {
     for (uint32_t t = 0; t < nt_trees; ++t) { trees[t] = beg[t]; trees[2 + t] = cnt[t]; trees[4 + t] = base[t]; }
     HIP_CHECK(hipMemcpyAsync(c->d_trees.p, trees, 24, hipMemcpyHostToDevice, s));
     if (c->geo_builder.build(s, c->d_prim9.p, c->d_power.p, c->d_ids.p, N, nt_trees, c->d_trees.p,
-                             c->d_trees.p + 2, c->d_trees.p + 4, base, c->d_nodes.p, c->d_depth.p,
+                             c->d_trees.p + 2, c->d_trees.p + 4, base, c->G().d_nodes.p, c->G().d_depth.p,
                              nullptr) != 0)
         throw HipError("geometry build exceeded its workspace");
     uint32_t dep[2] = {0, 0};
-    HIP_CHECK(hipMemcpyAsync(dep, c->d_depth.p, nt_trees * 4, hipMemcpyDeviceToHost, s));
+    unsigned long long S = 0;
+    HIP_CHECK(hipMemcpyAsync(dep, c->G().d_depth.p, nt_trees * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&S, c->geo_builder.leaf_depth_sum(), 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    // structural figures of the build (profile output)
+    c->prof.build_leaf_depth_sum = S;
+    c->prof.build_prims = (uint64_t)nt + nl;
+    c->prof.build_nodes = (uint64_t)gnodes + lnodes;
     uint32_t t = 0;
-    if (nt) c->geo_depth = dep[t++];
-    if (nl) c->lgeo_depth = dep[t++];
-    if (geom_stack_need(c->geo_depth) + 2 > VRGI_MAX_STACK || 3 * ((c->lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
+    if (nt) c->G().geo_depth = dep[t++];
+    if (nl) c->G().lgeo_depth = dep[t++];
+    if (geom_stack_need(c->G().geo_depth) + 2 > VRGI_MAX_STACK || 3 * ((c->G().lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
         throw std::runtime_error("BVH deeper than the traversal stack");
-    launch_wide(s, c->d_nodes.p, gnodes, c->d_gwide.p);
-    if (VRGI_GEOM4) launch_wide4(s, c->d_nodes.p, gnodes, c->d_gwide4.p);
-    launch_wide(s, c->d_nodes.p + gnodes, lnodes, c->d_lwide.p);
-    launch_wide4(s, c->d_nodes.p + gnodes, lnodes, c->d_lwide4.p);
+    launch_wide(s, c->G().d_nodes.p, gnodes, c->G().d_gwide.p);
+    if (VRGI_GEOM4) launch_wide4(s, c->G().d_nodes.p, gnodes, c->G().d_gwide4.p);
+    launch_wide(s, c->G().d_nodes.p + gnodes, lnodes, c->G().d_lwide.p);
+    launch_wide4(s, c->G().d_nodes.p + gnodes, lnodes, c->G().d_lwide4.p);
 }
 
 void free_object(vrgi_object *o) {
@@ -523,15 +594,24 @@ bool valid_xf(const float *xf) {
 vrgi_ctx::~vrgi_ctx() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (up_stream) (void)hipStreamSynchronize(up_stream);
+    for (int k = 0; k < 2; ++k)
+        for (hipEvent_t e : events[k]) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; ++k)
+        if (geo[k].ev_free) (void)hipEventDestroy(geo[k].ev_free);
+    if (ev_geo_ready) (void)hipEventDestroy(ev_geo_ready);
+    if (h_qcount) (void)hipHostFree(h_qcount);
     for (auto &kv : objects) release(kv.second.first);
     for (vrgi_object *o : old) release(o);
     if (stream) (void)hipStreamDestroy(stream);
     if (up_stream) (void)hipStreamDestroy(up_stream);
     if (side) (void)hipStreamSynchronize(side);
     if (side) (void)hipStreamDestroy(side);
-    for (hipEvent_t e : {ev_query, ev_infer, ev_train, ev_accum})
+    if (nee_s) (void)hipStreamSynchronize(nee_s);
+    if (nee_s) (void)hipStreamDestroy(nee_s);
+    for (hipEvent_t e : {ev_query, ev_infer, ev_train, ev_accum, ev_nee_done})
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_bounce) (void)hipEventDestroy(e);
 }
 
 extern "C" {
@@ -575,6 +655,10 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&c->nee_s, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_nee_done, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_geo_ready, hipEventDisableTiming));
+        for (int k = 0; k < 2; ++k) HIP_CHECK(hipEventCreateWithFlags(&c->geo[k].ev_free, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_query, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_infer, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_train, hipEventDisableTiming));
@@ -820,17 +904,32 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     const uint32_t P = c->P, B = c->B;
     const bool stages = (c->profile_flags & VRGI_PROFILE_STAGES) != 0;
     const bool touch = (c->profile_flags & VRGI_PROFILE_TOUCH) != 0;
-    // events: 0 start, 1 after the builds, 2+b after trace b, 2+B+b after NEE b, 2+2B end
-    const size_t n_ev = 3 + 2 * (size_t)B;
-    if (stages)
-        while (c->events.size() < n_ev) {
+    // events: 0 start, 1 after the builds, 2+b after trace b, 2+B+b after NEE b (2+2B-1:
+    // NEE joined), 2+2B end, 3+2B+b before NEE b, 3+3B / 4+3B around the build (on the
+    // stream it runs on)
+    const uint32_t par = (uint32_t)(c->frame_count & 1);
+    const size_t n_ev = 5 + 3 * (size_t)B;
+    if (stages) {
+        harvest_frame(c, par);  // a frame two renders back, if it is still unread
+        while (c->events[par].size() < n_ev) {
             hipEvent_t e;
             HIP_CHECK(hipEventCreate(&e));
-            c->events.push_back(e);
+            c->events[par].push_back(e);
         }
-    auto rec = [&](size_t k) {
-        if (stages) HIP_CHECK(hipEventRecord(c->events[k], s));
+    }
+    auto rec_on = [&](size_t k, hipStream_t st) {
+        if (stages) HIP_CHECK(hipEventRecord(c->events[par][k], st));
     };
+    auto rec = [&](size_t k) { rec_on(k, s); };
+    // NEE pdf of bounce b needs only bounce b's outgoing rays (slot b, queue b + 1) and
+    // writes only its slot of the pdf, while the trace of bounce b + 1 reads slot b and
+    // writes slot b + 1: the two run side by side
+    const bool nee_overlap = prefs->nee_type != 0 && !(c->cfg.flags & VRGI_FLAG_SERIAL_NEE);
+    while (c->ev_bounce.size() < B) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->ev_bounce.push_back(e);
+    }
     rec(0);
     bool built = false;
     if (c->inst_dirty) {
@@ -839,7 +938,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         c->geo_dirty = true;
     }
     if (c->geo_dirty || (c->cfg.flags & VRGI_FLAG_REBUILD_EVERY_FRAME)) {
-        rebuild_geometry(c);
+        rebuild_geometry(c, stages ? c->events[par][3 + 3 * (size_t)B] : nullptr,
+                         stages ? c->events[par][4 + 3 * (size_t)B] : nullptr);
         c->geo_dirty = false;
         built = true;
     }
@@ -849,7 +949,6 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     const bool cache_on = c->cache.on;
     // learned cache: training records of frame f go to half f % 2 (count word 0 or 3);
     // the side stream trains on the other half (frame f - 1) while this frame traces
-    const uint32_t par = (uint32_t)(c->frame_count & 1);
     float *trec_cur = c->cache.trec.p + (size_t)par * c->cache.tcap * 16;
     float *trec_prev = c->cache.trec.p + (size_t)(par ^ 1) * c->cache.tcap * 16;
     uint32_t *tcnt_cur = c->cache.counts.p + (par ? 3 : 0), *tcnt_prev = c->cache.counts.p + (par ? 0 : 3);
@@ -936,7 +1035,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         if (sort_rays && b + 1 < B) {
             SortArgs sa{};
             sa.bounce = b; sa.P = P; sa.bits = c->cfg.ray_sort_bits;
-            sa.root = c->d_gwide.p;
+            sa.root = c->G().d_gwide.p;
             sa.origin = c->f_origin.p; sa.dir = c->f_dir.p;
             sa.count = c->f_qcount.p + b + 1;
             sa.queue_in = ba.queue_out;
@@ -950,15 +1049,31 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             launch_popcount(s, ba.touch_tris, touch_words, c->touch_count.p + 2 * b + 1);
         }
         rec(2 + b);
+        if (nee_overlap && b + 1 < B) {  // interactive_rendering.rs:1206
+            HIP_CHECK(hipEventRecord(c->ev_bounce[b], s));
+            HIP_CHECK(hipStreamWaitEvent(c->nee_s, c->ev_bounce[b], 0));
+            BounceArgs na = ba;
+            na.queue_in = qin[b + 1];
+            na.touch_nodes = na.touch_tris = nullptr;
+            rec_on(3 + 2 * (size_t)B + b, c->nee_s);
+            launch_nee_pdf(c->nee_s, sv, na, P);
+            rec_on(2 + B + b, c->nee_s);
+        }
     }
     ba.touch_nodes = ba.touch_tris = nullptr;
-    for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
-        if (prefs->nee_type != 0) {
-            ba.bounce = b;
-            ba.queue_in = qin[b + 1];
-            launch_nee_pdf(s, sv, ba, P);
+    if (nee_overlap) {
+        HIP_CHECK(hipEventRecord(c->ev_nee_done, c->nee_s));
+        HIP_CHECK(hipStreamWaitEvent(s, c->ev_nee_done, 0));
+    } else {
+        for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
+            rec(3 + 2 * (size_t)B + b);
+            if (prefs->nee_type != 0) {
+                ba.bounce = b;
+                ba.queue_in = qin[b + 1];
+                launch_nee_pdf(s, sv, ba, P);
+            }
+            rec(2 + B + b);
         }
-        rec(2 + B + b);
     }
     if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_infer, 0));  // cache radiance is ready
     rec(2 + B + (B - 1));
@@ -1037,42 +1152,29 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     }
     if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_train, 0));  // join the side stream
     launch_frame_stats(s, c->f_qcount.p, B, P, (unsigned long long *)c->d_counters.p);
+    // the set this frame read may be rebuilt once this point has passed
+    HIP_CHECK(hipEventRecord(c->G().ev_free, s));
+    c->G().freed_once = true;
+    if (stages) {
+        if (!c->h_qcount) HIP_CHECK(hipHostMalloc((void **)&c->h_qcount, 2 * ((size_t)B + 2) * 4, hipHostMallocDefault));
+        HIP_CHECK(hipMemcpyAsync(c->h_qcount + (size_t)par * (B + 2), c->f_qcount.p, ((size_t)B + 2) * 4,
+                                 hipMemcpyDeviceToHost, s));
+    }
     rec(2 + 2 * (size_t)B);
-    if (stages || touch) {
+    if (stages) {
+        c->pend[par] = true;
+        c->pend_built[par] = built;
+        c->pend_P[par] = P;
+        harvest_frame(c, par ^ 1u);  // the previous frame: its work ran beside this one's build
+    }
+    if (touch) {
         HIP_CHECK(hipStreamSynchronize(s));
-        std::vector<uint32_t> q(B + 2);
-        HIP_CHECK(hipMemcpy(q.data(), c->f_qcount.p, (B + 2) * 4, hipMemcpyDeviceToHost));
         vrgi_profile_out &p = c->prof;
-        if (stages) {
-            auto el = [&](size_t a, size_t b) {
-                float ms = 0.0f;
-                HIP_CHECK(hipEventElapsedTime(&ms, c->events[a], c->events[b]));
-                return (double)ms;
-            };
-            if (built) p.build_ms += el(0, 1);
-            for (uint32_t b = 0; b < B && b < 16; ++b) {
-                p.trace_ms[b] += el(b == 0 ? 1 : 1 + b, 2 + b);
-                if (b + 1 < B) p.nee_ms[b] += el(b == 0 ? 1 + B : 1 + B + b, 2 + B + b);
-                p.rays[b] += b == 0 ? P : q[b];
-            }
-            p.resolve_ms += el(2 + B + (B - 1), 2 + 2 * (size_t)B);
-            p.frame_ms += el(0, 2 + 2 * (size_t)B);
-            p.frames += 1;
-        }
-        if (touch) {
-            std::vector<unsigned long long> tc(2 * (size_t)B);
-            HIP_CHECK(hipMemcpy(tc.data(), c->touch_count.p, 16 * (size_t)B, hipMemcpyDeviceToHost));
-            for (uint32_t b = 0; b < B && b < 16; ++b)
-                p.touched_bytes[b] = tc[2 * b] * VRGI_GEOM_NODE_BYTES + tc[2 * b + 1] * 48ull;
-            c->profile_flags &= ~VRGI_PROFILE_TOUCH;
-        }
-        if (built) {
-            unsigned long long S = 0;
-            HIP_CHECK(hipMemcpy(&S, c->geo_builder.leaf_depth_sum(), 8, hipMemcpyDeviceToHost));
-            p.build_leaf_depth_sum = S;
-            p.build_prims = (uint64_t)c->n_tris + c->n_ltri;
-            p.build_nodes = (c->n_tris ? 2ull * c->n_tris - 1 : 0) + (c->n_ltri ? 2ull * c->n_ltri - 1 : 0);
-        }
+        std::vector<unsigned long long> tc(2 * (size_t)B);
+        HIP_CHECK(hipMemcpy(tc.data(), c->touch_count.p, 16 * (size_t)B, hipMemcpyDeviceToHost));
+        for (uint32_t b = 0; b < B && b < 16; ++b)
+            p.touched_bytes[b] = tc[2 * b] * VRGI_GEOM_NODE_BYTES + tc[2 * b + 1] * 48ull;
+        c->profile_flags &= ~VRGI_PROFILE_TOUCH;
     }
     c->have_frame = true;
     if (out) {
@@ -1106,7 +1208,7 @@ int vrgi_stats(vrgi_ctx *c, vrgi_frame_stats *st, int reset) {
     st->rays_closest = h.rays_closest;
     st->rays_light = h.rays_light;
     st->bvh_nodes = c->n_tris ? 2ull * c->n_tris - 1 : 0;
-    st->bvh_depth = c->geo_depth;
+    st->bvh_depth = c->G().geo_depth;
     st->pad = 0;
     if (reset) {
         HIP_CHECK(hipMemsetAsync(c->d_counters.p, 0, sizeof(Counters), c->stream));
@@ -1117,14 +1219,23 @@ int vrgi_stats(vrgi_ctx *c, vrgi_frame_stats *st, int reset) {
 }
 
 int vrgi_profile(vrgi_ctx *c, uint32_t flags) {
+    API_BEGIN
     if (!c) return fail(VRGI_E_INVALID, "null context");
     if (flags & ~(VRGI_PROFILE_STAGES | VRGI_PROFILE_TOUCH)) return fail(VRGI_E_INVALID, "unknown profile flag");
+    set_dev(c);
+    harvest_frame(c, 0);
+    harvest_frame(c, 1);
     c->profile_flags = flags;
     return VRGI_OK;
+    API_END
 }
 
 int vrgi_profile_read(vrgi_ctx *c, vrgi_profile_out *out, int reset) {
+    API_BEGIN
     if (!c || !out) return fail(VRGI_E_INVALID, "null argument");
+    set_dev(c);
+    harvest_frame(c, 0);
+    harvest_frame(c, 1);
     *out = c->prof;
     if (reset) {
         vrgi_profile_out keep = c->prof;
@@ -1136,6 +1247,7 @@ int vrgi_profile_read(vrgi_ctx *c, vrgi_profile_out *out, int reset) {
         std::memcpy(c->prof.touched_bytes, keep.touched_bytes, sizeof(keep.touched_bytes));
     }
     return VRGI_OK;
+    API_END
 }
 
 int vrgi_buffer_size(vrgi_ctx *c, int which, size_t *bytes) {
@@ -1189,10 +1301,10 @@ int vrgi_read_buffer(vrgi_ctx *c, int which, void *dst, size_t bytes) {
     const void *src = nullptr;
     size_t P = c->P, nt = c->n_tris;
     switch (which) {
-        case VRGI_BUF_GEOM_BVH: src = c->d_nodes.p; break;
+        case VRGI_BUF_GEOM_BVH: src = c->G().d_nodes.p; break;
         case VRGI_BUF_GEOM_PRIM_ORDER: src = c->geo_builder.prim_order(); break;
         case VRGI_BUF_LIGHT_TL_BVH: src = c->d_tl.p; break;
-        case VRGI_BUF_LIGHT_GEOM_BVH: src = c->d_nodes.p + (nt ? 2 * nt - 1 : 0); break;
+        case VRGI_BUF_LIGHT_GEOM_BVH: src = c->G().d_nodes.p + (nt ? 2 * nt - 1 : 0); break;
         case VRGI_BUF_PATH_LEN: src = c->f_plen.p; break;
         case VRGI_BUF_BOUNCE_ORIGIN: src = c->f_origin.p; break;
         case VRGI_BUF_BOUNCE_DIRECTION: src = c->f_dir.p; break;

@@ -29,6 +29,8 @@ CACHE_WEIGHTS = 4 * 64 * 64 + 64 * 16
 
 FLAG_REBUILD_EVERY_FRAME = 1
 FLAG_KEEP_BOUNCES = 2
+FLAG_SERIAL_NEE = 4
+FLAG_SERIAL_BUILD = 8
 
 
 class Config(C.Structure):
