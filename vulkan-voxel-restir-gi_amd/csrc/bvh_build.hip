@@ -137,9 +137,10 @@ __global__ void k_roots(BuildArgs a, BuildState s) {
 }
 
 // ---------------------------------------------------------------- top phase
-// items for the open nodes of this level + reset of their bounds / bins
+// items for the open nodes of this level: a block-wide scan of ceil(count / CHUNK) per
+// node, then every thread writes items of any node (found by binary search over the
+// inclusive scan), so a node with many chunks is not written by one thread
 __global__ void k_items(BuildState s) {
-    // single block: scan over open nodes of ceil(count / CHUNK)
     __shared__ uint32_t s_sum[1024];
     const OpenNode *open = (const OpenNode *)s.open;
     uint32_t n_open = *s.n_open;
@@ -158,17 +159,26 @@ __global__ void k_items(BuildState s) {
         uint32_t excl = carry + s_sum[threadIdx.x] - c;
         if (i < n_open) {
             s.item_beg[i] = excl;
-            for (uint32_t k = 0; k < c; ++k) {
-                Item it;
-                it.open = i;
-                it.beg = open[i].first + k * CHUNK;
-                it.len = min((uint32_t)CHUNK, open[i].count - k * CHUNK);
-                it.pad = 0;
-                ((Item *)s.items)[excl + k] = it;
-            }
             s.item_end[i] = excl + c;
         }
-        carry += s_sum[1023];
+        const uint32_t total = s_sum[1023];
+        for (uint32_t j = threadIdx.x; j < total; j += 1024) {
+            uint32_t lo = 0, hi = 1023;  // first t with s_sum[t] > j
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_sum[mid] > j) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint32_t k = j - (lo ? s_sum[lo - 1] : 0u);
+            const OpenNode on = open[base + lo];
+            Item it;
+            it.open = base + lo;
+            it.beg = on.first + k * CHUNK;
+            it.len = min((uint32_t)CHUNK, on.count - k * CHUNK);
+            it.pad = 0;
+            ((Item *)s.items)[carry + j] = it;
+        }
+        carry += total;
         __syncthreads();
     }
     if (threadIdx.x == 0) *s.n_items = carry;
@@ -475,42 +485,93 @@ __global__ __launch_bounds__(TOP_THREADS) void k_part_b(BuildArgs a, BuildState 
 }
 
 // children of each open node: node records, leaves, pairs, jobs, next open nodes
+// slot in a list for every lane with `take`: one atomic per wave (all 64 lanes call it)
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool take) {
+    const uint64_t m = __ballot(take);
+    if (!m) return 0;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int lead = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == lead) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, lead, 64);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+
+// children of the open nodes: node records, leaves and pairs, and the next level's open
+// nodes and subtree jobs. List order is free (ranks come from counts), so list slots,
+// depth maxima and split sums take one atomic per wave.
 __global__ void k_children(BuildArgs a, BuildState s) {
-    uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= *s.n_open) return;
-    const OpenNode on = ((const OpenNode *)s.open)[o];
-    const SplitInfo si = ((const SplitInfo *)s.split)[o];
-    uint32_t nl = si.fallback ? on.count / 2 : si.T;
-    uint32_t nr = on.count - nl;
-    uint32_t base = a.tree_base[on.tree];
-    const float pad = 0.0001f;
-    float mn[3], mx[3];
-    for (int k = 0; k < 3; ++k) { mn[k] = funkey(si.box[k]) - pad; mx[k] = funkey(si.box[3 + k]) + pad; }
-    uint32_t li = 2 * on.rank + 1, ri = 2 * on.rank + 2;
-    write_node(a.nodes + base + on.node, li, ri, mn, 0.0f, mx, on.parent);
-    uint32_t ln = atomicAdd(s.n_level_nodes, 1u);
-    s.level_nodes[2 * ln] = base + on.node;
-    s.level_nodes[2 * ln + 1] = base;
-    atomicMax(a.depth + on.tree, on.depth + 1);
-    unsigned long long splits = on.count;
-    for (int side = 0; side < 2; ++side) {
-        uint32_t cf = side ? on.first + nl : on.first;
-        uint32_t cc = side ? nr : nl;
-        uint32_t cr = side ? on.rank + nl : on.rank + 1;
-        uint32_t ci = side ? ri : li;
-        if (cc == 1) {
-            emit_leaf(a, base + ci, on.node, a.perm[cf]);
-        } else if (cc == 2) {
-            emit_pair(a, base, ci, cr, on.node, a.perm[cf], a.perm[cf + 1]);
-            atomicMax(a.depth + on.tree, on.depth + 2);
-            splits += 2;
-        } else {
-            OpenNode ch{cf, cc, cr, ci, on.node, on.tree, on.depth + 1, 0};
-            if (cc <= (uint32_t)a.sub_max) ((OpenNode *)s.jobs)[atomicAdd(s.n_jobs, 1u)] = ch;
-            else ((OpenNode *)s.open_next)[atomicAdd(s.n_open_next, 1u)] = ch;
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_open = *s.n_open;
+    if (blockIdx.x * blockDim.x >= n_open) return;  // the whole block (uniform)
+    const bool live = o < n_open;
+    OpenNode on{};
+    uint32_t base = 0, dep = 0;
+    unsigned long long splits = 0;
+    uint32_t kind[2] = {0u, 0u};  // 1 = subtree job, 2 = open node
+    OpenNode ch[2];
+    if (live) {
+        on = ((const OpenNode *)s.open)[o];
+        const SplitInfo si = ((const SplitInfo *)s.split)[o];
+        uint32_t nl = si.fallback ? on.count / 2 : si.T;
+        uint32_t nr = on.count - nl;
+        base = a.tree_base[on.tree];
+        const float pad = 0.0001f;
+        float mn[3], mx[3];
+        for (int k = 0; k < 3; ++k) { mn[k] = funkey(si.box[k]) - pad; mx[k] = funkey(si.box[3 + k]) + pad; }
+        uint32_t li = 2 * on.rank + 1, ri = 2 * on.rank + 2;
+        write_node(a.nodes + base + on.node, li, ri, mn, 0.0f, mx, on.parent);
+        dep = on.depth + 1;
+        splits = on.count;
+        for (int side = 0; side < 2; ++side) {
+            uint32_t cf = side ? on.first + nl : on.first;
+            uint32_t cc = side ? nr : nl;
+            uint32_t cr = side ? on.rank + nl : on.rank + 1;
+            uint32_t ci = side ? ri : li;
+            if (cc == 1) {
+                emit_leaf(a, base + ci, on.node, a.perm[cf]);
+            } else if (cc == 2) {
+                emit_pair(a, base, ci, cr, on.node, a.perm[cf], a.perm[cf + 1]);
+                dep = max(dep, on.depth + 2);
+                splits += 2;
+            } else {
+                ch[side] = OpenNode{cf, cc, cr, ci, on.node, on.tree, on.depth + 1, 0};
+                kind[side] = cc <= (uint32_t)a.sub_max ? 1u : 2u;
+            }
         }
     }
-    atomicAdd(a.split_count_sum, splits);
+    const uint32_t ln = wave_append(s.n_level_nodes, live);
+    if (live) {
+        s.level_nodes[2 * ln] = base + on.node;
+        s.level_nodes[2 * ln + 1] = base;
+    }
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t j = wave_append(s.n_jobs, kind[side] == 1u);
+        if (kind[side] == 1u) ((OpenNode *)s.jobs)[j] = ch[side];
+        const uint32_t q = wave_append(s.n_open_next, kind[side] == 2u);
+        if (kind[side] == 2u) ((OpenNode *)s.open_next)[q] = ch[side];
+    }
+    // depth per tree: one atomic per (wave, tree present in the wave)
+    bool pend = live;
+    for (;;) {
+        const uint64_t m = __ballot(pend);
+        if (!m) break;
+        const int lead = __ffsll((unsigned long long)m) - 1;
+        const uint32_t t = __shfl(on.tree, lead, 64);
+        const bool mine = pend && on.tree == t;
+        const int d = wave_max(mine ? (int)dep : 0);
+        if ((int)(threadIdx.x & 63u) == lead) atomicMax(a.depth + t, (uint32_t)d);
+        if (mine) pend = false;
+    }
+    const unsigned long long sp = wave_sum64(splits);
+    if ((threadIdx.x & 63u) == 0u && sp) atomicAdd(a.split_count_sum, sp);
 }
 
 // ---------------------------------------------------------------- bottom phase
