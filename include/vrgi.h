@@ -321,12 +321,56 @@ int vrgi_mesh_chunks(vrgi_ctx *ctx, const uint16_t *blocks, const uint32_t dims[
                      const uint8_t *translucent, uint32_t n_chunks, const uint32_t *chunks,
                      vrgi_vertex3d *out, uint64_t cap, uint64_t *counts);
 
+/* ---- batched scenes ---------------------------------------------------------------
+ * No counterpart in the reference (its Renderer draws one Scene per frame); for callers
+ * that render many independent scenes or views of uneven size per step. A batch groups
+ * contexts made with vrgi_create (same device, same bounce count and ReSTIR mode; no
+ * learned cache, no ray sorting, no VRGI_FLAG_KEEP_BOUNCES, no FP64 accumulation). One
+ * batched frame:
+ *   - rebuilds, in ONE multi-tree build, the BVHs of every scene whose objects changed
+ *     (every scene with VRGI_FLAG_REBUILD_EVERY_FRAME);
+ *   - runs `spp` passes; each pass is exactly one vrgi_render frame of every scene (same
+ *     bits: each scene's frame counter advances by one per pass), with ONE launch per
+ *     stage for all scenes, replayed from a captured HIP graph after the first pass;
+ *   - keeps per scene the mean of the passes' ReSTIR outputs,
+ *     (((r_1 + r_2) + ...) + r_spp) * (1 / spp) in binary32.
+ * Rays and pixels per block are fixed by the config or picked by timing each candidate
+ * on the first batched frames (the results do not depend on them). */
+typedef struct vrgi_batch vrgi_batch;
+typedef struct vrgi_batch_config {
+    uint32_t spp;              /* passes per batched frame, >= 1 */
+    uint32_t rays_per_block;   /* 64, 128 or 256 (one LDS stack per 64); 0 = autotuned */
+    uint32_t pixels_per_block; /* 64, 128 or 256; 0 = autotuned */
+    uint32_t flags;            /* VRGI_BATCH_* */
+} vrgi_batch_config;
+#define VRGI_BATCH_NO_GRAPH 1u /* launch every pass directly (no graph capture) */
+typedef struct vrgi_batch_info {
+    uint32_t n_scenes, rays_per_block, pixels_per_block;
+    uint32_t tuned;             /* 1 once the autotuner has fixed both sizes */
+    uint32_t graph;             /* 1 when passes replay a captured graph */
+    uint32_t pad;
+    uint64_t triangles, light_triangles, pixels;
+    float build_ms, passes_ms;  /* GPU time of the last batched frame's build and passes */
+    uint64_t rays;              /* closest-hit rays since the previous vrgi_batch_info_get */
+} vrgi_batch_info;
+void vrgi_batch_config_default(vrgi_batch_config *cfg);
+int vrgi_batch_create(vrgi_ctx *const *scenes, uint32_t n_scenes, const vrgi_batch_config *cfg,
+                      vrgi_batch **out);
+int vrgi_batch_destroy(vrgi_batch *batch);
+/* cams: one per scene; prefs: shared. Returns when the work is queued. */
+int vrgi_batch_render(vrgi_batch *batch, const vrgi_camera *cams, const vrgi_prefs *prefs);
+int vrgi_batch_sync(vrgi_batch *batch);
+/* the spp mean of scene `scene` (its P x 3 floats) after the last batched frame */
+int vrgi_batch_read_mean(vrgi_batch *batch, uint32_t scene, float *rgb);
+int vrgi_batch_info_get(vrgi_batch *batch, vrgi_batch_info *info);
+
 /* ---- synthetic scenes (caller side: the game_system worldgen + chunk mesher) ---- */
 
 /* Seeded voxel world of one of the benchmark workloads: kind 0 = 64x64x64
  * value-noise terrain (4 block types, 1% emissive surface voxels); kind 1 =
- * 256x128x256 terrain + 200 boxes (0.5% emissive). blocks: u16 per voxel,
- * index x*(dy*dz) + y*dz + z; air = n_block_types. */
+ * 256x128x256 terrain + 200 boxes (0.5% emissive); kind 100 + i = scene i of the
+ * batched workload (each side 32..160 voxels, drawn per scene; 1% emissive).
+ * blocks: u16 per voxel, index x*(dy*dz) + y*dz + z; air = n_block_types. */
 int vrgi_world_dims(int kind, uint32_t dims[3], uint32_t *n_block_types);
 int vrgi_world_generate(int kind, uint64_t seed, uint16_t *blocks);
 /* material table of a generated world (6 slots per block type) and luminances. */

@@ -129,6 +129,10 @@ struct vrgi_ctx {
         bool freed_once = false;
     } geo[2];
     uint32_t gi = 0;  // the set the next frame reads
+    // canonical nodes of the last build when a batched build made it (its combined array)
+    const vrgi_bvh_node *ext_gnodes = nullptr, *ext_lnodes = nullptr;
+    bool in_batch = false;
+    hipStream_t batch_stream = nullptr;  // the batch's stream while the scene is in one
     GeoSet &G() { return geo[gi]; }
     hipEvent_t ev_geo_ready = nullptr;
     DevBuf<float> d_prim9, d_power;  // builder inputs (one build at a time)
@@ -201,7 +205,11 @@ struct vrgi_ctx {
 
 namespace {
 
-void set_dev(vrgi_ctx *c) { HIP_CHECK(hipSetDevice(c->device)); }
+// current device of the context; a scene in a batch first waits for the batch's queued work
+void set_dev(vrgi_ctx *c) {
+    HIP_CHECK(hipSetDevice(c->device));
+    if (c->batch_stream) HIP_CHECK(hipStreamSynchronize(c->batch_stream));
+}
 
 SceneView scene_view(vrgi_ctx *c) {
     SceneView sv{};
@@ -490,6 +498,7 @@ void rebuild_geometry(vrgi_ctx *c, hipEvent_t e0 = nullptr, hipEvent_t e1 = null
 }
 
 void build_geometry_on(vrgi_ctx *c, hipStream_t s) {
+    c->ext_gnodes = c->ext_lnodes = nullptr;
     uint32_t nt = c->n_tris, nl = c->n_ltri, N = nt + nl;
     c->G().d_tri.reserve(nt);
     c->G().d_tri_inst.reserve(nt);
@@ -690,6 +699,7 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
 int vrgi_destroy(vrgi_ctx *c) {
     API_BEGIN
     if (!c) return fail(VRGI_E_INVALID, "null context");
+    if (c->in_batch) return fail(VRGI_E_STATE, "the scene is in a batch (destroy the batch first)");
     delete c;
     return VRGI_OK;
     API_END
@@ -881,6 +891,7 @@ int vrgi_resize(vrgi_ctx *c, uint32_t w, uint32_t h) {
     if (!c) return fail(VRGI_E_INVALID, "null context");
     if (w == 0 || h == 0) return VRGI_OK;  // zero extent: frames are skipped (interactive_rendering.rs:978-981)
     if ((uint64_t)w * h > (1ull << 26)) return fail(VRGI_E_LIMIT, "extent above 2^26 pixels");
+    if (c->in_batch) return fail(VRGI_E_STATE, "the scene is in a batch (its extent is fixed there)");
     set_dev(c);
     HIP_CHECK(hipStreamSynchronize(c->stream));
     c->W = w;
@@ -1301,10 +1312,12 @@ int vrgi_read_buffer(vrgi_ctx *c, int which, void *dst, size_t bytes) {
     const void *src = nullptr;
     size_t P = c->P, nt = c->n_tris;
     switch (which) {
-        case VRGI_BUF_GEOM_BVH: src = c->G().d_nodes.p; break;
+        case VRGI_BUF_GEOM_BVH: src = c->ext_gnodes ? c->ext_gnodes : c->G().d_nodes.p; break;
         case VRGI_BUF_GEOM_PRIM_ORDER: src = c->geo_builder.prim_order(); break;
         case VRGI_BUF_LIGHT_TL_BVH: src = c->d_tl.p; break;
-        case VRGI_BUF_LIGHT_GEOM_BVH: src = c->G().d_nodes.p + (nt ? 2 * nt - 1 : 0); break;
+        case VRGI_BUF_LIGHT_GEOM_BVH:
+            src = c->ext_lnodes ? c->ext_lnodes : c->G().d_nodes.p + (nt ? 2 * nt - 1 : 0);
+            break;
         case VRGI_BUF_PATH_LEN: src = c->f_plen.p; break;
         case VRGI_BUF_BOUNCE_ORIGIN: src = c->f_origin.p; break;
         case VRGI_BUF_BOUNCE_DIRECTION: src = c->f_dir.p; break;
@@ -1821,6 +1834,540 @@ int vrgi_cache_train(vrgi_ctx *c, uint64_t n, const float *samples, const float 
     HIP_CHECK(hipMemcpyAsync(&l, C.loss.p, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     if (loss) *loss = l;
+    return VRGI_OK;
+    API_END
+}
+
+}  // extern "C"
+
+// ==================================================================== batched scenes
+struct vrgi_batch {
+    int device = 0;
+    std::vector<vrgi_ctx *> sc;
+    vrgi_batch_config cfg{};
+    uint32_t B = 0;
+    hipStream_t stream = nullptr;
+    Builder builder;
+    DevBuf<float> prim9, power;
+    DevBuf<uint32_t> ids, trees, depth;
+    DevBuf<vrgi_bvh_node> nodes;
+    // per-pass arguments: one device block, a ring of pinned host blocks (each reused only
+    // after its copy has run)
+    size_t off_sv = 0, off_tba = 0, off_nba = 0, off_fa = 0, off_da = 0, off_pass = 0, args_bytes = 0;
+    DevBuf<uint8_t> d_args;
+    static constexpr int RING = 4;
+    uint8_t *h_args[RING] = {};
+    hipEvent_t ev_args[RING] = {};
+    bool ev_used[RING] = {};
+    uint32_t ring = 0;
+    // block prefixes of the current tiles, mean buffers
+    DevBuf<uint32_t> d_blk;
+    uint32_t blk_wpb = 0, blk_pb = 0, ray_blocks = 0, px_blocks = 0;
+    std::vector<DevBuf<float>> mean;
+    DevBuf<float *> d_mean;
+    // graph of one pass
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint32_t> graph_key;
+    // tiles: fixed, or tuned over the first batched frames
+    uint32_t wpb = 0, pb = 0;
+    int tune_step = 0;  // 0..2: rays per block 64/128/256; 3..5: pixels per block 64/128/256
+    float tune_ms[6] = {};
+    bool tuned = false;
+    hipEvent_t e_b0 = nullptr, e_b1 = nullptr, e_end = nullptr;
+    bool have_times = false;
+    uint64_t rays_seen = 0;  // closest-hit rays counted at the last vrgi_batch_info_get
+    ~vrgi_batch();
+};
+
+namespace {
+
+const uint32_t BATCH_TILES[3] = {64, 128, 256};
+
+// one multi-tree build over the geometry and light-geometry trees of the listed scenes
+void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
+    hipStream_t s = bt->stream;
+    struct Part {
+        vrgi_ctx *c;
+        uint32_t off, nt, nl, gbase, lbase;
+        int gt, lt;
+    };
+    std::vector<Part> parts;
+    std::vector<uint32_t> beg, cnt, base;
+    uint64_t N = 0, nodes = 0;
+    for (vrgi_ctx *c : list) {
+        Part p{c, (uint32_t)N, c->n_tris, c->n_ltri, 0, 0, -1, -1};
+        if (p.nt) { p.gt = (int)beg.size(); beg.push_back(p.off); cnt.push_back(p.nt); base.push_back((uint32_t)nodes); p.gbase = (uint32_t)nodes; nodes += 2ull * p.nt - 1; }
+        if (p.nl) { p.lt = (int)beg.size(); beg.push_back(p.off + p.nt); cnt.push_back(p.nl); base.push_back((uint32_t)nodes); p.lbase = (uint32_t)nodes; nodes += 2ull * p.nl - 1; }
+        N += (uint64_t)p.nt + p.nl;
+        parts.push_back(p);
+        if (N >= (1ull << 31) || nodes >= (1ull << 32) - 1) throw std::runtime_error("batched build above 2^31 primitives");
+    }
+    const uint32_t nT = (uint32_t)beg.size();
+    for (Part &p : parts) {
+        vrgi_ctx *c = p.c;
+        auto &G = c->G();
+        G.geo_depth = G.lgeo_depth = 0;
+        c->ext_gnodes = c->ext_lnodes = nullptr;
+    }
+    if (N == 0) return;
+    bt->prim9.reserve((size_t)9 * N);
+    bt->power.reserve(N);
+    bt->ids.reserve(N);
+    bt->nodes.reserve(nodes);
+    bt->trees.reserve(3 * (size_t)nT);
+    bt->depth.reserve(nT);
+    for (Part &p : parts) {
+        vrgi_ctx *c = p.c;
+        auto &G = c->G();
+        G.d_tri.reserve(p.nt);
+        G.d_tri_inst.reserve(p.nt);
+        G.d_ltri.reserve(p.nl);
+        G.d_ltri_inst.reserve(p.nl);
+        G.d_ltri_node.reserve(p.nl);
+        G.d_gwide.reserve(p.nt > 1 ? p.nt - 1 : 1);
+        G.d_lwide.reserve(p.nl > 1 ? p.nl - 1 : 1);
+        G.d_lwide4.reserve(p.nl > 1 ? p.nl - 1 : 1);
+        G.d_lanc.reserve((size_t)p.nl * VRGI_LANC_MAX);
+        G.d_lanc_len.reserve(p.nl);
+        FlattenArgs f{};
+        f.inst = c->d_inst.p;
+        f.n_inst = c->n_inst;
+        f.vtx = c->d_vtxptr.p;
+        f.lvtx = c->d_lvtxptr.p;
+        f.light_off = c->d_light_off.p;
+        f.prim_stride = (uint32_t)N;
+        f.n_tris = p.nt;
+        f.tri = G.d_tri.p;
+        f.tri_inst = G.d_tri_inst.p;
+        f.prim9 = bt->prim9.p;
+        f.power = bt->power.p;
+        f.ids = bt->ids.p;
+        f.prim_off = p.off;
+        launch_flatten(s, f);
+        FlattenArgs fl = f;
+        fl.n_tris = p.nl;
+        fl.tri = G.d_ltri.p;
+        fl.tri_inst = G.d_ltri_inst.p;
+        fl.light_node = G.d_ltri_node.p;
+        fl.prim_off = p.off + p.nt;
+        launch_flatten_lights(s, fl);
+        launch_light_ancestry(s, G.d_ltri_inst.p, G.d_ltri_node.p, c->d_inst.p, c->d_tl.p, p.nl, G.d_lanc.p,
+                              G.d_lanc_len.p);
+    }
+    std::vector<uint32_t> tr(3 * (size_t)nT);
+    for (uint32_t t = 0; t < nT; ++t) { tr[t] = beg[t]; tr[nT + t] = cnt[t]; tr[2 * nT + t] = base[t]; }
+    HIP_CHECK(hipMemcpyAsync(bt->trees.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice, s));
+    if (bt->builder.build(s, bt->prim9.p, bt->power.p, bt->ids.p, (uint32_t)N, nT, bt->trees.p, bt->trees.p + nT,
+                          bt->trees.p + 2 * nT, base.data(), bt->nodes.p, bt->depth.p, nullptr) != 0)
+        throw HipError("batched build exceeded its workspace");
+    std::vector<uint32_t> dep(nT);
+    HIP_CHECK(hipMemcpyAsync(dep.data(), bt->depth.p, nT * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (Part &p : parts) {
+        vrgi_ctx *c = p.c;
+        auto &G = c->G();
+        if (p.gt >= 0) G.geo_depth = dep[p.gt];
+        if (p.lt >= 0) G.lgeo_depth = dep[p.lt];
+        if (geom_stack_need(G.geo_depth) + 2 > VRGI_MAX_STACK || 3 * ((G.lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
+            throw std::runtime_error("BVH deeper than the traversal stack");
+        if (p.nt) launch_wide(s, bt->nodes.p + p.gbase, 2 * p.nt - 1, G.d_gwide.p);
+        if (p.nl) {
+            launch_wide(s, bt->nodes.p + p.lbase, 2 * p.nl - 1, G.d_lwide.p);
+            launch_wide4(s, bt->nodes.p + p.lbase, 2 * p.nl - 1, G.d_lwide4.p);
+        }
+        c->ext_gnodes = bt->nodes.p + p.gbase;
+        c->ext_lnodes = bt->nodes.p + p.lbase;
+        c->geo_dirty = false;
+    }
+}
+
+// block prefixes for `wpb` waves of rays and `pb` pixels per block
+void batch_tables(vrgi_batch *bt, uint32_t wpb, uint32_t pb) {
+    if (bt->blk_wpb == wpb && bt->blk_pb == pb) return;
+    const uint32_t n = (uint32_t)bt->sc.size();
+    std::vector<uint32_t> t(2 * ((size_t)n + 1));
+    uint64_t r = 0, q = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        t[k] = (uint32_t)r;
+        t[n + 1 + k] = (uint32_t)q;
+        r += (bt->sc[k]->P + 64 * wpb - 1) / (64 * wpb);
+        q += (bt->sc[k]->P + pb - 1) / pb;
+    }
+    if (r >= (1ull << 31) || q >= (1ull << 31)) throw std::runtime_error("batch grid too large");
+    t[n] = (uint32_t)r;
+    t[2 * n + 1] = (uint32_t)q;
+    bt->d_blk.reserve(t.size());
+    HIP_CHECK(hipStreamSynchronize(bt->stream));
+    HIP_CHECK(hipMemcpy(bt->d_blk.p, t.data(), t.size() * 4, hipMemcpyHostToDevice));
+    bt->blk_wpb = wpb;
+    bt->blk_pb = pb;
+    bt->ray_blocks = (uint32_t)r;
+    bt->px_blocks = (uint32_t)q;
+}
+
+// the arguments of one pass of scene k, as vrgi_render forms them for a frame without the
+// learned cache, ray sorting or profiling
+void batch_fill(vrgi_batch *bt, uint8_t *h, uint32_t k, const vrgi_camera &cam, const vrgi_prefs &pr) {
+    vrgi_ctx *c = bt->sc[k];
+    const uint32_t P = c->P, B = c->B;
+    const uint32_t frame = (uint32_t)c->frame_count;
+    ((SceneView *)(h + bt->off_sv))[k] = scene_view(c);
+    BounceArgs ba{};
+    ba.B = B; ba.W = c->W; ba.H = c->H; ba.P = P;
+    ba.nee_type = pr.nee_type;
+    ba.rr_start = c->cfg.rr_start_bounce;
+    ba.rr_min = c->cfg.rr_min_survival;
+    ba.cam = cam;
+    ba.qcount = c->f_qcount.p;
+    ba.origin = c->f_origin.p; ba.dir = c->f_dir.p; ba.normal = c->f_normal.p;
+    ba.emis = c->f_emis.p; ba.refl = c->f_refl.p; ba.mis = c->f_mis.p; ba.pdf = c->f_pdf.p;
+    ba.nee = c->f_nee.p; ba.hit_t = c->f_hit_t.p; ba.hit = c->f_hit.p; ba.path_len = c->f_plen.p;
+    ba.counters = (unsigned long long *)c->d_counters.p;
+    BounceArgs *tba = (BounceArgs *)(h + bt->off_tba) + (size_t)k * B;
+    BounceArgs *nba = (BounceArgs *)(h + bt->off_nba) + (size_t)k * B;
+    for (uint32_t b = 0; b < B; ++b) {
+        ba.bounce = b;
+        ba.inv_seed = frame * B + b;  // interactive_rendering.rs:1189
+        ba.queue_in = b == 0 ? nullptr : c->f_queue.p + (size_t)b * P;
+        ba.queue_out = c->f_queue.p + (size_t)(b + 1) * P;
+        tba[b] = ba;
+        BounceArgs na = ba;  // NEE pdf of bounce b: the rays of bounce b + 1
+        na.queue_in = c->f_queue.p + (size_t)(b + 1) * P;
+        nba[b] = na;
+    }
+    FrameArgs fa{};
+    fa.W = c->W; fa.H = c->H; fa.P = P; fa.B = B; fa.frame = frame;
+    fa.slot = (uint32_t)(c->frame_count % 3);
+    fa.tseed = frame_seed_host(c->cfg.seed, frame, 2);
+    fa.sseed = frame_seed_host(c->cfg.seed, frame, 3);
+    fa.spatial_iters = pr.restir_spatial_iterations;
+    fa.radius = c->cfg.restir_spatial_radius;
+    fa.debug_view = pr.debug_view;
+    fa.origin = c->f_origin.p; fa.normal = c->f_normal.p; fa.emis = c->f_emis.p; fa.refl = c->f_refl.p;
+    fa.mis = c->f_mis.p; fa.pdf = c->f_pdf.p; fa.nee = pr.nee_type ? c->f_nee.p : nullptr;
+    fa.path_len = c->f_plen.p;
+    fa.radiance = c->f_radiance.p;
+    fa.temporal = c->f_temporal.p + (size_t)fa.slot * P * 20;
+    fa.spatial = c->f_spatial.p;
+    fa.restir = c->f_restir.p;
+    fa.debug = c->f_debug.p;
+    fa.bgra8 = c->f_bgra.p;
+    ((FrameArgs *)(h + bt->off_fa))[k] = fa;
+    DiArgs da{};
+    if (c->cfg.restir_mode == 1) {
+        da.W = c->W; da.H = c->H; da.P = P; da.frame = frame;
+        da.candidates = c->cfg.di_candidates;
+        da.spatial_iters = pr.restir_spatial_iterations;
+        da.radius = c->cfg.restir_spatial_radius;
+        da.dseed = frame_seed_host(c->cfg.seed, frame, 4);
+        da.origin = c->f_origin.p; da.normal = c->f_normal.p; da.emis = c->f_emis.p; da.hit = c->f_hit.p;
+        da.T = c->di_T.p;
+        da.hist_prev = c->di_hist.p + (size_t)((frame + 1) % 2) * P * 8;
+        da.hist_cur = c->di_hist.p + (size_t)(frame % 2) * P * 8;
+        da.restir = c->f_restir.p;
+        da.di_light = c->di_light.p;
+        da.di_weights = c->di_w.p;
+    }
+    ((DiArgs *)(h + bt->off_da))[k] = da;
+}
+
+const char *batch_scene_ok(const vrgi_ctx *c, const vrgi_ctx *first) {
+    if (c->device != first->device) return "batched scenes must share one device";
+    if (c->B != first->B) return "batched scenes must share the bounce count";
+    if (c->cfg.restir_mode != first->cfg.restir_mode) return "batched scenes must share the ReSTIR mode";
+    if (c->cache.on) return "the learned cache is not available in a batch";
+    if (c->cfg.ray_sort_bits) return "ray sorting is not available in a batch";
+    if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) return "VRGI_FLAG_KEEP_BOUNCES is not available in a batch";
+    if (c->accum_on) return "FP64 accumulation is not available in a batch";
+    return nullptr;
+}
+
+}  // namespace
+
+vrgi_batch::~vrgi_batch() {
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (int k = 0; k < RING; ++k) {
+        if (ev_args[k]) (void)hipEventDestroy(ev_args[k]);
+        if (h_args[k]) (void)hipHostFree(h_args[k]);
+    }
+    for (hipEvent_t e : {e_b0, e_b1, e_end})
+        if (e) (void)hipEventDestroy(e);
+    for (vrgi_ctx *c : sc) {
+        // the canonical nodes lived in this batch's build arrays; the next render rebuilds
+        c->ext_gnodes = c->ext_lnodes = nullptr;
+        c->geo_dirty = true;
+        c->in_batch = false;
+        c->batch_stream = nullptr;
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+void vrgi_batch_config_default(vrgi_batch_config *cfg) {
+    if (!cfg) return;
+    cfg->spp = 1;
+    cfg->rays_per_block = 0;
+    cfg->pixels_per_block = 0;
+    cfg->flags = 0;
+}
+
+int vrgi_batch_create(vrgi_ctx *const *scenes, uint32_t n, const vrgi_batch_config *cfg, vrgi_batch **out) {
+    API_BEGIN
+    if (!scenes || !cfg || !out || n == 0) return fail(VRGI_E_INVALID, "null argument or no scenes");
+    *out = nullptr;
+    if (n > 65536) return fail(VRGI_E_LIMIT, "more than 65536 scenes");
+    if (cfg->spp < 1 || cfg->spp > 1024) return fail(VRGI_E_INVALID, "spp must be in [1, 1024]");
+    auto tile_ok = [](uint32_t t) { return t == 0 || t == 64 || t == 128 || t == 256; };
+    if (!tile_ok(cfg->rays_per_block) || !tile_ok(cfg->pixels_per_block))
+        return fail(VRGI_E_INVALID, "rays_per_block and pixels_per_block must be 0, 64, 128 or 256");
+    if (cfg->flags & ~VRGI_BATCH_NO_GRAPH) return fail(VRGI_E_INVALID, "unknown batch flag");
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!scenes[k]) return fail(VRGI_E_INVALID, "null scene");
+        if (const char *e = batch_scene_ok(scenes[k], scenes[0])) return fail(VRGI_E_INVALID, e);
+        if (scenes[k]->in_batch) return fail(VRGI_E_STATE, "a scene is already in a batch");
+        for (uint32_t j = 0; j < k; ++j)
+            if (scenes[j] == scenes[k]) return fail(VRGI_E_INVALID, "a scene appears twice");
+    }
+    vrgi_batch *bt = new vrgi_batch();
+    try {
+        bt->device = scenes[0]->device;
+        HIP_CHECK(hipSetDevice(bt->device));
+        bt->sc.assign(scenes, scenes + n);
+        bt->cfg = *cfg;
+        bt->B = scenes[0]->B;
+        HIP_CHECK(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
+        for (hipEvent_t *e : {&bt->e_b0, &bt->e_b1, &bt->e_end}) HIP_CHECK(hipEventCreate(e));
+        auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        size_t o = 0;
+        bt->off_sv = o; o = align(o + n * sizeof(SceneView));
+        bt->off_tba = o; o = align(o + (size_t)n * bt->B * sizeof(BounceArgs));
+        bt->off_nba = o; o = align(o + (size_t)n * bt->B * sizeof(BounceArgs));
+        bt->off_fa = o; o = align(o + n * sizeof(FrameArgs));
+        bt->off_da = o; o = align(o + n * sizeof(DiArgs));
+        bt->off_pass = o; o = align(o + sizeof(BatchPass));
+        bt->args_bytes = o;
+        bt->d_args.reserve(o);
+        for (int k = 0; k < vrgi_batch::RING; ++k) {
+            HIP_CHECK(hipHostMalloc((void **)&bt->h_args[k], o, hipHostMallocDefault));
+            std::memset(bt->h_args[k], 0, o);
+            HIP_CHECK(hipEventCreateWithFlags(&bt->ev_args[k], hipEventDisableTiming));
+        }
+        bt->mean.resize(n);
+        std::vector<float *> mp(n);
+        for (uint32_t k = 0; k < n; ++k) {
+            bt->mean[k].reserve((size_t)scenes[k]->P * 3);
+            mp[k] = bt->mean[k].p;
+        }
+        bt->d_mean.reserve(n);
+        HIP_CHECK(hipMemcpy(bt->d_mean.p, mp.data(), n * sizeof(float *), hipMemcpyHostToDevice));
+        bt->wpb = cfg->rays_per_block / 64;
+        bt->pb = cfg->pixels_per_block;
+        bt->tuned = bt->wpb && bt->pb;
+        bt->tune_step = bt->wpb ? 3 : 0;  // rays per block fixed: tune pixels only
+        for (vrgi_ctx *c : bt->sc) {
+            c->in_batch = true;
+            c->batch_stream = bt->stream;
+        }
+    } catch (...) {
+        delete bt;
+        throw;
+    }
+    *out = bt;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_batch_destroy(vrgi_batch *bt) {
+    API_BEGIN
+    delete bt;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_batch_render(vrgi_batch *bt, const vrgi_camera *cams, const vrgi_prefs *prefs) {
+    API_BEGIN
+    if (!bt || !cams || !prefs) return fail(VRGI_E_INVALID, "null argument");
+    if (prefs->nee_type > 2) return fail(VRGI_E_INVALID, "nee_type must be 0, 1 or 2");
+    if (prefs->restir_spatial_iterations > 20) return fail(VRGI_E_INVALID, "restir_spatial_iterations above 20");
+    const uint32_t n = (uint32_t)bt->sc.size();
+    for (vrgi_ctx *c : bt->sc)
+        if (const char *e = batch_scene_ok(c, bt->sc[0])) return fail(VRGI_E_STATE, e);
+    HIP_CHECK(hipSetDevice(bt->device));
+    hipStream_t s = bt->stream;
+    // instance tables are rewritten in place: the previous batched frame must have ended
+    for (vrgi_ctx *c : bt->sc)
+        if (c->inst_dirty) {
+            HIP_CHECK(hipStreamSynchronize(s));
+            break;
+        }
+    // the scenes' own queued work (uploads, earlier single renders) ends first
+    for (vrgi_ctx *c : bt->sc) {
+        if (c->inst_dirty) {
+            rebuild_instances(c);
+            c->inst_dirty = false;
+            c->geo_dirty = true;
+        }
+        HIP_CHECK(hipStreamSynchronize(c->up_stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    HIP_CHECK(hipEventRecord(bt->e_b0, s));
+    std::vector<vrgi_ctx *> list;
+    for (vrgi_ctx *c : bt->sc)
+        if (c->geo_dirty || (c->cfg.flags & VRGI_FLAG_REBUILD_EVERY_FRAME)) list.push_back(c);
+    if (!list.empty()) batch_build(bt, list);
+    HIP_CHECK(hipEventRecord(bt->e_b1, s));
+    uint32_t gneed = 0, lneed = 0;
+    for (vrgi_ctx *c : bt->sc) {
+        gneed = std::max(gneed, geom_stack_need(c->G().geo_depth));
+        lneed = std::max(lneed, 3 * ((c->G().lgeo_depth + 1) / 2));
+    }
+    // tiles of this batched frame
+    uint32_t wpb = bt->wpb, pb = bt->pb;
+    const bool tuning = !bt->tuned;
+    if (tuning) {
+        const int st = bt->tune_step;
+        wpb = st < 3 ? BATCH_TILES[st] / 64 : bt->wpb;
+        pb = st < 3 ? (bt->cfg.pixels_per_block ? bt->cfg.pixels_per_block : 256u) : BATCH_TILES[st - 3];
+    }
+    if (gneed + 2 > 64 || lneed + 2 > 64) wpb = 1;  // a stack above 64 entries runs one wave per block
+    batch_tables(bt, wpb, pb);
+    BatchLaunch L{};
+    uint8_t *dA = bt->d_args.p;
+    L.svs = (const SceneView *)(dA + bt->off_sv);
+    L.tba = (const BounceArgs *)(dA + bt->off_tba);
+    L.nba = (const BounceArgs *)(dA + bt->off_nba);
+    L.fas = (const FrameArgs *)(dA + bt->off_fa);
+    L.das = (const DiArgs *)(dA + bt->off_da);
+    L.pass = (const BatchPass *)(dA + bt->off_pass);
+    L.blk_ray = bt->d_blk.p;
+    L.blk_px = bt->d_blk.p + n + 1;
+    L.mean = bt->d_mean.p;
+    L.n = n; L.B = bt->B;
+    L.ray_blocks = bt->ray_blocks; L.px_blocks = bt->px_blocks;
+    L.gneed = gneed; L.lneed = lneed;
+    L.wpb = wpb; L.pb = pb;
+    L.nee = prefs->nee_type != 0;
+    L.di = bt->sc[0]->cfg.restir_mode == 1;
+    auto cls = [](uint32_t need) { return need + 2 <= 32 ? 32u : need + 2 <= 48 ? 48u : need + 2 <= 64 ? 64u : 160u; };
+    const bool use_graph = !(bt->cfg.flags & VRGI_BATCH_NO_GRAPH) && !tuning;
+    std::vector<uint32_t> key = {cls(gneed), cls(lneed), wpb, pb, L.ray_blocks, L.px_blocks, (uint32_t)L.nee,
+                                 (uint32_t)L.di, bt->B};
+    for (uint32_t pass = 0; pass < bt->cfg.spp; ++pass) {
+        const uint32_t r = bt->ring;
+        bt->ring = (r + 1) % vrgi_batch::RING;
+        if (bt->ev_used[r]) HIP_CHECK(hipEventSynchronize(bt->ev_args[r]));
+        uint8_t *h = bt->h_args[r];
+        for (uint32_t k = 0; k < n; ++k) batch_fill(bt, h, k, cams[k], *prefs);
+        BatchPass bp{};
+        bp.first = pass == 0;
+        bp.last = pass + 1 == bt->cfg.spp;
+        bp.inv_spp = 1.0f / (float)bt->cfg.spp;
+        *(BatchPass *)(h + bt->off_pass) = bp;
+        HIP_CHECK(hipMemcpyAsync(dA, h, bt->args_bytes, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipEventRecord(bt->ev_args[r], s));
+        bt->ev_used[r] = true;
+        if (use_graph) {
+            if (!bt->exec || bt->graph_key != key) {
+                if (bt->exec) HIP_CHECK(hipGraphExecDestroy(bt->exec));
+                if (bt->graph) HIP_CHECK(hipGraphDestroy(bt->graph));
+                bt->exec = nullptr;
+                bt->graph = nullptr;
+                HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                launch_batch_pass(s, L);
+                HIP_CHECK(hipStreamEndCapture(s, &bt->graph));
+                HIP_CHECK(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+                bt->graph_key = key;
+            }
+            HIP_CHECK(hipGraphLaunch(bt->exec, s));
+        } else {
+            launch_batch_pass(s, L);
+        }
+        for (vrgi_ctx *c : bt->sc) {
+            c->have_frame = true;
+            ++c->frame_count;
+        }
+    }
+    HIP_CHECK(hipEventRecord(bt->e_end, s));
+    bt->have_times = true;
+    if (tuning) {  // steps 0-2: 64 / 128 / 256 rays per block; steps 3-5: pixels per block
+        HIP_CHECK(hipEventSynchronize(bt->e_end));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, bt->e_b1, bt->e_end));
+        const int st = bt->tune_step;
+        bt->tune_ms[st] = ms;
+        if (st == 2) {
+            int best = 0;
+            for (int k = 1; k < 3; ++k)
+                if (bt->tune_ms[k] < bt->tune_ms[best]) best = k;
+            bt->wpb = BATCH_TILES[best] / 64;
+            if (bt->cfg.pixels_per_block) bt->tuned = true;
+        }
+        if (st == 5) {
+            int best = 3;
+            for (int k = 4; k < 6; ++k)
+                if (bt->tune_ms[k] < bt->tune_ms[best]) best = k;
+            bt->pb = BATCH_TILES[best - 3];
+            bt->tuned = true;
+        }
+        bt->tune_step = st + 1;
+    }
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_batch_sync(vrgi_batch *bt) {
+    API_BEGIN
+    if (!bt) return fail(VRGI_E_INVALID, "null batch");
+    HIP_CHECK(hipSetDevice(bt->device));
+    HIP_CHECK(hipStreamSynchronize(bt->stream));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_batch_read_mean(vrgi_batch *bt, uint32_t scene, float *rgb) {
+    API_BEGIN
+    if (!bt || !rgb) return fail(VRGI_E_INVALID, "null argument");
+    if (scene >= bt->sc.size()) return fail(VRGI_E_INVALID, "scene index out of range");
+    HIP_CHECK(hipSetDevice(bt->device));
+    HIP_CHECK(hipMemcpyAsync(rgb, bt->mean[scene].p, (size_t)bt->sc[scene]->P * 12, hipMemcpyDeviceToHost, bt->stream));
+    HIP_CHECK(hipStreamSynchronize(bt->stream));
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_batch_info_get(vrgi_batch *bt, vrgi_batch_info *info) {
+    API_BEGIN
+    if (!bt || !info) return fail(VRGI_E_INVALID, "null argument");
+    HIP_CHECK(hipSetDevice(bt->device));
+    *info = vrgi_batch_info{};
+    info->n_scenes = (uint32_t)bt->sc.size();
+    info->rays_per_block = bt->wpb * 64;
+    info->pixels_per_block = bt->pb;
+    info->tuned = bt->tuned ? 1u : 0u;
+    info->graph = bt->exec ? 1u : 0u;
+    for (vrgi_ctx *c : bt->sc) {
+        info->triangles += c->n_tris;
+        info->light_triangles += c->n_ltri;
+        info->pixels += c->P;
+    }
+    HIP_CHECK(hipStreamSynchronize(bt->stream));
+    if (bt->have_times) {
+        HIP_CHECK(hipEventElapsedTime(&info->build_ms, bt->e_b0, bt->e_b1));
+        HIP_CHECK(hipEventElapsedTime(&info->passes_ms, bt->e_b1, bt->e_end));
+    }
+    uint64_t rays = 0;
+    for (vrgi_ctx *c : bt->sc) {
+        Counters k{};
+        HIP_CHECK(hipMemcpy(&k, c->d_counters.p, sizeof(k), hipMemcpyDeviceToHost));
+        rays += k.rays_closest;
+    }
+    info->rays = rays - bt->rays_seen;
+    bt->rays_seen = rays;
     return VRGI_OK;
     API_END
 }

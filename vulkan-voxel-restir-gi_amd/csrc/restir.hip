@@ -46,8 +46,7 @@ __device__ __forceinline__ void update_res(uint32_t seed, Reservoir &r, const Sa
 }
 
 // outgoing_radiance (backward recurrence) + initial sample + temporal update
-__global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void accum_temporal_px(const FrameArgs &fa, uint32_t id) {
     if (id >= fa.P) return;
     const uint32_t plen = fa.path_len[id];
     const float bsp = 1.0f / (2.0f * VRGI_PI_F);
@@ -124,10 +123,12 @@ __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
     R.ucw = ph > 0.0f ? R.wsum / ((float)R.m * ph) : 0.0f;
     res_store(slot, R);
 }
+__global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
+    accum_temporal_px(fa, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // restir_spatial_resampling.rs:316-406 + restir_finalize.rs:121-141
-__global__ __launch_bounds__(256) void k_spatial_finalize(const FrameArgs fa) {
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void spatial_finalize_px(const FrameArgs &fa, uint32_t id) {
     if (id >= fa.P) return;
     const uint32_t qx = id % fa.W, qy = id / fa.W;
     const uint32_t ps = mm_combine(fa.sseed, id);
@@ -175,11 +176,13 @@ __global__ __launch_bounds__(256) void k_spatial_finalize(const FrameArgs fa) {
     float dv = Rs.ucw / 10.0f;
     st3(fa.debug + (size_t)id * 3, mk(dv, dv, dv));
 }
+__global__ __launch_bounds__(256) void k_spatial_finalize(const FrameArgs fa) {
+    spatial_finalize_px(fa, blockIdx.x * blockDim.x + threadIdx.x);
+}
 
 // postprocess.rs:39-88 with scale 1: view select, BGRA8 (clamped; the reference leaves
 // out-of-range conversion undefined)
-__global__ __launch_bounds__(256) void k_postprocess(const FrameArgs fa) {
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void postprocess_px(const FrameArgs &fa, uint32_t id) {
     if (id >= fa.P) return;
     const float *src = fa.debug_view == 0 ? fa.radiance : (fa.debug_view == 1 ? fa.restir : fa.debug);
     f3 c = ld3(src + (size_t)id * 3) / (float)(1 * 1);
@@ -191,6 +194,54 @@ __global__ __launch_bounds__(256) void k_postprocess(const FrameArgs fa) {
     };
     uint32_t px = cv(c.z) | (cv(c.y) << 8) | (cv(c.x) << 16) | (255u << 24);
     ((uint32_t *)fa.bgra8)[id] = px;
+}
+__global__ __launch_bounds__(256) void k_postprocess(const FrameArgs fa) {
+    postprocess_px(fa, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---------------------------------------------------------------- batched scenes
+template <int PB>
+__global__ __launch_bounds__(PB) void k_accum_temporal_b(const FrameArgs *__restrict__ fas,
+                                                         const uint32_t *__restrict__ blk, uint32_t n) {
+    const uint32_t s = batch_scene(blk, n, blockIdx.x);
+    accum_temporal_px(fas[s], (blockIdx.x - blk[s]) * PB + threadIdx.x);
+}
+template <int PB>
+__global__ __launch_bounds__(PB) void k_spatial_finalize_b(const FrameArgs *__restrict__ fas,
+                                                           const uint32_t *__restrict__ blk, uint32_t n) {
+    const uint32_t s = batch_scene(blk, n, blockIdx.x);
+    spatial_finalize_px(fas[s], (blockIdx.x - blk[s]) * PB + threadIdx.x);
+}
+// postprocess, then the spp mean of the ReSTIR output: (((r_1 + r_2) + ...) + r_spp) / spp
+template <int PB>
+__global__ __launch_bounds__(PB) void k_postprocess_b(const FrameArgs *__restrict__ fas, const uint32_t *__restrict__ blk,
+                                                      uint32_t n, float *const *__restrict__ mean,
+                                                      const BatchPass *__restrict__ pass) {
+    const uint32_t s = batch_scene(blk, n, blockIdx.x);
+    const FrameArgs &fa = fas[s];
+    const uint32_t id = (blockIdx.x - blk[s]) * PB + threadIdx.x;
+    postprocess_px(fa, id);
+    if (id >= fa.P) return;
+    float *m = mean[s] + (size_t)id * 3;
+    const f3 r = ld3(fa.restir + (size_t)id * 3);
+    f3 v = pass->first ? r : ld3(m) + r;
+    if (pass->last) v = v * pass->inv_spp;
+    st3(m, v);
+}
+#define VRGI_PB_LAUNCH(K, ...)                                                                   \
+    do {                                                                                         \
+        if (L.pb == 64) K<64><<<L.px_blocks, 64, 0, s>>>(__VA_ARGS__);                            \
+        else if (L.pb == 128) K<128><<<L.px_blocks, 128, 0, s>>>(__VA_ARGS__);                    \
+        else K<256><<<L.px_blocks, 256, 0, s>>>(__VA_ARGS__);                                     \
+    } while (0)
+void launch_batch_pixels_gi(hipStream_t s, const BatchLaunch &L) {
+    VRGI_PB_LAUNCH(k_accum_temporal_b, L.fas, L.blk_px, L.n);
+}
+void launch_batch_pixels_spatial(hipStream_t s, const BatchLaunch &L) {
+    VRGI_PB_LAUNCH(k_spatial_finalize_b, L.fas, L.blk_px, L.n);
+}
+void launch_batch_pixels_post(hipStream_t s, const BatchLaunch &L) {
+    VRGI_PB_LAUNCH(k_postprocess_b, L.fas, L.blk_px, L.n, L.mean, L.pass);
 }
 
 void launch_accum_temporal(hipStream_t s, const FrameArgs &fa) {

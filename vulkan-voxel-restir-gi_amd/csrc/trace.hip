@@ -460,7 +460,7 @@ __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const Bou
         const bool q = active && !train && h.prim != VRGI_INVALID;
         const uint64_t qb = __ballot(q);
         if (qb) {
-            const uint32_t lane = threadIdx.x;
+            const uint32_t lane = threadIdx.x & 63u;
             uint32_t base = 0;
             if (lane == (uint32_t)(__ffsll((unsigned long long)qb) - 1))
                 base = atomicAdd(ba.qrec_count, (uint32_t)__popcll(qb));
@@ -477,7 +477,7 @@ __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const Bou
     // compaction of the rays alive at the next bounce: ballot + one atomic per wave
     uint64_t bal = __ballot(cont);
     if (bal) {
-        uint32_t lane = threadIdx.x;
+        uint32_t lane = threadIdx.x & 63u;
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(ba.qcount + b + 1, (uint32_t)__popcll(bal));
         base = __shfl(base, 0, 64);
@@ -514,7 +514,7 @@ __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArg
         f3 n = ld3(ba.normal + bi * 3);
         RayPre r = ray_pre(o, d);
         // every light triangle crossed in [1e-4, 1000], left-to-right leaf order
-        uint32_t *stk = stack + threadIdx.x;
+        uint32_t *stk = stack + (threadIdx.x & 63u);
         int sp = 0;
         uint32_t ref;
         bool have;
@@ -610,8 +610,7 @@ __device__ __forceinline__ f3 di_fr(const SceneView &sv, uint32_t hit) {
 }
 
 // initial candidates + temporal merge with the previous frame (M capped at 20x)
-__global__ __launch_bounds__(64) void k_di_initial(const SceneView sv, const DiArgs da) {
-    const uint32_t id = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void di_initial_px(const SceneView &sv, const DiArgs &da, uint32_t id) {
     if (id >= da.P) return;
     DiRes r;
     r.light = VRGI_INVALID; r.y = mk(0, 0, 0); r.wsum = 0.0f; r.W = 0.0f; r.ph = 0.0f; r.M = 0;
@@ -655,12 +654,13 @@ __global__ __launch_bounds__(64) void k_di_initial(const SceneView sv, const DiA
     }
     di_store(da.T + (size_t)id * 8, r);
 }
+__global__ __launch_bounds__(64) void k_di_initial(const SceneView sv, const DiArgs da) {
+    di_initial_px(sv, da, blockIdx.x * 64 + threadIdx.x);
+}
 
 // spatial merge, shadow ray, final radiance, history for the next frame
 template <int STACK>
-__global__ __launch_bounds__(64) void k_di_final(const SceneView sv, const DiArgs da) {
-    __shared__ uint32_t stack[STACK * 64];
-    const uint32_t id = blockIdx.x * 64 + threadIdx.x;
+__device__ __forceinline__ void di_final_px(const SceneView &sv, const DiArgs &da, uint32_t *stack, uint32_t id) {
     if (id >= da.P) return;
     float *hc = da.hist_cur + (size_t)id * 8;
     const f3 E0 = ld3(da.emis + (size_t)id * 3);
@@ -713,6 +713,124 @@ __global__ __launch_bounds__(64) void k_di_final(const SceneView sv, const DiArg
     float4 *q = (float4 *)hc;
     q[0] = make_float4(__uint_as_float(r.light), r.y.x, r.y.y, r.y.z);
     q[1] = make_float4(r.W, __uint_as_float(r.M), 0.0f, 0.0f);
+}
+template <int STACK>
+__global__ __launch_bounds__(64) void k_di_final(const SceneView sv, const DiArgs da) {
+    __shared__ uint32_t stack[STACK * 64];
+    di_final_px<STACK>(sv, da, stack, blockIdx.x * 64 + threadIdx.x);
+}
+
+// ---------------------------------------------------------------- batched scenes
+// The per-scene kernels above, over many scenes per launch: a block finds its scene by
+// binary search over the block prefix; WPB waves per block, one LDS stack per wave.
+template <int STACK, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_trace_shade_b(const SceneView *__restrict__ svs,
+                                                            const BounceArgs *__restrict__ bas,
+                                                            const uint32_t *__restrict__ blk, uint32_t n,
+                                                            uint32_t stride, uint32_t b) {
+    __shared__ uint32_t stack[WPB * STACK * 64];
+    const uint32_t s = batch_scene(blk, n, blockIdx.x);
+    const BounceArgs &ba = bas[(size_t)s * stride + b];
+    const uint32_t count = ba.bounce == 0 ? ba.P : ba.qcount[ba.bounce];
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t first = ((blockIdx.x - blk[s]) * WPB + w) * 64u;
+    if (first >= count) return;  // the whole wave
+    trace_shade_slice<STACK, false, false>(svs[s], ba, stack + w * STACK * 64, first + (threadIdx.x & 63u), count);
+}
+template <int STACK, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_nee_pdf_b(const SceneView *__restrict__ svs,
+                                                        const BounceArgs *__restrict__ bas,
+                                                        const uint32_t *__restrict__ blk, uint32_t n,
+                                                        uint32_t stride, uint32_t b) {
+    __shared__ uint32_t stack[WPB * STACK * 64];
+    const uint32_t s = batch_scene(blk, n, blockIdx.x);
+    const BounceArgs &ba = bas[(size_t)s * stride + b];
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t i = ((blockIdx.x - blk[s]) * WPB + w) * 64u + (threadIdx.x & 63u);
+    nee_pdf_ray<STACK>(svs[s], ba, stack + w * STACK * 64, i, ba.qcount[ba.bounce + 1]);
+}
+template <int PB>
+__global__ __launch_bounds__(PB) void k_di_initial_b(const SceneView *__restrict__ svs, const DiArgs *__restrict__ das,
+                                                     const uint32_t *__restrict__ blk, uint32_t n) {
+    const uint32_t s = batch_scene(blk, n, blockIdx.x);
+    di_initial_px(svs[s], das[s], (blockIdx.x - blk[s]) * PB + threadIdx.x);
+}
+template <int STACK, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_di_final_b(const SceneView *__restrict__ svs,
+                                                         const DiArgs *__restrict__ das,
+                                                         const uint32_t *__restrict__ blk, uint32_t n) {
+    __shared__ uint32_t stack[WPB * STACK * 64];
+    const uint32_t s = batch_scene(blk, n, blockIdx.x);
+    const uint32_t w = threadIdx.x >> 6;
+    di_final_px<STACK>(svs[s], das[s], stack + w * STACK * 64, (blockIdx.x - blk[s]) * 64u * WPB + threadIdx.x);
+}
+// queue counts of every scene to zero (a kernel, so that the pass can be graph-captured)
+__global__ void k_batch_reset(const BounceArgs *__restrict__ bas, uint32_t n, uint32_t stride, uint32_t words) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * words) return;
+    bas[(size_t)(i / words) * stride].qcount[i % words] = 0u;
+}
+// per scene: closest-hit rays and frames (k_frame_stats)
+__global__ void k_batch_stats(const BounceArgs *__restrict__ bas, uint32_t n, uint32_t stride, uint32_t B) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const BounceArgs &ba = bas[(size_t)s * stride];
+    unsigned long long r = ba.P;
+    for (uint32_t b = 1; b < B; ++b) r += ba.qcount[b];
+    ba.counters[0] += r;
+    ba.counters[2] += 1;
+}
+
+template <int STACK, int WPB>
+static void batch_rays(hipStream_t s, const BatchLaunch &L) {
+    for (uint32_t b = 0; b < L.B; ++b)
+        k_trace_shade_b<STACK, WPB><<<L.ray_blocks, 64 * WPB, 0, s>>>(L.svs, L.tba, L.blk_ray, L.n, L.B, b);
+}
+template <int STACK, int WPB>
+static void batch_nee(hipStream_t s, const BatchLaunch &L) {
+    for (uint32_t b = 0; b + 1 < L.B; ++b)
+        k_nee_pdf_b<STACK, WPB><<<L.ray_blocks, 64 * WPB, 0, s>>>(L.svs, L.nba, L.blk_ray, L.n, L.B, b);
+}
+template <int STACK, int WPB>
+static void batch_di_final(hipStream_t s, const BatchLaunch &L) {
+    k_di_final_b<STACK, WPB><<<L.ray_blocks, 64 * WPB, 0, s>>>(L.svs, L.das, L.blk_ray, L.n);
+}
+// STACK x WPB dispatch; an LDS stack of more than 64 entries runs one wave per block
+#define VRGI_BATCH_DISPATCH(FN)                                                              \
+    static void FN##_any(hipStream_t s, const BatchLaunch &L, uint32_t need) {               \
+        const uint32_t w = L.wpb;                                                              \
+        if (need + 2 <= 32) {                                                                  \
+            if (w >= 4) FN<32, 4>(s, L); else if (w == 2) FN<32, 2>(s, L); else FN<32, 1>(s, L); \
+        } else if (need + 2 <= 48) {                                                           \
+            if (w >= 4) FN<48, 4>(s, L); else if (w == 2) FN<48, 2>(s, L); else FN<48, 1>(s, L); \
+        } else if (need + 2 <= 64) {                                                           \
+            if (w >= 4) FN<64, 4>(s, L); else if (w == 2) FN<64, 2>(s, L); else FN<64, 1>(s, L); \
+        } else {                                                                               \
+            FN<VRGI_MAX_STACK, 1>(s, L);                                                       \
+        }                                                                                      \
+    }
+VRGI_BATCH_DISPATCH(batch_rays)
+VRGI_BATCH_DISPATCH(batch_nee)
+VRGI_BATCH_DISPATCH(batch_di_final)
+
+// one spp pass of every scene: reset, B bounces, NEE pdfs, resolve, postprocess, stats
+void launch_batch_pass(hipStream_t s, const BatchLaunch &L) {
+    if (!L.n) return;
+    const uint32_t words = L.B + 2;
+    k_batch_reset<<<(L.n * words + 255) / 256, 256, 0, s>>>(L.tba, L.n, L.B, words);
+    batch_rays_any(s, L, L.gneed);
+    if (L.nee && L.B > 1) batch_nee_any(s, L, L.lneed);
+    launch_batch_pixels_gi(s, L);
+    if (L.di) {
+        if (L.pb == 64) k_di_initial_b<64><<<L.px_blocks, 64, 0, s>>>(L.svs, L.das, L.blk_px, L.n);
+        else if (L.pb == 128) k_di_initial_b<128><<<L.px_blocks, 128, 0, s>>>(L.svs, L.das, L.blk_px, L.n);
+        else k_di_initial_b<256><<<L.px_blocks, 256, 0, s>>>(L.svs, L.das, L.blk_px, L.n);
+        batch_di_final_any(s, L, L.gneed);
+    } else {
+        launch_batch_pixels_spatial(s, L);
+    }
+    launch_batch_pixels_post(s, L);
+    k_batch_stats<<<(L.n + 63) / 64, 64, 0, s>>>(L.tba, L.n, L.B, L.B);
 }
 
 // ---------------------------------------------------------------- host launchers

@@ -433,6 +433,44 @@ __device__ __forceinline__ bool in_fallback(const uint8_t *fb, uint32_t px, uint
     return fb && fb[((px / W) / tile) * tx + (px % W) / tile];
 }
 
+// ---------------------------------------------------------------- batched scenes
+// Many scenes in one launch per stage: each scene owns a contiguous range of blocks
+// (`blk[s]` .. `blk[s + 1]`), so every wave belongs to one scene and reads that scene's
+// arguments from the per-scene arrays below at a uniform address.
+struct BatchPass {   // per pass of a batched frame
+    uint32_t first, last;  // first / last of the frame's spp passes
+    float inv_spp;
+    uint32_t pad;
+};
+struct BatchLaunch {
+    const SceneView *svs;     // [n]
+    const BounceArgs *tba;    // [n][B]: the trace of bounce b
+    const BounceArgs *nba;    // [n][B]: the NEE pdf of bounce b
+    const FrameArgs *fas;     // [n]
+    const DiArgs *das;        // [n]
+    const uint32_t *blk_ray;  // [n + 1] block prefix for 64 * wpb rays (or pixels) per block
+    const uint32_t *blk_px;   // [n + 1] block prefix for pb pixels per block
+    float *const *mean;       // [n] spp mean of the ReSTIR output
+    const BatchPass *pass;
+    uint32_t n, B, ray_blocks, px_blocks;
+    uint32_t gneed, lneed;    // traversal stack entries of the deepest geometry / light tree
+    uint32_t wpb, pb;
+    bool nee, di;
+};
+__device__ __forceinline__ uint32_t batch_scene(const uint32_t *__restrict__ blk, uint32_t n, uint32_t b) {
+    uint32_t lo = 0, hi = n - 1;  // the last scene whose first block is <= b
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (blk[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+void launch_batch_pass(hipStream_t s, const BatchLaunch &L);
+void launch_batch_pixels_gi(hipStream_t s, const BatchLaunch &L);  // accum + temporal
+void launch_batch_pixels_spatial(hipStream_t s, const BatchLaunch &L);
+void launch_batch_pixels_post(hipStream_t s, const BatchLaunch &L);  // postprocess + spp mean
+
 // exposed-face meshing of 32^3 chunks (mesher.hip)
 struct MeshArgsHost {
     const uint16_t *blocks;      // dims[0] x dims[1] x dims[2], x-major

@@ -46,6 +46,17 @@ bool spec_of(int kind, WorldSpec *s) {
         *s = WorldSpec{256, 128, 256, 8, 120, 3.0, 4, 200, 0.005, 4, 32};
         return true;
     }
+    if (kind >= 100 && kind < 100 + 65536) {
+        // scene i = kind - 100 of the batched workload: each side 32..160 voxels (a multiple
+        // of 32, drawn per scene), terrain up to 3/4 of the height, caves and slabs in
+        // proportion to the footprint, 1% emissive surface voxels
+        const uint64_t h = mix64(0xBA7Cull ^ (uint64_t)(kind - 100));
+        const uint32_t sx = 32u * (1u + (uint32_t)(mix64(h + 1) % 5)), sy = 32u * (1u + (uint32_t)(mix64(h + 2) % 5));
+        const uint32_t sz = 32u * (1u + (uint32_t)(mix64(h + 3) % 5));
+        const double period = 4.0 + (double)(mix64(h + 4) % 12);
+        *s = WorldSpec{sx, sy, sz, (int)(sy / 8), (int)(sy * 3 / 4), period, 4, (int)(sx * sz / 1024), 0.01, 4, 32};
+        return true;
+    }
     return false;
 }
 

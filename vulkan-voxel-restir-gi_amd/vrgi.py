@@ -430,6 +430,79 @@ class Context:
         return nodes
 
 
+# ------------------------------------------------------------------ batched scenes
+class BatchConfig(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("rays_per_block", C.c_uint32), ("pixels_per_block", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("n_scenes", C.c_uint32), ("rays_per_block", C.c_uint32), ("pixels_per_block", C.c_uint32),
+                ("tuned", C.c_uint32), ("graph", C.c_uint32), ("pad", C.c_uint32), ("triangles", C.c_uint64),
+                ("light_triangles", C.c_uint64), ("pixels", C.c_uint64), ("build_ms", C.c_float),
+                ("passes_ms", C.c_float), ("rays", C.c_uint64)]
+
+
+BATCH_NO_GRAPH = 1
+
+
+class Batch:
+    """vrgi_batch over Context objects: one multi-tree build and one launch per stage for
+    all scenes; each of the `spp` passes is one frame of every scene."""
+
+    def __init__(self, contexts, spp=1, rays_per_block=0, pixels_per_block=0, flags=0):
+        self.L = lib()
+        self.contexts = list(contexts)
+        cfg = BatchConfig(spp, rays_per_block, pixels_per_block, flags)
+        hs = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        self.h = C.c_void_p()
+        check(self.L.vrgi_batch_create(hs, len(self.contexts), C.byref(cfg), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            check(self.L.vrgi_batch_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def render(self, cams, nee_type=0, debug_view=0, spatial=0):
+        arr = (Camera * len(cams))(*cams)
+        p = Prefs(nee_type, debug_view, spatial)
+        check(self.L.vrgi_batch_render(self.h, arr, C.byref(p)))
+
+    def sync(self):
+        check(self.L.vrgi_batch_sync(self.h))
+
+    def mean(self, k):
+        c = self.contexts[k]
+        out = np.zeros((c.cfg.width * c.cfg.height, 3), np.float32)
+        check(self.L.vrgi_batch_read_mean(self.h, C.c_uint32(k), ptr(out)))
+        return out
+
+    def info(self):
+        i = BatchInfo()
+        check(self.L.vrgi_batch_info_get(self.h, C.byref(i)))
+        return i
+
+
+def batch_scene(i):
+    """Scene i of the batched workload: (objects, mats, lum, (W, H), camera). The world is
+    kind 100 + i (sides 32..160 voxels); the square resolution is 128, 256, 384 or 512
+    pixels, drawn per scene; the camera looks from above one corner to the far side."""
+    blocks, nt = world(100 + i, i)
+    mats, lum = world_materials(100 + i, i)
+    tr = world_translucent(100 + i, nt)
+    objs = chunk_objects(blocks, nt, tr)
+    h = (i * 2654435761 + 12345) & 0xFFFFFFFF
+    res = 128 * (1 + (h >> 13) % 4)
+    dx, dy, dz = blocks.shape
+    cam = camera_look([0.1 * dx, 0.95 * dy, 0.1 * dz], [0.6 * dx, 0.2 * dy, 0.6 * dz])
+    return objs, mats, lum, (res, res), cam
+
 def config(width, height, bounces=3, rr_start=0xFFFFFFFF, rr_min=0.1, seed=0, flags=0, radius=15.0,
            mode=0, di_candidates=32, sort_bits=0):
     cfg = Config()
