@@ -10,8 +10,14 @@ Second part of the metric (field "learned_cache"): time to equal RMSE of the lea
 radiance cache against the plain path tracer on the same scene at 1920x1080, static
 camera, both against an FP64 reference of the plain 8-bounce tracer at 256 spp.
 
+Third part (field "batched", not the headline): 64 independent seeded scenes of uneven
+size (sides 32..160 voxels, 128^2..512^2 pixels), 2 spp, 3 bounces, NEE, ReSTIR DI (32
+candidates, 5 spatial neighbours within 30 px), every scene's BVHs rebuilt in one
+multi-tree build per batched frame, one launch per stage for all scenes; beside it the
+same frames through 64 separate contexts (one render call per scene and pass).
+
 Prints ONE JSON line (rank 0). Progress goes to stderr.
-  python bench.py [--gpus 1] [--steps 16] [--warmup 3] [--no-cpu-baseline] [--no-cache-study]
+  python bench.py [--gpus 1] [--steps 16] [--warmup 3] [--no-cpu-baseline] [--no-cache-study] [--no-batch]
 """
 import argparse
 import ctypes as C
@@ -113,6 +119,61 @@ def cpu_baseline(objs, mats, lum, cams):
                       f"frame with NEE and ReSTIR GI ({t_frame:.2f} s, {threads} threads)"}
 
 
+def batched_bench(steps):
+    """The 64-scene batched workload: ms per batched frame (2 passes of every scene, BVH
+    rebuilds included), closest-hit Mrays/s, and the same 2 frames of every scene through its
+    own context (64 render calls per pass)."""
+    import vrgi
+    n = 64
+    log(f"batched workload: generating {n} scenes")
+    scenes = [vrgi.batch_scene(i) for i in range(n)]
+    flags = vrgi.FLAG_REBUILD_EVERY_FRAME
+    ctxs = []
+    try:
+        for objs, mats, lum, (W, H), cam in scenes:
+            c = vrgi.Context(vrgi.config(W, H, 3, mode=1, radius=30.0, di_candidates=32, flags=flags), mats, lum)
+            c.add_objects(objs)
+            ctxs.append(c)
+        cams = [sc[4] for sc in scenes]
+        # the same frames through the scenes' own contexts (each rebuilds; 2 frames = 2 spp)
+        for c, cam in zip(ctxs, cams):
+            c.render(cam, nee_type=NEE, spatial=SPATIAL)
+        for c in ctxs:
+            c.sync()
+        t0 = time.perf_counter()
+        for c, cam in zip(ctxs, cams):
+            for _ in range(2):
+                c.render(cam, nee_type=NEE, spatial=SPATIAL)
+        for c in ctxs:
+            c.sync()
+        seq_ms = 1000.0 * (time.perf_counter() - t0)
+        with vrgi.Batch(ctxs, spp=2) as bt:
+            log("batched workload: autotuning and warm-up")
+            for _ in range(8):
+                bt.render(cams, nee_type=NEE, spatial=SPATIAL)
+            bt.sync()
+            bt.info()
+            log(f"batched workload: timing {steps} batched frames")
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                bt.render(cams, nee_type=NEE, spatial=SPATIAL)
+            bt.sync()
+            dt = time.perf_counter() - t0
+            info = bt.info()
+    finally:
+        for c in ctxs:
+            c.close()
+    ms = 1000.0 * dt / steps
+    return {"workload": f"{n} seeded scenes (sides 32..160 voxels), 128^2..512^2 px, 2 spp, 3 bounces, NEE, "
+                        "ReSTIR DI 32 candidates + temporal + 5 spatial (30 px), BVH rebuild every batched frame",
+            "scenes": n, "triangles": int(info.triangles), "light_triangles": int(info.light_triangles),
+            "pixels": int(info.pixels), "ms_per_batched_frame": round(ms, 3),
+            "mrays_per_s": round(info.rays / dt / 1e6, 2),
+            "last_frame_build_ms": round(float(info.build_ms), 3), "last_frame_passes_ms": round(float(info.passes_ms), 3),
+            "rays_per_block": int(info.rays_per_block), "pixels_per_block": int(info.pixels_per_block),
+            "graph": bool(info.graph), "scenes_separately_ms": round(seq_ms, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +184,7 @@ def main():
                     help="skip the learned-cache time-to-equal-RMSE study")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (pmc counter scaling)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched workload")
     ap.add_argument("--serial-build", action="store_true",
                     help="one BVH buffer set built on the frame stream (default: frame f + 1 builds beside frame f)")
     ap.add_argument("--serial-nee", action="store_true",
@@ -212,6 +274,9 @@ def main():
         cache = convergence.study(objs, mats, lum, cams[0], W_PX, H_PX, bounces=8, rr_start=RR_START, ref_spp=256,
                                   frames=32, warmup=32, nee=NEE, spatial=SPATIAL, cache_kw=dict(query_vertex=0),
                                   log=log)
+    batched = None
+    if not a.no_batch:
+        batched = batched_bench(a.steps)
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline (oracle on the host cores, bounded sample)")
@@ -228,6 +293,7 @@ def main():
         "roofline": roofline,
         "kernels": others,
         "learned_cache": cache,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
