@@ -364,6 +364,26 @@ int vrgi_batch_sync(vrgi_batch *batch);
 int vrgi_batch_read_mean(vrgi_batch *batch, uint32_t scene, float *rgb);
 int vrgi_batch_info_get(vrgi_batch *batch, vrgi_batch_info *info);
 
+/* ---- device-resident voxel world -------------------------------------------------
+ * game_system's chunk manager (chunk_manager.rs:173-399) on the GPU, for callers that
+ * edit the world every frame. vrgi_voxels_load keeps the block grid on the device and
+ * makes every 32^3 chunk an object of the scene, with the keys and transforms the CPU
+ * chunk path gives: key (cx * ny + cy) * nz + cz, translation (32 cx, 32 cy, 32 cz),
+ * vertices of chunk::gen_mesh (chunk.rs:158-292), the light BL-BVH and light vertices of
+ * upload_object (scene.rs:547-725). vrgi_voxels_set writes voxels; the chunks whose
+ * meshes can change (the voxel's chunk and, on a chunk border, the neighbour across it)
+ * are re-meshed and their objects replaced at the next vrgi_render (or
+ * vrgi_voxels_commit): one mesher pass and one multi-tree light build for all of them.
+ * Replaced objects are released at the next commit. dims: multiples of 32; blocks:
+ * x-major u16, air = n_types; translucent: one flag per block type. */
+int vrgi_voxels_load(vrgi_ctx *ctx, const uint16_t *blocks, const uint32_t dims[3], uint32_t n_types,
+                     const uint8_t *translucent);
+/* writes blocks[i] at voxel (xyz[3i], xyz[3i+1], xyz[3i+2]); the n positions must be distinct */
+int vrgi_voxels_set(vrgi_ctx *ctx, uint64_t n, const uint32_t *xyz, const uint16_t *blocks);
+int vrgi_voxels_commit(vrgi_ctx *ctx);
+/* test hook: the device grid (may be NULL), commits and chunks re-meshed so far */
+int vrgi_voxels_read(vrgi_ctx *ctx, uint16_t *blocks, uint64_t *commits, uint64_t *chunks_remeshed);
+
 /* ---- synthetic scenes (caller side: the game_system worldgen + chunk mesher) ---- */
 
 /* Seeded voxel world of one of the benchmark workloads: kind 0 = 64x64x64
@@ -378,6 +398,16 @@ int vrgi_world_materials(int kind, uint64_t seed, vrgi_material *mats, float *lu
                          uint32_t cap_slots, uint32_t *n_slots);
 /* translucent flag per block type of a generated world (air excluded). */
 int vrgi_world_translucent(int kind, uint8_t *translucent, uint32_t cap);
+
+/* Edits of the large dynamic workload (caller side): n_steps steps of n_walkers seeded
+ * random walkers (walkers: 3 x n_walkers voxel positions, updated in place); a step moves
+ * walker (step % n_walkers) by one voxel along a random axis and toggles the voxel it
+ * reaches (solid -> air, air -> block type 2). blocks is updated in place; the distinct
+ * voxels touched (first-touch order) and their final ids go to xyz / vals (capacity
+ * n_steps), their count to *n_out. */
+int vrgi_world_walk(uint16_t *blocks, const uint32_t dims[3], uint32_t n_types, uint64_t seed,
+                    uint32_t n_walkers, uint32_t *walkers, uint64_t n_steps, uint32_t *xyz,
+                    uint16_t *vals, uint64_t *n_out);
 
 /* chunk::gen_mesh (chunk.rs:158-292) for the 32^3 chunk at chunk coordinates
  * (cx, cy, cz) of a world of dims; voxels outside the world are air. Writes at

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <memory>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -75,6 +76,9 @@ struct vrgi_object {
     bool light_empty = true;
     float light_aabb[6] = {0, 0, 0, 0, 0, 0};
     float power = 0.0f;
+    // set for a chunk object made by a voxel-world commit: the device pointers above are
+    // slices of this shared allocation (freed with its last object)
+    std::shared_ptr<void> arena;
 };
 
 struct vrgi_ctx {
@@ -187,6 +191,19 @@ struct vrgi_ctx {
     bool accum_on = false;
     uint64_t accum_frames = 0;
     DevBuf<double> accum;
+
+    // device-resident voxel world (vrgi_voxels_*): grid, per-type data, pending edits
+    struct Voxels {
+        bool on = false, pending = false;
+        uint32_t dims[3] = {0, 0, 0}, nc[3] = {0, 0, 0}, n_types = 0;
+        DevBuf<uint16_t> blocks, edit_val;
+        DevBuf<uint8_t> transl, em, dirty, mask;
+        DevBuf<float> lum, prim9;
+        DevBuf<uint32_t> edit_xyz, chunks, face_off, faces, lface_off, lfaces, trees, depth, tree_base;
+        DevBuf<uint64_t> vert_off, light_off;
+        Builder builder;
+        uint64_t commits = 0, chunks_remeshed = 0;
+    } vx;
 
     // GPU mesher workspace
     DevBuf<uint16_t> m_blocks;
@@ -580,15 +597,213 @@ void free_object(vrgi_object *o) {
         (void)hipSetDevice(o->ctx->device);
         (void)hipStreamSynchronize(o->ctx->stream);
     }
-    if (o->d_verts) (void)hipFree(o->d_verts);
-    if (o->d_light_bvh) (void)hipFree(o->d_light_bvh);
-    if (o->d_lpairs) (void)hipFree(o->d_lpairs);
-    if (o->d_light_verts) (void)hipFree(o->d_light_verts);
-    delete o;
+    if (!o->arena) {
+        if (o->d_verts) (void)hipFree(o->d_verts);
+        if (o->d_light_bvh) (void)hipFree(o->d_light_bvh);
+        if (o->d_lpairs) (void)hipFree(o->d_lpairs);
+        if (o->d_light_verts) (void)hipFree(o->d_light_verts);
+    }
+    delete o;  // drops its share of the arena
 }
 
 void release(vrgi_object *o) {
     if (o && o->refs.fetch_sub(1) == 1) free_object(o);
+}
+
+// device memory of the chunk objects of one voxel-world commit
+struct VxArena {
+    DevBuf<vrgi_vertex3d> verts;
+    DevBuf<float> tri9, power;
+    DevBuf<uint32_t> ids;
+    DevBuf<vrgi_bvh_node> nodes;
+    DevBuf<LPair> lpairs;
+    DevBuf<vrgi_light_vertex3d> lverts;
+};
+
+// Re-meshes the dirty chunks of the voxel world and replaces their objects: what the
+// reference's chunk manager does per dirty chunk (chunk_manager.rs:338-399: gen_mesh,
+// upload_object, add_object), for all of them at once. Vertices, the emissive gather and
+// the light BL-BVHs (one multi-tree build) stay on the device; the light vertices in
+// leaf order (scene.rs:509-524) and each object's (box, power) are formed on the host
+// from one readback, as upload_object forms them.
+void voxels_commit(vrgi_ctx *c) {
+    auto &V = c->vx;
+    if (!V.on || !V.pending) return;
+    hipStream_t s = c->stream;
+    HIP_CHECK(hipStreamSynchronize(c->up_stream));
+    const uint32_t nx = V.nc[0], ny = V.nc[1], nz = V.nc[2], nch = nx * ny * nz;
+    std::vector<uint8_t> dirty(nch);
+    HIP_CHECK(hipMemcpyAsync(dirty.data(), V.dirty.p, nch, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    V.pending = false;
+    // every earlier frame has ended: objects replaced before now can go
+    for (vrgi_object *o : c->old) release(o);
+    c->old.clear();
+    std::vector<uint32_t> coords, keys;
+    for (uint32_t k = 0; k < nch; ++k)
+        if (dirty[k]) {
+            keys.push_back(k);
+            coords.push_back(k / (ny * nz));
+            coords.push_back((k / nz) % ny);
+            coords.push_back(k % nz);
+        }
+    if (keys.empty()) return;
+    HIP_CHECK(hipMemsetAsync(V.dirty.p, 0, nch, s));
+    const uint32_t n = (uint32_t)keys.size();
+    V.chunks.reserve(3 * (size_t)n);
+    V.mask.reserve((size_t)n * 32768);
+    V.face_off.reserve((size_t)n * 32768);
+    V.lface_off.reserve((size_t)n * 32768);
+    V.faces.reserve(n);
+    V.lfaces.reserve(n);
+    V.vert_off.reserve(n);
+    V.light_off.reserve(n);
+    HIP_CHECK(hipMemcpyAsync(V.chunks.p, coords.data(), (size_t)12 * n, hipMemcpyHostToDevice, s));
+    MeshArgsHost h{};
+    h.blocks = V.blocks.p;
+    for (int k = 0; k < 3; ++k) h.dims[k] = V.dims[k];
+    h.air = V.n_types;
+    h.translucent = V.transl.p;
+    h.chunks = V.chunks.p;
+    h.n_chunks = n;
+    h.mask = V.mask.p;
+    h.face_off = V.face_off.p;
+    h.faces = V.faces.p;
+    launch_mesh_count(s, h);
+    launch_light_scan(s, h, V.em.p, V.lface_off.p, V.lfaces.p);
+    std::vector<uint32_t> faces(n), lfaces(n);
+    HIP_CHECK(hipMemcpyAsync(faces.data(), V.faces.p, (size_t)4 * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(lfaces.data(), V.lfaces.p, (size_t)4 * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint64_t> voff(n), loff(n);
+    std::vector<uint32_t> beg, cnt, base, tree_of(n, VRGI_INVALID);
+    uint64_t nv = 0, nl = 0, nodes = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        voff[i] = nv;
+        loff[i] = nl;
+        nv += 6ull * faces[i];
+        const uint64_t m = 2ull * lfaces[i];
+        if (m) {
+            tree_of[i] = (uint32_t)beg.size();
+            beg.push_back((uint32_t)nl);
+            cnt.push_back((uint32_t)m);
+            base.push_back((uint32_t)nodes);
+            nodes += 2 * m - 1;
+        }
+        nl += m;
+    }
+    if (nl >= (1ull << 31) || nodes >= (1ull << 32) - 1 || nv >= (1ull << 40))
+        throw std::runtime_error("voxel commit above 2^31 light triangles");
+    auto A = std::make_shared<VxArena>();
+    A->verts.reserve(nv);
+    A->tri9.reserve(9 * nl);
+    A->power.reserve(nl);
+    A->ids.reserve(nl);
+    A->nodes.reserve(nodes);
+    A->lpairs.reserve(nodes);
+    A->lverts.reserve(3 * nl);
+    V.prim9.reserve(9 * nl);
+    HIP_CHECK(hipMemcpyAsync(V.vert_off.p, voff.data(), (size_t)8 * n, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(V.light_off.p, loff.data(), (size_t)8 * n, hipMemcpyHostToDevice, s));
+    h.vert_off = V.vert_off.p;
+    h.out = A->verts.p;
+    if (nv) launch_mesh_emit(s, h);
+    launch_light_gather(s, h, V.lum.p, V.em.p, V.lface_off.p, V.light_off.p, A->tri9.p, A->power.p, V.prim9.p,
+                        A->ids.p, (uint32_t)nl);
+    const uint32_t nT = (uint32_t)beg.size();
+    std::vector<vrgi_bvh_node> hn(nodes);
+    std::vector<float> t9(9 * nl), pw(nl);
+    if (nT) {
+        std::vector<uint32_t> tr(3 * (size_t)nT);
+        for (uint32_t t = 0; t < nT; ++t) { tr[t] = beg[t]; tr[nT + t] = cnt[t]; tr[2 * nT + t] = base[t]; }
+        V.trees.reserve(tr.size());
+        V.depth.reserve(nT);
+        V.tree_base.reserve(nT + 1);
+        std::vector<uint32_t> tb(base);
+        tb.push_back((uint32_t)nodes);
+        HIP_CHECK(hipMemcpyAsync(V.trees.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(V.tree_base.p, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, s));
+        if (V.builder.build(s, V.prim9.p, A->power.p, A->ids.p, (uint32_t)nl, nT, V.trees.p, V.trees.p + nT,
+                            V.trees.p + 2 * nT, base.data(), A->nodes.p, V.depth.p, nullptr) != 0)
+            throw HipError("voxel light build exceeded its workspace");
+        launch_light_pairs_multi(s, A->nodes.p, V.tree_base.p, nT, (uint32_t)nodes, A->lpairs.p);
+        HIP_CHECK(hipMemcpyAsync(hn.data(), A->nodes.p, nodes * sizeof(vrgi_bvh_node), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(t9.data(), A->tri9.p, t9.size() * 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(pw.data(), A->power.p, pw.size() * 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        // light vertices in leaf order: the corners of the light triangle each leaf holds
+        // (right_or_prim = its triangle index in the chunk; the light list is in triangle
+        // order, so the chunk's light j with ids[j] == prim is found through an index map)
+        std::vector<vrgi_light_vertex3d> lv(3 * nl);
+        std::vector<uint32_t> ids(nl);
+        HIP_CHECK(hipMemcpy(ids.data(), A->ids.p, ids.size() * 4, hipMemcpyDeviceToHost));
+        std::map<uint32_t, uint32_t> slot;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (tree_of[i] == VRGI_INVALID) continue;
+            const uint32_t t = tree_of[i], m = cnt[t];
+            slot.clear();
+            for (uint32_t j = 0; j < m; ++j) slot[ids[loff[i] + j]] = j;
+            size_t w = 3 * loff[i];
+            for (uint32_t k = 0; k < 2 * m - 1; ++k) {
+                const vrgi_bvh_node &nd = hn[base[t] + k];
+                if (nd.left != VRGI_INVALID) continue;
+                const float *tri = t9.data() + 9 * (loff[i] + slot.at(nd.right_or_prim));
+                for (int q = 0; q < 3; ++q) {
+                    std::memcpy(lv[w].position, tri + 3 * q, 12);
+                    lv[w].bvh_node_idx = k;
+                    ++w;
+                }
+            }
+        }
+        HIP_CHECK(hipMemcpyAsync(A->lverts.p, lv.data(), lv.size() * sizeof(vrgi_light_vertex3d), hipMemcpyHostToDevice,
+                                 s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t key = keys[i];
+        vrgi_object *o = nullptr;
+        if (faces[i]) {
+            o = new vrgi_object();
+            o->ctx = c;
+            o->arena = A;
+            o->n_tris = 2 * faces[i];
+            o->d_verts = A->verts.p + voff[i];
+            const uint32_t t = tree_of[i];
+            if (t != VRGI_INVALID) {
+                const uint32_t m = cnt[t];
+                o->n_light = m;
+                o->d_light_bvh = A->nodes.p + base[t];
+                o->d_lpairs = A->lpairs.p + base[t];
+                o->d_light_verts = A->lverts.p + 3 * loff[i];
+                o->h_light_bvh.assign(hn.begin() + base[t], hn.begin() + base[t] + (2 * m - 1));
+                // the (Aabb, power) build_bl_bvh returns (build.rs:359-371)
+                const vrgi_bvh_node &r = o->h_light_bvh[0];
+                o->light_empty = false;
+                if (r.left == VRGI_INVALID) {
+                    const float *tri = t9.data() + 9 * loff[i];
+                    for (int k = 0; k < 3; ++k) {
+                        o->light_aabb[k] = std::min(std::min(tri[k], tri[3 + k]), tri[6 + k]);
+                        o->light_aabb[3 + k] = std::max(std::max(tri[k], tri[3 + k]), tri[6 + k]);
+                    }
+                    o->power = pw[loff[i]];
+                } else {
+                    for (int k = 0; k < 3; ++k) { o->light_aabb[k] = r.min[k]; o->light_aabb[3 + k] = r.max[k]; }
+                    o->power = r.power;
+                }
+            }
+        }
+        auto it = c->objects.find(key);
+        if (it != c->objects.end()) {
+            if (it->second.first) c->old.push_back(it->second.first);
+            c->objects.erase(it);
+        }
+        const float cx = (float)(32 * coords[3 * i]), cy = (float)(32 * coords[3 * i + 1]);
+        const float cz = (float)(32 * coords[3 * i + 2]);
+        c->objects[key] = {o, std::vector<float>{1, 0, 0, 0, 1, 0, 0, 0, 1, cx, cy, cz}};
+    }
+    V.commits += 1;
+    V.chunks_remeshed += n;
+    c->inst_dirty = c->geo_dirty = true;
 }
 
 bool valid_xf(const float *xf) {
@@ -943,6 +1158,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     }
     rec(0);
     bool built = false;
+    if (c->vx.pending) voxels_commit(c);
     if (c->inst_dirty) {
         rebuild_instances(c);
         c->inst_dirty = false;
@@ -1781,6 +1997,95 @@ int vrgi_mesh_chunks(vrgi_ctx *c, const uint16_t *blocks, const uint32_t dims[3]
     API_END
 }
 
+int vrgi_voxels_load(vrgi_ctx *c, const uint16_t *blocks, const uint32_t dims[3], uint32_t n_types,
+                     const uint8_t *translucent) {
+    API_BEGIN
+    if (!c || !blocks || !dims || !translucent) return fail(VRGI_E_INVALID, "null argument");
+    for (int k = 0; k < 3; ++k)
+        if (dims[k] == 0 || dims[k] % 32 || dims[k] > 32768) return fail(VRGI_E_INVALID, "dims must be multiples of 32 in [32, 32768]");
+    const size_t nvox = (size_t)dims[0] * dims[1] * dims[2];
+    if (nvox > ((size_t)1 << 32)) return fail(VRGI_E_LIMIT, "more than 2^32 voxels");
+    if (n_types == 0 || (size_t)n_types * 6 > c->lum.size()) return fail(VRGI_E_INVALID, "n_types * 6 exceeds the material slots");
+    for (size_t i = 0; i < nvox; ++i)
+        if (blocks[i] > n_types) return fail(VRGI_E_INVALID, "block id above n_types (air)");
+    set_dev(c);
+    auto &V = c->vx;
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) { V.dims[k] = dims[k]; V.nc[k] = dims[k] / 32; }
+    V.n_types = n_types;
+    const uint32_t nch = V.nc[0] * V.nc[1] * V.nc[2];
+    V.blocks.reserve(nvox);
+    V.transl.reserve(n_types + 1);
+    V.em.reserve(n_types + 1);
+    V.lum.reserve(c->lum.size());
+    V.dirty.reserve(nch);
+    std::vector<uint8_t> tr(translucent, translucent + n_types), em(n_types + 1, 0);
+    tr.push_back(0);
+    for (uint32_t b = 0; b < n_types; ++b)
+        for (int f = 0; f < 6; ++f)
+            if (c->lum[(size_t)b * 6 + f] > 0.0f) em[b] |= (uint8_t)(1u << f);
+    HIP_CHECK(hipMemcpyAsync(V.blocks.p, blocks, nvox * 2, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(V.transl.p, tr.data(), tr.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(V.em.p, em.data(), em.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(V.lum.p, c->lum.data(), c->lum.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemsetAsync(V.dirty.p, 1, nch, c->stream));
+    V.on = true;
+    V.pending = true;
+    voxels_commit(c);
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_voxels_set(vrgi_ctx *c, uint64_t n, const uint32_t *xyz, const uint16_t *blocks) {
+    API_BEGIN
+    if (!c || (n && (!xyz || !blocks))) return fail(VRGI_E_INVALID, "null argument");
+    auto &V = c->vx;
+    if (!V.on) return fail(VRGI_E_STATE, "no voxel world loaded");
+    if (n > (1ull << 31)) return fail(VRGI_E_LIMIT, "more than 2^31 voxel writes");
+    for (uint64_t i = 0; i < n; ++i) {
+        if (blocks[i] > V.n_types) return fail(VRGI_E_INVALID, "block id above n_types (air)");
+        if (xyz[3 * i] >= V.dims[0] || xyz[3 * i + 1] >= V.dims[1] || xyz[3 * i + 2] >= V.dims[2])
+            return fail(VRGI_E_INVALID, "voxel outside the world");
+    }
+    if (n == 0) return VRGI_OK;
+    set_dev(c);
+    V.edit_xyz.reserve(3 * n);
+    V.edit_val.reserve(n);
+    HIP_CHECK(hipMemcpyAsync(V.edit_xyz.p, xyz, (size_t)12 * n, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(V.edit_val.p, blocks, (size_t)2 * n, hipMemcpyHostToDevice, c->stream));
+    launch_vx_set(c->stream, V.blocks.p, V.dims, n, V.edit_xyz.p, V.edit_val.p, V.dirty.p);
+    V.pending = true;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_voxels_commit(vrgi_ctx *c) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    if (!c->vx.on) return fail(VRGI_E_STATE, "no voxel world loaded");
+    set_dev(c);
+    voxels_commit(c);
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_voxels_read(vrgi_ctx *c, uint16_t *blocks, uint64_t *commits, uint64_t *chunks_remeshed) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    auto &V = c->vx;
+    if (!V.on) return fail(VRGI_E_STATE, "no voxel world loaded");
+    set_dev(c);
+    if (blocks) {
+        HIP_CHECK(hipMemcpyAsync(blocks, V.blocks.p, (size_t)V.dims[0] * V.dims[1] * V.dims[2] * 2,
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    if (commits) *commits = V.commits;
+    if (chunks_remeshed) *chunks_remeshed = V.chunks_remeshed;
+    return VRGI_OK;
+    API_END
+}
+
 int vrgi_frame_set(vrgi_ctx *c, uint64_t frame) {
     API_BEGIN
     if (!c) return fail(VRGI_E_INVALID, "null context");
@@ -2207,6 +2512,10 @@ int vrgi_batch_render(vrgi_batch *bt, const vrgi_camera *cams, const vrgi_prefs 
         }
     // the scenes' own queued work (uploads, earlier single renders) ends first
     for (vrgi_ctx *c : bt->sc) {
+        if (c->vx.pending) {
+            HIP_CHECK(hipStreamSynchronize(s));
+            voxels_commit(c);
+        }
         if (c->inst_dirty) {
             rebuild_instances(c);
             c->inst_dirty = false;

@@ -486,6 +486,15 @@ struct MeshArgsHost {
     vrgi_vertex3d *out;
 };
 void launch_mesh_count(hipStream_t s, const MeshArgsHost &h);
+// device-resident voxel edits (mesher.hip)
+void launch_vx_set(hipStream_t s, uint16_t *blocks, const uint32_t dims[3], uint64_t n, const uint32_t *xyz,
+                   const uint16_t *val, uint8_t *dirty);
+void launch_light_scan(hipStream_t s, const MeshArgsHost &h, const uint8_t *em, uint32_t *lface_off, uint32_t *lfaces);
+void launch_light_gather(hipStream_t s, const MeshArgsHost &h, const float *lum, const uint8_t *em,
+                         const uint32_t *lface_off, const uint64_t *light_off, float *tri9, float *power,
+                         float *prim9, uint32_t *ids, uint32_t n_light);
+void launch_light_pairs_multi(hipStream_t s, const vrgi_bvh_node *nodes, const uint32_t *base, uint32_t n_trees,
+                              uint32_t n_nodes, LPair *out);
 void launch_mesh_emit(hipStream_t s, const MeshArgsHost &h);
 
 // queue reordering by (direction octant, origin Morton code) between bounces

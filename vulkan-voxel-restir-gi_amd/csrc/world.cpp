@@ -46,6 +46,10 @@ bool spec_of(int kind, WorldSpec *s) {
         *s = WorldSpec{256, 128, 256, 8, 120, 3.0, 4, 200, 0.005, 4, 32};
         return true;
     }
+    if (kind == 4) {  // 512x256x512 terrain + 400 boxes, 1.1% emissive surface voxels
+        *s = WorldSpec{512, 256, 512, 16, 240, 8.0, 4, 400, 0.011, 4, 32};
+        return true;
+    }
     if (kind >= 100 && kind < 100 + 65536) {
         // scene i = kind - 100 of the batched workload: each side 32..160 voxels (a multiple
         // of 32, drawn per scene), terrain up to 3/4 of the height, caves and slabs in
@@ -235,6 +239,40 @@ int vrgi_gen_mesh_chunk(const uint16_t *blocks, const uint32_t dims[3], uint32_t
             }
     *n_out = n;
     return n <= cap ? VRGI_OK : VRGI_E_LIMIT;
+}
+
+int vrgi_world_walk(uint16_t *blocks, const uint32_t dims[3], uint32_t n_types, uint64_t seed, uint32_t n_walkers,
+                    uint32_t *walkers, uint64_t n_steps, uint32_t *xyz, uint16_t *vals, uint64_t *n_out) {
+    if (!blocks || !dims || !walkers || !n_out || n_walkers == 0 || (n_steps && (!xyz || !vals))) return VRGI_E_INVALID;
+    const uint64_t dx = dims[0], dy = dims[1], dz = dims[2], nvox = dx * dy * dz;
+    if (nvox == 0 || n_types < 3) return VRGI_E_INVALID;
+    std::vector<uint64_t> seen((nvox + 63) / 64, 0);
+    std::vector<uint64_t> order;
+    order.reserve(n_steps);
+    for (uint64_t st = 0; st < n_steps; ++st) {
+        uint32_t *w = walkers + 3 * (st % n_walkers);
+        const uint64_t h = mix64(seed ^ mix64(st + 0x57A1Cull));
+        const int axis = (int)(h % 3);
+        const uint64_t lim = axis == 0 ? dx : axis == 1 ? dy : dz;
+        if ((h >> 8) & 1u) { if (w[axis] + 1 < lim) w[axis] += 1; }
+        else if (w[axis] > 0) w[axis] -= 1;
+        const uint64_t i = ((uint64_t)w[0] * dy + w[1]) * dz + w[2];
+        uint16_t &b = blocks[i];
+        b = b == (uint16_t)n_types ? (uint16_t)2 : (uint16_t)n_types;  // solid <-> air
+        if (!((seen[i >> 6] >> (i & 63)) & 1u)) {
+            seen[i >> 6] |= 1ull << (i & 63);
+            order.push_back(i);
+        }
+    }
+    for (size_t k = 0; k < order.size(); ++k) {
+        const uint64_t i = order[k];
+        xyz[3 * k] = (uint32_t)(i / (dy * dz));
+        xyz[3 * k + 1] = (uint32_t)((i / dz) % dy);
+        xyz[3 * k + 2] = (uint32_t)(i % dz);
+        vals[k] = blocks[i];
+    }
+    *n_out = order.size();
+    return VRGI_OK;
 }
 
 }  // extern "C"

@@ -161,6 +161,19 @@ def gen_mesh_chunk(blocks, n_types, translucent, cx, cy, cz):
     return out
 
 
+def world_walk(blocks, n_types, seed, walkers, n_steps):
+    """Random-walk toggles of the dynamic workload: updates `blocks` and `walkers`
+    (uint32 [n, 3]) in place; returns the distinct touched voxels and their final ids."""
+    assert blocks.flags.c_contiguous and blocks.dtype == np.uint16
+    assert walkers.flags.c_contiguous and walkers.dtype == np.uint32
+    dims = np.array(blocks.shape, np.uint32)
+    xyz = np.zeros((n_steps, 3), np.uint32)
+    vals = np.zeros(n_steps, np.uint16)
+    n = C.c_uint64()
+    check(lib().vrgi_world_walk(ptr(blocks), ptr(dims), n_types, C.c_uint64(seed), len(walkers), ptr(walkers),
+                                C.c_uint64(n_steps), ptr(xyz), ptr(vals), C.byref(n)))
+    return xyz[:n.value], vals[:n.value]
+
 def chunk_objects(blocks, n_types, translucent):
     """Meshes every 32^3 chunk: list of (key, transform[12], vertices); empty chunks
     are kept (they become Empty handles, as in the reference's chunk manager)."""
@@ -284,6 +297,28 @@ class Context:
         st = FrameStats()
         check(self.L.vrgi_stats(self.h, C.byref(st), int(reset)))
         return st
+
+    # ---- device-resident voxel world
+    def voxels_load(self, blocks, n_types, translucent):
+        blocks = np.ascontiguousarray(blocks, np.uint16)
+        dims = np.array(blocks.shape, np.uint32)
+        check(self.L.vrgi_voxels_load(self.h, ptr(blocks), ptr(dims), n_types,
+                                      ptr(np.ascontiguousarray(translucent, np.uint8))))
+        self.vx_shape = blocks.shape
+
+    def voxels_set(self, xyz, vals):
+        xyz = np.ascontiguousarray(xyz, np.uint32).reshape(-1, 3)
+        vals = np.ascontiguousarray(vals, np.uint16)
+        check(self.L.vrgi_voxels_set(self.h, C.c_uint64(len(vals)), ptr(xyz), ptr(vals)))
+
+    def voxels_commit(self):
+        check(self.L.vrgi_voxels_commit(self.h))
+
+    def voxels_read(self):
+        out = np.zeros(self.vx_shape, np.uint16)
+        commits, remeshed = C.c_uint64(), C.c_uint64()
+        check(self.L.vrgi_voxels_read(self.h, ptr(out), C.byref(commits), C.byref(remeshed)))
+        return out, commits.value, remeshed.value
 
     # ---- GPU mesher
     def mesh_chunks(self, blocks, n_types, translucent, chunks):
