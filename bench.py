@@ -16,6 +16,12 @@ candidates, 5 spatial neighbours within 30 px), every scene's BVHs rebuilt in on
 multi-tree build per batched frame, one launch per stage for all scenes; beside it the
 same frames through 64 separate contexts (one render call per scene and pass).
 
+Fourth part (field "large_dynamic", not the headline): the 512x256x512 world (~5.6M
+exposed faces, ~40k emissive voxels) as a device-resident voxel world at 3840x2160, 4
+bounces, NEE, ReSTIR DI, learned cache; every frame 1% of the voxels are toggled by 64
+seeded random walkers, the dirty chunks re-meshed on the GPU and every BVH rebuilt;
+beside it the same frames without the cache.
+
 Prints ONE JSON line (rank 0). Progress goes to stderr.
   python bench.py [--gpus 1] [--steps 16] [--warmup 3] [--no-cpu-baseline] [--no-cache-study] [--no-batch]
 """
@@ -174,6 +180,55 @@ def batched_bench(steps):
             "graph": bool(info.graph), "scenes_separately_ms": round(seq_ms, 3)}
 
 
+def large_dynamic_bench(frames=8):
+    """The large dynamic workload: ms per frame including the voxel edits, the re-mesh of
+    the dirty chunks, the light and geometry rebuilds and the 4K frame."""
+    import vrgi
+    log("large dynamic workload: 512x256x512 world")
+    blocks0, nt = vrgi.world(4, 0)
+    mats, lum = vrgi.world_materials(4, 0)
+    tr = vrgi.world_translucent(4, nt)
+    nvox = blocks0.size
+    steps = nvox // 100  # 1% of the voxels per frame
+    dx, dy, dz = blocks0.shape
+    cam = vrgi.camera_look([0.1 * dx, 0.9 * dy, 0.1 * dz], [0.6 * dx, 0.15 * dy, 0.6 * dz])
+    out = {"workload": "512x256x512 terrain+400 boxes as a device voxel world, 3840x2160, 1 spp, 4 bounces, "
+                       "RR from bounce 2, NEE, ReSTIR DI 32 candidates + temporal + 5 spatial; 1% of the voxels "
+                       "toggled per frame by 64 seeded random walkers; dirty chunks re-meshed, all BVHs rebuilt",
+           "voxels_toggled_per_frame": steps, "frames": frames}
+    for cache in (True, False):
+        blocks = blocks0.copy()
+        rng = np.random.default_rng(4)
+        walkers = np.stack([rng.integers(0, dx, 64), rng.integers(dy // 4, dy // 2, 64), rng.integers(0, dz, 64)],
+                           1).astype(np.uint32)
+        cfg = vrgi.config(3840, 2160, 4, rr_start=RR_START, rr_min=0.1, mode=1, radius=30.0, di_candidates=32)
+        with vrgi.Context(cfg, mats, lum) as ctx:
+            ctx.voxels_load(blocks, nt, tr)
+            if cache:
+                ctx.cache_enable(train_bounces=4, query_vertex=0)
+            ctx.render(cam, nee_type=NEE, spatial=SPATIAL)
+            ctx.sync()
+            st0 = ctx.stats(reset=True)
+            writes = 0
+            t0 = time.perf_counter()
+            for f in range(frames):
+                xyz, vals = vrgi.world_walk(blocks, nt, 9000 + f, walkers, steps)
+                writes += len(vals)
+                ctx.voxels_set(xyz, vals)
+                ctx.render(cam, nee_type=NEE, spatial=SPATIAL)
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            st = ctx.stats(reset=True)
+            _, commits, remeshed = ctx.voxels_read()
+        key = "with_cache" if cache else "plain"
+        out[key] = {"ms_per_frame": round(1000.0 * dt / frames, 2), "mrays_per_s": round(st.rays_closest / dt / 1e6, 2),
+                    "distinct_voxel_writes_per_frame": int(writes // frames),
+                    "chunks_remeshed_per_frame": round(remeshed / max(commits, 1), 1),
+                    "triangles": int(st.bvh_nodes + 1) // 2 if st.bvh_nodes else 0}
+        log(f"large dynamic ({key}): {out[key]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +240,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (pmc counter scaling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched workload")
+    ap.add_argument("--no-large", action="store_true", help="skip the large dynamic workload")
     ap.add_argument("--serial-build", action="store_true",
                     help="one BVH buffer set built on the frame stream (default: frame f + 1 builds beside frame f)")
     ap.add_argument("--serial-nee", action="store_true",
@@ -277,6 +333,9 @@ def main():
     batched = None
     if not a.no_batch:
         batched = batched_bench(a.steps)
+    large = None
+    if not a.no_large:
+        large = large_dynamic_bench()
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline (oracle on the host cores, bounded sample)")
@@ -294,6 +353,7 @@ def main():
         "kernels": others,
         "learned_cache": cache,
         "batched": batched,
+        "large_dynamic": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

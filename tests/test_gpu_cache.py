@@ -284,3 +284,40 @@ def test_cache_gain_over_plain(world0):
     # at this size a frame costs the same in both modes (fixed per-frame work), so the
     # gain is in error: the plain tracer's 4-frame RMSE takes the cache under half the time
     assert r["time_to_equal_rmse_ratio"]["4"] is not None and r["time_to_equal_rmse_ratio"]["4"] < 0.5
+
+
+def test_cache_with_restir_di(world0):
+    """The learned cache with ReSTIR DI: the DI outputs are those of DI without the cache
+    (DI reads only the camera rays' hits, which the cache leaves as they are). With the
+    network frozen, the path-traced radiance is that of the cache with ReSTIR GI (same
+    paths, same network); with training on, the weights depend on the order of the
+    gradient atomics, so only the DI equality is exact."""
+    import vrgi
+    W, H = 64, 48
+    cam = vrgi.camera_look([6.0, 62.0, 8.0], [40.0, 0.0, 44.0])
+    for freeze in (True, False):
+        kw = dict(box_lo=(0, 0, 0), box_hi=(0, 0, 0), train_fraction=0.25, train_bounces=5, batch=1024,
+                  steps_per_frame=2, seed=3, query_vertex=0, flags=vrgi.CACHE_FREEZE if freeze else 0)
+        ctx = []
+        for mode, cache in [(1, True), (1, False), (0, True)]:
+            c = vrgi.Context(vrgi.config(W, H, 5, mode=mode, radius=30.0 if mode else 15.0), world0["mats"],
+                             world0["lum"])
+            c.add_objects(world0["objs"])
+            if cache:
+                _enable(c, **kw)
+            ctx.append(c)
+        try:
+            for f in range(4):
+                say(f"cache + ReSTIR DI frame {f}, frozen {freeze}")
+                for c in ctx:
+                    c.render(cam, nee_type=1, spatial=5)
+                    c.sync()
+                di_c, di, gi_c = ctx
+                for b in (vrgi.BUF_DI_LIGHT, vrgi.BUF_DI_WEIGHTS, vrgi.BUF_RESTIR):
+                    assert di_c.read(b, np.uint8).tobytes() == di.read(b, np.uint8).tobytes()
+                assert di_c.read(vrgi.BUF_RADIANCE).tobytes() != di.read(vrgi.BUF_RADIANCE).tobytes()
+                if freeze:
+                    assert di_c.read(vrgi.BUF_RADIANCE).tobytes() == gi_c.read(vrgi.BUF_RADIANCE).tobytes()
+        finally:
+            for c in ctx:
+                c.close()

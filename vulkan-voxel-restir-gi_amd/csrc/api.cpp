@@ -1781,7 +1781,7 @@ int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
     if (!(cfg->train_fraction >= 0.0f && cfg->train_fraction <= 1.0f))
         return fail(VRGI_E_INVALID, "train_fraction must be in [0, 1]");
     if (!(cfg->lr > 0.0f) || !(cfg->loss_scale > 0.0f)) return fail(VRGI_E_INVALID, "lr and loss_scale must be positive");
-    if (c->cfg.restir_mode != 0) return fail(VRGI_E_STATE, "the learned cache runs with ReSTIR GI (restir_mode 0)");
+    if (c->in_batch) return fail(VRGI_E_STATE, "the scene is in a batch (no learned cache there)");
     bool box_zero = true;
     for (int k = 0; k < 3; ++k) box_zero = box_zero && cfg->box_lo[k] == 0.0f && cfg->box_hi[k] == 0.0f;
     if (!box_zero)
