@@ -72,6 +72,38 @@ def _camera0():
 NO_RR = 0xFFFFFFFF
 
 
+def _check_frame(c, ref, P, B, nee_type):
+    """Every per-bounce buffer of the live bounces, the radiance, the reservoirs and the
+    image of the frame just rendered by context c equal the oracle's frame ref bit for bit."""
+    import vrgi
+    plen = c.read(vrgi.BUF_PATH_LEN, np.uint32)
+    assert np.array_equal(plen, ref["path_len"])
+    live = np.arange(B)[:, None] < plen[None, :]  # [B, P]
+    hit = c.read(vrgi.BUF_HIT_PRIM, np.uint32).reshape(B, P)
+    assert np.array_equal(hit[live], ref["hit_prim"][live])
+    for buf, key, k in [(vrgi.BUF_BOUNCE_ORIGIN, "origin", 3), (vrgi.BUF_BOUNCE_DIRECTION, "direction", 3)]:
+        g = c.read(buf).reshape(B, P, k)
+        o = ref[key][1:]
+        assert g[live].tobytes() == o[live].tobytes(), key
+    for buf, key, k in [(vrgi.BUF_BOUNCE_NORMAL, "normal", 3), (vrgi.BUF_BOUNCE_EMISSIVITY, "emissivity", 3),
+                        (vrgi.BUF_BOUNCE_REFLECTIVITY, "reflectivity", 3),
+                        (vrgi.BUF_BOUNCE_MIS_WEIGHT, "mis_weight", 1), (vrgi.BUF_BOUNCE_BSDF_PDF, "bsdf_pdf", 1)]:
+        g = c.read(buf).reshape(B, P, k)
+        o = ref[key].reshape(B, P, k)
+        assert g[live].tobytes() == o[live].tobytes(), key
+    if nee_type:
+        cont = np.arange(B)[:, None] + 1 < plen[None, :]
+        g = c.read(vrgi.BUF_BOUNCE_NEE_PDF).reshape(B, P)
+        assert g[cont].tobytes() == ref["nee_pdf"][cont].tobytes()
+        assert (ref["nee_pdf"] > 0).any() or nee_type == 2
+    assert c.read(vrgi.BUF_RADIANCE).tobytes() == ref["l_out"][0].tobytes()
+    assert c.read(vrgi.BUF_TEMPORAL_RESERVOIR).tobytes() == ref["temporal"].tobytes()
+    assert c.read(vrgi.BUF_SPATIAL_RESERVOIR).tobytes() == ref["spatial"].tobytes()
+    assert c.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
+    assert c.read(vrgi.BUF_BGRA8, np.uint8).tobytes() == ref["bgra8"].tobytes()
+    return plen
+
+
 @pytest.mark.parametrize("nee_type,spatial,rr_start,sort_bits,serial", [
     (0, 0, NO_RR, 0, False), (1, 5, NO_RR, 0, False), (2, 3, NO_RR, 0, False),
     # NEE pdf launches after all bounces on the frame stream
@@ -94,32 +126,65 @@ def test_frames_parity(world0, scenes, nee_type, spatial, rr_start, sort_bits, s
             c.render(cam, nee_type=nee_type, spatial=spatial)
             c.sync()
             ref = r.render(s, cam, f, nee_type=nee_type, spatial=spatial)
-            P = W * H
-            plen = c.read(vrgi.BUF_PATH_LEN, np.uint32)
-            assert np.array_equal(plen, ref["path_len"])
-            live = np.arange(B)[:, None] < plen[None, :]  # [B, P]
-            hit = c.read(vrgi.BUF_HIT_PRIM, np.uint32).reshape(B, P)
-            assert np.array_equal(hit[live], ref["hit_prim"][live])
-            for buf, key, k in [(vrgi.BUF_BOUNCE_ORIGIN, "origin", 3), (vrgi.BUF_BOUNCE_DIRECTION, "direction", 3)]:
-                g = c.read(buf).reshape(B, P, k)
-                o = ref[key][1:]
-                assert g[live].tobytes() == o[live].tobytes(), key
-            for buf, key, k in [(vrgi.BUF_BOUNCE_NORMAL, "normal", 3), (vrgi.BUF_BOUNCE_EMISSIVITY, "emissivity", 3),
-                                (vrgi.BUF_BOUNCE_REFLECTIVITY, "reflectivity", 3),
-                                (vrgi.BUF_BOUNCE_MIS_WEIGHT, "mis_weight", 1), (vrgi.BUF_BOUNCE_BSDF_PDF, "bsdf_pdf", 1)]:
-                g = c.read(buf).reshape(B, P, k)
-                o = ref[key].reshape(B, P, k)
-                assert g[live].tobytes() == o[live].tobytes(), key
-            if nee_type:
-                cont = np.arange(B)[:, None] + 1 < plen[None, :]
-                g = c.read(vrgi.BUF_BOUNCE_NEE_PDF).reshape(B, P)
-                assert g[cont].tobytes() == ref["nee_pdf"][cont].tobytes()
-                assert (ref["nee_pdf"] > 0).any() or nee_type == 2
-            assert c.read(vrgi.BUF_RADIANCE).tobytes() == ref["l_out"][0].tobytes()
-            assert c.read(vrgi.BUF_TEMPORAL_RESERVOIR).tobytes() == ref["temporal"].tobytes()
-            assert c.read(vrgi.BUF_SPATIAL_RESERVOIR).tobytes() == ref["spatial"].tobytes()
-            assert c.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
-            assert c.read(vrgi.BUF_BGRA8, np.uint8).tobytes() == ref["bgra8"].tobytes()
+            _check_frame(c, ref, W * H, B, nee_type)
+
+
+def _cubes_crossed(o, d, lo, tmin=1e-4, tmax=1000.0):
+    """Number of unit cubes [lo, lo + 1] each ray crosses within [tmin, tmax] (slab test)."""
+    n = np.zeros(len(o), np.int64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / d.astype(np.float64)
+        for k in range(0, len(lo), 512):
+            c = lo[None, k:k + 512, :].astype(np.float64)
+            t0 = (c - o[:, None, :]) * inv[:, None, :]
+            t1 = (c + 1.0 - o[:, None, :]) * inv[:, None, :]
+            tn = np.maximum(np.nanmax(np.minimum(t0, t1), axis=2), tmin)
+            tf = np.minimum(np.nanmin(np.maximum(t0, t1), axis=2), tmax)
+            n += (tn < tf).sum(axis=1)
+    return n
+
+
+def test_frames_parity_many_light_crossings(world0):
+    """Four layers of separated emissive blocks above the terrain: the NEE pdf rays cross
+    many light triangles each (more than the kernel keeps per lane before summing), and
+    the frames stay bit-exact against the oracle, whose sum runs over the crossings in
+    the order they are met."""
+    import oracle
+    import vrgi
+    blocks = world0["blocks"].copy()
+    nt = world0["n_types"]
+    lit = 4  # first emissive block type of the 64^3 world's materials
+    assert world0["lum"][6 * lit] > 0
+    cubes = []
+    for y in (50, 53, 56, 59):
+        for x in range(2, 62, 2):
+            for z in range(2, 62, 2):
+                blocks[x, y, z] = lit
+                cubes.append((x, y, z))
+    objs = vrgi.chunk_objects(blocks, nt, world0["tr"])
+    s = oracle.Scene(world0["mats"], world0["lum"])
+    s.add_objects(objs)
+    s.build()
+    W, H, B = 96, 64, 3
+    cam = vrgi.camera_look([6.0, 48.0, 8.0], [40.0, 0.0, 44.0])
+    r = oracle.Renderer(W, H, B)
+    cfg = vrgi.config(W, H, B, flags=vrgi.FLAG_KEEP_BOUNCES)
+    with vrgi.Context(cfg, world0["mats"], world0["lum"]) as c:
+        c.add_objects(objs)
+        for f in range(2):
+            say(f"many light crossings: frame {f}")
+            c.render(cam, nee_type=1, spatial=5)
+            c.sync()
+            ref = r.render(s, cam, f, nee_type=1, spatial=5)
+            plen = _check_frame(c, ref, W * H, B, 1)
+    # the scene does what it is for: many pdf rays cross 3 or more emissive cubes, i.e.
+    # at least 6 light triangles
+    P = W * H
+    cont = (1 < plen) & (ref["mis_weight"][0] != 0)
+    o = ref["origin"][1][cont]
+    d = ref["direction"][1][cont]
+    n = _cubes_crossed(o[:4000], d[:4000], np.array(cubes, np.float64))
+    assert (n >= 3).sum() >= 50, np.bincount(n)
 
 
 @pytest.mark.parametrize("serial_build", [False, True])

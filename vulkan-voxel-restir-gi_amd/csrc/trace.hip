@@ -495,6 +495,7 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
 }
 
 // NEE pdf of bounce b (nee_pdf.rs:281-337) for the rays alive at bounce b+1
+constexpr int NEE_DEFER = 4;  // crossed light triangles kept per lane before their pdfs are summed
 template <int STACK>
 __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArgs &ba, uint32_t *stack, uint32_t i,
                                             uint32_t count) {
@@ -520,14 +521,36 @@ __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArg
         bool have;
         if (sv.n_ltri == 1) { ref = 0x80000000u | sv.lroot_prim; have = true; }
         else { ref = 0; have = true; }
+        // Crossed triangles are kept (at most NEE_DEFER) and their pdfs summed after the
+        // walk, in the order they were met: the long reverse walks then run together for
+        // all lanes of the wave instead of one lane at a time inside the traversal loop.
+        uint32_t hl0 = 0, hl1 = 0, hl2 = 0, hl3 = 0;
+        float hu0 = 0.0f, hu1 = 0.0f, hu2 = 0.0f, hu3 = 0.0f, hv0 = 0.0f, hv1 = 0.0f, hv2 = 0.0f, hv3 = 0.0f;
+        int nh = 0;
+        auto flush = [&]() {
+#pragma unroll 1
+            for (int j = 0; j < nh; ++j) {
+                const uint32_t lid = j == 0 ? hl0 : j == 1 ? hl1 : j == 2 ? hl2 : hl3;
+                const float u = j == 0 ? hu0 : j == 1 ? hu1 : j == 2 ? hu2 : hu3;
+                const float v = j == 0 ? hv0 : j == 1 ? hv1 : j == 2 ? hv2 : hv3;
+                np += nee_pdf_one(sv, o, n, d, lid, u, v);
+            }
+            nh = 0;
+        };
         while (have) {
             if (ref & 0x80000000u) {
                 uint32_t lid = ref & 0x7FFFFFFFu;
                 Tri t = sv.ltri[lid];
                 float tt, u, v;
                 if (tri_hit(r, mk(t.v0.x, t.v0.y, t.v0.z), mk(t.v1.x, t.v1.y, t.v1.z),
-                            mk(t.v2.x, t.v2.y, t.v2.z), VRGI_EPS_NEE, 1000.0f, &tt, &u, &v))
-                    np += nee_pdf_one(sv, o, n, d, lid, u, v);
+                            mk(t.v2.x, t.v2.y, t.v2.z), VRGI_EPS_NEE, 1000.0f, &tt, &u, &v)) {
+                    if (nh == NEE_DEFER) flush();
+                    if (nh == 0) { hl0 = lid; hu0 = u; hv0 = v; }
+                    else if (nh == 1) { hl1 = lid; hu1 = u; hv1 = v; }
+                    else if (nh == 2) { hl2 = lid; hu2 = u; hv2 = v; }
+                    else { hl3 = lid; hu3 = u; hv3 = v; }
+                    ++nh;
+                }
                 have = false;
             } else {
                 // 4-ary node: descendants in leaf order; the first one hit is next, the
@@ -548,6 +571,7 @@ __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArg
             }
             if (!have && sp > 0) { ref = stk[(--sp) * 64]; have = true; }
         }
+        flush();
     }
     ba.nee[bi] = np;
 }
