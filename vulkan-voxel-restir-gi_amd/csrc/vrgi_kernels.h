@@ -142,7 +142,7 @@ __device__ __forceinline__ void write_prim9(float *prim9, size_t n, uint32_t g, 
 
 // Woop, Benthin, Wald: per-ray shear constants
 struct RayPre {
-    f3 o, d, inv;
+    f3 o, d, inv, oi;  // oi = o * inv (slab tests as one fma per plane)
     int kx, ky, kz;
     float sx, sy, sz;
 };
@@ -161,6 +161,7 @@ __device__ __forceinline__ RayPre ray_pre(f3 o, f3 d) {
     r.sz = 1.0f / dz;
     r.inv = mk(d.x != 0.0f ? 1.0f / d.x : 1e30f, d.y != 0.0f ? 1.0f / d.y : 1e30f,
                d.z != 0.0f ? 1.0f / d.z : 1e30f);
+    r.oi = mk(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
     return r;
 }
 
@@ -197,12 +198,14 @@ __device__ __forceinline__ bool tri_hit(const RayPre &r, f3 v0, f3 v1, f3 v2, fl
 
 // conservative slab test (Ize 2013: far distance scaled by 1 + 2 gamma(3)); boxes of
 // the traversal copies carry VRGI_BOX_PAD of slack so that a triangle hit reported at
-// rounding distance outside its exact box is never culled
+// rounding distance outside its exact box is never culled. Each plane distance is one
+// fma with o * inv rounded once: its absolute error, |o| * 2^-23 / |d_axis| at most, is
+// far below the slack's VRGI_BOX_PAD / |d_axis| for scene coordinates below 10^4.
 __device__ __forceinline__ bool box_hit_t(const RayPre &r, float mnx, float mny, float mnz, float mxx,
                                           float mxy, float mxz, float tmin, float tmax, float *tn) {
-    float tx0 = (mnx - r.o.x) * r.inv.x, tx1 = (mxx - r.o.x) * r.inv.x;
-    float ty0 = (mny - r.o.y) * r.inv.y, ty1 = (mxy - r.o.y) * r.inv.y;
-    float tz0 = (mnz - r.o.z) * r.inv.z, tz1 = (mxz - r.o.z) * r.inv.z;
+    float tx0 = __builtin_fmaf(mnx, r.inv.x, -r.oi.x), tx1 = __builtin_fmaf(mxx, r.inv.x, -r.oi.x);
+    float ty0 = __builtin_fmaf(mny, r.inv.y, -r.oi.y), ty1 = __builtin_fmaf(mxy, r.inv.y, -r.oi.y);
+    float tz0 = __builtin_fmaf(mnz, r.inv.z, -r.oi.z), tz1 = __builtin_fmaf(mxz, r.inv.z, -r.oi.z);
     float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
     *tn = t0;
@@ -223,11 +226,16 @@ struct Hit {
 // stack: STACK * 64 LDS words of this wave, entry k of lane l at [k * 64 + l].
 // COUNT (measurement only): marks every visited internal node / tested triangle in the
 // bitmaps touch_nodes / touch_tris.
+// The triangle tests of a wave run together: a lane that reaches a leaf keeps its
+// triangle (at most 4 pending) and goes on with its internal nodes while other lanes
+// still have none; once every lane that is still walking has a pending triangle, all
+// lanes test theirs at once. The tests then run with most lanes active instead of one
+// or two lanes per step; the result does not depend on the order of the tests.
 template <int STACK, bool COUNT = false>
 __device__ __forceinline__ Hit closest_hit2(const SceneView &sv, const RayPre &r, float tmin,
-                                           float tmax, uint32_t *stack,
-                                           uint32_t *touch_nodes = nullptr,
-                                           uint32_t *touch_tris = nullptr) {
+                                            float tmax, uint32_t *stack,
+                                            uint32_t *touch_nodes = nullptr,
+                                            uint32_t *touch_tris = nullptr) {
     Hit h;
     h.prim = VRGI_INVALID;
     h.t = 0.0f; h.u = 0.0f; h.v = 0.0f;
@@ -252,15 +260,33 @@ __device__ __forceinline__ Hit closest_hit2(const SceneView &sv, const RayPre &r
     }
     uint32_t ref = 0;
     int sp = 0;
+    bool walk = true;  // ref is an internal node still to visit
+    uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    int np = 0;
+    auto keep = [&](uint32_t prim) {  // a shift register: no indexed private array
+        p3 = p2; p2 = p1; p1 = p0; p0 = prim;
+        ++np;
+    };
     for (;;) {
+        // wave-uniform choice: test when no walking lane is without a pending triangle
+        if (!__any(walk && np == 0)) {
+            if (!__any(np > 0)) break;
+#pragma unroll 1
+            for (; np > 0; --np) {
+                test(p0);
+                p0 = p1; p1 = p2; p2 = p3;
+            }
+            continue;
+        }
+        if (!walk || np > 2) continue;  // this lane waits (no node, or no room for two more)
         if (COUNT) atomicOr(touch_nodes + (ref >> 5), 1u << (ref & 31));
         WideNode wn = sv.gwide[ref];
         float tn0, tn1;
         bool h0 = box_hit_t(r, wn.a.x, wn.a.y, wn.a.z, wn.a.w, wn.b.x, wn.b.y, tmin, best, &tn0);
         bool h1 = box_hit_t(r, wn.b.z, wn.b.w, wn.c.x, wn.c.y, wn.c.z, wn.c.w, tmin, best, &tn1);
         uint32_t r0 = wn.r.x, r1 = wn.r.y;
-        if (h0 && (r0 & 0x80000000u)) { test(r0 & 0x7FFFFFFFu); h0 = false; }
-        if (h1 && (r1 & 0x80000000u)) { test(r1 & 0x7FFFFFFFu); h1 = false; }
+        if (h0 && (r0 & 0x80000000u)) { keep(r0 & 0x7FFFFFFFu); h0 = false; }
+        if (h1 && (r1 & 0x80000000u)) { keep(r1 & 0x7FFFFFFFu); h1 = false; }
         if (h0 && h1) {
             bool first0 = tn0 <= tn1;
             stk[(sp++) * 64] = first0 ? r1 : r0;
@@ -269,8 +295,9 @@ __device__ __forceinline__ Hit closest_hit2(const SceneView &sv, const RayPre &r
             ref = r0;
         } else if (h1) {
             ref = r1;
+        } else if (sp == 0) {
+            walk = false;
         } else {
-            if (sp == 0) break;
             ref = stk[(--sp) * 64];
         }
     }
