@@ -82,21 +82,25 @@ __device__ __forceinline__ void write_node(vrgi_bvh_node *dst, uint32_t left, ui
 
 // leaf (build.rs:311-326) for primitive p
 __device__ __forceinline__ void emit_leaf(const BuildArgs &a, uint32_t gnode, uint32_t parent, uint32_t p) {
-    float mn[3] = {a.prim9[0 * a.n + p], a.prim9[1 * a.n + p], a.prim9[2 * a.n + p]};
-    float mx[3] = {a.prim9[3 * a.n + p], a.prim9[4 * a.n + p], a.prim9[5 * a.n + p]};
-    write_node(a.nodes + gnode, VRGI_INVALID, a.ids[p], mn, a.power[p], mx, parent);
+    const float4 b0 = a.box8[2 * p], b1 = a.box8[2 * p + 1];
+    float mn[3] = {b0.x, b0.y, b0.z};
+    float mx[3] = {b0.w, b1.x, b1.y};
+    write_node(a.nodes + gnode, VRGI_INVALID, __float_as_uint(b1.w), mn, b1.z, mx, parent);
 }
 
 // 2-primitive node (build.rs:203-230) of tree-local rank r / index idx, leaves at p0, p1
 __device__ __forceinline__ void emit_pair(const BuildArgs &a, uint32_t base, uint32_t idx, uint32_t r,
                                           uint32_t parent, uint32_t p0, uint32_t p1) {
     const float pad = 0.0001f;
+    const float4 a0 = a.box8[2 * p0], a1 = a.box8[2 * p0 + 1];
+    const float4 c0 = a.box8[2 * p1], c1 = a.box8[2 * p1 + 1];
+    const float l0[6] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, l1[6] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y};
     float mn[3], mx[3];
     for (int k = 0; k < 3; ++k) {
-        mn[k] = gmin(a.prim9[k * a.n + p0], a.prim9[k * a.n + p1]) - pad;
-        mx[k] = gmax(a.prim9[(3 + k) * a.n + p0], a.prim9[(3 + k) * a.n + p1]) + pad;
+        mn[k] = gmin(l0[k], l1[k]) - pad;
+        mx[k] = gmax(l0[3 + k], l1[3 + k]) + pad;
     }
-    float pw = (0.0f + a.power[p0]) + a.power[p1];
+    float pw = (0.0f + a1.z) + c1.z;
     write_node(a.nodes + base + idx, 2 * r + 1, 2 * r + 2, mn, pw, mx, parent);
     emit_leaf(a, base + 2 * r + 1, idx, p0);
     emit_leaf(a, base + 2 * r + 2, idx, p1);
@@ -110,6 +114,8 @@ __global__ void k_init(BuildArgs a) {
     a.pc[0 * a.n + i] = a.prim9[6 * a.n + i];
     a.pc[1 * a.n + i] = a.prim9[7 * a.n + i];
     a.pc[2 * a.n + i] = a.prim9[8 * a.n + i];
+    a.box8[2 * i] = make_float4(a.prim9[0 * a.n + i], a.prim9[1 * a.n + i], a.prim9[2 * a.n + i], a.prim9[3 * a.n + i]);
+    a.box8[2 * i + 1] = make_float4(a.prim9[4 * a.n + i], a.prim9[5 * a.n + i], a.power[i], __uint_as_float(a.ids[i]));
 }
 
 // one thread per tree: place the root (leaf / pair / job / open node)
@@ -256,8 +262,8 @@ __global__ __launch_bounds__(TOP_THREADS) void k_bin(BuildArgs a, BuildState s) 
     for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
         uint32_t i = it.beg + k;
         uint32_t p = a.perm[i];
-        int bk[6];
-        for (int q = 0; q < 6; ++q) bk[q] = fkey(a.prim9[q * a.n + p]);
+        const float4 b0 = a.box8[2 * p], b1 = a.box8[2 * p + 1];
+        const int bk[6] = {fkey(b0.x), fkey(b0.y), fkey(b0.z), fkey(b0.w), fkey(b1.x), fkey(b1.y)};
         for (int d = 0; d < 3; ++d) {
             uint32_t b = bin_index(a.pc[d * a.n + i], bmin[d], scale[d]);
             int *bin = sb + (d * BINS + b) * 7;
@@ -577,8 +583,8 @@ __global__ void k_children(BuildArgs a, BuildState s) {
 // ---------------------------------------------------------------- bottom phase
 // One 64-lane wave builds a whole subtree of <= SUB primitives held in LDS.
 // Phase A: ranges larger than SMALL are split one at a time by the whole wave (an LDS
-// stack of pending ranges). Phase B: ranges of 3..SMALL primitives are split by one
-// lane each, sequentially, 64 subtrees at a time. Phase C: power of every internal
+// stack of pending ranges). Phase B: ranges of 3..SMALL primitives are split by three
+// lanes each (one per axis), 21 ranges at a time, one subtree level per pass. Phase C: power of every internal
 // node of the subtree, children before parents: internal nodes of a subtree have
 // consecutive preorder ranks and descendants have larger ranks, so one pass over
 // decreasing ranks suffices (build.rs:345-357).
@@ -598,71 +604,69 @@ struct RankInfo {  // per internal node of the subtree, indexed by rank - job ra
 #endif
 #define VRGI_PRAGMA(x) _Pragma(#x)
 
-// SAH plane search of one range by a single lane (build.rs:45-128). Every plane of a run
-// of planes with the same (left set, right set) has the same cost, so only the plane at
-// each occupied bin (and plane 0 when bin 0 is empty) is evaluated; ascending order
-// with a strict < keeps the reference's first minimum. The lane writes the bin of each
-// of its primitives into bin8[f..f+c), a range no other lane touches.
+// SAH plane search (build.rs:45-128) of one range of at most 12 primitives along one
+// axis d, by one lane. The bins of the range are packed 5 bits each into a 64-bit word. A sweep
+// from the highest occupied bin down stores the right-side cost of each occupied bin's
+// plane in tmp[f..f+occupied) (descending order); a sweep from bin 0 up then forms the
+// left side and evaluates the planes in ascending order with a strict < (the first
+// minimum). Each primitive's box is read once per sweep instead of once per plane; the
+// unions are exact min/max, so every cost equals the per-plane evaluation's bit for bit.
 template <int SUB>
-__device__ void lane_best_plane(const float (*box)[SUB], uint8_t *bin8, uint32_t f, uint32_t c,
-                                uint32_t *dim_out, float *split_out, float nb[6]) {
-    float best_c = 3.40282347e+38f;
-    uint32_t best_d = 0;
-    float best_s = 0.0f;
+__device__ void axis_best_plane(const float (*box)[SUB], float *tmp, uint32_t f, uint32_t c, uint32_t d,
+                                float *cost_out, float *split_out, float nb[6]) {
     for (int q = 0; q < 3; ++q) { nb[q] = box[q][f]; nb[3 + q] = box[3 + q][f]; }
     for (uint32_t k = 1; k < c; ++k)
         for (int q = 0; q < 3; ++q) { nb[q] = gmin(nb[q], box[q][f + k]); nb[3 + q] = gmax(nb[3 + q], box[3 + q][f + k]); }
     const float all_cost = area6(nb) * (float)c;
-    for (uint32_t d = 0; d < 3; ++d) {
-        float lo = box[6 + d][f], hi = lo;
-        for (uint32_t k = 1; k < c; ++k) { lo = gmin(lo, box[6 + d][f + k]); hi = gmax(hi, box[6 + d][f + k]); }
-        const float scale = (float)BINS / (hi - lo);
-        uint32_t occ = 0;
-        for (uint32_t k = 0; k < c; ++k) {
-            uint32_t b = bin_index(box[6 + d][f + k], lo, scale);
-            bin8[f + k] = (uint8_t)b;
-            occ |= 1u << b;
+    float lo = box[6 + d][f], hi = lo;
+    for (uint32_t k = 1; k < c; ++k) { lo = gmin(lo, box[6 + d][f + k]); hi = gmax(hi, box[6 + d][f + k]); }
+    const float scale = (float)BINS / (hi - lo);
+    uint64_t pk = 0;
+    uint32_t occ = 0;
+    for (uint32_t k = 0; k < c; ++k) {
+        const uint32_t b = bin_index(box[6 + d][f + k], lo, scale);
+        pk |= (uint64_t)b << (5 * k);
+        occ |= 1u << b;
+    }
+    float dbest = 3.40282347e+38f;
+    uint32_t dplane = 0;
+    if (!(occ & 1u)) { dbest = 0.0f * 0.0f + all_cost; dplane = 0; }  // left side empty
+    auto add = [&](float (&u)[6], uint32_t n, uint32_t e) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float vn = box[q][e], vx = box[3 + q][e];
+            u[q] = n == 0 ? vn : gmin(u[q], vn);
+            u[3 + q] = n == 0 ? vx : gmax(u[3 + q], vx);
         }
-        float dbest = 3.40282347e+38f;
-        uint32_t dplane = 0;
-        if (!(occ & 1u)) { dbest = 0.0f * 0.0f + all_cost; dplane = 0; }  // left side empty
-        uint32_t rest = occ & 0x7fffffffu;                                  // no plane after bin 31
+    };
+    {  // right sides, highest occupied bin first: the plane at bin b has bins > b on the right
+        float r[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t nr = 0, j = 0, rest = occ;
+        while (rest) {
+            const uint32_t b = 31u - (uint32_t)__builtin_clz(rest);
+            rest &= ~(1u << b);
+            tmp[f + j++] = nr ? area6(r) * (float)nr : 0.0f * 0.0f;
+            for (uint32_t k = 0; k < c; ++k)
+                if ((uint32_t)((pk >> (5 * k)) & 31u) == b) { add(r, nr, f + k); ++nr; }
+        }
+    }
+    {  // left sides and plane costs, bin 0 up (no plane after bin 31)
+        float l[6] = {0, 0, 0, 0, 0, 0};
+        const uint32_t nocc = (uint32_t)__builtin_popcount(occ);
+        uint32_t nl = 0, i = 0, rest = occ & 0x7fffffffu;
         while (rest) {
             const uint32_t b = (uint32_t)__builtin_ctz(rest);
             rest &= rest - 1u;
-            float l[6], r[6];
-            uint32_t nl = 0;
-            VRGI_PRAGMA(unroll VRGI_PLANE_UNROLL)
-            for (uint32_t k = 0; k < c; ++k) {
-                const uint32_t e = f + k;
-                const bool left = bin8[e] <= b;
-                const bool lf = nl == 0, rf = (k - nl) == 0;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const float vn = box[q][e], vx = box[3 + q][e];
-                    if (left) {
-                        l[q] = lf ? vn : gmin(l[q], vn);
-                        l[3 + q] = lf ? vx : gmax(l[3 + q], vx);
-                    } else {
-                        r[q] = rf ? vn : gmin(r[q], vn);
-                        r[3 + q] = rf ? vx : gmax(r[3 + q], vx);
-                    }
-                }
-                nl += left ? 1u : 0u;
-            }
+            for (uint32_t k = 0; k < c; ++k)
+                if ((uint32_t)((pk >> (5 * k)) & 31u) == b) { add(l, nl, f + k); ++nl; }
             const float al = area6(l) * (float)nl;
-            const float ar = nl < c ? area6(r) * (float)(c - nl) : 0.0f * 0.0f;
+            const float ar = tmp[f + (nocc - 1u - i++)];
             const float cost = al + ar;
             if (cost < dbest) { dbest = cost; dplane = b; }
         }
-        if (dbest < best_c) {
-            best_c = dbest;
-            best_d = d;
-            best_s = lo + ((float)dplane + 1.0f) * ((hi - lo) / (float)BINS);
-        }
     }
-    *dim_out = best_d;
-    *split_out = best_s;
+    *cost_out = dbest;
+    *split_out = lo + ((float)dplane + 1.0f) * ((hi - lo) / (float)BINS);
 }
 
 template <int SUB, int SMALL>
@@ -670,6 +674,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     constexpr int STK = SUB / (SMALL + 1) + 2;  // pending ranges are disjoint and hold > SMALL
     constexpr int SMQ = SUB / 3 + 2;            // small ranges are disjoint and hold >= 3
     static_assert(SUB <= 1024 && SMALL <= 127, "small-range queue packs 10 + 7 + 10 bits");
+    static_assert(SMALL <= 12, "bins of a small range are packed 5 bits each into 64 bits");
     __shared__ uint32_t s_perm[SUB];
     __shared__ float s_box[9][SUB];  // min xyz, max xyz, centroid xyz
     __shared__ uint32_t s_pair[SUB];
@@ -679,7 +684,6 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     __shared__ float s_ipow[SUB];
     __shared__ uint32_t s_stack[STK][6];  // first, count, rank, node, depth, parent
     __shared__ uint4 s_q[2][SMQ];         // small ranges: first|count<<10|(rank-job rank)<<17, node, depth, parent
-    __shared__ uint8_t s_bin8[SUB];
     __shared__ uint32_t s_nsmall;
 
     const uint32_t job = blockIdx.x;
@@ -691,7 +695,9 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     for (uint32_t k = lane; k < c; k += 64) {
         uint32_t p = a.perm[jn.first + k];
         s_perm[k] = p;
-        for (int q = 0; q < 6; ++q) s_box[q][k] = a.prim9[q * a.n + p];
+        const float4 b0 = a.box8[2 * p], b1 = a.box8[2 * p + 1];
+        s_box[0][k] = b0.x; s_box[1][k] = b0.y; s_box[2][k] = b0.z;
+        s_box[3][k] = b0.w; s_box[4][k] = b1.x; s_box[5][k] = b1.y;
         for (int q = 0; q < 3; ++q) s_box[6 + q][k] = a.pc[q * a.n + jn.first + k];
     }
     uint32_t sp = 0, maxdepth = jn.depth;
@@ -816,20 +822,35 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         }
         __syncthreads();
     }
-    // ---------------- phase B: ranges of 3..SMALL primitives, one lane each, one level
-    // of the subtree per pass; the ranges of a pass are disjoint, so at most SUB/3
+    // ---------------- phase B: ranges of 3..SMALL primitives, three lanes each, one
+    // level of the subtree per pass; the ranges of a pass are disjoint, so at most SUB/3
     uint32_t cur = 0, ncur = s_nsmall;
     while (ncur > 0) {
         __syncthreads();
         if (lane == 0) s_nsmall = 0;
         __syncthreads();
-        for (uint32_t w = lane; w < ncur; w += 64) {
-            const uint4 e = s_q[cur][w];
+        // three lanes per range, one per axis; the range's first lane picks the axis
+        // (strict < in axis order, as find_best_plane's outer loop) and splits it
+        for (uint32_t w0 = 0; w0 < ncur; w0 += 21) {
+            const uint32_t w = w0 + (uint32_t)lane / 3u, ax = (uint32_t)lane % 3u;
+            const bool act = lane < 63 && w < ncur;
+            uint4 e = make_uint4(0, 0, 0, 0);
+            float acost = 0.0f, asplit = 0.0f, nb[6];
+            if (act) {
+                e = s_q[cur][w];
+                float *tmp = ax == 0 ? (float *)s_pair : ax == 1 ? (float *)s_cj : s_ipow;
+                axis_best_plane<SUB>(s_box, tmp, e.x & 0x3ffu, (e.x >> 10) & 0x7fu, ax, &acost, &asplit, nb);
+            }
+            const float c1 = __shfl(acost, lane + 1, 64), c2 = __shfl(acost, lane + 2, 64);
+            const float s1 = __shfl(asplit, lane + 1, 64), s2 = __shfl(asplit, lane + 2, 64);
+            if (!act || ax != 0) continue;
             const uint32_t nf = e.x & 0x3ffu, nc = (e.x >> 10) & 0x7fu, rank = jn.rank + (e.x >> 17);
             const uint32_t gidx = e.y, dep = e.z, parent = e.w;
-            uint32_t dim;
-            float split, nb[6];
-            lane_best_plane<SUB>(s_box, s_bin8, nf, nc, &dim, &split, nb);
+            uint32_t dim = 0;
+            float split = 0.0f, best_c = 3.40282347e+38f;
+            if (acost < best_c) { best_c = acost; dim = 0; split = asplit; }
+            if (c1 < best_c) { best_c = c1; dim = 1; split = s1; }
+            if (c2 < best_c) { best_c = c2; dim = 2; split = s2; }
             // partition (build.rs:145): the crate's l/r scan, in place
             uint32_t l = 0, r = nc - 1, T;
             for (;;) {
@@ -875,7 +896,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     // ---------------- phase C: power, decreasing rank (children first). Leaf powers are
     // gathered into LDS by all lanes first (s_cj is free after phase A); the left child of
     // rank k, when internal, is rank k + 1, the value computed one step earlier.
-    for (uint32_t k = lane; k < c; k += 64) s_cj[k] = __float_as_uint(a.power[s_perm[k]]);
+    for (uint32_t k = lane; k < c; k += 64) s_cj[k] = __float_as_uint(a.box8[2 * s_perm[k] + 1].z);
     __syncthreads();
     if (lane == 0) {
         float prev = 0.0f;
@@ -914,6 +935,7 @@ void Builder::reserve(uint32_t n, uint32_t n_trees) {
     auto A = [&](void **p, size_t bytes) { dev_alloc(p, bytes); };
     A((void **)&perm, (size_t)cap_n * 4);
     A((void **)&pc, (size_t)cap_n * 12);
+    A((void **)&box8, (size_t)cap_n * 32);
     A((void **)&st.rank_tmp, (size_t)cap_n * 4);
     A((void **)&st.pair, (size_t)cap_n * 4);
     A((void **)&st.open, (size_t)open_cap * sizeof(OpenNode));
@@ -940,12 +962,12 @@ void Builder::reserve(uint32_t n, uint32_t n_trees) {
 }
 
 void Builder::free_all() {
-    void *ptrs[] = {perm, pc, st.rank_tmp, st.pair, st.open, st.open_next, st.jobs, st.items,
+    void *ptrs[] = {perm, pc, box8, st.rank_tmp, st.pair, st.open, st.open_next, st.jobs, st.items,
                     st.item_true, st.item_off, st.item_beg, st.item_end, st.bins, st.bounds,
                     st.split, st.level_nodes, counters, level_start};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    perm = nullptr; pc = nullptr; counters = nullptr; level_start = nullptr;
+    perm = nullptr; pc = nullptr; box8 = nullptr; counters = nullptr; level_start = nullptr;
     st = BuildState{};
     cap_n = cap_t = 0;
 }
@@ -994,6 +1016,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     a.depth = d_depth;
     a.perm = perm;
     a.pc = pc;
+    a.box8 = box8;
     a.sub_max = SUB_MAX_DEFAULT;
     a.split_count_sum = (unsigned long long *)(counters + 8);
     (void)h_tree_base;

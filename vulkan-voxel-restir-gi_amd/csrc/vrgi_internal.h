@@ -52,6 +52,9 @@ struct BuildArgs {
     unsigned long long *split_count_sum;
     uint32_t *perm;  // working permutation -> final primitive order per leaf position
     float *pc;       // centroids permuted along with perm (3 planes)
+    // per primitive p, one 32-byte record gathered by primitive index:
+    // box8[2p] = (min x, min y, min z, max x), box8[2p+1] = (max y, max z, power, id bits)
+    float4 *box8;
     int sub_max;
 };
 
@@ -98,6 +101,7 @@ class Builder {
     uint32_t cap_n = 0, cap_t = 0, open_capacity = 0;
     uint32_t *perm = nullptr;
     float *pc = nullptr;
+    float4 *box8 = nullptr;
     uint32_t *counters = nullptr;
     uint32_t *level_start = nullptr;  // first level-node entry of each top-phase level
     uint32_t level_cap = 0;
