@@ -200,16 +200,26 @@ __global__ void k_clear_node_data(BuildState s) {
     if (threadIdx.x < 6) s.bounds[o * 6 + threadIdx.x] = threadIdx.x < 3 ? KEY_MIN_ID : KEY_MAX_ID;
 }
 
-__device__ __forceinline__ int wave_min(int v) {
-    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+// wave-wide reductions over all 64 lanes (all lanes active at every call site): the
+// device library's DPP wavefront reductions
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+extern "C" __device__ unsigned int __ockl_wfred_add_u32(unsigned int);
+extern "C" __device__ unsigned int __ockl_wfred_min_u32(unsigned int);
+__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return __ockl_wfred_add_u32(v); }
+
+// inclusive scans within the two 32-lane halves of the wave: row shifts by 1, 2, 4, 8
+// (16-lane rows) and the broadcast of lane 15 of rows 0 and 2 into rows 1 and 3 (DPP);
+// lanes without a source take the identity `id`
+template <class Op>
+__device__ __forceinline__ int seg32_scan(int v, int id, Op op) {
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x111, 0xF, 0xF, false));  // row_shr:1
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x112, 0xF, 0xF, false));  // row_shr:2
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x114, 0xF, 0xF, false));  // row_shr:4
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x118, 0xF, 0xF, false));  // row_shr:8
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
     return v;
 }
 
@@ -286,68 +296,63 @@ __global__ __launch_bounds__(TOP_THREADS) void k_bin(BuildArgs a, BuildState s) 
 // SAH sweep over the 3 x 31 planes (build.rs:87-127) for one node held as bins in
 // `bins` (3*32*7 ints, LDS or global). Executed by one full wave; returns the
 // chosen (dim, split) and the node box keys on every lane.
+// Bins accumulate in 32-lane scans: forward (lane j holds bin j, giving the left side
+// of plane j) and backward (lane j holds bin 31 - j, giving the right side of plane
+// 30 - j). Pass 0 scans axes 0 and 1 (one per half) both ways; pass 1 scans axis 2
+// forward in the low half and backward in the high half. Each plane's cost is
+// area(L) * n(L) + area(R) * n(R) as cost_function (build.rs:287-289) forms it; the
+// first minimum in (axis, plane) order wins, as the strict < of the reference's loops.
+struct BinScan { int c, k[6]; };
+__device__ __forceinline__ BinScan bin_scan(const int *bins, int d, int b) {
+    const int *bn = bins + (d * BINS + b) * 7;
+    BinScan r;
+    r.c = seg32_scan(bn[0], 0, [](int x, int y) { return x + y; });
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        r.k[q] = seg32_scan(bn[1 + q], KEY_MIN_ID, [](int x, int y) { return min(x, y); });
+        r.k[3 + q] = seg32_scan(bn[4 + q], KEY_MAX_ID, [](int x, int y) { return max(x, y); });
+    }
+    return r;
+}
+__device__ __forceinline__ float side_cost(const BinScan &r) {
+    const float a = r.c ? area_keys(r.k[0], r.k[1], r.k[2], r.k[3], r.k[4], r.k[5]) : 0.0f;
+    return a * (float)(uint32_t)r.c;
+}
 __device__ void sah_sweep_wave(const int *bins, const int *bounds, uint32_t *dim_out,
                                float *split_out, int box_out[6]) {
     const int lane = threadIdx.x % 64;
-    float best_c = __int_as_float(0x7f800000);  // +inf marks "no candidate"
-    uint32_t best_i = 0xFFFFu;
-    for (int pass = 0; pass < 2; ++pass) {
-        int d = pass == 0 ? lane / 32 : 2;
-        int b = lane % 32;
-        bool active = pass == 0 || lane < 32;
-        const int *bn = bins + (d * BINS + b) * 7;
-        uint32_t cnt = active ? (uint32_t)bn[0] : 0u;
-        int k[6];
-        for (int q = 0; q < 6; ++q) k[q] = active ? bn[1 + q] : (q < 3 ? KEY_MIN_ID : KEY_MAX_ID);
-        // inclusive prefix (bins 0..b) and suffix (bins b..31) within 32-lane segments
-        uint32_t lc = cnt, rc = cnt;
-        int lk[6], rk[6];
-        for (int q = 0; q < 6; ++q) { lk[q] = k[q]; rk[q] = k[q]; }
-        for (int off = 1; off < 32; off <<= 1) {
-            uint32_t uc = __shfl_up(lc, off, 32);
-            uint32_t dc = __shfl_down(rc, off, 32);
-            int uk[6], dk[6];
-            for (int q = 0; q < 6; ++q) { uk[q] = __shfl_up(lk[q], off, 32); dk[q] = __shfl_down(rk[q], off, 32); }
-            if (b >= off) {
-                lc += uc;
-                for (int q = 0; q < 6; ++q) lk[q] = q < 3 ? min(lk[q], uk[q]) : max(lk[q], uk[q]);
-            }
-            if (b + off < 32) {
-                rc += dc;
-                for (int q = 0; q < 6; ++q) rk[q] = q < 3 ? min(rk[q], dk[q]) : max(rk[q], dk[q]);
-            }
+    const int half = lane / 32, j = lane % 32;
+    // per lane best candidate: cost bits (costs are >= +0, so their bits order as the
+    // floats) and plane index d * 31 + b
+    uint32_t best_c = 0xFFFFFFFFu, best_i = 0xFFFFFFFFu;
+    auto offer = [&](float cost, uint32_t idx) {
+        if (cost < 3.40282347e+38f) {
+            const uint32_t cb = __float_as_uint(cost);
+            if (cb < best_c || (cb == best_c && idx < best_i)) { best_c = cb; best_i = idx; }
         }
-        // plane b: left = prefix(b), right = suffix(b + 1)
-        uint32_t nr = __shfl_down(rc, 1, 32);
-        int rr[6];
-        for (int q = 0; q < 6; ++q) rr[q] = __shfl_down(rk[q], 1, 32);
-        if (active && b < 31) {
-            float al = lc ? area_keys(lk[0], lk[1], lk[2], lk[3], lk[4], lk[5]) : 0.0f;
-            float ar = nr ? area_keys(rr[0], rr[1], rr[2], rr[3], rr[4], rr[5]) : 0.0f;
-            float cost = al * (float)lc + ar * (float)nr;  // cost_function, build.rs:287-289
-            uint32_t idx = (uint32_t)(d * 31 + b);
-            if (cost < 3.40282347e+38f) {
-                if (cost < best_c || (cost == best_c && idx < best_i)) { best_c = cost; best_i = idx; }
-            }
-        }
-        if (pass == 0 && lane == 0) {
-            // node box: union over all bins of axis 0 = lane 31's... prefix at b = 31 of segment 0
-        }
-        if (pass == 0) {
-            // the segment-0 prefix at lane 31 is the union of all primitive boxes
-            for (int q = 0; q < 6; ++q) box_out[q] = __shfl(lk[q], 31, 64);
-        }
+    };
+    {  // pass 0: axis `half`
+        const BinScan f = bin_scan(bins, half, j), r = bin_scan(bins, half, 31 - j);
+        const float x = side_cost(f), y = side_cost(r);
+        const float yr = __shfl(y, half * 32 + (j < 31 ? 30 - j : 0), 64);  // right side of plane j
+        if (j < 31) offer(x + yr, (uint32_t)(half * 31 + j));
+        // the forward prefix at lane 31 is the union of all primitive boxes
+#pragma unroll
+        for (int q = 0; q < 6; ++q) box_out[q] = __shfl(f.k[q], 31, 64);
     }
-    for (int m = 32; m >= 1; m >>= 1) {
-        float oc = __shfl_xor(best_c, m, 64);
-        uint32_t oi = __shfl_xor(best_i, m, 64);
-        if (oc < best_c || (oc == best_c && oi < best_i)) { best_c = oc; best_i = oi; }
+    {  // pass 1: axis 2, forward in the low half, backward in the high half
+        const BinScan g = bin_scan(bins, 2, half == 0 ? j : 31 - j);
+        const float v = side_cost(g);
+        const float yr = __shfl(v, j < 31 ? 62 - j : 32, 64);
+        if (half == 0 && j < 31) offer(v + yr, (uint32_t)(2 * 31 + j));
     }
-    if (best_i == 0xFFFFu) {  // no cost below f32::MAX: initial values (0, 0.0)
+    const uint32_t wc = __ockl_wfred_min_u32(best_c);
+    const uint32_t wi = __ockl_wfred_min_u32(best_c == wc ? best_i : 0xFFFFFFFFu);
+    if (wc == 0xFFFFFFFFu) {  // no cost below f32::MAX: initial values (0, 0.0)
         *dim_out = 0;
         *split_out = 0.0f;
     } else {
-        uint32_t d = best_i / 31, pl = best_i % 31;
+        uint32_t d = wi / 31, pl = wi % 31;
         float lo = funkey(bounds[d]), hi = funkey(bounds[3 + d]);
         float sc = (hi - lo) / (float)BINS;
         *dim_out = d;
