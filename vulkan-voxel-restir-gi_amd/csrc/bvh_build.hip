@@ -610,12 +610,14 @@ struct RankInfo {  // per internal node of the subtree, indexed by rank - job ra
 #define VRGI_PRAGMA(x) _Pragma(#x)
 
 // SAH plane search (build.rs:45-128) of one range of at most 12 primitives along one
-// axis d, by one lane. The bins of the range are packed 5 bits each into a 64-bit word. A sweep
-// from the highest occupied bin down stores the right-side cost of each occupied bin's
-// plane in tmp[f..f+occupied) (descending order); a sweep from bin 0 up then forms the
-// left side and evaluates the planes in ascending order with a strict < (the first
-// minimum). Each primitive's box is read once per sweep instead of once per plane; the
-// unions are exact min/max, so every cost equals the per-plane evaluation's bit for bit.
+// axis d, by one lane. The primitives are put in bin order once (a counting sort held in
+// registers: 5-bit bins and 4-bit per-bin counts in 64-bit words, prefix sums of the
+// counts by one multiply since no partial sum exceeds 12). A sweep from the highest bin
+// down stores the right-side cost of each occupied bin's plane in tmp[f..f+occupied)
+// (descending order); a sweep from bin 0 up forms the left side and evaluates the planes
+// in ascending order with a strict < (the first minimum). Each primitive's box is read
+// once per sweep; the unions are exact min/max, so every cost equals the per-plane
+// evaluation's bit for bit.
 template <int SUB>
 __device__ void axis_best_plane(const float (*box)[SUB], float *tmp, uint32_t f, uint32_t c, uint32_t d,
                                 float *cost_out, float *split_out, float nb[6]) {
@@ -626,13 +628,35 @@ __device__ void axis_best_plane(const float (*box)[SUB], float *tmp, uint32_t f,
     float lo = box[6 + d][f], hi = lo;
     for (uint32_t k = 1; k < c; ++k) { lo = gmin(lo, box[6 + d][f + k]); hi = gmax(hi, box[6 + d][f + k]); }
     const float scale = (float)BINS / (hi - lo);
-    uint64_t pk = 0;
+    uint64_t pk = 0, cnt_lo = 0, cnt_hi = 0;  // bins; per-bin counts of bins 0-15 and 16-31
     uint32_t occ = 0;
     for (uint32_t k = 0; k < c; ++k) {
         const uint32_t b = bin_index(box[6 + d][f + k], lo, scale);
         pk |= (uint64_t)b << (5 * k);
         occ |= 1u << b;
+        if (b < 16) cnt_lo += 1ull << (4 * b);
+        else cnt_hi += 1ull << (4 * (b - 16));
     }
+    const uint64_t ones = 0x1111111111111111ull;
+    const uint64_t inc_lo = cnt_lo * ones;                                // counts of bins <= b
+    const uint64_t inc_hi = cnt_hi * ones + (inc_lo >> 60) * ones;
+    uint64_t ord = 0, run_lo = 0, run_hi = 0;  // ord: primitive of each sorted position, 4 bits each
+    for (uint32_t k = 0; k < c; ++k) {
+        const uint32_t b = (uint32_t)((pk >> (5 * k)) & 31u);
+        uint32_t pos;
+        if (b < 16) {
+            const uint32_t sh = 4 * b;
+            pos = (uint32_t)((inc_lo >> sh) & 15u) - (uint32_t)((cnt_lo >> sh) & 15u) + (uint32_t)((run_lo >> sh) & 15u);
+            run_lo += 1ull << sh;
+        } else {
+            const uint32_t sh = 4 * (b - 16);
+            pos = (uint32_t)((inc_hi >> sh) & 15u) - (uint32_t)((cnt_hi >> sh) & 15u) + (uint32_t)((run_hi >> sh) & 15u);
+            run_hi += 1ull << sh;
+        }
+        ord |= (uint64_t)k << (4 * pos);
+    }
+    auto kth = [&](uint32_t i) { return (uint32_t)((ord >> (4 * i)) & 15u); };
+    auto bin_of = [&](uint32_t k) { return (uint32_t)((pk >> (5 * k)) & 31u); };
     float dbest = 3.40282347e+38f;
     uint32_t dplane = 0;
     if (!(occ & 1u)) { dbest = 0.0f * 0.0f + all_cost; dplane = 0; }  // left side empty
@@ -644,30 +668,33 @@ __device__ void axis_best_plane(const float (*box)[SUB], float *tmp, uint32_t f,
             u[3 + q] = n == 0 ? vx : gmax(u[3 + q], vx);
         }
     };
-    {  // right sides, highest occupied bin first: the plane at bin b has bins > b on the right
+    {  // right sides, highest bin first: the plane at bin b has bins > b on the right
         float r[6] = {0, 0, 0, 0, 0, 0};
-        uint32_t nr = 0, j = 0, rest = occ;
-        while (rest) {
-            const uint32_t b = 31u - (uint32_t)__builtin_clz(rest);
-            rest &= ~(1u << b);
-            tmp[f + j++] = nr ? area6(r) * (float)nr : 0.0f * 0.0f;
-            for (uint32_t k = 0; k < c; ++k)
-                if ((uint32_t)((pk >> (5 * k)) & 31u) == b) { add(r, nr, f + k); ++nr; }
+        uint32_t nr = 0, j = 0, prev = 0xFFFFFFFFu;
+        for (int i = (int)c - 1; i >= 0; --i) {
+            const uint32_t k = kth((uint32_t)i), b = bin_of(k);
+            if (b != prev) { tmp[f + j++] = nr ? area6(r) * (float)nr : 0.0f * 0.0f; prev = b; }
+            add(r, nr, f + k);
+            ++nr;
         }
     }
     {  // left sides and plane costs, bin 0 up (no plane after bin 31)
         float l[6] = {0, 0, 0, 0, 0, 0};
         const uint32_t nocc = (uint32_t)__builtin_popcount(occ);
-        uint32_t nl = 0, i = 0, rest = occ & 0x7fffffffu;
-        while (rest) {
-            const uint32_t b = (uint32_t)__builtin_ctz(rest);
-            rest &= rest - 1u;
-            for (uint32_t k = 0; k < c; ++k)
-                if ((uint32_t)((pk >> (5 * k)) & 31u) == b) { add(l, nl, f + k); ++nl; }
-            const float al = area6(l) * (float)nl;
-            const float ar = tmp[f + (nocc - 1u - i++)];
-            const float cost = al + ar;
-            if (cost < dbest) { dbest = cost; dplane = b; }
+        uint32_t nl = 0, j = 0;
+        for (uint32_t i = 0; i < c; ++i) {
+            const uint32_t k = kth(i), b = bin_of(k);
+            add(l, nl, f + k);
+            ++nl;
+            if (i + 1 == c || bin_of(kth(i + 1)) != b) {  // last primitive of bin b
+                if (b < 31) {
+                    const float al = area6(l) * (float)nl;
+                    const float ar = tmp[f + (nocc - 1u - j)];
+                    const float cost = al + ar;
+                    if (cost < dbest) { dbest = cost; dplane = b; }
+                }
+                ++j;
+            }
         }
     }
     *cost_out = dbest;
@@ -681,7 +708,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     static_assert(SUB <= 1024 && SMALL <= 127, "small-range queue packs 10 + 7 + 10 bits");
     static_assert(SMALL <= 12, "bins of a small range are packed 5 bits each into 64 bits");
     __shared__ uint32_t s_perm[SUB];
-    __shared__ float s_box[9][SUB];  // min xyz, max xyz, centroid xyz
+    __shared__ float s_box[11][SUB];  // min xyz, max xyz, centroid xyz, power, id bits
     __shared__ uint32_t s_pair[SUB];
     __shared__ uint32_t s_cj[SUB];
     __shared__ int s_bins[BIN_WORDS];
@@ -704,6 +731,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         s_box[0][k] = b0.x; s_box[1][k] = b0.y; s_box[2][k] = b0.z;
         s_box[3][k] = b0.w; s_box[4][k] = b1.x; s_box[5][k] = b1.y;
         for (int q = 0; q < 3; ++q) s_box[6 + q][k] = a.pc[q * a.n + jn.first + k];
+        s_box[9][k] = b1.z; s_box[10][k] = b1.w;
     }
     uint32_t sp = 0, maxdepth = jn.depth;
     uint32_t splitsA = 0, splitsB = 0;  // split sizes (sum = sum of leaf depths): wave-wide, per lane
@@ -718,6 +746,24 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     if (c > (uint32_t)SMALL) sp = 1;
     __syncthreads();
     const float pad = 0.0001f;
+    // leaves and 2-primitive nodes (build.rs:311-326, 203-230) from the LDS copy of the
+    // primitives at job positions e, e + 1
+    auto lds_leaf = [&](uint32_t gnode, uint32_t parent, uint32_t e) {
+        float mn[3] = {s_box[0][e], s_box[1][e], s_box[2][e]};
+        float mx[3] = {s_box[3][e], s_box[4][e], s_box[5][e]};
+        write_node(a.nodes + gnode, VRGI_INVALID, __float_as_uint(s_box[10][e]), mn, s_box[9][e], mx, parent);
+    };
+    auto lds_pair = [&](uint32_t idx, uint32_t r, uint32_t parent, uint32_t e) {
+        float mn[3], mx[3];
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = gmin(s_box[k][e], s_box[k][e + 1]) - pad;
+            mx[k] = gmax(s_box[3 + k][e], s_box[3 + k][e + 1]) + pad;
+        }
+        float pw = (0.0f + s_box[9][e]) + s_box[9][e + 1];
+        write_node(a.nodes + base + idx, 2 * r + 1, 2 * r + 2, mn, pw, mx, parent);
+        lds_leaf(base + 2 * r + 1, idx, e);
+        lds_leaf(base + 2 * r + 2, idx, e + 1);
+    };
     // ---------------- phase A: ranges larger than SMALL, one wave per range
     while (sp > 0) {
         --sp;
@@ -786,7 +832,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                 if (cj & 0x80000000u) continue;
                 uint32_t e0 = nf + k, e1 = nf + s_pair[k - cj];
                 uint32_t tp = s_perm[e0]; s_perm[e0] = s_perm[e1]; s_perm[e1] = tp;
-                for (int q = 0; q < 9; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
+                for (int q = 0; q < 11; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
             }
             __syncthreads();
         }
@@ -806,10 +852,10 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             uint32_t cr = side ? rank + nl : rank + 1;
             uint32_t ci = side ? ri : li;
             if (cc == 1) {
-                if (lane == 0) emit_leaf(a, base + ci, gidx, s_perm[cf]);
+                if (lane == 0) lds_leaf(base + ci, gidx, cf);
             } else if (cc == 2) {
                 if (lane == 0) {
-                    emit_pair(a, base, ci, cr, gidx, s_perm[cf], s_perm[cf + 1]);
+                    lds_pair(ci, cr, gidx, cf);
                     s_rank[cr - jn.rank] = RankInfo{(uint16_t)cf, 1, 2, 0, ci};
                 }
                 splitsA += 2;
@@ -827,15 +873,14 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         }
         __syncthreads();
     }
-    // ---------------- phase B: ranges of 3..SMALL primitives, three lanes each, one
-    // level of the subtree per pass; the ranges of a pass are disjoint, so at most SUB/3
+    // ---------------- phase B: ranges of 3..SMALL primitives, three lanes each (one per
+    // axis; the first leads), one level of the subtree per pass; the ranges of a pass are
+    // disjoint, so at most SUB/3
     uint32_t cur = 0, ncur = s_nsmall;
     while (ncur > 0) {
         __syncthreads();
         if (lane == 0) s_nsmall = 0;
         __syncthreads();
-        // three lanes per range, one per axis; the range's first lane picks the axis
-        // (strict < in axis order, as find_best_plane's outer loop) and splits it
         for (uint32_t w0 = 0; w0 < ncur; w0 += 21) {
             const uint32_t w = w0 + (uint32_t)lane / 3u, ax = (uint32_t)lane % 3u;
             const bool act = lane < 63 && w < ncur;
@@ -865,7 +910,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                 const uint32_t e0 = nf + l, e1 = nf + r;
                 const uint32_t tp = s_perm[e0]; s_perm[e0] = s_perm[e1]; s_perm[e1] = tp;
 #pragma unroll
-                for (int q = 0; q < 9; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
+                for (int q = 0; q < 11; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
             }
             const uint32_t nl = (T == 0 || T == nc) ? nc / 2 : T, nr = nc - nl;
             const uint32_t li = 2 * rank + 1, ri = 2 * rank + 2;
@@ -881,9 +926,9 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                 const uint32_t cr = side ? rank + nl : rank + 1;
                 const uint32_t ci = side ? ri : li;
                 if (cc == 1) {
-                    emit_leaf(a, base + ci, gidx, s_perm[cf]);
+                    lds_leaf(base + ci, gidx, cf);
                 } else if (cc == 2) {
-                    emit_pair(a, base, ci, cr, gidx, s_perm[cf], s_perm[cf + 1]);
+                    lds_pair(ci, cr, gidx, cf);
                     s_rank[cr - jn.rank] = RankInfo{(uint16_t)cf, 1, 2, 0, ci};
                     splitsB += 2;
                     maxdepth = max(maxdepth, dep + 2);
@@ -898,18 +943,20 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         cur ^= 1;
     }
     __syncthreads();
-    // ---------------- phase C: power, decreasing rank (children first). Leaf powers are
-    // gathered into LDS by all lanes first (s_cj is free after phase A); the left child of
-    // rank k, when internal, is rank k + 1, the value computed one step earlier.
-    for (uint32_t k = lane; k < c; k += 64) s_cj[k] = __float_as_uint(a.box8[2 * s_perm[k] + 1].z);
-    __syncthreads();
-    if (lane == 0) {
+    // ---------------- phase C: power, decreasing rank (children first); the left child of
+    // rank k, when internal, is rank k + 1, the value computed one step earlier. With every
+    // leaf power zero (the geometry trees) each sum is +0, the value the nodes were
+    // written with, and the pass is skipped.
+    bool any_power = false;
+    for (uint32_t k = lane; k < c; k += 64) any_power |= s_box[9][k] != 0.0f;
+    const bool powered = __any(any_power);
+    if (lane == 0 && powered) {
         float prev = 0.0f;
         for (int k = (int)c - 2; k >= 0; --k) {
             const RankInfo ri = s_rank[k];
             const uint32_t nr = (uint32_t)ri.count - ri.nl;
-            const float pl = ri.nl == 1 ? __uint_as_float(s_cj[ri.first]) : prev;
-            const float pr = nr == 1 ? __uint_as_float(s_cj[ri.first + ri.nl]) : s_ipow[k + ri.nl];
+            const float pl = ri.nl == 1 ? s_box[9][ri.first] : prev;
+            const float pr = nr == 1 ? s_box[9][ri.first + ri.nl] : s_ipow[k + ri.nl];
             float p = 0.0f;
             p += pl;
             p += pr;
