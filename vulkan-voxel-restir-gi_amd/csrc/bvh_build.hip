@@ -31,6 +31,8 @@ static constexpr int CHUNK = 4096;      // primitives per top-phase work item
 static constexpr int TOP_THREADS = 256;
 static constexpr int PER_THREAD = CHUNK / TOP_THREADS;  // 16
 static constexpr int BIN_WORDS = 3 * BINS * 7;          // count + 6 box keys per bin
+static constexpr uint32_t GRID_CAP_ITEMS = 4096;         // blocks of the per-item kernels
+static constexpr uint32_t GRID_CAP_NODES = 8192;         // blocks of the per-node kernels
 
 struct OpenNode {
     uint32_t first, count, rank, node, parent, tree, depth, pad;
@@ -146,10 +148,23 @@ __global__ void k_roots(BuildArgs a, BuildState s) {
 // items for the open nodes of this level: a block-wide scan of ceil(count / CHUNK) per
 // node, then every thread writes items of any node (found by binary search over the
 // inclusive scan), so a node with many chunks is not written by one thread
-__global__ void k_items(BuildState s) {
+// It also starts the level: the open list of the previous level's children becomes
+// current, the level's first entry in the level-node list is recorded and, when the level
+// holds open nodes, the count of non-empty levels is raised to include it.
+__global__ void k_items(BuildState s, uint32_t *level_start, uint32_t lvl, uint32_t *levels_used) {
     __shared__ uint32_t s_sum[1024];
+    __shared__ uint32_t s_nopen;
+    if (threadIdx.x == 0) {
+        const uint32_t n = *s.n_open_next;
+        s_nopen = n;
+        *s.n_open = n;
+        *s.n_open_next = 0;
+        level_start[lvl] = *s.n_level_nodes;
+        if (n) atomicMax(levels_used, lvl + 1u);
+    }
+    __syncthreads();
     const OpenNode *open = (const OpenNode *)s.open;
-    uint32_t n_open = *s.n_open;
+    const uint32_t n_open = s_nopen;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < n_open; base += 1024) {
         uint32_t i = base + threadIdx.x;
@@ -190,14 +205,19 @@ __global__ void k_items(BuildState s) {
     if (threadIdx.x == 0) *s.n_items = carry;
 }
 
+// Grids of the per-level kernels are capped on the host (an upper bound of a level's
+// nodes or items can be far above its actual count); blocks stride over the device count.
 __global__ void k_clear_node_data(BuildState s) {
-    uint32_t o = blockIdx.x;
-    if (o >= *s.n_open) return;
-    for (uint32_t k = threadIdx.x; k < BIN_WORDS; k += blockDim.x) {
-        uint32_t w = k % 7;
-        s.bins[(size_t)o * BIN_WORDS + k] = w == 0 ? 0u : (w <= 3 ? (uint32_t)KEY_MIN_ID : (uint32_t)KEY_MAX_ID);
+    const uint32_t n_open = *s.n_open;
+    for (uint32_t o = blockIdx.x; o < n_open; o += gridDim.x) {
+        if (s.item_end[o] - s.item_beg[o] > 1u) {  // a node of one item never reads its global bins
+            for (uint32_t k = threadIdx.x; k < BIN_WORDS; k += blockDim.x) {
+                uint32_t w = k % 7;
+                s.bins[(size_t)o * BIN_WORDS + k] = w == 0 ? 0u : (w <= 3 ? (uint32_t)KEY_MIN_ID : (uint32_t)KEY_MAX_ID);
+            }
+        }
+        if (threadIdx.x < 6) s.bounds[o * 6 + threadIdx.x] = threadIdx.x < 3 ? KEY_MIN_ID : KEY_MAX_ID;
     }
-    if (threadIdx.x < 6) s.bounds[o * 6 + threadIdx.x] = threadIdx.x < 3 ? KEY_MIN_ID : KEY_MAX_ID;
 }
 
 // wave-wide reductions over all 64 lanes (all lanes active at every call site): the
@@ -225,8 +245,9 @@ __device__ __forceinline__ int seg32_scan(int v, int id, Op op) {
 
 // centroid bounds of each open node (find_best_plane's first loop, build.rs:60-66)
 __global__ __launch_bounds__(TOP_THREADS) void k_bounds(BuildArgs a, BuildState s) {
-    if (blockIdx.x >= *s.n_items) return;
-    Item it = ((const Item *)s.items)[blockIdx.x];
+  const uint32_t n_items = *s.n_items;
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    Item it = ((const Item *)s.items)[item];
     int mn[3] = {KEY_MIN_ID, KEY_MIN_ID, KEY_MIN_ID}, mx[3] = {KEY_MAX_ID, KEY_MAX_ID, KEY_MAX_ID};
     for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
         uint32_t i = it.beg + k;
@@ -251,15 +272,52 @@ __global__ __launch_bounds__(TOP_THREADS) void k_bounds(BuildArgs a, BuildState 
         if (threadIdx.x < 3) atomicMin(dst, v);
         else atomicMax(dst, v);
     }
+    __syncthreads();  // red is written again by the next item
+  }
 }
 
 // bin counts and bin boxes (build.rs:68-85)
+// Consecutive primitives of a range are spatial neighbours, so many lanes of a wave fall
+// in the same bin and their LDS atomics on one address serialise. The block therefore
+// keeps up to BIN_REPL copies of its bins (fewer for short items, whose clearing and
+// merging of the copies would cost more than the conflicts), lane l updating copy
+// l % copies; copies are
+// BIN_WORDS + 1 words apart, so one word of the BIN_REPL copies lies in BIN_REPL
+// different banks. A wave whose live lanes all fall in one bin of an axis reduces them
+// across the wave first (one LDS update). The copies are merged before the node's global
+// bins are updated; every union is an exact min / max, so the order is free.
+#ifndef VRGI_BIN_REPL
+#define VRGI_BIN_REPL 8
+#endif
+static constexpr int BIN_REPL = VRGI_BIN_REPL;
+static constexpr int BIN_STRIDE = BIN_WORDS + 1;
+__device__ void sah_sweep_wave(const int *bins, const int *bounds, uint32_t *dim_out, float *split_out,
+                               int box_out[6]);
+// SAH sweep of open node o from its bins, by one full wave; the split record of the node
+__device__ __forceinline__ void store_split(BuildState &s, uint32_t o, const int *bins) {
+    uint32_t dim;
+    float split;
+    int box[6];
+    sah_sweep_wave(bins, s.bounds + o * 6, &dim, &split, box);
+    if ((threadIdx.x & 63u) == 0) {
+        SplitInfo si;
+        si.dim = dim;
+        si.split = split;
+        si.T = 0;
+        si.fallback = 0;
+        for (int q = 0; q < 6; ++q) si.box[q] = box[q];
+        ((SplitInfo *)s.split)[o] = si;
+    }
+}
 __global__ __launch_bounds__(TOP_THREADS) void k_bin(BuildArgs a, BuildState s) {
-    if (blockIdx.x >= *s.n_items) return;
-    Item it = ((const Item *)s.items)[blockIdx.x];
-    __shared__ int sb[BIN_WORDS];
-    for (int k = threadIdx.x; k < BIN_WORDS; k += TOP_THREADS) {
-        int w = k % 7;
+  __shared__ int sb[BIN_REPL * BIN_STRIDE];
+  const uint32_t n_items = *s.n_items;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    Item it = ((const Item *)s.items)[item];
+    const int nrep = it.len >= 1536 ? BIN_REPL : (it.len >= 384 ? min(4, BIN_REPL) : min(2, BIN_REPL));
+    for (int k = threadIdx.x; k < nrep * BIN_STRIDE; k += TOP_THREADS) {
+        int w = (k % BIN_STRIDE) % 7;
         sb[k] = w == 0 ? 0 : (w <= 3 ? KEY_MIN_ID : KEY_MAX_ID);
     }
     float bmin[3], scale[3];
@@ -269,28 +327,78 @@ __global__ __launch_bounds__(TOP_THREADS) void k_bin(BuildArgs a, BuildState s) 
         scale[d] = (float)BINS / (hi - lo);
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
-        uint32_t i = it.beg + k;
-        uint32_t p = a.perm[i];
-        const float4 b0 = a.box8[2 * p], b1 = a.box8[2 * p + 1];
-        const int bk[6] = {fkey(b0.x), fkey(b0.y), fkey(b0.z), fkey(b0.w), fkey(b1.x), fkey(b1.y)};
+    int *mine_copy = sb + (lane % (uint32_t)nrep) * BIN_STRIDE;
+    const uint32_t rounds = (it.len + TOP_THREADS - 1) / TOP_THREADS;  // uniform over the block
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t k = r * TOP_THREADS + threadIdx.x;
+        const bool valid = k < it.len;
+        int bk[6] = {KEY_MIN_ID, KEY_MIN_ID, KEY_MIN_ID, KEY_MAX_ID, KEY_MAX_ID, KEY_MAX_ID};
+        uint32_t bb[3] = {0u, 0u, 0u};
+        if (valid) {
+            const uint32_t i = it.beg + k;
+            const uint32_t p = a.perm[i];
+            const float4 b0 = a.box8[2 * p], b1 = a.box8[2 * p + 1];
+            bk[0] = fkey(b0.x); bk[1] = fkey(b0.y); bk[2] = fkey(b0.z);
+            bk[3] = fkey(b0.w); bk[4] = fkey(b1.x); bk[5] = fkey(b1.y);
+            for (int d = 0; d < 3; ++d) bb[d] = bin_index(a.pc[d * a.n + i], bmin[d], scale[d]);
+        }
+        const uint64_t live = __ballot(valid);
         for (int d = 0; d < 3; ++d) {
-            uint32_t b = bin_index(a.pc[d * a.n + i], bmin[d], scale[d]);
-            int *bin = sb + (d * BINS + b) * 7;
-            atomicAdd(bin, 1);
-            atomicMin(bin + 1, bk[0]); atomicMin(bin + 2, bk[1]); atomicMin(bin + 3, bk[2]);
-            atomicMax(bin + 4, bk[3]); atomicMax(bin + 5, bk[4]); atomicMax(bin + 6, bk[5]);
+            const int lead = __ffsll((unsigned long long)live) - 1;
+            const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)bb[d], lead < 0 ? 0 : lead);
+            if (live && __ballot(valid && bb[d] != lb) == 0) {  // one bin for the whole wave (uniform)
+                const int m0 = wave_min(bk[0]), m1 = wave_min(bk[1]), m2 = wave_min(bk[2]);
+                const int x0 = wave_max(bk[3]), x1 = wave_max(bk[4]), x2 = wave_max(bk[5]);
+                if ((int)lane == lead) {
+                    int *bin = mine_copy + (d * BINS + lb) * 7;
+                    atomicAdd(bin, (int)__popcll(live));
+                    atomicMin(bin + 1, m0); atomicMin(bin + 2, m1); atomicMin(bin + 3, m2);
+                    atomicMax(bin + 4, x0); atomicMax(bin + 5, x1); atomicMax(bin + 6, x2);
+                }
+            } else if (valid) {
+                int *bin = mine_copy + (d * BINS + bb[d]) * 7;
+                atomicAdd(bin, 1);
+                atomicMin(bin + 1, bk[0]); atomicMin(bin + 2, bk[1]); atomicMin(bin + 3, bk[2]);
+                atomicMax(bin + 4, bk[3]); atomicMax(bin + 5, bk[4]); atomicMax(bin + 6, bk[5]);
+            }
         }
     }
     __syncthreads();
     int *g = (int *)s.bins + (size_t)it.open * BIN_WORDS;
     for (int k = threadIdx.x; k < BIN_WORDS; k += TOP_THREADS) {
-        int bin = k / 7, w = k % 7;
-        if (sb[bin * 7] == 0) continue;
-        if (w == 0) atomicAdd(g + k, sb[k]);
-        else if (w <= 3) atomicMin(g + k, sb[k]);
-        else atomicMax(g + k, sb[k]);
+        const int w = k % 7;
+        int v = sb[k];
+        for (int c = 1; c < nrep; ++c) {
+            const int u = sb[c * BIN_STRIDE + k];
+            v = w == 0 ? v + u : (w <= 3 ? min(v, u) : max(v, u));
+        }
+        sb[k] = v;  // copy 0 holds the merged bins (each word read and written by one thread)
     }
+    __syncthreads();
+    // A node of one item has its complete bins in LDS: its SAH sweep (build.rs:87-127)
+    // runs here. The items of a larger node add their bins to the node's global bins,
+    // swept by k_split.
+    const uint32_t o = it.open;
+    if (s.item_end[o] - s.item_beg[o] > 1u) {
+        for (int k = threadIdx.x; k < BIN_WORDS; k += TOP_THREADS) {
+            const int bin = k / 7, w = k % 7;
+            if (sb[bin * 7] == 0) continue;
+            if (w == 0) atomicAdd(g + k, sb[k]);
+            else if (w <= 3) atomicMin(g + k, sb[k]);
+            else atomicMax(g + k, sb[k]);
+        }
+    } else if (threadIdx.x < 64) {  // wave 0, all lanes
+        store_split(s, o, sb);
+    }
+    __syncthreads();  // sb is cleared again by the next item
+  }
+}
+
+// SAH sweep of the nodes of several items, over their global bins
+__global__ __launch_bounds__(64) void k_split(BuildState s) {
+    const uint32_t n_open = *s.n_open;
+    for (uint32_t o = blockIdx.x; o < n_open; o += gridDim.x)
+        if (s.item_end[o] - s.item_beg[o] > 1u) store_split(s, o, (const int *)s.bins + (size_t)o * BIN_WORDS);
 }
 
 // SAH sweep over the 3 x 31 planes (build.rs:87-127) for one node held as bins in
@@ -360,121 +468,115 @@ __device__ void sah_sweep_wave(const int *bins, const int *bounds, uint32_t *dim
     }
 }
 
-__global__ __launch_bounds__(64) void k_split(BuildState s) {
-    uint32_t o = blockIdx.x;
-    if (o >= *s.n_open) return;
-    uint32_t dim;
-    float split;
-    int box[6];
-    sah_sweep_wave((const int *)s.bins + (size_t)o * BIN_WORDS, s.bounds + o * 6, &dim, &split, box);
-    if (threadIdx.x == 0) {
-        SplitInfo si;
-        si.dim = dim;
-        si.split = split;
-        si.T = 0;
-        si.fallback = 0;
-        for (int q = 0; q < 6; ++q) si.box[q] = box[q];
-        ((SplitInfo *)s.split)[o] = si;
-    }
-}
-
 // per-item count of `centroid[dim] < split` (the partition predicate, build.rs:148)
 __global__ __launch_bounds__(TOP_THREADS) void k_count(BuildArgs a, BuildState s) {
-    if (blockIdx.x >= *s.n_items) return;
-    Item it = ((const Item *)s.items)[blockIdx.x];
+  __shared__ uint32_t red[TOP_THREADS / 64];
+  const uint32_t n_items = *s.n_items;
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    Item it = ((const Item *)s.items)[item];
     SplitInfo si = ((const SplitInfo *)s.split)[it.open];
     uint32_t c = 0;
     for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS)
         c += a.pc[si.dim * a.n + it.beg + k] < si.split ? 1u : 0u;
     c = wave_sum(c);
-    __shared__ uint32_t red[TOP_THREADS / 64];
     if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = 0;
         for (int k = 0; k < TOP_THREADS / 64; ++k) t += red[k];
-        s.item_true[blockIdx.x] = t;
+        s.item_true[item] = t;
     }
-}
-
-// per node (one wave): exclusive scan of its items' true counts, T, fallback
-__global__ __launch_bounds__(64) void k_prep(BuildState s) {
-    uint32_t o = blockIdx.x;
-    if (o >= *s.n_open) return;
-    uint32_t ib = s.item_beg[o], ie = s.item_end[o];
-    uint32_t carry = 0;
-    int lane = threadIdx.x;
-    for (uint32_t base = ib; base < ie; base += 64) {
-        uint32_t i = base + lane;
-        uint32_t v = i < ie ? s.item_true[i] : 0u;
-        uint32_t inc = v;
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t u = __shfl_up(inc, off, 64);
-            if (lane >= off) inc += u;
-        }
-        if (i < ie) s.item_off[i] = carry + inc - v;
-        carry += __shfl(inc, 63, 64);
-    }
-    if (lane == 0) {
-        const OpenNode &on = ((const OpenNode *)s.open)[o];
-        SplitInfo *si = (SplitInfo *)s.split + o;
-        si->T = carry;
-        si->fallback = (carry == 0 || carry == on.count) ? 1u : 0u;  // build.rs:152
-    }
-}
-
-// block-wide exclusive scan of one value per thread
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tmp) {
-    int lane = threadIdx.x % 64, w = threadIdx.x / 64;
-    uint32_t inc = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t u = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += u;
-    }
-    if (lane == 63) tmp[w] = inc;
-    __syncthreads();
-    uint32_t pre = 0;
-    for (int k = 0; k < w; ++k) pre += tmp[k];
-    __syncthreads();
-    return pre + inc - v;
+    __syncthreads();  // red is written again by the next item
+  }
 }
 
 // partition, pass A: every element learns its exclusive count of trues c_j within the
-// node; right-region trues publish their position at slot T-1-c_j
+// node; right-region trues publish their position at slot T-1-c_j. The item's count of
+// trues before it (and the node's total T, hence the half-split fallback of
+// build.rs:152) come from the per-item counts. Element e of an item is read by thread
+// e % 256 in row e / 256 (coalesced); its rank among the item's trues is the count of
+// earlier (row, wave) groups plus the earlier lanes of its own ballot.
 __global__ __launch_bounds__(TOP_THREADS) void k_part_a(BuildArgs a, BuildState s) {
-    if (blockIdx.x >= *s.n_items) return;
-    Item it = ((const Item *)s.items)[blockIdx.x];
-    SplitInfo si = ((const SplitInfo *)s.split)[it.open];
-    if (si.fallback) return;
-    const OpenNode on = ((const OpenNode *)s.open)[it.open];
-    __shared__ uint32_t tmp[TOP_THREADS / 64];
-    uint32_t k0 = threadIdx.x * PER_THREAD;
-    uint32_t flags = 0, cnt = 0;
+  constexpr int WAVES = TOP_THREADS / 64;
+  __shared__ uint32_t s_cnt[PER_THREAD * WAVES];  // trues per (row, wave), then their exclusive prefix
+  __shared__ uint32_t s_off, s_T;
+  const uint32_t n_items = *s.n_items;
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x / 64;
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const Item it = ((const Item *)s.items)[item];
+    const uint32_t o = it.open;
+    SplitInfo *sip = (SplitInfo *)s.split + o;
+    const uint32_t dim = sip->dim;
+    const float split = sip->split;
+    const OpenNode on = ((const OpenNode *)s.open)[o];
+    if (w == 0) {
+        const uint32_t ib = s.item_beg[o], ie = s.item_end[o];
+        uint32_t before = 0, total = 0;
+        for (uint32_t j = ib + lane; j < ie; j += 64) {
+            const uint32_t v = s.item_true[j];
+            total += v;
+            before += j < item ? v : 0u;
+        }
+        before = wave_sum(before);
+        total = wave_sum(total);
+        if (lane == 0) {
+            s_off = before;
+            s_T = total;
+            if (item == ib) {
+                sip->T = total;
+                sip->fallback = (total == 0 || total == on.count) ? 1u : 0u;  // build.rs:152
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t T = s_T;
+    if (T == 0 || T == on.count) {  // fallback: no partition (uniform over the block)
+        __syncthreads();
+        continue;
+    }
+    uint32_t flags = 0;
     for (int q = 0; q < PER_THREAD; ++q) {
-        uint32_t k = k0 + q;
-        bool p = k < it.len && a.pc[si.dim * a.n + it.beg + k] < si.split;
+        const uint32_t e = (uint32_t)q * TOP_THREADS + threadIdx.x;
+        const bool p = e < it.len && a.pc[dim * a.n + it.beg + e] < split;
         flags |= (p ? 1u : 0u) << q;
-        cnt += p ? 1u : 0u;
+        const uint64_t m = __ballot(p);
+        if (lane == 0) s_cnt[q * WAVES + w] = (uint32_t)__popcll(m);
     }
-    uint32_t c = s.item_off[blockIdx.x] + block_excl_scan(cnt, tmp);
+    __syncthreads();
+    if (w == 0) {  // exclusive prefix of the PER_THREAD * WAVES = 64 group counts
+        const uint32_t v = s_cnt[lane];
+        uint32_t inc = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(inc, off, 64);
+            if (lane >= (uint32_t)off) inc += u;
+        }
+        s_cnt[lane] = inc - v;
+    }
+    __syncthreads();
+    const uint32_t off = s_off;
     for (int q = 0; q < PER_THREAD; ++q) {
-        uint32_t k = k0 + q;
-        if (k >= it.len) break;
-        uint32_t i = it.beg + k;
-        uint32_t j = i - on.first;
-        bool p = (flags >> q) & 1u;
+        const uint32_t e = (uint32_t)q * TOP_THREADS + threadIdx.x;
+        const bool p = (flags >> q) & 1u;
+        const uint64_t m = __ballot(p);
+        if (e >= it.len) continue;
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const uint32_t c = off + s_cnt[q * WAVES + w] + below;
+        const uint32_t i = it.beg + e;
+        const uint32_t j = i - on.first;
         s.rank_tmp[i] = c | (p ? 0x80000000u : 0u);
-        if (p && j >= si.T) s.pair[on.first + (si.T - 1u - c)] = i;
-        c += p ? 1u : 0u;
+        if (p && j >= T) s.pair[on.first + (T - 1u - c)] = i;
     }
+    __syncthreads();  // s_cnt, s_off, s_T are written again by the next item
+  }
 }
 
 // partition, pass B: the r-th left-region false swaps with the r-th true from the right
 __global__ __launch_bounds__(TOP_THREADS) void k_part_b(BuildArgs a, BuildState s) {
-    if (blockIdx.x >= *s.n_items) return;
-    Item it = ((const Item *)s.items)[blockIdx.x];
+  const uint32_t n_items = *s.n_items;
+  for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    Item it = ((const Item *)s.items)[item];
     SplitInfo si = ((const SplitInfo *)s.split)[it.open];
-    if (si.fallback) return;
+    if (si.fallback) continue;
     const OpenNode on = ((const OpenNode *)s.open)[it.open];
     for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
         uint32_t i = it.beg + k;
@@ -493,6 +595,7 @@ __global__ __launch_bounds__(TOP_THREADS) void k_part_b(BuildArgs a, BuildState 
             a.pc[d * a.n + p] = f;
         }
     }
+  }
 }
 
 // children of each open node: node records, leaves, pairs, jobs, next open nodes
@@ -519,9 +622,9 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
 // nodes and subtree jobs. List order is free (ranks come from counts), so list slots,
 // depth maxima and split sums take one atomic per wave.
 __global__ void k_children(BuildArgs a, BuildState s) {
-    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n_open = *s.n_open;
-    if (blockIdx.x * blockDim.x >= n_open) return;  // the whole block (uniform)
+  const uint32_t n_open = *s.n_open;
+  for (uint32_t o0 = blockIdx.x * blockDim.x; o0 < n_open; o0 += gridDim.x * blockDim.x) {  // uniform
+    const uint32_t o = o0 + threadIdx.x;
     const bool live = o < n_open;
     OpenNode on{};
     uint32_t base = 0, dep = 0;
@@ -583,6 +686,7 @@ __global__ void k_children(BuildArgs a, BuildState s) {
     }
     const unsigned long long sp = wave_sum64(splits);
     if ((threadIdx.x & 63u) == 0u && sp) atomicAdd(a.split_count_sum, sp);
+  }
 }
 
 // ---------------------------------------------------------------- bottom phase
@@ -995,7 +1099,7 @@ void Builder::reserve(uint32_t n, uint32_t n_trees) {
     A((void **)&st.jobs, (size_t)(cap_n / 3 + cap_t + 16) * sizeof(OpenNode));
     A((void **)&st.items, (size_t)item_cap * sizeof(Item));
     A((void **)&st.item_true, (size_t)item_cap * 4);
-    A((void **)&st.item_off, (size_t)item_cap * 4);
+
     A((void **)&st.item_beg, (size_t)open_cap * 4);
     A((void **)&st.item_end, (size_t)open_cap * 4);
     A((void **)&st.bins, (size_t)open_cap * BIN_WORDS * 4);
@@ -1015,7 +1119,7 @@ void Builder::reserve(uint32_t n, uint32_t n_trees) {
 
 void Builder::free_all() {
     void *ptrs[] = {perm, pc, box8, st.rank_tmp, st.pair, st.open, st.open_next, st.jobs, st.items,
-                    st.item_true, st.item_off, st.item_beg, st.item_end, st.bins, st.bounds,
+                    st.item_true, st.item_beg, st.item_end, st.bins, st.bounds,
                     st.split, st.level_nodes, counters, level_start};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1038,14 +1142,14 @@ __global__ void k_advance(BuildState s, uint32_t *level_start, uint32_t lvl) {
 // power pass over the (global node, tree base) pairs of level l
 __global__ void k_power_pairs(vrgi_bvh_node *nodes, const uint32_t *pairs, const uint32_t *level_start, uint32_t l) {
     const uint32_t beg = level_start[l], end = level_start[l + 1];
-    uint32_t i = beg + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= end) return;
-    uint32_t gi = pairs[2 * i], b = pairs[2 * i + 1];
-    vrgi_bvh_node nd = nodes[gi];
-    float p = 0.0f;
-    p += nodes[b + nd.left].power;
-    p += nodes[b + nd.right_or_prim].power;
-    nodes[gi].power = p;
+    for (uint32_t i = beg + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
+        uint32_t gi = pairs[2 * i], b = pairs[2 * i + 1];
+        vrgi_bvh_node nd = nodes[gi];
+        float p = 0.0f;
+        p += nodes[b + nd.left].power;
+        p += nodes[b + nd.right_or_prim].power;
+        nodes[gi].power = p;
+    }
 }
 
 int Builder::build(hipStream_t stream, const float *prim9, const float *power, const uint32_t *ids,
@@ -1076,11 +1180,14 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     k_init<<<(n + 255) / 256, 256, 0, stream>>>(a);
     k_roots<<<(n_trees + 255) / 256, 256, 0, stream>>>(a, st);
     // Top phase, level-synchronous. Level l holds at most n_trees * 2^l open nodes, so the
-    // grids are sized on the host without reading the device; kernels exit on the device
-    // counts. Levels are queued in groups of 1, 2, 4, 4, ... and the host reads the
+    // grids are bounded on the host without reading the device (and capped: blocks stride
+    // over the device counts). Levels are queued in groups and the host reads the
     // counters once per group to see whether any node is still open (extra levels are
-    // empty).
-    uint32_t group = 1;
+    // empty). The first group has as many levels as the previous build of the same size
+    // had non-empty levels (a frame-by-frame rebuild of a similar scene then takes one
+    // readback), later groups 1, 2, 4, 4, ...
+    uint32_t group = (prev_n == n && prev_trees == n_trees && prev_levels) ? prev_levels : 1u;
+    bool first_group = true;
     auto bound = [&](uint32_t lvl) -> uint32_t {
         uint64_t b = lvl >= 32 ? ~0ull : ((uint64_t)n_trees << lvl);
         return (uint32_t)std::min<uint64_t>(b, open_capacity);
@@ -1091,34 +1198,38 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
         for (uint32_t j = 0; j < group; ++j, ++lvl) {
             if (lvl + 2 >= level_cap) return -5;
             std::swap(st.open, st.open_next);
-            k_advance<<<1, 64, 0, stream>>>(st, level_start, lvl);
             const uint32_t nb = bound(lvl);
-            k_items<<<1, 1024, 0, stream>>>(st);
-            k_clear_node_data<<<nb, 256, 0, stream>>>(st);
-            const uint32_t max_items = n / CHUNK + nb + 1;
-            k_bounds<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-            k_bin<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-            k_split<<<nb, 64, 0, stream>>>(st);
-            k_count<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-            k_prep<<<nb, 64, 0, stream>>>(st);
-            k_part_a<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-            k_part_b<<<max_items, TOP_THREADS, 0, stream>>>(a, st);
-            k_children<<<(nb + 127) / 128, 128, 0, stream>>>(a, st);
+            k_items<<<1, 1024, 0, stream>>>(st, level_start, lvl, counters + 5);
+            const uint32_t node_grid = std::min(nb, GRID_CAP_NODES);
+            const uint32_t item_grid = std::min(n / CHUNK + nb + 1, GRID_CAP_ITEMS);
+            k_clear_node_data<<<node_grid, 256, 0, stream>>>(st);
+            k_bounds<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+            k_bin<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+            k_split<<<node_grid, 64, 0, stream>>>(st);
+            k_count<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+            k_part_a<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+            k_part_b<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+            k_children<<<std::min((nb + 127) / 128, GRID_CAP_NODES / 2), 128, 0, stream>>>(a, st);
         }
-        HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 5 * 4, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 6 * 4, hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         if (h_cnt[1] == 0) break;  // no node left open for the next level
-        group = std::min(group * 2, 4u);
+        group = first_group ? 1u : std::min(group * 2, 4u);
+        first_group = false;
     }
+    prev_n = n;
+    prev_trees = n_trees;
+    prev_levels = h_cnt[5];
     std::swap(st.open, st.open_next);
     k_advance<<<1, 64, 0, stream>>>(st, level_start, lvl);  // level_start[lvl] = end of the list
-    const uint32_t n_levels = lvl;
+    const uint32_t n_levels = std::min(lvl, h_cnt[5]);  // levels past h_cnt[5] held no node
     const uint32_t n_jobs = h_cnt[2];
     if (n_jobs) k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL><<<n_jobs, 64, 0, stream>>>(a, st);
     // power of the top-phase nodes, deepest level first
     for (int l = (int)n_levels - 1; l >= 0; --l) {
         const uint32_t nb = bound((uint32_t)l);
-        k_power_pairs<<<(nb + 255) / 256, 256, 0, stream>>>(nodes, st.level_nodes, level_start, (uint32_t)l);
+        k_power_pairs<<<std::min((nb + 255) / 256, GRID_CAP_NODES), 256, 0, stream>>>(nodes, st.level_nodes, level_start,
+                                                                                     (uint32_t)l);
     }
     if (prim_order_out && prim_order_out != perm)
         HIP_CHECK(hipMemcpyAsync(prim_order_out, perm, (size_t)n * 4, hipMemcpyDeviceToDevice, stream));

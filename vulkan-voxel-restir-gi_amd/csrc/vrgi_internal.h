@@ -66,7 +66,6 @@ struct BuildState {
     void *jobs = nullptr;
     void *items = nullptr;
     uint32_t *item_true = nullptr;
-    uint32_t *item_off = nullptr;
     uint32_t *item_beg = nullptr;
     uint32_t *item_end = nullptr;
     uint32_t *bins = nullptr;
@@ -84,8 +83,9 @@ struct BuildState {
 class Builder {
   public:
     ~Builder();
-    // Queues a batched build on `stream`; synchronises the stream after top-phase levels
-    // 1, 3, 7, 11, ... (to see whether any node is still open). prim_order_out (may be null) receives the
+    // Queues a batched build on `stream`; synchronises the stream after each group of
+    // top-phase levels (to see whether any node is still open): the first group has the
+    // previous same-sized build's level count, later groups 1, 2, 4, 4, ... levels. prim_order_out (may be null) receives the
     // final primitive index per leaf position (global over all trees).
     int build(hipStream_t stream, const float *prim9, const float *power, const uint32_t *ids,
               uint32_t n, uint32_t n_trees, const uint32_t *d_tree_beg, const uint32_t *d_tree_n,
@@ -105,6 +105,7 @@ class Builder {
     uint32_t *counters = nullptr;
     uint32_t *level_start = nullptr;  // first level-node entry of each top-phase level
     uint32_t level_cap = 0;
+    uint32_t prev_n = 0, prev_trees = 0, prev_levels = 0;  // last build's size and non-empty top levels
     BuildState st;
 };
 
