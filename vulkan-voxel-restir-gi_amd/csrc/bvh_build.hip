@@ -112,6 +112,15 @@ __device__ __forceinline__ void emit_pair(const BuildArgs &a, uint32_t base, uin
 __global__ void k_init(BuildArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
+    if (a.power[i] != 0.0f) {  // the tree holding i has nodes of non-zero power
+        uint32_t lo = 0, hi = a.n_trees - 1;  // last tree whose first primitive is <= i
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (a.tree_beg[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        if (!a.powered[lo]) a.powered[lo] = 1u;
+    }
     a.perm[i] = i;
     a.pc[0 * a.n + i] = a.prim9[6 * a.n + i];
     a.pc[1 * a.n + i] = a.prim9[7 * a.n + i];
@@ -166,17 +175,31 @@ __global__ void k_items(BuildState s, uint32_t *level_start, uint32_t lvl, uint3
     const OpenNode *open = (const OpenNode *)s.open;
     const uint32_t n_open = s_nopen;
     uint32_t carry = 0;
+    __shared__ uint32_t s_wave[16];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     for (uint32_t base = 0; base < n_open; base += 1024) {
         uint32_t i = base + threadIdx.x;
         uint32_t c = i < n_open ? (open[i].count + CHUNK - 1) / CHUNK : 0;
-        s_sum[threadIdx.x] = c;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            uint32_t v = threadIdx.x >= (uint32_t)off ? s_sum[threadIdx.x - off] : 0;
-            __syncthreads();
-            s_sum[threadIdx.x] += v;
-            __syncthreads();
+        // inclusive scan: within each wave, then over the 16 wave totals
+        uint32_t inc = c;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(inc, off, 64);
+            if (lane >= (uint32_t)off) inc += u;
         }
+        if (lane == 63) s_wave[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            const uint32_t v = lane < 16 ? s_wave[lane] : 0u;
+            uint32_t wi = v;
+            for (int off = 1; off < 16; off <<= 1) {
+                const uint32_t u = __shfl_up(wi, off, 64);
+                if (lane >= (uint32_t)off) wi += u;
+            }
+            if (lane < 16) s_wave[lane] = wi;
+        }
+        __syncthreads();
+        s_sum[threadIdx.x] = (w ? s_wave[w - 1] : 0u) + inc;
+        __syncthreads();
         uint32_t excl = carry + s_sum[threadIdx.x] - c;
         if (i < n_open) {
             s.item_beg[i] = excl;
@@ -621,7 +644,10 @@ __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
 // children of the open nodes: node records, leaves and pairs, and the next level's open
 // nodes and subtree jobs. List order is free (ranks come from counts), so list slots,
 // depth maxima and split sums take one atomic per wave.
-__global__ void k_children(BuildArgs a, BuildState s) {
+// Only nodes of trees with some non-zero leaf power enter the level-node list of the power
+// pass (every power of another tree is (0 + 0) + 0 = +0, the value written here), and the
+// count of levels holding such nodes is raised to include this one.
+__global__ void k_children(BuildArgs a, BuildState s, uint32_t lvl, uint32_t *power_levels) {
   const uint32_t n_open = *s.n_open;
   for (uint32_t o0 = blockIdx.x * blockDim.x; o0 < n_open; o0 += gridDim.x * blockDim.x) {  // uniform
     const uint32_t o = o0 + threadIdx.x;
@@ -661,8 +687,10 @@ __global__ void k_children(BuildArgs a, BuildState s) {
             }
         }
     }
-    const uint32_t ln = wave_append(s.n_level_nodes, live);
-    if (live) {
+    const bool pw = live && a.powered[on.tree] != 0u;
+    const uint32_t ln = wave_append(s.n_level_nodes, pw);
+    if (__ballot(pw) && (threadIdx.x & 63u) == 0u) atomicMax(power_levels, lvl + 1u);
+    if (pw) {
         s.level_nodes[2 * ln] = base + on.node;
         s.level_nodes[2 * ln + 1] = base;
     }
@@ -1107,6 +1135,7 @@ void Builder::reserve(uint32_t n, uint32_t n_trees) {
     A((void **)&st.split, (size_t)open_cap * sizeof(SplitInfo));
     A((void **)&st.level_nodes, (size_t)cap_n * 8 + 64);
     A((void **)&counters, 64 * 4);
+    A((void **)&powered, (size_t)cap_t * 4);
     level_cap = 65536;
     A((void **)&level_start, (size_t)level_cap * 4);
     st.n_open = counters + 0;
@@ -1120,10 +1149,10 @@ void Builder::reserve(uint32_t n, uint32_t n_trees) {
 void Builder::free_all() {
     void *ptrs[] = {perm, pc, box8, st.rank_tmp, st.pair, st.open, st.open_next, st.jobs, st.items,
                     st.item_true, st.item_beg, st.item_end, st.bins, st.bounds,
-                    st.split, st.level_nodes, counters, level_start};
+                    st.split, st.level_nodes, counters, level_start, powered};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    perm = nullptr; pc = nullptr; box8 = nullptr; counters = nullptr; level_start = nullptr;
+    perm = nullptr; pc = nullptr; box8 = nullptr; counters = nullptr; level_start = nullptr; powered = nullptr;
     st = BuildState{};
     cap_n = cap_t = 0;
 }
@@ -1174,9 +1203,11 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     a.pc = pc;
     a.box8 = box8;
     a.sub_max = SUB_MAX_DEFAULT;
+    a.powered = powered;
     a.split_count_sum = (unsigned long long *)(counters + 8);
     (void)h_tree_base;
     HIP_CHECK(hipMemsetAsync(counters, 0, 64 * 4, stream));
+    HIP_CHECK(hipMemsetAsync(powered, 0, (size_t)n_trees * 4, stream));
     k_init<<<(n + 255) / 256, 256, 0, stream>>>(a);
     k_roots<<<(n_trees + 255) / 256, 256, 0, stream>>>(a, st);
     // Top phase, level-synchronous. Level l holds at most n_trees * 2^l open nodes, so the
@@ -1209,9 +1240,9 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
             k_count<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
             k_part_a<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
             k_part_b<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
-            k_children<<<std::min((nb + 127) / 128, GRID_CAP_NODES / 2), 128, 0, stream>>>(a, st);
+            k_children<<<std::min((nb + 127) / 128, GRID_CAP_NODES / 2), 128, 0, stream>>>(a, st, lvl, counters + 6);
         }
-        HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 6 * 4, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 7 * 4, hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         if (h_cnt[1] == 0) break;  // no node left open for the next level
         group = first_group ? 1u : std::min(group * 2, 4u);
@@ -1222,7 +1253,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     prev_levels = h_cnt[5];
     std::swap(st.open, st.open_next);
     k_advance<<<1, 64, 0, stream>>>(st, level_start, lvl);  // level_start[lvl] = end of the list
-    const uint32_t n_levels = std::min(lvl, h_cnt[5]);  // levels past h_cnt[5] held no node
+    const uint32_t n_levels = std::min(lvl, h_cnt[6]);  // levels past h_cnt[6] held no powered node
     const uint32_t n_jobs = h_cnt[2];
     if (n_jobs) k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL><<<n_jobs, 64, 0, stream>>>(a, st);
     // power of the top-phase nodes, deepest level first

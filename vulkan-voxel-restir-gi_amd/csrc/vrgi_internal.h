@@ -56,6 +56,7 @@ struct BuildArgs {
     // box8[2p] = (min x, min y, min z, max x), box8[2p+1] = (max y, max z, power, id bits)
     float4 *box8;
     int sub_max;
+    uint32_t *powered;  // per tree: 1 when some primitive has non-zero power
 };
 
 struct BuildState {
@@ -104,6 +105,7 @@ class Builder {
     float4 *box8 = nullptr;
     uint32_t *counters = nullptr;
     uint32_t *level_start = nullptr;  // first level-node entry of each top-phase level
+    uint32_t *powered = nullptr;      // per tree: some primitive has non-zero power
     uint32_t level_cap = 0;
     uint32_t prev_n = 0, prev_trees = 0, prev_levels = 0;  // last build's size and non-empty top levels
     BuildState st;
