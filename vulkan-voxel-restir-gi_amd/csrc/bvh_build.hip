@@ -833,26 +833,36 @@ __device__ void axis_best_plane(const float (*box)[SUB], float *tmp, uint32_t f,
     *split_out = lo + ((float)dplane + 1.0f) * ((hi - lo) / (float)BINS);
 }
 
-template <int SUB, int SMALL>
+// POWER: the variant for the jobs of trees with non-zero leaf power (powers in LDS, phase
+// C). The other variant keeps no power, id or rank planes and shares the bin area with
+// the partition and plane-search scratch: less LDS per wave, more waves per CU. Both are
+// launched over the job list; each takes the jobs of its kind.
+template <int SUB, int SMALL, bool POWER>
 __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     constexpr int STK = SUB / (SMALL + 1) + 2;  // pending ranges are disjoint and hold > SMALL
     constexpr int SMQ = SUB / 3 + 2;            // small ranges are disjoint and hold >= 3
+    constexpr int NBOX = POWER ? 11 : 9;
     static_assert(SUB <= 1024 && SMALL <= 127, "small-range queue packs 10 + 7 + 10 bits");
     static_assert(SMALL <= 12, "bins of a small range are packed 5 bits each into 64 bits");
+    static_assert(3 * SUB <= BIN_WORDS, "partition and plane-search scratch share the bin area");
     __shared__ uint32_t s_perm[SUB];
-    __shared__ float s_box[11][SUB];  // min xyz, max xyz, centroid xyz, power, id bits
-    __shared__ uint32_t s_pair[SUB];
-    __shared__ uint32_t s_cj[SUB];
+    __shared__ float s_box[NBOX][SUB];  // min xyz, max xyz, centroid xyz (, power, id bits)
+    // bins of phase A; after each sweep the partition's ranks and pairs; in phase B the
+    // three axes' plane-search scratch
     __shared__ int s_bins[BIN_WORDS];
-    __shared__ RankInfo s_rank[SUB];
-    __shared__ float s_ipow[SUB];
+    __shared__ RankInfo s_rank[POWER ? SUB : 1];
+    __shared__ float s_ipow[POWER ? SUB : 1];
     __shared__ uint32_t s_stack[STK][6];  // first, count, rank, node, depth, parent
     __shared__ uint4 s_q[2][SMQ];         // small ranges: first|count<<10|(rank-job rank)<<17, node, depth, parent
     __shared__ uint32_t s_nsmall;
+    uint32_t *const s_pair = (uint32_t *)s_bins;
+    uint32_t *const s_cj = s_pair + SUB;
+    float *const s_tmp2 = (float *)(s_cj + SUB);
 
     const uint32_t job = blockIdx.x;
     if (job >= *s.n_jobs) return;
     const OpenNode jn = ((const OpenNode *)s.jobs)[job];
+    if ((a.powered[jn.tree] != 0u) != POWER) return;  // the other variant's job
     const int lane = threadIdx.x;
     const uint32_t c = jn.count;
     const uint32_t base = a.tree_base[jn.tree];
@@ -863,7 +873,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         s_box[0][k] = b0.x; s_box[1][k] = b0.y; s_box[2][k] = b0.z;
         s_box[3][k] = b0.w; s_box[4][k] = b1.x; s_box[5][k] = b1.y;
         for (int q = 0; q < 3; ++q) s_box[6 + q][k] = a.pc[q * a.n + jn.first + k];
-        s_box[9][k] = b1.z; s_box[10][k] = b1.w;
+        if constexpr (POWER) { s_box[NBOX - 2][k] = b1.z; s_box[NBOX - 1][k] = b1.w; }
     }
     uint32_t sp = 0, maxdepth = jn.depth;
     uint32_t splitsA = 0, splitsB = 0;  // split sizes (sum = sum of leaf depths): wave-wide, per lane
@@ -880,10 +890,17 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     const float pad = 0.0001f;
     // leaves and 2-primitive nodes (build.rs:311-326, 203-230) from the LDS copy of the
     // primitives at job positions e, e + 1
+    // power and id bits of the primitive at job position e (from LDS, or its record)
+    auto pw_id = [&](uint32_t e) -> float2 {
+        if constexpr (POWER) return make_float2(s_box[NBOX - 2][e], s_box[NBOX - 1][e]);
+        const float4 b1 = a.box8[2 * s_perm[e] + 1];
+        return make_float2(b1.z, b1.w);
+    };
     auto lds_leaf = [&](uint32_t gnode, uint32_t parent, uint32_t e) {
         float mn[3] = {s_box[0][e], s_box[1][e], s_box[2][e]};
         float mx[3] = {s_box[3][e], s_box[4][e], s_box[5][e]};
-        write_node(a.nodes + gnode, VRGI_INVALID, __float_as_uint(s_box[10][e]), mn, s_box[9][e], mx, parent);
+        const float2 pi = pw_id(e);
+        write_node(a.nodes + gnode, VRGI_INVALID, __float_as_uint(pi.y), mn, pi.x, mx, parent);
     };
     auto lds_pair = [&](uint32_t idx, uint32_t r, uint32_t parent, uint32_t e) {
         float mn[3], mx[3];
@@ -891,7 +908,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             mn[k] = gmin(s_box[k][e], s_box[k][e + 1]) - pad;
             mx[k] = gmax(s_box[3 + k][e], s_box[3 + k][e + 1]) + pad;
         }
-        float pw = (0.0f + s_box[9][e]) + s_box[9][e + 1];
+        float pw = (0.0f + pw_id(e).x) + pw_id(e + 1).x;
         write_node(a.nodes + base + idx, 2 * r + 1, 2 * r + 2, mn, pw, mx, parent);
         lds_leaf(base + 2 * r + 1, idx, e);
         lds_leaf(base + 2 * r + 2, idx, e + 1);
@@ -937,6 +954,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
         float split;
         int box[6];
         sah_sweep_wave(s_bins, bounds, &dim, &split, box);
+        __syncthreads();  // the bin area becomes the partition scratch
         const uint32_t iters = (nc + 63) / 64;
         uint32_t T = 0;
         for (uint32_t itn = 0; itn < iters; ++itn) {
@@ -964,7 +982,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                 if (cj & 0x80000000u) continue;
                 uint32_t e0 = nf + k, e1 = nf + s_pair[k - cj];
                 uint32_t tp = s_perm[e0]; s_perm[e0] = s_perm[e1]; s_perm[e1] = tp;
-                for (int q = 0; q < 11; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
+                for (int q = 0; q < NBOX; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
             }
             __syncthreads();
         }
@@ -974,7 +992,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             float mn[3], mx[3];
             for (int k = 0; k < 3; ++k) { mn[k] = funkey(box[k]) - pad; mx[k] = funkey(box[3 + k]) + pad; }
             write_node(a.nodes + base + gidx, li, ri, mn, 0.0f, mx, parent);
-            s_rank[rank - jn.rank] = RankInfo{(uint16_t)nf, (uint16_t)nl, (uint16_t)nc, 0, gidx};
+            if constexpr (POWER) s_rank[rank - jn.rank] = RankInfo{(uint16_t)nf, (uint16_t)nl, (uint16_t)nc, 0, gidx};
         }
         splitsA += nc;
         maxdepth = max(maxdepth, dep + 1);
@@ -988,7 +1006,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             } else if (cc == 2) {
                 if (lane == 0) {
                     lds_pair(ci, cr, gidx, cf);
-                    s_rank[cr - jn.rank] = RankInfo{(uint16_t)cf, 1, 2, 0, ci};
+                    if constexpr (POWER) s_rank[cr - jn.rank] = RankInfo{(uint16_t)cf, 1, 2, 0, ci};
                 }
                 splitsA += 2;
                 maxdepth = max(maxdepth, dep + 2);
@@ -1020,7 +1038,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
             float acost = 0.0f, asplit = 0.0f, nb[6];
             if (act) {
                 e = s_q[cur][w];
-                float *tmp = ax == 0 ? (float *)s_pair : ax == 1 ? (float *)s_cj : s_ipow;
+                float *tmp = ax == 0 ? (float *)s_pair : ax == 1 ? (float *)s_cj : s_tmp2;
                 axis_best_plane<SUB>(s_box, tmp, e.x & 0x3ffu, (e.x >> 10) & 0x7fu, ax, &acost, &asplit, nb);
             }
             const float c1 = __shfl(acost, lane + 1, 64), c2 = __shfl(acost, lane + 2, 64);
@@ -1042,14 +1060,14 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                 const uint32_t e0 = nf + l, e1 = nf + r;
                 const uint32_t tp = s_perm[e0]; s_perm[e0] = s_perm[e1]; s_perm[e1] = tp;
 #pragma unroll
-                for (int q = 0; q < 11; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
+                for (int q = 0; q < NBOX; ++q) { float f = s_box[q][e0]; s_box[q][e0] = s_box[q][e1]; s_box[q][e1] = f; }
             }
             const uint32_t nl = (T == 0 || T == nc) ? nc / 2 : T, nr = nc - nl;
             const uint32_t li = 2 * rank + 1, ri = 2 * rank + 2;
             float mn[3], mx[3];
             for (int k = 0; k < 3; ++k) { mn[k] = nb[k] - pad; mx[k] = nb[3 + k] + pad; }
             write_node(a.nodes + base + gidx, li, ri, mn, 0.0f, mx, parent);
-            s_rank[rank - jn.rank] = RankInfo{(uint16_t)nf, (uint16_t)nl, (uint16_t)nc, 0, gidx};
+            if constexpr (POWER) s_rank[rank - jn.rank] = RankInfo{(uint16_t)nf, (uint16_t)nl, (uint16_t)nc, 0, gidx};
             splitsB += nc;
             maxdepth = max(maxdepth, dep + 1);
             for (int side = 1; side >= 0; --side) {
@@ -1061,7 +1079,7 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
                     lds_leaf(base + ci, gidx, cf);
                 } else if (cc == 2) {
                     lds_pair(ci, cr, gidx, cf);
-                    s_rank[cr - jn.rank] = RankInfo{(uint16_t)cf, 1, 2, 0, ci};
+                    if constexpr (POWER) s_rank[cr - jn.rank] = RankInfo{(uint16_t)cf, 1, 2, 0, ci};
                     splitsB += 2;
                     maxdepth = max(maxdepth, dep + 2);
                 } else {
@@ -1080,15 +1098,16 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
     // leaf power zero (the geometry trees) each sum is +0, the value the nodes were
     // written with, and the pass is skipped.
     bool any_power = false;
-    for (uint32_t k = lane; k < c; k += 64) any_power |= s_box[9][k] != 0.0f;
+    if constexpr (POWER)
+        for (uint32_t k = lane; k < c; k += 64) any_power |= s_box[NBOX - 2][k] != 0.0f;
     const bool powered = __any(any_power);
-    if (lane == 0 && powered) {
+    if constexpr (POWER) if (lane == 0 && powered) {
         float prev = 0.0f;
         for (int k = (int)c - 2; k >= 0; --k) {
             const RankInfo ri = s_rank[k];
             const uint32_t nr = (uint32_t)ri.count - ri.nl;
-            const float pl = ri.nl == 1 ? s_box[9][ri.first] : prev;
-            const float pr = nr == 1 ? s_box[9][ri.first + ri.nl] : s_ipow[k + ri.nl];
+            const float pl = ri.nl == 1 ? s_box[NBOX - 2][ri.first] : prev;
+            const float pr = nr == 1 ? s_box[NBOX - 2][ri.first + ri.nl] : s_ipow[k + ri.nl];
             float p = 0.0f;
             p += pl;
             p += pr;
@@ -1255,7 +1274,10 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     k_advance<<<1, 64, 0, stream>>>(st, level_start, lvl);  // level_start[lvl] = end of the list
     const uint32_t n_levels = std::min(lvl, h_cnt[6]);  // levels past h_cnt[6] held no powered node
     const uint32_t n_jobs = h_cnt[2];
-    if (n_jobs) k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL><<<n_jobs, 64, 0, stream>>>(a, st);
+    if (n_jobs) {
+        k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL, false><<<n_jobs, 64, 0, stream>>>(a, st);
+        k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL, true><<<n_jobs, 64, 0, stream>>>(a, st);
+    }
     // power of the top-phase nodes, deepest level first
     for (int l = (int)n_levels - 1; l >= 0; --l) {
         const uint32_t nb = bound((uint32_t)l);
