@@ -400,6 +400,8 @@ void rebuild_instances(vrgi_ctx *c) {
         std::memcpy(d.xf, kv.second.second.data(), 48);
         d.tri_off = tri_off;
         d.tl_idx = 0;  // prim2node.get(i).unwrap_or(0)
+        d.lin_id = (d.xf[0] == 1.0f && d.xf[1] == 0.0f && d.xf[2] == 0.0f && d.xf[3] == 0.0f && d.xf[4] == 1.0f &&
+                    d.xf[5] == 0.0f && d.xf[6] == 0.0f && d.xf[7] == 0.0f && d.xf[8] == 1.0f) ? 1u : 0u;
         d.light_bvh = o->d_light_bvh;
         d.lpairs = o->d_lpairs;
         hi.push_back(d);

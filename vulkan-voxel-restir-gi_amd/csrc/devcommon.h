@@ -160,6 +160,7 @@ struct DevInst {
     float xf[12];                     // transform (column-major mat4x3)
     uint32_t tri_off;                 // first global triangle id of the instance
     uint32_t tl_idx;                  // light_bvh_tl_idx
+    uint32_t lin_id;                  // 1 when the transform's linear part is the identity
     const vrgi_bvh_node *light_bvh;   // bvh_node_buffer_addr: the object's light BL-BVH
     const LPair *lpairs;              // child records of that BL-BVH
 };

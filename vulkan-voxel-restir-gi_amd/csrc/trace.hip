@@ -174,6 +174,27 @@ __device__ __forceinline__ float box_importance(f3 p, f3 n, const float *m, f3 m
     float dsq = gmax(dot3(diag, diag), dot3(mid, mid));
     return power / dsq * ((float)vis / 8.0f);
 }
+// The same importance for a transform whose linear part is the identity: xform_pt then
+// gives ((x + 0) + 0) + t_x = x + t_x exactly, up to the sign of a zero result, which no
+// corner test, difference or square below can see; the instance-to-world offset t is
+// added once per corner coordinate instead of the 3x3 product.
+__device__ __forceinline__ float box_importance_t(f3 p, f3 n, f3 t, f3 mn, f3 mx, float power, float eps) {
+    f3 a = mn + t;
+    f3 b = mx + t;
+    uint32_t vis = 0;
+    vis += dot3(a - p, n) >= eps;
+    vis += dot3(mk(a.x, a.y, b.z) - p, n) >= eps;
+    vis += dot3(mk(a.x, b.y, a.z) - p, n) >= eps;
+    vis += dot3(mk(a.x, b.y, b.z) - p, n) >= eps;
+    vis += dot3(mk(b.x, a.y, a.z) - p, n) >= eps;
+    vis += dot3(mk(b.x, a.y, b.z) - p, n) >= eps;
+    vis += dot3(mk(b.x, b.y, a.z) - p, n) >= eps;
+    vis += dot3(b - p, n) >= eps;
+    f3 diag = b - a;
+    f3 mid = 0.5f * (a + b) - p;
+    float dsq = gmax(dot3(diag, diag), dot3(mid, mid));
+    return power / dsq * ((float)vis / 8.0f);
+}
 __device__ __forceinline__ float node_importance(f3 p, f3 n, const float *m,
                                                  const vrgi_bvh_node &nd, float eps) {
     return box_importance(p, n, m, mk(nd.min[0], nd.min[1], nd.min[2]), mk(nd.max[0], nd.max[1], nd.max[2]),
@@ -186,6 +207,15 @@ __device__ __forceinline__ PairImp pair_importance(f3 p, f3 n, const float *m, c
     PairImp q;
     q.li = box_importance(p, n, m, mk(r.a.x, r.a.y, r.a.z), mk(r.b.x, r.b.y, r.b.z), r.a.w, eps);
     q.ri = box_importance(p, n, m, mk(r.c.x, r.c.y, r.c.z), mk(r.d.x, r.d.y, r.d.z), r.b.w, eps);
+    float tot = q.li + q.ri;
+    q.ln = q.li / tot;
+    q.rn = q.ri / tot;
+    return q;
+}
+__device__ __forceinline__ PairImp pair_importance_t(f3 p, f3 n, f3 t, const LPair &r, float eps) {
+    PairImp q;
+    q.li = box_importance_t(p, n, t, mk(r.a.x, r.a.y, r.a.z), mk(r.b.x, r.b.y, r.b.z), r.a.w, eps);
+    q.ri = box_importance_t(p, n, t, mk(r.c.x, r.c.y, r.c.z), mk(r.d.x, r.d.y, r.d.z), r.b.w, eps);
     float tot = q.li + q.ri;
     q.ln = q.li / tot;
     q.rn = q.ri / tot;
@@ -210,6 +240,8 @@ __device__ Pick traverse_light(const SceneView &sv, f3 p, f3 n, uint32_t seed) {
     if (tl0.left == VRGI_INVALID && tl0.right_or_prim == VRGI_INVALID) return res;  // dummy root
     float prob = 1.0f, imp = 0.0f;
     const float *m = IDENTITY12;
+    bool lin = true;               // linear part of m is the identity: offset `off` instead of m
+    f3 off = mk(0.0f, 0.0f, 0.0f);
     const LPair *pairs = sv.tlpairs;
     uint32_t inst = VRGI_INVALID;
     bool top = true;
@@ -220,6 +252,8 @@ __device__ Pick traverse_light(const SceneView &sv, f3 p, f3 n, uint32_t seed) {
             inst = idx;
             const DevInst &I = sv.inst[inst];
             m = I.xf;
+            lin = I.lin_id != 0u;
+            off = mk(I.xf[9], I.xf[10], I.xf[11]);
             pairs = I.lpairs;
             top = false;
             const vrgi_bvh_node bl0 = I.light_bvh[0];
@@ -232,7 +266,7 @@ __device__ Pick traverse_light(const SceneView &sv, f3 p, f3 n, uint32_t seed) {
             return res;
         }
         const LPair r = ld_pair(pairs + idx);
-        const PairImp q = pair_importance(p, n, m, r, VRGI_EPS_RT);
+        const PairImp q = lin ? pair_importance_t(p, n, off, r, VRGI_EPS_RT) : pair_importance(p, n, m, r, VRGI_EPS_RT);
         uint32_t ref;
         if (mm_finalizef(seed) < q.ln) { ref = __float_as_uint(r.c.w); prob *= q.ln; imp = q.li; }
         else { ref = __float_as_uint(r.d.w); prob *= q.rn; imp = q.ri; }
@@ -285,10 +319,10 @@ __device__ float reverse_traverse_anc(const SceneView &sv, f3 p, f3 n, uint32_t 
     const uint32_t in = sv.ltri_inst[lid];
     if (len == VRGI_INVALID) return reverse_traverse(sv, p, n, in, sv.ltri_node[lid]);
     const DevInst &I = sv.inst[in];
-    float xf[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) xf[k] = I.xf[k];
-    const float idm[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    // BL levels: the instance's offset when its linear part is the identity (the TL levels'
+    // transform is the identity: offset 0), else the full transform
+    const bool lin = I.lin_id != 0u;
+    const f3 off = mk(I.xf[9], I.xf[10], I.xf[11]), zero = mk(0.0f, 0.0f, 0.0f);
     const LPair *blp = I.lpairs;
     const uint32_t *anc = sv.lanc + (size_t)lid * VRGI_LANC_MAX;
     float prob = 1.0f;
@@ -298,7 +332,8 @@ __device__ float reverse_traverse_anc(const SceneView &sv, f3 p, f3 n, uint32_t 
         const bool tl = (e & 0x40000000u) != 0u;
         const uint32_t ni = e & 0x3FFFFFFFu;
         const LPair r = ld_pair((tl ? sv.tlpairs : blp) + ni);
-        const PairImp q = tl ? pair_importance(p, n, idm, r, VRGI_EPS_NEE) : pair_importance(p, n, xf, r, VRGI_EPS_NEE);
+        const PairImp q = (tl || lin) ? pair_importance_t(p, n, tl ? zero : off, r, VRGI_EPS_NEE)
+                                      : pair_importance(p, n, I.xf, r, VRGI_EPS_NEE);
         prob *= (e & 0x80000000u) ? q.rn : q.ln;
     }
     return prob;
