@@ -45,10 +45,11 @@ __device__ __forceinline__ void update_res(uint32_t seed, Reservoir &r, const Sa
     if (float_construct(seed) < w / r.wsum) r.z = z;
 }
 
-// outgoing_radiance (backward recurrence) + initial sample + temporal update
+// outgoing_radiance (backward recurrence) + initial sample + temporal update. Called by
+// every thread of the block (a block-wide record allocation when the cache is on).
 __device__ __forceinline__ void accum_temporal_px(const FrameArgs &fa, uint32_t id) {
-    if (id >= fa.P) return;
-    const uint32_t plen = fa.path_len[id];
+    const bool live = id < fa.P;
+    const uint32_t plen = live ? fa.path_len[id] : 0u;
     const float bsp = 1.0f / (2.0f * VRGI_PI_F);
     f3 L = mk(0, 0, 0), L1 = mk(0, 0, 0);
     float q0 = bsp;
@@ -57,51 +58,70 @@ __device__ __forceinline__ void accum_temporal_px(const FrameArgs &fa, uint32_t 
     // radiance is known (not the last vertex of a path cut by the bounce cap)
     bool train = false, queried = false;
     const uint32_t cv = fa.cache_vertex;
-    if (fa.cache_on) {
+    if (fa.cache_on && live) {
         train = mm_finalizef(mm_combine(fa.train_seed, id)) < fa.train_frac ||
                 in_fallback(fa.fallback, id, fa.W, fa.tile, fa.tiles_x);
         queried = !train && plen == cv + 1 && fa.hit[(size_t)cv * fa.P + id] != VRGI_INVALID;
     }
-    for (int b = (int)plen - 1; b >= 0; --b) {
-        const size_t bi = (size_t)b * fa.P + id;
-        float bp = fa.pdf[bi], w = fa.mis[bi];
-        // written only for continued rays, and only when NEE is on (nee == nullptr otherwise:
-        // mis_weight is then 0 everywhere and the reference's nee_pdf is 0)
-        float np = (fa.nee != nullptr && b + 1 < (int)plen) ? fa.nee[bi] : 0.0f;
-        float q = np * w + (1.0f - w) * bsp;
-        float rf = bp / q;
-        f3 term = ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;
-        if (queried && b == (int)cv) term = ld3(fa.crad + (size_t)id * 3);
-        L = ld3(fa.emis + bi * 3) + term;
-        const bool rec = train && fa.hit[bi] != VRGI_INVALID &&
-                         (b + 1 < (int)plen || zero3(ld3(fa.dir + bi * 3))) &&
-                         isfinite(term.x) && isfinite(term.y) && isfinite(term.z);
-        // one counter update per wave and iteration (lanes still in the loop)
-        const uint64_t rb = __ballot(rec);
-        uint32_t slot = 0;
-        if (rb) {
-            const uint32_t lane = threadIdx.x & 63u;
-            const int lead = __ffsll((unsigned long long)rb) - 1;
-            uint32_t base = 0;
-            if ((int)lane == lead) base = atomicAdd(fa.trec_count, (uint32_t)__popcll(rb));
-            base = __shfl(base, lead, 64);
-            slot = base + (uint32_t)__popcll(rb & ((1ull << lane) - 1ull));
-        }
-        if (rec) {
-            if (slot < fa.tcap) {
+    // the recurrence from the path's last vertex to its first; with `write`, the records
+    // go to slots slot, slot + 1, ... in that order (counted, not written, otherwise)
+    auto walk = [&](bool write, uint32_t slot) -> uint32_t {
+        uint32_t n_rec = 0;
+        L = mk(0, 0, 0);
+        for (int b = (int)plen - 1; b >= 0; --b) {
+            const size_t bi = (size_t)b * fa.P + id;
+            float bp = fa.pdf[bi], w = fa.mis[bi];
+            // written only for continued rays, and only when NEE is on (nee == nullptr
+            // otherwise: mis_weight is then 0 everywhere and the reference's nee_pdf is 0)
+            float np = (fa.nee != nullptr && b + 1 < (int)plen) ? fa.nee[bi] : 0.0f;
+            float q = np * w + (1.0f - w) * bsp;
+            float rf = bp / q;
+            f3 term = ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;
+            if (queried && b == (int)cv) term = ld3(fa.crad + (size_t)id * 3);
+            L = ld3(fa.emis + bi * 3) + term;
+            const bool rec = train && fa.hit[bi] != VRGI_INVALID &&
+                             (b + 1 < (int)plen || zero3(ld3(fa.dir + bi * 3))) &&
+                             isfinite(term.x) && isfinite(term.y) && isfinite(term.z);
+            if (rec && write && slot + n_rec < fa.tcap) {
                 const f3 p = ld3(fa.origin + bi * 3);
                 const f3 dd = b == 0 ? ld3(fa.d0 + (size_t)id * 3) : ld3(fa.dir + (bi - fa.P) * 3);
                 const f3 nn = ld3(fa.normal + bi * 3), aa = ld3(fa.albedo + bi * 3);
-                float4 *r = (float4 *)(fa.trec + (size_t)slot * 16);
+                float4 *r = (float4 *)(fa.trec + (size_t)(slot + n_rec) * 16);
                 r[0] = make_float4(p.x, p.y, p.z, dd.x);
                 r[1] = make_float4(dd.y, dd.z, nn.x, nn.y);
                 r[2] = make_float4(nn.z, aa.x, aa.y, aa.z);
                 r[3] = make_float4(term.x, term.y, term.z, __uint_as_float(id * 16u + (uint32_t)b));
             }
+            n_rec += rec ? 1u : 0u;
+            if (b == 1) L1 = L;
+            if (b == 0) q0 = q;
         }
-        if (b == 1) L1 = L;
-        if (b == 0) q0 = q;
+        return n_rec;
+    };
+    const uint32_t n_rec = walk(false, 0u);
+    if (fa.cache_on) {  // uniform over the block
+        // the block's records take one range of the record buffer (one counter update
+        // per block); each thread's records are consecutive, in decreasing vertex order
+        __shared__ uint32_t s_wave[16];
+        __shared__ uint32_t s_base;
+        const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+        uint32_t inc = n_rec;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(inc, off, 64);
+            if (lane >= (uint32_t)off) inc += u;
+        }
+        if (lane == 63u) s_wave[wv] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (uint32_t k = 0; k < nw; ++k) { const uint32_t v = s_wave[k]; s_wave[k] = t; t += v; }
+            s_base = t ? atomicAdd(fa.trec_count, t) : 0u;
+        }
+        __syncthreads();
+        if (n_rec) walk(true, s_base + s_wave[wv] + inc - n_rec);
+        __syncthreads();  // s_wave / s_base may be reused by a later call in this block
     }
+    if (!live) return;
     st3(fa.radiance + (size_t)id * 3, L);
     // initial sample (interactive_rendering.rs:1359-1414)
     Sample S;
