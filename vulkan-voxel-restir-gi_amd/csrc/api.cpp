@@ -87,8 +87,11 @@ struct vrgi_ctx {
     // learned-cache work of a frame that overlaps the training paths' bounces (default
     // priority, like every stream here); joined into `stream` before the frame ends
     hipStream_t side = nullptr;
-    hipStream_t nee_s = nullptr;  // NEE pdf of bounce b beside the trace of bounce b + 1
-    hipEvent_t ev_nee_done = nullptr;
+    // NEE pdf of bounce b beside the trace of bounce b + 1, on stream b % VRGI_NEE_STREAMS:
+    // the pdfs of different bounces are independent, so a slow one does not hold back the
+    // next (the long latency-bound walks of short training-path queues)
+    hipStream_t nee_s[VRGI_NEE_STREAMS] = {};
+    hipEvent_t ev_nee_done[VRGI_NEE_STREAMS] = {};
     std::vector<hipEvent_t> ev_bounce;  // trace (and sort) of bounce b done
     hipEvent_t ev_query = nullptr, ev_infer = nullptr, ev_train = nullptr, ev_accum = nullptr;
     std::mutex up_mu;
@@ -833,9 +836,14 @@ vrgi_ctx::~vrgi_ctx() {
     if (up_stream) (void)hipStreamDestroy(up_stream);
     if (side) (void)hipStreamSynchronize(side);
     if (side) (void)hipStreamDestroy(side);
-    if (nee_s) (void)hipStreamSynchronize(nee_s);
-    if (nee_s) (void)hipStreamDestroy(nee_s);
-    for (hipEvent_t e : {ev_query, ev_infer, ev_train, ev_accum, ev_nee_done})
+    for (hipStream_t ns : nee_s)
+        if (ns) {
+            (void)hipStreamSynchronize(ns);
+            (void)hipStreamDestroy(ns);
+        }
+    for (hipEvent_t e : ev_nee_done)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {ev_query, ev_infer, ev_train, ev_accum})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_bounce) (void)hipEventDestroy(e);
 }
@@ -881,8 +889,10 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-        HIP_CHECK(hipStreamCreateWithFlags(&c->nee_s, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreateWithFlags(&c->ev_nee_done, hipEventDisableTiming));
+        for (int k = 0; k < VRGI_NEE_STREAMS; ++k) {
+            HIP_CHECK(hipStreamCreateWithFlags(&c->nee_s[k], hipStreamNonBlocking));
+            HIP_CHECK(hipEventCreateWithFlags(&c->ev_nee_done[k], hipEventDisableTiming));
+        }
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_geo_ready, hipEventDisableTiming));
         for (int k = 0; k < 2; ++k) HIP_CHECK(hipEventCreateWithFlags(&c->geo[k].ev_free, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_query, hipEventDisableTiming));
@@ -1244,7 +1254,9 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             ba.touch_nodes = c->touch_bits.p;
             ba.touch_tris = c->touch_bits.p + touch_words;
         }
-        launch_trace_shade(s, sv, ba, P);
+        // past the cache's query vertex only training paths (and fallback tiles) go on
+        const uint32_t small_q = cache_on && b > c->cache.cfg.query_vertex ? VRGI_SMALL_QUEUE_GRID : 0u;
+        launch_trace_shade(s, sv, ba, P, small_q);
         qin[b + 1] = ba.queue_out;
         if (cache_on && b == c->cache.cfg.query_vertex) {
             // side stream: cache radiance of this bounce's queries, then the training
@@ -1279,27 +1291,30 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         }
         rec(2 + b);
         if (nee_overlap && b + 1 < B) {  // interactive_rendering.rs:1206
+            hipStream_t ns = c->nee_s[b % VRGI_NEE_STREAMS];
             HIP_CHECK(hipEventRecord(c->ev_bounce[b], s));
-            HIP_CHECK(hipStreamWaitEvent(c->nee_s, c->ev_bounce[b], 0));
+            HIP_CHECK(hipStreamWaitEvent(ns, c->ev_bounce[b], 0));
             BounceArgs na = ba;
             na.queue_in = qin[b + 1];
             na.touch_nodes = na.touch_tris = nullptr;
-            rec_on(3 + 2 * (size_t)B + b, c->nee_s);
-            launch_nee_pdf(c->nee_s, sv, na, P);
-            rec_on(2 + B + b, c->nee_s);
+            rec_on(3 + 2 * (size_t)B + b, ns);
+            launch_nee_pdf(ns, sv, na, P, cache_on && b >= c->cache.cfg.query_vertex ? VRGI_SMALL_QUEUE_GRID : 0u);
+            rec_on(2 + B + b, ns);
         }
     }
     ba.touch_nodes = ba.touch_tris = nullptr;
     if (nee_overlap) {
-        HIP_CHECK(hipEventRecord(c->ev_nee_done, c->nee_s));
-        HIP_CHECK(hipStreamWaitEvent(s, c->ev_nee_done, 0));
+        for (uint32_t k = 0; k < VRGI_NEE_STREAMS && k + 1 < B; ++k) {
+            HIP_CHECK(hipEventRecord(c->ev_nee_done[k], c->nee_s[k]));
+            HIP_CHECK(hipStreamWaitEvent(s, c->ev_nee_done[k], 0));
+        }
     } else {
         for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
             rec(3 + 2 * (size_t)B + b);
             if (prefs->nee_type != 0) {
                 ba.bounce = b;
                 ba.queue_in = qin[b + 1];
-                launch_nee_pdf(s, sv, ba, P);
+                launch_nee_pdf(s, sv, ba, P, cache_on && b >= c->cache.cfg.query_vertex ? VRGI_SMALL_QUEUE_GRID : 0u);
             }
             rec(2 + B + b);
         }

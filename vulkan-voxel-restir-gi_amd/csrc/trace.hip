@@ -529,6 +529,17 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
     trace_shade_slice<STACK, COUNT, CACHE>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, count);
 }
 
+// The same with a capped grid whose blocks stride over the queue: for queues known to be
+// small (the training paths past the cache's query vertex), where a grid sized for every
+// pixel would be mostly blocks that exit at once
+template <int STACK, bool CACHE>
+__global__ __launch_bounds__(64) void k_trace_shade_gs(const SceneView sv, const BounceArgs ba) {
+    __shared__ uint32_t stack[STACK * 64];
+    const uint32_t count = ba.bounce == 0 ? ba.P : ba.qcount[ba.bounce];
+    for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u)
+        trace_shade_slice<STACK, false, CACHE>(sv, ba, stack, base + threadIdx.x, count);
+}
+
 // NEE pdf of bounce b (nee_pdf.rs:281-337) for the rays alive at bounce b+1
 constexpr int NEE_DEFER = 4;  // crossed light triangles kept per lane before their pdfs are summed
 template <int STACK>
@@ -614,6 +625,13 @@ template <int STACK>
 __global__ __launch_bounds__(64) void k_nee_pdf(const SceneView sv, const BounceArgs ba) {
     __shared__ uint32_t stack[STACK * 64];
     nee_pdf_ray<STACK>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, ba.qcount[ba.bounce + 1]);
+}
+template <int STACK>
+__global__ __launch_bounds__(64) void k_nee_pdf_gs(const SceneView sv, const BounceArgs ba) {
+    __shared__ uint32_t stack[STACK * 64];
+    const uint32_t count = ba.qcount[ba.bounce + 1];
+    for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u)
+        nee_pdf_ray<STACK>(sv, ba, stack, base + threadIdx.x, count);
 }
 
 // ---------------------------------------------------------------- ReSTIR DI (benchmark mode)
@@ -937,13 +955,17 @@ void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, co
         k_trace_closest<decltype(S)::value><<<(n + 63) / 64, 64, 0, s>>>(sv, o, d, n, tmin, tmax, hit, t, uv);
     });
 }
-void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
+void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays,
+                        uint32_t grid_cap) {
     if (!max_rays) return;
     bool cnt = ba.touch_nodes != nullptr;
     with_stack(geom_stack_need(sv.depth), [&](auto S) {
         constexpr int K = decltype(S)::value;
         const uint32_t grid = (max_rays + 63) / 64;
-        if (ba.cache_on) k_trace_shade<K, false, true><<<grid, 64, 0, s>>>(sv, ba);
+        if (grid_cap && !cnt && grid > grid_cap) {
+            if (ba.cache_on) k_trace_shade_gs<K, true><<<grid_cap, 64, 0, s>>>(sv, ba);
+            else k_trace_shade_gs<K, false><<<grid_cap, 64, 0, s>>>(sv, ba);
+        } else if (ba.cache_on) k_trace_shade<K, false, true><<<grid, 64, 0, s>>>(sv, ba);
         else if (cnt) k_trace_shade<K, true, false><<<grid, 64, 0, s>>>(sv, ba);
         else k_trace_shade<K, false, false><<<grid, 64, 0, s>>>(sv, ba);
     });
@@ -963,11 +985,13 @@ __global__ void k_popcount(const uint32_t *bits, size_t n_words, unsigned long l
 void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsigned long long *out) {
     if (n_words) k_popcount<<<(unsigned)((n_words + 255) / 256), 256, 0, s>>>(bits, n_words, out);
 }
-void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
+void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays, uint32_t grid_cap) {
     if (!max_rays) return;
     // the 4-ary walk pushes at most 3 references per two binary levels
     with_stack(3 * ((sv.ldepth + 1) / 2), [&](auto S) {
-        k_nee_pdf<decltype(S)::value><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+        const uint32_t grid = (max_rays + 63) / 64;
+        if (grid_cap && grid > grid_cap) k_nee_pdf_gs<decltype(S)::value><<<grid_cap, 64, 0, s>>>(sv, ba);
+        else k_nee_pdf<decltype(S)::value><<<grid, 64, 0, s>>>(sv, ba);
     });
 }
 

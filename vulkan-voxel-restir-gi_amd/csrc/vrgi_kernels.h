@@ -7,6 +7,8 @@
 namespace vrgi {
 
 #define VRGI_MAX_STACK 160  // traversal stack entries per lane (LDS); deeper trees are rejected
+#define VRGI_SMALL_QUEUE_GRID 2048  // blocks of a striding ray launch over a queue expected to be short
+#define VRGI_NEE_STREAMS 4  // streams the NEE pdf launches of a frame rotate over
 #define VRGI_BOX_PAD 0.001f  // slack added to every box of the traversal copies (see box_hit)
 
 // ---------------------------------------------------------------- argument blocks
@@ -397,8 +399,12 @@ void launch_wide(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, Wi
 void launch_wide4(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, Wide4 *wide4);
 void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, const float *d,
                           uint32_t n, float tmin, float tmax, uint32_t *hit, float *t, float *uv);
-void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays);
-void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays);
+// grid_cap > 0: a grid of at most grid_cap blocks striding over the queue (for queues
+// expected to be far shorter than max_rays)
+void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays,
+                        uint32_t grid_cap = 0);
+void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays,
+                    uint32_t grid_cap = 0);
 void launch_accum_temporal(hipStream_t s, const FrameArgs &fa);
 void launch_spatial_finalize(hipStream_t s, const FrameArgs &fa);
 void launch_postprocess(hipStream_t s, const FrameArgs &fa);
