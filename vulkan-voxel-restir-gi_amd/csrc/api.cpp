@@ -94,6 +94,7 @@ struct vrgi_ctx {
     hipEvent_t ev_nee_done[VRGI_NEE_STREAMS] = {};
     std::vector<hipEvent_t> ev_bounce;  // trace (and sort) of bounce b done
     hipEvent_t ev_query = nullptr, ev_infer = nullptr, ev_train = nullptr, ev_accum = nullptr;
+    hipEvent_t ev_fstart = nullptr, ev_snap = nullptr, ev_held = nullptr;
     std::mutex up_mu;
     vrgi_config cfg{};
     uint32_t W = 0, H = 0, P = 0, B = 0;
@@ -182,6 +183,10 @@ struct vrgi_ctx {
         uint32_t plane_cap = 0, tcap = 0, hcap = 0;
         DevBuf<float> w32, mw, vw, table, mt, vt, gw, gtable, loss;
         DevBuf<uint16_t> wt, wb, xplane, gplane;
+        // the network as the frame starts: the frame's inference reads it while the
+        // training steps update the live table and weights beside it
+        DevBuf<float> table_snap;
+        DevBuf<uint16_t> wt_snap;
         DevBuf<float> trec, qrec, crad, hrec, hout, albedo, d0;
         // per-tile held-out error and fallback flags
         uint32_t tiles_x = 0, tiles_y = 0;
@@ -843,7 +848,7 @@ vrgi_ctx::~vrgi_ctx() {
         }
     for (hipEvent_t e : ev_nee_done)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {ev_query, ev_infer, ev_train, ev_accum})
+    for (hipEvent_t e : {ev_query, ev_infer, ev_train, ev_accum, ev_fstart, ev_snap, ev_held})
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_bounce) (void)hipEventDestroy(e);
 }
@@ -899,6 +904,9 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_infer, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_train, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_accum, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_fstart, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_snap, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_held, hipEventDisableTiming));
         c->cfg = *cfg;
         c->W = cfg->width;
         c->H = cfg->height;
@@ -1191,10 +1199,27 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     float *trec_cur = c->cache.trec.p + (size_t)par * c->cache.tcap * 16;
     float *trec_prev = c->cache.trec.p + (size_t)(par ^ 1) * c->cache.tcap * 16;
     uint32_t *tcnt_cur = c->cache.counts.p + (par ? 3 : 0), *tcnt_prev = c->cache.counts.p + (par ? 0 : 3);
+    const bool cache_train = cache_on && !(c->cache.cfg.flags & VRGI_CACHE_FREEZE) && c->cache.cfg.steps_per_frame;
     if (cache_on) {
         cache_box_from_scene(c);
         HIP_CHECK(hipMemsetAsync(c->cache.counts.p + 1, 0, 4, s));
         HIP_CHECK(hipMemsetAsync(tcnt_cur, 0, 4, s));
+    }
+    if (cache_train) {
+        // the frame's training steps on the previous frame's records, on the build stream
+        // (idle once this frame's BVHs are built), after a snapshot of the network for this
+        // frame's inference; they run beside the frame's rays and inference
+        auto &C = c->cache;
+        hipStream_t ts = c->up_stream;
+        HIP_CHECK(hipEventRecord(c->ev_fstart, s));
+        HIP_CHECK(hipStreamWaitEvent(ts, c->ev_fstart, 0));
+        HIP_CHECK(hipMemcpyAsync(C.table_snap.p, C.table.p, C.n_table * sizeof(float), hipMemcpyDeviceToDevice, ts));
+        HIP_CHECK(hipMemcpyAsync(C.wt_snap.p, C.wt.p, NRC_N_WEIGHTS * sizeof(uint16_t), hipMemcpyDeviceToDevice, ts));
+        HIP_CHECK(hipEventRecord(c->ev_snap, ts));
+        HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, ts));
+        for (uint32_t k = 0; k < C.cfg.steps_per_frame; ++k)
+            cache_step(c, ts, trec_prev, tcnt_prev, C.tcap, k, C.cfg.steps_per_frame, C.cfg.batch, true);
+        HIP_CHECK(hipEventRecord(c->ev_train, ts));
     }
     if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) {
         size_t n = (size_t)B * P;
@@ -1259,19 +1284,19 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         launch_trace_shade(s, sv, ba, P, small_q);
         qin[b + 1] = ba.queue_out;
         if (cache_on && b == c->cache.cfg.query_vertex) {
-            // side stream: cache radiance of this bounce's queries, then the training
-            // steps on the previous frame's records
+            // side stream: cache radiance of this bounce's queries, by the network as the
+            // frame started (the snapshot while the training steps run)
             auto &C = c->cache;
             HIP_CHECK(hipEventRecord(c->ev_query, s));
             HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_query, 0));
-            launch_nrc_infer(c->side, cache_net(c), C.qrec.p, C.counts.p + 1, P, nullptr, C.crad.p);
-            HIP_CHECK(hipEventRecord(c->ev_infer, c->side));
-            if (!(C.cfg.flags & VRGI_CACHE_FREEZE) && C.cfg.steps_per_frame) {
-                HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, c->side));
-                for (uint32_t k = 0; k < C.cfg.steps_per_frame; ++k)
-                    cache_step(c, c->side, trec_prev, tcnt_prev, C.tcap, k, C.cfg.steps_per_frame, C.cfg.batch, true);
+            NrcNet net = cache_net(c);
+            if (cache_train) {
+                HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_snap, 0));
+                net.table = C.table_snap.p;
+                net.wt = C.wt_snap.p;
             }
-            HIP_CHECK(hipEventRecord(c->ev_train, c->side));
+            launch_nrc_infer(c->side, net, C.qrec.p, C.counts.p + 1, P, nullptr, C.crad.p);
+            HIP_CHECK(hipEventRecord(c->ev_infer, c->side));
         }
         if (sort_rays && b + 1 < B) {
             SortArgs sa{};
@@ -1362,13 +1387,14 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         auto &C = c->cache;
         HIP_CHECK(hipEventRecord(c->ev_accum, s));
         HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_accum, 0));
+        if (cache_train) HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_train, 0));
         C.hpred.reserve((size_t)C.tcap * 3);
         launch_nrc_infer(c->side, cache_net(c), trec_cur, tcnt_cur, C.tcap, C.hpred.p, nullptr);
         launch_tile_stats(c->side, trec_cur, tcnt_cur, C.tcap, C.hpred.p, C.cfg.query_vertex, c->W,
                           C.cfg.fallback_tile, C.tiles_x, C.t_acc.p);
         launch_tile_update(c->side, C.t_acc.p, C.t_ema.p, C.t_fb.p, C.tiles_x * C.tiles_y, C.cfg.fallback_ema,
                            C.cfg.fallback_bound);
-        HIP_CHECK(hipEventRecord(c->ev_train, c->side));
+        HIP_CHECK(hipEventRecord(c->ev_held, c->side));
     }
     if (c->cfg.restir_mode == 1) {
         DiArgs da{};
@@ -1394,7 +1420,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         launch_accumulate(s, c->f_radiance.p, c->accum.p, (size_t)P * 3);
         c->accum_frames += 1;
     }
-    if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_train, 0));  // join the side stream
+    if (cache_train) HIP_CHECK(hipStreamWaitEvent(s, c->ev_train, 0));  // join the training steps
+    if (cache_on && c->cache.cfg.fallback_bound > 0.0f) HIP_CHECK(hipStreamWaitEvent(s, c->ev_held, 0));
     launch_frame_stats(s, c->f_qcount.p, B, P, (unsigned long long *)c->d_counters.p);
     // the set this frame read may be rebuilt once this point has passed
     HIP_CHECK(hipEventRecord(c->G().ev_free, s));
@@ -1815,6 +1842,7 @@ int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
     C.w32.reserve(NRC_N_WEIGHTS); C.mw.reserve(NRC_N_WEIGHTS); C.vw.reserve(NRC_N_WEIGHTS); C.gw.reserve(NRC_N_WEIGHTS);
     C.table.reserve(C.n_table); C.mt.reserve(C.n_table); C.vt.reserve(C.n_table); C.gtable.reserve(C.n_table);
     C.wt.reserve(NRC_N_WEIGHTS); C.wb.reserve(NRC_N_WEIGHTS);
+    C.table_snap.reserve(C.n_table); C.wt_snap.reserve(NRC_N_WEIGHTS);
     C.loss.reserve(1);
     C.counts.reserve(4);
     C.plane_cap = 0;
