@@ -153,11 +153,36 @@ def _by_key(rec):
 def test_cache_frame_parity(world0, scenes, nee_type, spatial, qv, fb):
     """Cache-mode frames against the oracle given the GPU's cache radiance; with fb > 0
     the per-tile fallback flags of the previous frame select full paths as well."""
-    import oracle
     import vrgi
     s, objs = scenes
+    _cache_frames_vs_oracle(world0, s, objs, vrgi.camera_look([6.0, 62.0, 8.0], [40.0, 0.0, 44.0]), nee_type,
+                            spatial, qv, fb)
+
+
+def test_cache_frame_parity_many_light_crossings(world0):
+    """Cache-mode frames over four layers of separated emissive blocks: the training paths'
+    NEE pdf rays (past the query vertex, computed in enumeration / walk / overflow passes)
+    cross many light triangles, more than a ray keeps for the walk pass, and the pdfs stay
+    bit-exact against the oracle."""
+    import oracle
+    import vrgi
+    blocks = world0["blocks"].copy()
+    lit = 4  # first emissive block type of the 64^3 world's materials
+    for y in (50, 53, 56, 59):
+        for x in range(2, 62, 2):
+            for z in range(2, 62, 2):
+                blocks[x, y, z] = lit
+    objs = vrgi.chunk_objects(blocks, world0["n_types"], world0["tr"])
+    s = oracle.Scene(world0["mats"], world0["lum"])
+    s.add_objects(objs)
+    s.build()
+    _cache_frames_vs_oracle(world0, s, objs, vrgi.camera_look([6.0, 48.0, 8.0], [40.0, 0.0, 44.0]), 1, 2, 0, 0.0)
+
+
+def _cache_frames_vs_oracle(world0, s, objs, cam, nee_type, spatial, qv, fb):
+    import oracle
+    import vrgi
     W, H, TB = 96, 64, 5
-    cam = vrgi.camera_look([6.0, 62.0, 8.0], [40.0, 0.0, 44.0])
     frac, seed = 0.25, 3
     with _ctx(world0, W, H, flags=vrgi.FLAG_KEEP_BOUNCES) as c:
         c.add_objects(objs)

@@ -150,6 +150,11 @@ struct vrgi_ctx {
     DevBuf<float> f_origin, f_dir, f_normal, f_emis, f_refl, f_mis, f_pdf, f_nee, f_hit_t;
     DevBuf<uint32_t> f_hit, f_plen, f_queue, f_qcount;
     DevBuf<uint32_t> f_squeue, f_skeys, f_shist;  // Morton-ordered queues, keys, buckets
+    // NEE pdf passes, one set per NEE stream: crossed-triangle counts and records per queue
+    // slot, overflow list and its length
+    DevBuf<uint32_t> nee_cnt[VRGI_NEE_STREAMS], nee_lid[VRGI_NEE_STREAMS], nee_over[VRGI_NEE_STREAMS],
+        nee_oc[VRGI_NEE_STREAMS];
+    DevBuf<float2> nee_uv[VRGI_NEE_STREAMS];
     DevBuf<float> f_radiance, f_restir, f_debug, f_temporal, f_spatial;
     DevBuf<uint8_t> f_bgra;
     DevBuf<float> di_T, di_hist, di_w;
@@ -326,6 +331,22 @@ void cache_step(vrgi_ctx *c, hipStream_t s, const float *rec, const uint32_t *co
     ad.inv_scale = 1.0f / C.cfg.loss_scale;
     launch_nrc_train_step(s, cache_net(c), tr, C.w32.p, C.mw.p, C.vw.p, C.table.p, C.mt.p, C.vt.p,
                           C.n_table, ad);
+}
+
+// scratch of NEE stream slot k for the enumeration / walk passes of one NEE pdf launch
+void nee_passes(vrgi_ctx *c, hipStream_t s, uint32_t k, BounceArgs &ba) {
+    const size_t P = c->P;
+    c->nee_cnt[k].reserve(P);
+    c->nee_lid[k].reserve(P * VRGI_NEE_RECS);
+    c->nee_uv[k].reserve(P * VRGI_NEE_RECS);
+    c->nee_over[k].reserve(P);
+    c->nee_oc[k].reserve(1);
+    HIP_CHECK(hipMemsetAsync(c->nee_oc[k].p, 0, 4, s));
+    ba.nee_cnt = c->nee_cnt[k].p;
+    ba.nee_lid = c->nee_lid[k].p;
+    ba.nee_uv = c->nee_uv[k].p;
+    ba.nee_over = c->nee_over[k].p;
+    ba.nee_over_count = c->nee_oc[k].p;
 }
 
 void alloc_frame(vrgi_ctx *c) {
@@ -1322,8 +1343,13 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             BounceArgs na = ba;
             na.queue_in = qin[b + 1];
             na.touch_nodes = na.touch_tris = nullptr;
+            // short queues (the training paths past the cache's query vertex): the pdf in
+            // passes, whose walks then run at full occupancy instead of in the tail of a
+            // few waves that also hold the traversal
+            const bool short_q = cache_on && b >= c->cache.cfg.query_vertex;
+            if (short_q) nee_passes(c, ns, b % VRGI_NEE_STREAMS, na);
             rec_on(3 + 2 * (size_t)B + b, ns);
-            launch_nee_pdf(ns, sv, na, P, cache_on && b >= c->cache.cfg.query_vertex ? VRGI_SMALL_QUEUE_GRID : 0u);
+            launch_nee_pdf(ns, sv, na, P, short_q ? VRGI_SMALL_QUEUE_GRID : 0u);
             rec_on(2 + B + b, ns);
         }
     }
@@ -1339,7 +1365,10 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             if (prefs->nee_type != 0) {
                 ba.bounce = b;
                 ba.queue_in = qin[b + 1];
-                launch_nee_pdf(s, sv, ba, P, cache_on && b >= c->cache.cfg.query_vertex ? VRGI_SMALL_QUEUE_GRID : 0u);
+                const bool short_q = cache_on && b >= c->cache.cfg.query_vertex;
+                BounceArgs na = ba;
+                if (short_q) nee_passes(c, s, 0, na);
+                launch_nee_pdf(s, sv, na, P, short_q ? VRGI_SMALL_QUEUE_GRID : 0u);
             }
             rec(2 + B + b);
         }

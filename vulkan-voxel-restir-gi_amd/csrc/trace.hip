@@ -542,7 +542,7 @@ __global__ __launch_bounds__(64) void k_trace_shade_gs(const SceneView sv, const
 
 // NEE pdf of bounce b (nee_pdf.rs:281-337) for the rays alive at bounce b+1
 constexpr int NEE_DEFER = 4;  // crossed light triangles kept per lane before their pdfs are summed
-template <int STACK>
+template <int STACK, bool COUNT_RAYS = true>
 __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArgs &ba, uint32_t *stack, uint32_t i,
                                             uint32_t count) {
     const uint32_t b = ba.bounce;  // pdf of bounce b, rays of bounce b + 1
@@ -553,9 +553,11 @@ __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArg
     f3 d = ld3(ba.dir + bi * 3);
     float np = 0.0f;
     const bool traced = !(len3(d) == 0.0f || w == 0.0f) && sv.n_ltri > 0;
-    uint64_t bal = __ballot(traced);
-    if ((threadIdx.x & 63) == (uint32_t)(__ffsll((unsigned long long)bal) - 1))
-        atomicAdd(ba.counters + 1, (unsigned long long)__popcll(bal));
+    if (COUNT_RAYS) {
+        uint64_t bal = __ballot(traced);
+        if ((threadIdx.x & 63) == (uint32_t)(__ffsll((unsigned long long)bal) - 1))
+            atomicAdd(ba.counters + 1, (unsigned long long)__popcll(bal));
+    }
     if (traced) {
         f3 o = ld3(ba.origin + bi * 3);
         f3 n = ld3(ba.normal + bi * 3);
@@ -632,6 +634,119 @@ __global__ __launch_bounds__(64) void k_nee_pdf_gs(const SceneView sv, const Bou
     const uint32_t count = ba.qcount[ba.bounce + 1];
     for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u)
         nee_pdf_ray<STACK>(sv, ba, stack, base + threadIdx.x, count);
+}
+
+// The NEE pdf in passes: the enumeration keeps no reverse-walk state (fewer registers,
+// more waves), the walks run in their own pass. Per ray the pdfs are summed in the order
+// the triangles were crossed, as in nee_pdf_ray; a ray crossing more than VRGI_NEE_RECS
+// goes to an overflow list and is done again whole by nee_pdf_ray.
+template <int STACK>
+__device__ __forceinline__ void nee_enum_ray(const SceneView &sv, const BounceArgs &ba, uint32_t *stack, uint32_t i,
+                                             uint32_t count) {
+    const uint32_t b = ba.bounce;
+    if (i >= count) return;
+    const uint32_t px = ba.queue_in[i];
+    const size_t bi = (size_t)b * ba.P + px;
+    const float w = ba.mis[bi];
+    const f3 d = ld3(ba.dir + bi * 3);
+    const bool traced = !(len3(d) == 0.0f || w == 0.0f) && sv.n_ltri > 0;
+    const uint64_t bal = __ballot(traced);
+    if ((threadIdx.x & 63) == (uint32_t)(__ffsll((unsigned long long)bal) - 1))
+        atomicAdd(ba.counters + 1, (unsigned long long)__popcll(bal));
+    uint32_t nh = 0;
+    if (traced) {
+        const f3 o = ld3(ba.origin + bi * 3);
+        const RayPre r = ray_pre(o, d);
+        uint32_t *stk = stack + (threadIdx.x & 63u);
+        int sp = 0;
+        uint32_t ref = sv.n_ltri == 1 ? 0x80000000u | sv.lroot_prim : 0u;
+        bool have = true;
+        while (have) {
+            if (ref & 0x80000000u) {
+                const uint32_t lid = ref & 0x7FFFFFFFu;
+                const Tri t = sv.ltri[lid];
+                float tt, u, v;
+                if (tri_hit(r, mk(t.v0.x, t.v0.y, t.v0.z), mk(t.v1.x, t.v1.y, t.v1.z), mk(t.v2.x, t.v2.y, t.v2.z),
+                            VRGI_EPS_NEE, 1000.0f, &tt, &u, &v)) {
+                    if (nh < VRGI_NEE_RECS) {
+                        ba.nee_lid[(size_t)nh * ba.P + i] = lid;
+                        ba.nee_uv[(size_t)nh * ba.P + i] = make_float2(u, v);
+                    }
+                    ++nh;
+                    if (nh > VRGI_NEE_RECS) sp = 0;  // overflow: the whole ray is done again
+                }
+                have = false;
+            } else {  // 4-ary node, descendants in leaf order (as in nee_pdf_ray)
+                const Wide4 *w4 = sv.lwide4 + ref;
+                const float4 lx = w4->lox, ly = w4->loy, lz = w4->loz, hx = w4->hix, hy = w4->hiy, hz = w4->hiz;
+                const uint4 rf = w4->ref;
+                uint32_t m = 0;
+                m |= (rf.x != VRGI_INVALID && box_hit(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, VRGI_EPS_NEE, 1000.0f)) ? 1u : 0u;
+                m |= (rf.y != VRGI_INVALID && box_hit(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, VRGI_EPS_NEE, 1000.0f)) ? 2u : 0u;
+                m |= (rf.z != VRGI_INVALID && box_hit(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, VRGI_EPS_NEE, 1000.0f)) ? 4u : 0u;
+                m |= (rf.w != VRGI_INVALID && box_hit(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, VRGI_EPS_NEE, 1000.0f)) ? 8u : 0u;
+                have = m != 0u;
+                if (m & 8u && m != 8u) stk[(sp++) * 64] = rf.w;
+                if (m & 4u && (m & 3u)) stk[(sp++) * 64] = rf.z;
+                if (m & 2u && (m & 1u)) stk[(sp++) * 64] = rf.y;
+                ref = (m & 1u) ? rf.x : (m & 2u) ? rf.y : (m & 4u) ? rf.z : rf.w;
+            }
+            if (!have && sp > 0) { ref = stk[(--sp) * 64]; have = true; }
+        }
+    }
+    if (nh > VRGI_NEE_RECS) {
+        nh = VRGI_NEE_RECS + 1;
+        ba.nee_over[atomicAdd(ba.nee_over_count, 1u)] = i;
+    }
+    ba.nee_cnt[i] = nh;
+    if (nh == 0) ba.nee[bi] = 0.0f;
+}
+template <int STACK>
+__global__ __launch_bounds__(64) void k_nee_enum(const SceneView sv, const BounceArgs ba, uint32_t grid_stride) {
+    __shared__ uint32_t stack[STACK * 64];
+    const uint32_t count = ba.qcount[ba.bounce + 1];
+    if (!grid_stride) {
+        nee_enum_ray<STACK>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, count);
+        return;
+    }
+    for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u)
+        nee_enum_ray<STACK>(sv, ba, stack, base + threadIdx.x, count);
+}
+// the reverse walks of the kept triangles, summed in crossing order (nee_pdf.rs:281-337)
+__device__ __forceinline__ void nee_walk_ray(const SceneView &sv, const BounceArgs &ba, uint32_t i, uint32_t count) {
+    if (i >= count) return;
+    const uint32_t nh = ba.nee_cnt[i];
+    if (nh == 0 || nh > VRGI_NEE_RECS) return;
+    const uint32_t px = ba.queue_in[i];
+    const size_t bi = (size_t)ba.bounce * ba.P + px;
+    const f3 o = ld3(ba.origin + bi * 3), n = ld3(ba.normal + bi * 3), d = ld3(ba.dir + bi * 3);
+    float np = 0.0f;
+#pragma unroll 1
+    for (uint32_t j = 0; j < nh; ++j) {
+        const uint32_t lid = ba.nee_lid[(size_t)j * ba.P + i];
+        const float2 uv = ba.nee_uv[(size_t)j * ba.P + i];
+        np += nee_pdf_one(sv, o, n, d, lid, uv.x, uv.y);
+    }
+    ba.nee[bi] = np;
+}
+__global__ __launch_bounds__(64) void k_nee_walk(const SceneView sv, const BounceArgs ba, uint32_t grid_stride) {
+    const uint32_t count = ba.qcount[ba.bounce + 1];
+    if (!grid_stride) {
+        nee_walk_ray(sv, ba, blockIdx.x * 64u + threadIdx.x, count);
+        return;
+    }
+    for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u)
+        nee_walk_ray(sv, ba, base + threadIdx.x, count);
+}
+// the rays on the overflow list, whole (a small grid striding over the list)
+template <int STACK>
+__global__ __launch_bounds__(64) void k_nee_overflow(const SceneView sv, const BounceArgs ba) {
+    __shared__ uint32_t stack[STACK * 64];
+    const uint32_t n = *ba.nee_over_count, count = ba.qcount[ba.bounce + 1];
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t k = base + threadIdx.x;
+        nee_pdf_ray<STACK, false>(sv, ba, stack, k < n ? ba.nee_over[k] : count, count);
+    }
 }
 
 // ---------------------------------------------------------------- ReSTIR DI (benchmark mode)
@@ -989,9 +1104,18 @@ void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, ui
     if (!max_rays) return;
     // the 4-ary walk pushes at most 3 references per two binary levels
     with_stack(3 * ((sv.ldepth + 1) / 2), [&](auto S) {
+        constexpr int K = decltype(S)::value;
         const uint32_t grid = (max_rays + 63) / 64;
-        if (grid_cap && grid > grid_cap) k_nee_pdf_gs<decltype(S)::value><<<grid_cap, 64, 0, s>>>(sv, ba);
-        else k_nee_pdf<decltype(S)::value><<<grid, 64, 0, s>>>(sv, ba);
+        const bool gs = grid_cap && grid > grid_cap;
+        if (ba.nee_cnt) {  // enumeration, walks, overflow rays
+            k_nee_enum<K><<<gs ? grid_cap : grid, 64, 0, s>>>(sv, ba, gs ? 1u : 0u);
+            k_nee_walk<<<gs ? grid_cap : grid, 64, 0, s>>>(sv, ba, gs ? 1u : 0u);
+            k_nee_overflow<K><<<64, 64, 0, s>>>(sv, ba);
+        } else if (gs) {
+            k_nee_pdf_gs<K><<<grid_cap, 64, 0, s>>>(sv, ba);
+        } else {
+            k_nee_pdf<K><<<grid, 64, 0, s>>>(sv, ba);
+        }
     });
 }
 

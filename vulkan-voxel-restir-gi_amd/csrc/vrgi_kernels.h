@@ -9,6 +9,7 @@ namespace vrgi {
 #define VRGI_MAX_STACK 160  // traversal stack entries per lane (LDS); deeper trees are rejected
 #define VRGI_SMALL_QUEUE_GRID 2048  // blocks of a striding ray launch over a queue expected to be short
 #define VRGI_NEE_STREAMS 4  // streams the NEE pdf launches of a frame rotate over
+#define VRGI_NEE_RECS 8     // crossed light triangles a ray keeps for the NEE walk pass
 #define VRGI_BOX_PAD 0.001f  // slack added to every box of the traversal copies (see box_hit)
 
 // ---------------------------------------------------------------- argument blocks
@@ -68,6 +69,12 @@ struct BounceArgs {
     float *albedo;          // [B][P][3] material reflectivity at each hit
     float *qrec;            // [P][16] query records
     uint32_t *qrec_count;
+    // NEE pdf in three passes (nee_rec set): per queue slot i, the crossed light
+    // triangles (at most VRGI_NEE_RECS: lid plane [k][P], barycentric pair plane [k][P]),
+    // their count (VRGI_NEE_RECS + 1: more, the ray is on the overflow list)
+    uint32_t *nee_cnt, *nee_lid;
+    float2 *nee_uv;
+    uint32_t *nee_over, *nee_over_count;
 };
 
 struct FlattenArgs {
@@ -400,7 +407,8 @@ void launch_wide4(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, W
 void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, const float *d,
                           uint32_t n, float tmin, float tmax, uint32_t *hit, float *t, float *uv);
 // grid_cap > 0: a grid of at most grid_cap blocks striding over the queue (for queues
-// expected to be far shorter than max_rays)
+// expected to be far shorter than max_rays). NEE pdf with ba.nee_cnt set: enumeration,
+// walk and overflow passes instead of one kernel.
 void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays,
                         uint32_t grid_cap = 0);
 void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays,
