@@ -212,7 +212,9 @@ struct vrgi_ctx {
         DevBuf<uint16_t> blocks, edit_val;
         DevBuf<uint8_t> transl, em, dirty, mask;
         DevBuf<float> lum, prim9;
-        DevBuf<uint32_t> edit_xyz, chunks, face_off, faces, lface_off, lfaces, trees, depth, tree_base;
+        DevBuf<uint32_t> edit_xyz, chunks, face_off, faces, lface_off, lfaces, trees, depth, tree_base, rank;
+        DevBuf<uint8_t> scan_tmp;
+        DevBuf<vrgi_bvh_node> roots;
         DevBuf<uint64_t> vert_off, light_off;
         Builder builder;
         uint64_t commits = 0, chunks_remeshed = 0;
@@ -742,8 +744,7 @@ void voxels_commit(vrgi_ctx *c) {
     launch_light_gather(s, h, V.lum.p, V.em.p, V.lface_off.p, V.light_off.p, A->tri9.p, A->power.p, V.prim9.p,
                         A->ids.p, (uint32_t)nl);
     const uint32_t nT = (uint32_t)beg.size();
-    std::vector<vrgi_bvh_node> hn(nodes);
-    std::vector<float> t9(9 * nl), pw(nl);
+    std::vector<vrgi_bvh_node> roots(nT);
     if (nT) {
         std::vector<uint32_t> tr(3 * (size_t)nT);
         for (uint32_t t = 0; t < nT; ++t) { tr[t] = beg[t]; tr[nT + t] = cnt[t]; tr[2 * nT + t] = base[t]; }
@@ -758,36 +759,16 @@ void voxels_commit(vrgi_ctx *c) {
                             V.trees.p + 2 * nT, base.data(), A->nodes.p, V.depth.p, nullptr) != 0)
             throw HipError("voxel light build exceeded its workspace");
         launch_light_pairs_multi(s, A->nodes.p, V.tree_base.p, nT, (uint32_t)nodes, A->lpairs.p);
-        HIP_CHECK(hipMemcpyAsync(hn.data(), A->nodes.p, nodes * sizeof(vrgi_bvh_node), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipMemcpyAsync(t9.data(), A->tri9.p, t9.size() * 4, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipMemcpyAsync(pw.data(), A->power.p, pw.size() * 4, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-        // light vertices in leaf order: the corners of the light triangle each leaf holds
-        // (right_or_prim = its triangle index in the chunk; the light list is in triangle
-        // order, so the chunk's light j with ids[j] == prim is found through an index map)
-        std::vector<vrgi_light_vertex3d> lv(3 * nl);
-        std::vector<uint32_t> ids(nl);
-        HIP_CHECK(hipMemcpy(ids.data(), A->ids.p, ids.size() * 4, hipMemcpyDeviceToHost));
-        std::map<uint32_t, uint32_t> slot;
-        for (uint32_t i = 0; i < n; ++i) {
-            if (tree_of[i] == VRGI_INVALID) continue;
-            const uint32_t t = tree_of[i], m = cnt[t];
-            slot.clear();
-            for (uint32_t j = 0; j < m; ++j) slot[ids[loff[i] + j]] = j;
-            size_t w = 3 * loff[i];
-            for (uint32_t k = 0; k < 2 * m - 1; ++k) {
-                const vrgi_bvh_node &nd = hn[base[t] + k];
-                if (nd.left != VRGI_INVALID) continue;
-                const float *tri = t9.data() + 9 * (loff[i] + slot.at(nd.right_or_prim));
-                for (int q = 0; q < 3; ++q) {
-                    std::memcpy(lv[w].position, tri + 3 * q, 12);
-                    lv[w].bvh_node_idx = k;
-                    ++w;
-                }
-            }
-        }
-        HIP_CHECK(hipMemcpyAsync(A->lverts.p, lv.data(), lv.size() * sizeof(vrgi_light_vertex3d), hipMemcpyHostToDevice,
-                                 s));
+        // light vertices in leaf order (each leaf's right_or_prim is its triangle index in
+        // the chunk, found among the chunk's lights, which are in triangle order) and the
+        // trees' roots, on the device; only the roots come back
+        V.rank.reserve(2 * (size_t)nodes);
+        const size_t sb = light_verts_scratch_bytes((uint32_t)nodes);
+        V.scan_tmp.reserve(sb);
+        V.roots.reserve(nT);
+        launch_light_verts_multi(s, A->nodes.p, (uint32_t)nodes, V.tree_base.p, V.trees.p, nT, A->ids.p, A->tri9.p,
+                                 V.rank.p, V.scan_tmp.p, sb, A->lverts.p, V.roots.p);
+        HIP_CHECK(hipMemcpyAsync(roots.data(), V.roots.p, nT * sizeof(vrgi_bvh_node), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
     for (uint32_t i = 0; i < n; ++i) {
@@ -806,21 +787,12 @@ void voxels_commit(vrgi_ctx *c) {
                 o->d_light_bvh = A->nodes.p + base[t];
                 o->d_lpairs = A->lpairs.p + base[t];
                 o->d_light_verts = A->lverts.p + 3 * loff[i];
-                o->h_light_bvh.assign(hn.begin() + base[t], hn.begin() + base[t] + (2 * m - 1));
-                // the (Aabb, power) build_bl_bvh returns (build.rs:359-371)
-                const vrgi_bvh_node &r = o->h_light_bvh[0];
+                // the (Aabb, power) build_bl_bvh returns (build.rs:359-371): the root's box
+                // and power (a single-leaf root holds its triangle's box and power)
+                const vrgi_bvh_node &r = roots[t];
                 o->light_empty = false;
-                if (r.left == VRGI_INVALID) {
-                    const float *tri = t9.data() + 9 * loff[i];
-                    for (int k = 0; k < 3; ++k) {
-                        o->light_aabb[k] = std::min(std::min(tri[k], tri[3 + k]), tri[6 + k]);
-                        o->light_aabb[3 + k] = std::max(std::max(tri[k], tri[3 + k]), tri[6 + k]);
-                    }
-                    o->power = pw[loff[i]];
-                } else {
-                    for (int k = 0; k < 3; ++k) { o->light_aabb[k] = r.min[k]; o->light_aabb[3 + k] = r.max[k]; }
-                    o->power = r.power;
-                }
+                for (int k = 0; k < 3; ++k) { o->light_aabb[k] = r.min[k]; o->light_aabb[3 + k] = r.max[k]; }
+                o->power = r.power;
             }
         }
         auto it = c->objects.find(key);
@@ -1081,13 +1053,21 @@ int vrgi_object_release(vrgi_object *o) {
     API_END
 }
 int vrgi_object_light_bvh(vrgi_object *o, vrgi_bvh_node *dst, size_t cap, size_t *n_nodes) {
+    API_BEGIN
     if (!o || !n_nodes) return fail(VRGI_E_INVALID, "null argument");
+    if (o->h_light_bvh.empty() && o->n_light && o->d_light_bvh) {  // an object of a voxel commit: read on demand
+        HIP_CHECK(hipSetDevice(o->ctx->device));
+        o->h_light_bvh.resize(2 * (size_t)o->n_light - 1);
+        HIP_CHECK(hipMemcpy(o->h_light_bvh.data(), o->d_light_bvh, o->h_light_bvh.size() * sizeof(vrgi_bvh_node),
+                            hipMemcpyDeviceToHost));
+    }
     *n_nodes = o->h_light_bvh.size();
     if (dst) {
         if (cap < o->h_light_bvh.size()) return fail(VRGI_E_LIMIT, "destination too small");
         std::memcpy(dst, o->h_light_bvh.data(), o->h_light_bvh.size() * sizeof(vrgi_bvh_node));
     }
     return VRGI_OK;
+    API_END
 }
 
 int vrgi_scene_add(vrgi_ctx *c, uint32_t key, vrgi_object *o, const float xf[12]) {

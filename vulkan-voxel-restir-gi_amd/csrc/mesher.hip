@@ -7,6 +7,9 @@
 // is air); one block per chunk forms the exclusive scan of the face counts in voxel
 // order; every voxel then writes its faces at the scanned position. The order is thus
 // the sequential loop's order by construction.
+#include <rocprim/device/device_scan.hpp>
+
+#include "vrgi_internal.h"
 #include "vrgi_kernels.h"
 
 namespace vrgi {
@@ -271,6 +274,68 @@ void launch_light_gather(hipStream_t s, const MeshArgsHost &h, const float *lum,
 void launch_light_pairs_multi(hipStream_t s, const vrgi_bvh_node *nodes, const uint32_t *base, uint32_t n_trees,
                               uint32_t n_nodes, LPair *out) {
     if (n_trees && n_nodes) k_light_pairs_multi<<<(n_nodes + 255) / 256, 256, 0, s>>>(nodes, base, n_trees, out);
+}
+
+// light_bvh_buffer_to_light_vertexes (scene.rs:509-524) for the light BL-BVHs of a voxel
+// commit: leaf flags, their exclusive scan (the rank of each leaf among its tree's leaves
+// in node order), then each leaf writes the corners of its triangle (the light whose id
+// is the leaf's right_or_prim, found by binary search) at its rank, with its node index
+__global__ void k_leaf_flags(const vrgi_bvh_node *nodes, uint32_t n, uint32_t *flags) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) flags[g] = nodes[g].left == VRGI_INVALID ? 1u : 0u;
+}
+__global__ void k_light_verts_multi(const vrgi_bvh_node *nodes, uint32_t n_nodes, const uint32_t *tree_base,
+                                    const uint32_t *tree_light, uint32_t n_trees, const uint32_t *ids,
+                                    const float *tri9, const uint32_t *rank, vrgi_light_vertex3d *out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_nodes) return;
+    const vrgi_bvh_node nd = nodes[g];
+    if (nd.left != VRGI_INVALID) return;
+    uint32_t lo = 0, hi = n_trees - 1;  // last tree whose first node is <= g
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (tree_base[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint32_t b0 = tree_base[lo], m = (tree_base[lo + 1] - b0 + 1) / 2, lb = tree_light[lo];
+    uint32_t a = 0, c = m;  // first light of the tree with id >= right_or_prim
+    while (a < c) {
+        const uint32_t mid = (a + c) >> 1;
+        if (ids[lb + mid] < nd.right_or_prim) a = mid + 1;
+        else c = mid;
+    }
+    const float *tri = tri9 + 9 * ((size_t)lb + a);
+    vrgi_light_vertex3d *o = out + 3 * ((size_t)lb + rank[g] - rank[b0]);
+    for (int q = 0; q < 3; ++q) {
+        o[q].position[0] = tri[3 * q];
+        o[q].position[1] = tri[3 * q + 1];
+        o[q].position[2] = tri[3 * q + 2];
+        o[q].bvh_node_idx = g - b0;
+    }
+}
+__global__ void k_tree_roots(const vrgi_bvh_node *nodes, const uint32_t *tree_base, uint32_t n_trees,
+                             vrgi_bvh_node *roots) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_trees) roots[t] = nodes[tree_base[t]];
+}
+size_t light_verts_scratch_bytes(uint32_t n_nodes) {
+    size_t bytes = 0;
+    HIP_CHECK(rocprim::exclusive_scan(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, 0u,
+                                      (size_t)n_nodes, rocprim::plus<uint32_t>()));
+    return bytes;
+}
+void launch_light_verts_multi(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, const uint32_t *tree_base,
+                              const uint32_t *tree_light, uint32_t n_trees, const uint32_t *ids, const float *tri9,
+                              uint32_t *rank, void *scratch, size_t scratch_bytes, vrgi_light_vertex3d *out,
+                              vrgi_bvh_node *roots) {
+    if (!n_trees || !n_nodes) return;
+    const uint32_t grid = (n_nodes + 255) / 256;
+    k_leaf_flags<<<grid, 256, 0, s>>>(nodes, n_nodes, rank + n_nodes);  // flags after the ranks
+    size_t bytes = scratch_bytes;
+    HIP_CHECK(rocprim::exclusive_scan(scratch, bytes, (const uint32_t *)(rank + n_nodes), rank, 0u,
+                                      (size_t)n_nodes, rocprim::plus<uint32_t>(), s));
+    k_light_verts_multi<<<grid, 256, 0, s>>>(nodes, n_nodes, tree_base, tree_light, n_trees, ids, tri9, rank, out);
+    k_tree_roots<<<(n_trees + 255) / 256, 256, 0, s>>>(nodes, tree_base, n_trees, roots);
 }
 
 void launch_mesh_count(hipStream_t s, const MeshArgsHost &h) {

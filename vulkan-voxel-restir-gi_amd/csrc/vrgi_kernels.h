@@ -537,6 +537,15 @@ void launch_light_gather(hipStream_t s, const MeshArgsHost &h, const float *lum,
 void launch_light_pairs_multi(hipStream_t s, const vrgi_bvh_node *nodes, const uint32_t *base, uint32_t n_trees,
                               uint32_t n_nodes, LPair *out);
 void launch_mesh_emit(hipStream_t s, const MeshArgsHost &h);
+// light vertices of many light BL-BVHs (tree t: nodes tree_base[t] .. tree_base[t + 1], its
+// lights from tree_light[t], ids ascending within a tree) in leaf order, and each tree's
+// root node; `rank` holds 2 * n_nodes words, `scratch` the scan's
+// light_verts_scratch_bytes(n_nodes) bytes
+void launch_light_verts_multi(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, const uint32_t *tree_base,
+                              const uint32_t *tree_light, uint32_t n_trees, const uint32_t *ids, const float *tri9,
+                              uint32_t *rank, void *scratch, size_t scratch_bytes, vrgi_light_vertex3d *out,
+                              vrgi_bvh_node *roots);
+size_t light_verts_scratch_bytes(uint32_t n_nodes);
 
 // queue reordering by (direction octant, origin Morton code) between bounces
 struct SortArgs {
