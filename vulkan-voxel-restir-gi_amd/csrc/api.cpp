@@ -43,11 +43,14 @@ template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t cap = 0;
+    // grows to n (a regrowth by 25 % more, so a scene that grows frame by frame does not
+    // reallocate every frame)
     void reserve(size_t n) {
         if (n <= cap && p) return;
+        const size_t want = std::max<size_t>(p ? n + n / 4 : n, 1);
         release();
-        dev_alloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T));
-        cap = std::max<size_t>(n, 1);
+        dev_alloc((void **)&p, want * sizeof(T));
+        cap = want;
     }
     void release() {
         if (p) (void)hipFree(p);
@@ -61,6 +64,10 @@ struct Counters {
     unsigned long long rays_closest, rays_light, frames, pad;
 };
 
+}  // namespace
+
+namespace {
+struct VxArenaPool;  // voxel commits' arenas for reuse (below)
 }  // namespace
 
 struct vrgi_object {
@@ -217,6 +224,7 @@ struct vrgi_ctx {
         DevBuf<vrgi_bvh_node> roots;
         DevBuf<uint64_t> vert_off, light_off;
         Builder builder;
+        std::shared_ptr<VxArenaPool> pool;
         uint64_t commits = 0, chunks_remeshed = 0;
     } vx;
 
@@ -652,13 +660,24 @@ struct VxArena {
     DevBuf<LPair> lpairs;
     DevBuf<vrgi_light_vertex3d> lverts;
 };
+// arenas whose objects have all been released, for the next commits; shared by the
+// context and the arenas' deleters (an object may outlive its context)
+struct VxArenaPool {
+    std::mutex mu;
+    std::vector<VxArena *> free_list;
+    ~VxArenaPool() {
+        for (VxArena *a : free_list) delete a;
+    }
+};
 
 // Re-meshes the dirty chunks of the voxel world and replaces their objects: what the
 // reference's chunk manager does per dirty chunk (chunk_manager.rs:338-399: gen_mesh,
-// upload_object, add_object), for all of them at once. Vertices, the emissive gather and
-// the light BL-BVHs (one multi-tree build) stay on the device; the light vertices in
-// leaf order (scene.rs:509-524) and each object's (box, power) are formed on the host
-// from one readback, as upload_object forms them.
+// upload_object, add_object), for all of them at once. Vertices, the emissive gather,
+// the light BL-BVHs (one multi-tree build) and the light vertices in leaf order
+// (scene.rs:509-524) stay on the device; each object's (box, power) comes from its
+// tree's root, the one readback. The commit's arena is taken from a pool of arenas whose
+// objects have all been released (their buffers only grow), so a commit per frame does
+// not allocate and free device memory each time.
 void voxels_commit(vrgi_ctx *c) {
     auto &V = c->vx;
     if (!V.on || !V.pending) return;
@@ -727,7 +746,21 @@ void voxels_commit(vrgi_ctx *c) {
     }
     if (nl >= (1ull << 31) || nodes >= (1ull << 32) - 1 || nv >= (1ull << 40))
         throw std::runtime_error("voxel commit above 2^31 light triangles");
-    auto A = std::make_shared<VxArena>();
+    if (!V.pool) V.pool = std::make_shared<VxArenaPool>();
+    VxArena *raw = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(V.pool->mu);
+        if (!V.pool->free_list.empty()) {
+            raw = V.pool->free_list.back();
+            V.pool->free_list.pop_back();
+        }
+    }
+    if (!raw) raw = new VxArena();
+    auto pool = V.pool;
+    std::shared_ptr<VxArena> A(raw, [pool](VxArena *a) {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        pool->free_list.push_back(a);
+    });
     A->verts.reserve(nv);
     A->tri9.reserve(9 * nl);
     A->power.reserve(nl);

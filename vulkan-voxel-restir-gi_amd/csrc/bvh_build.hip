@@ -1130,9 +1130,10 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
 // ---------------------------------------------------------------- host driver
 void Builder::reserve(uint32_t n, uint32_t n_trees) {
     if (n <= cap_n && n_trees <= cap_t) return;
+    const bool regrow = cap_n != 0;  // 25 % headroom when a workspace grows again
     free_all();
-    cap_n = std::max(n, 16u);
-    cap_t = std::max(n_trees, 1u);
+    cap_n = std::max(regrow ? n + n / 4 : n, 16u);
+    cap_t = std::max(regrow ? n_trees + n_trees / 4 : n_trees, 1u);
     uint32_t open_cap = cap_n / SUB_MAX_DEFAULT + cap_t + 16;
     uint32_t item_cap = cap_n / CHUNK + open_cap + 16;
     auto A = [&](void **p, size_t bytes) { dev_alloc(p, bytes); };
