@@ -49,6 +49,42 @@ def test_bl_bvh_bit_exact(ctx, n, seed, grid):
     assert g_box.tobytes() == o_box.tobytes() and np.float32(g_pw) == np.float32(o_pw)
 
 
+def test_bl_bvh_same_size_other_depth(ctx):
+    """Builds of the same size whose trees differ in depth, back to back in one context:
+    the builder queues as many top levels as the previous same-sized build had, so it
+    sees a build with more levels (further level groups) and one with fewer (empty
+    levels); every tree stays bit-exact."""
+    import oracle
+    n = 40000
+    shallow = random_soup(n, 11, spread=50.0)
+    rng = np.random.default_rng(12)
+    c = np.geomspace(1.0, 1e4, n)[:, None] * np.array([1.0, 0.01, 0.02])[None, :]  # geometric spacing
+    deep = (c[:, None, :] + rng.uniform(-0.5, 0.5, (n, 3, 3))).reshape(n, 9).astype(np.float32)
+    lum = rng.uniform(0.0, 10.0, n).astype(np.float32)
+    ids = np.arange(n, dtype=np.uint32)
+    depths = []
+    for k, tri in enumerate([shallow, deep, shallow, deep]):
+        say(f"build_bl_bvh same size, build {k}")
+        g_nodes, g_order, _, _ = ctx.build_bl_bvh(tri, lum, ids)
+        o_nodes, o_order, _, _, _ = oracle.build_bl_bvh(tri, lum, ids)
+        _same_nodes(g_nodes, o_nodes)
+        assert np.array_equal(g_order, o_order)
+        depths.append(_depth(g_nodes))
+    assert depths[1] > depths[0] + 4, depths
+
+
+def _depth(nodes):
+    """Tree depth from the node records (left child, right child / primitive)."""
+    raw = nodes.view(np.uint32).reshape(len(nodes), 10)
+    left = raw[:, 0]
+    depth = np.zeros(len(nodes), np.int64)
+    for i in range(len(nodes)):  # parents precede children in index order
+        if left[i] != 0xFFFFFFFF:
+            depth[left[i]] = depth[i] + 1
+            depth[raw[i, 1]] = depth[i] + 1
+    return int(depth.max())
+
+
 def test_bl_bvh_degenerate_centroids(ctx):
     """Every centroid equal: scale = 32/0 = inf, NaN bins -> 0, the partition is
     empty and the half split fallback (build.rs:152-157) runs at every level."""
