@@ -27,9 +27,12 @@
 namespace vrgi {
 
 static constexpr int BINS = 32;
-static constexpr int CHUNK = 4096;      // primitives per top-phase work item
+#ifndef VRGI_CHUNK
+#define VRGI_CHUNK 2048
+#endif
+static constexpr int CHUNK = VRGI_CHUNK;  // primitives per top-phase work item
 static constexpr int TOP_THREADS = 256;
-static constexpr int PER_THREAD = CHUNK / TOP_THREADS;  // 16
+static constexpr int PER_THREAD = CHUNK / TOP_THREADS;  // 8
 static constexpr int BIN_WORDS = 3 * BINS * 7;          // count + 6 box keys per bin
 static constexpr uint32_t GRID_CAP_ITEMS = 4096;         // blocks of the per-item kernels
 static constexpr uint32_t GRID_CAP_NODES = 8192;         // blocks of the per-node kernels
@@ -521,6 +524,7 @@ __global__ __launch_bounds__(TOP_THREADS) void k_count(BuildArgs a, BuildState s
 // earlier (row, wave) groups plus the earlier lanes of its own ballot.
 __global__ __launch_bounds__(TOP_THREADS) void k_part_a(BuildArgs a, BuildState s) {
   constexpr int WAVES = TOP_THREADS / 64;
+  static_assert(PER_THREAD * WAVES <= 64, "one wave forms the prefix of the group counts");
   __shared__ uint32_t s_cnt[PER_THREAD * WAVES];  // trues per (row, wave), then their exclusive prefix
   __shared__ uint32_t s_off, s_T;
   const uint32_t n_items = *s.n_items;
@@ -566,14 +570,14 @@ __global__ __launch_bounds__(TOP_THREADS) void k_part_a(BuildArgs a, BuildState 
         if (lane == 0) s_cnt[q * WAVES + w] = (uint32_t)__popcll(m);
     }
     __syncthreads();
-    if (w == 0) {  // exclusive prefix of the PER_THREAD * WAVES = 64 group counts
-        const uint32_t v = s_cnt[lane];
+    if (w == 0) {  // exclusive prefix of the PER_THREAD * WAVES (<= 64) group counts
+        const uint32_t v = lane < (uint32_t)(PER_THREAD * WAVES) ? s_cnt[lane] : 0u;
         uint32_t inc = v;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t u = __shfl_up(inc, off, 64);
             if (lane >= (uint32_t)off) inc += u;
         }
-        s_cnt[lane] = inc - v;
+        if (lane < (uint32_t)(PER_THREAD * WAVES)) s_cnt[lane] = inc - v;
     }
     __syncthreads();
     const uint32_t off = s_off;
