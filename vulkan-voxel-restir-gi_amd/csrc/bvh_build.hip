@@ -31,10 +31,16 @@ static constexpr int BINS = 32;
 #define VRGI_CHUNK 2048
 #endif
 static constexpr int CHUNK = VRGI_CHUNK;  // primitives per top-phase work item
-static constexpr int TOP_THREADS = 256;
+#ifndef VRGI_TOP_THREADS
+#define VRGI_TOP_THREADS 256
+#endif
+static constexpr int TOP_THREADS = VRGI_TOP_THREADS;
 static constexpr int PER_THREAD = CHUNK / TOP_THREADS;  // 8
 static constexpr int BIN_WORDS = 3 * BINS * 7;          // count + 6 box keys per bin
-static constexpr uint32_t GRID_CAP_ITEMS = 4096;         // blocks of the per-item kernels
+#ifndef VRGI_GRID_CAP_ITEMS
+#define VRGI_GRID_CAP_ITEMS 4096
+#endif
+static constexpr uint32_t GRID_CAP_ITEMS = VRGI_GRID_CAP_ITEMS;  // blocks of the per-item kernels
 static constexpr uint32_t GRID_CAP_NODES = 8192;         // blocks of the per-node kernels
 
 struct OpenNode {
