@@ -8,7 +8,9 @@ namespace vrgi {
 
 #define VRGI_MAX_STACK 160  // traversal stack entries per lane (LDS); deeper trees are rejected
 #define VRGI_SMALL_QUEUE_GRID 2048  // blocks of a striding ray launch over a queue expected to be short
+#ifndef VRGI_NEE_STREAMS
 #define VRGI_NEE_STREAMS 4  // streams the NEE pdf launches of a frame rotate over
+#endif
 #define VRGI_NEE_RECS 8     // crossed light triangles a ray keeps for the NEE walk pass
 #define VRGI_BOX_PAD 0.001f  // slack added to every box of the traversal copies (see box_hit)
 
