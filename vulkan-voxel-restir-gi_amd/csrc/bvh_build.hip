@@ -326,12 +326,15 @@ static constexpr int BIN_STRIDE = BIN_WORDS + 1;
 __device__ void sah_sweep_wave(const int *bins, const int *bounds, uint32_t *dim_out, float *split_out,
                                int box_out[6]);
 // SAH sweep of open node o from its bins, by one full wave; the split record of the node
-__device__ __forceinline__ void store_split(BuildState &s, uint32_t o, const int *bins) {
+// returns the split axis on every lane; lane 0 also stores the plane to *split_out
+__device__ __forceinline__ uint32_t store_split(BuildState &s, uint32_t o, const int *bins,
+                                                float *split_out = nullptr) {
     uint32_t dim;
     float split;
     int box[6];
     sah_sweep_wave(bins, s.bounds + o * 6, &dim, &split, box);
     if ((threadIdx.x & 63u) == 0) {
+        if (split_out) *split_out = split;
         SplitInfo si;
         si.dim = dim;
         si.split = split;
@@ -340,6 +343,7 @@ __device__ __forceinline__ void store_split(BuildState &s, uint32_t o, const int
         for (int q = 0; q < 6; ++q) si.box[q] = box[q];
         ((SplitInfo *)s.split)[o] = si;
     }
+    return dim;
 }
 __global__ __launch_bounds__(TOP_THREADS) void k_bin(BuildArgs a, BuildState s) {
   __shared__ int sb[BIN_REPL * BIN_STRIDE];
@@ -500,13 +504,33 @@ __device__ void sah_sweep_wave(const int *bins, const int *bounds, uint32_t *dim
     }
 }
 
-// per-item count of `centroid[dim] < split` (the partition predicate, build.rs:148)
+// per-item count of `centroid[dim] < split` (the partition predicate, build.rs:148).
+// With VRGI_SPLIT_IN_COUNT, the SAH sweep of a node of several items runs here, in
+// wave 0 of every block that counts one of its items, over the node's merged global
+// bins (complete once k_bin has ended): each such block derives the same split and
+// stores the same record, and no separate split launch sits between k_bin and k_count.
+#ifndef VRGI_SPLIT_IN_COUNT
+#define VRGI_SPLIT_IN_COUNT 1
+#endif
 __global__ __launch_bounds__(TOP_THREADS) void k_count(BuildArgs a, BuildState s) {
   __shared__ uint32_t red[TOP_THREADS / 64];
+  __shared__ uint32_t s_dim;
+  __shared__ float s_split;
   const uint32_t n_items = *s.n_items;
   for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     Item it = ((const Item *)s.items)[item];
-    SplitInfo si = ((const SplitInfo *)s.split)[it.open];
+    SplitInfo si;
+    if (VRGI_SPLIT_IN_COUNT && s.item_end[it.open] - s.item_beg[it.open] > 1u) {
+        if (threadIdx.x < 64) {
+            const uint32_t dim = store_split(s, it.open, (const int *)s.bins + (size_t)it.open * BIN_WORDS, &s_split);
+            if (threadIdx.x == 0) s_dim = dim;
+        }
+        __syncthreads();
+        si.dim = s_dim;
+        si.split = s_split;
+    } else {
+        si = ((const SplitInfo *)s.split)[it.open];
+    }
     uint32_t c = 0;
     for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS)
         c += a.pc[si.dim * a.n + it.beg + k] < si.split ? 1u : 0u;
@@ -1266,7 +1290,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
             k_clear_node_data<<<node_grid, 256, 0, stream>>>(st);
             k_bounds<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
             k_bin<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
-            k_split<<<node_grid, 64, 0, stream>>>(st);
+            if (!VRGI_SPLIT_IN_COUNT) k_split<<<node_grid, 64, 0, stream>>>(st);
             k_count<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
             k_part_a<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
             k_part_b<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
