@@ -344,6 +344,9 @@ void cache_step(vrgi_ctx *c, hipStream_t s, const float *rec, const uint32_t *co
 }
 
 // scratch of NEE stream slot k for the enumeration / walk passes of one NEE pdf launch
+#ifndef VRGI_NEE_PASSES_ALL
+#define VRGI_NEE_PASSES_ALL 0
+#endif
 void nee_passes(vrgi_ctx *c, hipStream_t s, uint32_t k, BounceArgs &ba) {
     const size_t P = c->P;
     c->nee_cnt[k].reserve(P);
@@ -1360,7 +1363,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             // passes, whose walks then run at full occupancy instead of in the tail of a
             // few waves that also hold the traversal
             const bool short_q = cache_on && b >= c->cache.cfg.query_vertex;
-            if (short_q) nee_passes(c, ns, b % VRGI_NEE_STREAMS, na);
+            if (short_q || VRGI_NEE_PASSES_ALL) nee_passes(c, ns, b % VRGI_NEE_STREAMS, na);
             rec_on(3 + 2 * (size_t)B + b, ns);
             launch_nee_pdf(ns, sv, na, P, short_q ? VRGI_SMALL_QUEUE_GRID : 0u);
             rec_on(2 + B + b, ns);

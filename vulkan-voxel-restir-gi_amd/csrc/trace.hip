@@ -232,6 +232,33 @@ __device__ const float IDENTITY12[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
 
 struct Pick { bool ok; uint32_t inst, prim; float prob, importance; };
 
+// Importances of the TL light tree's top two levels for one shading point, shared by
+// the many descents that start there (ReSTIR DI's candidates). Records 0 (the root's
+// children), 1 and 2 (the children of the root's left and right child, when inner)
+// each hold li, ri, ln, rn and the two child references; `memo` points at the thread's
+// slot in LDS, MEMO_FIELDS words with a stride of one word per record field.
+static constexpr int MEMO_FIELDS = 18;
+__device__ __forceinline__ void memo_store(float *memo, int rec, const PairImp &q, const LPair &r) {
+    float *m = memo + rec * 6;
+    m[0] = q.li; m[1] = q.ri; m[2] = q.ln; m[3] = q.rn; m[4] = r.c.w; m[5] = r.d.w;
+}
+// fills the memo at (p, n); false when the TL root is a leaf (the walk then reads no
+// TL record, and the plain walk is used)
+__device__ __forceinline__ bool light_memo(const SceneView &sv, f3 p, f3 n, float *memo) {
+    const vrgi_bvh_node tl0 = sv.tl[0];
+    if (tl0.left == VRGI_INVALID) return false;
+    const f3 zero = mk(0.0f, 0.0f, 0.0f);
+    const LPair r0 = ld_pair(sv.tlpairs);
+    memo_store(memo, 0, pair_importance_t(p, n, zero, r0, VRGI_EPS_RT), r0);
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t ref = __float_as_uint(side ? r0.d.w : r0.c.w);
+        if (ref & 0x80000000u) continue;  // a TL leaf: no record below it
+        const LPair r1 = ld_pair(sv.tlpairs + ref);
+        memo_store(memo, 1 + side, pair_importance_t(p, n, zero, r1, VRGI_EPS_RT), r1);
+    }
+    return true;
+}
+
 // traverseBvh, raytrace.rs:218-281. Each level reads one child record: the same
 // boxes and powers as the two child nodes, so the same arithmetic.
 __device__ Pick traverse_light(const SceneView &sv, f3 p, f3 n, uint32_t seed) {
@@ -270,6 +297,66 @@ __device__ Pick traverse_light(const SceneView &sv, f3 p, f3 n, uint32_t seed) {
         uint32_t ref;
         if (mm_finalizef(seed) < q.ln) { ref = __float_as_uint(r.c.w); prob *= q.ln; imp = q.li; }
         else { ref = __float_as_uint(r.d.w); prob *= q.rn; imp = q.ri; }
+        leaf = (ref & 0x80000000u) != 0u;
+        idx = ref & 0x7FFFFFFFu;
+        seed = mm_combine(seed, 0);
+    }
+    return res;
+}
+
+// traverse_light with `memo` holding light_memo at the same p, n: the top two TL levels
+// take the stored importances, which are the values the walk would compute there. A
+// separate copy of the walk keeps the plain one (the shading's NEE pick) compiled as
+// before: folding the memo into it as a template switch, or leaving ReSTIR DI without a
+// call to it, changed its machine code and cost the 1080p frame 0.06 ms.
+__device__ Pick traverse_light_memo(const SceneView &sv, f3 p, f3 n, uint32_t seed, const float *memo) {
+    const vrgi_bvh_node tl0 = sv.tl[0];
+    Pick res{false, 0, 0, 1.0f, 0.0f};
+    if (tl0.left == VRGI_INVALID && tl0.right_or_prim == VRGI_INVALID) return res;  // dummy root
+    float prob = 1.0f, imp = 0.0f;
+    const float *m = IDENTITY12;
+    bool lin = true;               // linear part of m is the identity: offset `off` instead of m
+    f3 off = mk(0.0f, 0.0f, 0.0f);
+    const LPair *pairs = sv.tlpairs;
+    uint32_t inst = VRGI_INVALID;
+    bool top = true;
+    bool leaf = tl0.left == VRGI_INVALID;
+    uint32_t idx = leaf ? tl0.right_or_prim : 0u;  // leaf: primitive / instance, else node index
+    int mrec = 0;  // memo record of the next TL level (3: none)
+    for (int guard = 0; guard < 4096; ++guard) {
+        if (top && leaf) {  // TL leaf: continue in the instance's BL-BVH from its root
+            inst = idx;
+            const DevInst &I = sv.inst[inst];
+            m = I.xf;
+            lin = I.lin_id != 0u;
+            off = mk(I.xf[9], I.xf[10], I.xf[11]);
+            pairs = I.lpairs;
+            top = false;
+            const vrgi_bvh_node bl0 = I.light_bvh[0];
+            if (imp == 0.0f) imp = node_importance(p, n, m, bl0, VRGI_EPS_RT);
+            leaf = bl0.left == VRGI_INVALID;
+            idx = leaf ? bl0.right_or_prim : 0u;
+        }
+        if (!top && leaf) {
+            res.ok = true; res.inst = inst; res.prim = idx; res.prob = prob; res.importance = imp;
+            return res;
+        }
+        uint32_t ref;
+        PairImp q;
+        uint32_t ref_l, ref_r;
+        if (top && mrec < 3) {
+            const float *mm = memo + mrec * 6;
+            q.li = mm[0]; q.ri = mm[1]; q.ln = mm[2]; q.rn = mm[3];
+            ref_l = __float_as_uint(mm[4]); ref_r = __float_as_uint(mm[5]);
+        } else {
+            const LPair r = ld_pair(pairs + idx);
+            q = lin ? pair_importance_t(p, n, off, r, VRGI_EPS_RT) : pair_importance(p, n, m, r, VRGI_EPS_RT);
+            ref_l = __float_as_uint(r.c.w); ref_r = __float_as_uint(r.d.w);
+        }
+        const bool go_left = mm_finalizef(seed) < q.ln;
+        if (go_left) { ref = ref_l; prob *= q.ln; imp = q.li; }
+        else { ref = ref_r; prob *= q.rn; imp = q.ri; }
+        mrec = mrec == 0 ? (go_left ? 1 : 2) : 3;
         leaf = (ref & 0x80000000u) != 0u;
         idx = ref & 0x7FFFFFFFu;
         seed = mm_combine(seed, 0);
@@ -804,7 +891,7 @@ __device__ __forceinline__ f3 di_fr(const SceneView &sv, uint32_t hit) {
 }
 
 // initial candidates + temporal merge with the previous frame (M capped at 20x)
-__device__ __forceinline__ void di_initial_px(const SceneView &sv, const DiArgs &da, uint32_t id) {
+__device__ __forceinline__ void di_initial_px(const SceneView &sv, const DiArgs &da, float *memo, uint32_t id) {
     if (id >= da.P) return;
     DiRes r;
     r.light = VRGI_INVALID; r.y = mk(0, 0, 0); r.wsum = 0.0f; r.W = 0.0f; r.ph = 0.0f; r.M = 0;
@@ -813,9 +900,11 @@ __device__ __forceinline__ void di_initial_px(const SceneView &sv, const DiArgs 
         f3 x = ld3(da.origin + (size_t)id * 3), n = ld3(da.normal + (size_t)id * 3);
         f3 fr = di_fr(sv, hp);
         const uint32_t pd = mm_combine(da.dseed, id);
+        const bool shared = light_memo(sv, x, n, memo);
         for (uint32_t k = 0; k < da.candidates; ++k) {
             uint32_t sk = mm_combine(pd, k);
-            Pick pk = traverse_light(sv, x, n, mm_combine(sk, 0));
+            Pick pk = shared ? traverse_light_memo(sv, x, n, mm_combine(sk, 0), memo)
+                             : traverse_light(sv, x, n, mm_combine(sk, 0));
             float w = 0.0f, ph = 0.0f;
             uint32_t lg = VRGI_INVALID;
             f3 y = mk(0, 0, 0);
@@ -849,7 +938,8 @@ __device__ __forceinline__ void di_initial_px(const SceneView &sv, const DiArgs 
     di_store(da.T + (size_t)id * 8, r);
 }
 __global__ __launch_bounds__(64) void k_di_initial(const SceneView sv, const DiArgs da) {
-    di_initial_px(sv, da, blockIdx.x * 64 + threadIdx.x);
+    __shared__ float s_memo[64 * (MEMO_FIELDS + 1)];
+    di_initial_px(sv, da, s_memo + threadIdx.x * (MEMO_FIELDS + 1), blockIdx.x * 64 + threadIdx.x);
 }
 
 // spatial merge, shadow ray, final radiance, history for the next frame
@@ -946,8 +1036,9 @@ __global__ __launch_bounds__(64 * WPB) void k_nee_pdf_b(const SceneView *__restr
 template <int PB>
 __global__ __launch_bounds__(PB) void k_di_initial_b(const SceneView *__restrict__ svs, const DiArgs *__restrict__ das,
                                                      const uint32_t *__restrict__ blk, uint32_t n) {
+    __shared__ float s_memo[PB * (MEMO_FIELDS + 1)];
     const uint32_t s = batch_scene(blk, n, blockIdx.x);
-    di_initial_px(svs[s], das[s], (blockIdx.x - blk[s]) * PB + threadIdx.x);
+    di_initial_px(svs[s], das[s], s_memo + threadIdx.x * (MEMO_FIELDS + 1), (blockIdx.x - blk[s]) * PB + threadIdx.x);
 }
 template <int STACK, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_di_final_b(const SceneView *__restrict__ svs,
