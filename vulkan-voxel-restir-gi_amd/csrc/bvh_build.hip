@@ -413,7 +413,7 @@ __global__ __launch_bounds__(TOP_THREADS) void k_bin(BuildArgs a, BuildState s) 
     __syncthreads();
     // A node of one item has its complete bins in LDS: its SAH sweep (build.rs:87-127)
     // runs here. The items of a larger node add their bins to the node's global bins,
-    // swept by k_split.
+    // swept by k_count (by k_split when VRGI_SPLIT_IN_COUNT is 0).
     const uint32_t o = it.open;
     if (s.item_end[o] - s.item_beg[o] > 1u) {
         for (int k = threadIdx.x; k < BIN_WORDS; k += TOP_THREADS) {
