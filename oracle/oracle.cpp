@@ -11,7 +11,15 @@
 //   build (A1-A6)            invariants + brute-force plane search + golden node arrays
 //   partition order          "parity unpinned": `partition` 0.1.2 source absent
 //   watertight hit           brute force vs BVH, watertight edge rays, closed forms
-//   shading / NEE / ReSTIR   closed-form scenes, probability sums, reservoir invariants
+//   shading                  closed-form scenes (sky, Lambert floor, mirror)
+//   light-BVH walks          probability sums, chi-square of the forward picks
+//   computeNeePdf            solid-angle integral = 1, barycentric order at known hits,
+//                            NEE-on mean = NEE-off mean
+//   ReSTIR GI                closed-form mean with 0 / 5 neighbours, first-frame weights,
+//                            Z filter (back-facing emitters), Jacobian known answers
+//   ReSTIR DI                mean = Lambert's polygon formula (float64)
+// Each of these checks is shown to fail with a planted mistake (OR_MUT_*, g_mut).
+
 #include "oracle.h"
 
 #include <algorithm>
@@ -122,6 +130,11 @@ inline v3 canon(v3 a) { return V(a.x + 0.0f, a.y + 0.0f, a.z + 0.0f); }
 const float PI_F = 3.1415926535897932384626433832795f;  // M_PI as a GLSL float literal
 const float EPS_RT = 0.001f;                            // raytrace.rs:13
 const float EPS_NEE = 0.0001f;                          // nee_pdf.rs:15
+
+// Planted mistake (OR_MUT_* in oracle.h), 0 = none. Set only by the tests that show a
+// pinning check fails when the restatement is wrong in a plausible way.
+int g_mut = 0;
+inline bool mut(int m) { return g_mut == m; }
 
 // mat4x3 (column-major, 4 columns of xyz) times vec4(p, 1)
 inline v3 xform_point(const float *m, v3 p) {
@@ -822,13 +835,24 @@ float reverse_traverse(const or_scene *s, v3 p, v3 n, uint32_t inst, uint32_t bl
 // world light triangle `lid` (the instance transform of the light vertices applied)
 float nee_pdf_one(const or_scene *s, v3 p, v3 n, v3 w, uint32_t lid, float bx, float by) {
     v3 t0 = tri_v(s->ltri, lid, 0), t1 = tri_v(s->ltri, lid, 1), t2 = tri_v(s->ltri, lid, 2);
-    v3 q = (bx * t0 + by * t1) + ((1.0f - bx) - by) * t2;
+    v3 q = mut(OR_MUT_NEE_BARY) ? (((1.0f - bx) - by) * t0 + bx * t1) + by * t2
+                                : (bx * t0 + by * t1) + ((1.0f - bx) - by) * t2;
     float dist = length(q - p);
-    float area = 0.5f * length(cross(t0 - t1, t2 - t1));
-    float cs = dot(n, w);
-    float ppp = dist * dist / (cs * area);
+    float area = mut(OR_MUT_NEE_AREA) ? length(cross(t0 - t1, t2 - t1))
+                                      : 0.5f * length(cross(t0 - t1, t2 - t1));
+    float cs = mut(OR_MUT_NEE_COS) ? 1.0f : dot(n, w);
+    float ppp = mut(OR_MUT_NEE_DIST) ? dist / (cs * area) : dist * dist / (cs * area);
     float prim_p = reverse_traverse(s, p, n, s->ltri_inst[lid], s->ltri_node[lid]);
     return prim_p * ppp;
+}
+
+// nee_pdf.rs:281-337 for one ray leaving `o` (shading normal n) along d: the sum of
+// computeNeePdf over every light triangle crossed in [1e-4, 1000] (nee_pdf.rs:305-334)
+float nee_pdf_ray(const or_scene *s, v3 o, v3 n, v3 d, std::vector<LHit> &hits) {
+    float np = 0.0f;
+    light_hits(s, o, d, EPS_NEE, 1000.0f, hits);
+    for (const LHit &lh : hits) np += nee_pdf_one(s, o, n, d, lh.id, lh.u, lh.v);
+    return np;
 }
 
 // ---------------------------------------------------------------- per-pixel kernels
@@ -974,6 +998,18 @@ inline void update_res(uint32_t seed, Reservoir &r, const Sample &z, float w) {
     r.m += 1;
     if (fconstruct(seed) < w / r.wsum) r.z = z;
 }
+// computeJacobianQToR (restir_spatial_resampling.rs:299-311): reuse of q's sample at
+// pixel r; sky samples (n_s = 0) give 1. The reference's form: cosine ratio times the
+// ratio of the distances (not their squares).
+inline float gi_jacobian(const Sample &q, const Sample &r) {
+    if (is_zero(q.ns)) return 1.0f;
+    v3 qv = q.xv - q.xs, rv = r.xv - q.xs;
+    float cq = std::fabs(dot(normalize(qv), q.ns));
+    float cr = std::fabs(dot(normalize(rv), q.ns));
+    if (mut(OR_MUT_GI_JAC_SQ)) return (cr / cq) * (dot(qv, qv) / dot(rv, rv));
+    if (mut(OR_MUT_GI_JAC_SWAP)) return (cq / cr) * (length(rv) / length(qv));
+    return (cr / cq) * (length(qv) / length(rv));
+}
 
 }  // namespace
 
@@ -1007,7 +1043,7 @@ inline v3 di_contrib(const or_scene *s, v3 x, v3 n, v3 fr, uint32_t lg, v3 y) {
     float dist = std::sqrt(d2);
     v3 wn = w / dist;
     float cx = std::max(0.0f, dot(n, wn));
-    float cy = std::max(0.0f, -dot(nl, wn));
+    float cy = mut(OR_MUT_DI_COSY) ? 1.0f : std::max(0.0f, -dot(nl, wn));
     const Material &m = s->mats[s->tri_t[lg]];
     v3 le = 1000.0f * V(m.emis[0], m.emis[1], m.emis[2]);
     return (fr * le) * ((cx * cy) / d2);
@@ -1062,7 +1098,7 @@ void restir_di(or_renderer *rr, const or_scene *s, const or_prefs *prefs, uint32
                 v3 bb = V((1.0f - ux) - uy, ux, uy);
                 y = (bb.x * l0 + bb.y * l1) + bb.z * l2;
                 float area = 0.5f * length(cross(l1 - l0, l2 - l0));
-                float psrc = pk.prob / area;
+                float psrc = mut(OR_MUT_DI_AREA) ? pk.prob : pk.prob / area;
                 ph = luminance(di_contrib(s, x, n, fr, lg, y));
                 w = psrc > 0.0f ? ph / psrc : 0.0f;
             }
@@ -1079,7 +1115,7 @@ void restir_di(or_renderer *rr, const or_scene *s, const or_prefs *prefs, uint32
             di_update(finalize(combine(pd, 0x10000u)), r, hl, hy, (ph * h[4]) * (float)hm, ph);
             r.M = m0 + hm;
         }
-        r.W = r.ph > 0.0f ? r.wsum / ((float)r.M * r.ph) : 0.0f;
+        r.W = r.ph > 0.0f ? (mut(OR_MUT_DI_UCW_NO_M) ? r.wsum / r.ph : r.wsum / ((float)r.M * r.ph)) : 0.0f;
         T[id] = r;
     }
     // spatial merge, visibility, final radiance, history
@@ -1110,7 +1146,7 @@ void restir_di(or_renderer *rr, const or_scene *s, const or_prefs *prefs, uint32
             float ph = luminance(di_contrib(s, x, n, fr, rq.light, rq.y));
             uint32_t m0 = r.M;
             di_update(finalize(combine(is, 2)), r, rq.light, rq.y, (ph * rq.W) * (float)rq.M, ph);
-            r.M = m0 + rq.M;
+            r.M = mut(OR_MUT_DI_SPATIAL_M) ? m0 + 1u : m0 + rq.M;
         }
         r.W = r.ph > 0.0f ? r.wsum / ((float)r.M * r.ph) : 0.0f;
         v3 L = E0;
@@ -1199,13 +1235,8 @@ void trace_pixel(const or_scene *s, const Cfg &c, uint32_t nee_type, uint32_t fr
         v3 d = ld3(&f.dir[bo * 3]);
         float w = f.mis[bi];
         float np = 0.0f;
-        if (!(length(d) == 0.0f || w == 0.0f)) {
-            v3 o = ld3(&f.origin[bo * 3]);
-            v3 n = ld3(&f.normal[bi * 3]);
-            light_hits(s, o, d, EPS_NEE, 1000.0f, hits);
-            for (const LHit &lh : hits)
-                np += nee_pdf_one(s, o, n, d, lh.id, lh.u, lh.v);
-        }
+        if (!(length(d) == 0.0f || w == 0.0f))
+            np = nee_pdf_ray(s, ld3(&f.origin[bo * 3]), ld3(&f.normal[bi * 3]), d, hits);
         f.nee[bi] = np;
     }
     // outgoing_radiance.rs:58-94
@@ -1494,6 +1525,24 @@ float or_reverse_traverse_light_bvh(or_scene *s, const float *p, const float *n,
     return reverse_traverse(s, ld3(p), ld3(n), inst, bl_node);
 }
 
+void or_set_mutation(int m) { g_mut = m; }
+int or_get_mutation(void) { return g_mut; }
+
+int or_nee_pdf(or_scene *s, uint64_t n, const float *o, const float *nrm, const float *d, float *out) {
+#pragma omp parallel
+    {
+        std::vector<LHit> hits;
+#pragma omp for schedule(static)
+        for (long i = 0; i < (long)n; ++i)
+            out[i] = nee_pdf_ray(s, ld3(o + 3 * i), ld3(nrm + 3 * i), ld3(d + 3 * i), hits);
+    }
+    return 0;
+}
+
+float or_restir_gi_jacobian(const float *q_res, const float *r_res) {
+    return gi_jacobian(res_load(q_res).z, res_load(r_res).z);
+}
+
 or_renderer *or_renderer_create(const or_config *cfg) {
     or_renderer *r = new or_renderer();
     r->c = Cfg{cfg->width, cfg->height, cfg->num_bounces, cfg->rr_start_bounce,
@@ -1558,7 +1607,7 @@ static int render_impl(or_renderer *rr, or_scene *s, const or_camera *cam, const
         float w = luminance(S.lo) / S.pw;
         update_res(finalize(combine(ps, 1)), R, S, w);
         float ph = luminance(R.z.lo);
-        R.ucw = ph > 0.0f ? R.wsum / ((float)R.m * ph) : 0.0f;
+        R.ucw = ph > 0.0f ? (mut(OR_MUT_GI_UCW_NO_M) ? R.wsum / ph : R.wsum / ((float)R.m * ph)) : 0.0f;
         res_store(&tr[id * 20], R);
     }
     // spatial (restir_spatial_resampling.rs:316-406)
@@ -1582,25 +1631,17 @@ static int render_impl(or_renderer *rr, or_scene *s, const or_camera *cam, const
             // geometricallySimilar, :277-288
             if (is_zero(S.nv) || is_zero(Rn.z.nv)) continue;
             if (dot(S.nv, Rn.z.nv) < cos25) continue;
-            // computeJacobianQToR, :299-311
-            float J;
-            if (is_zero(Rn.z.ns)) J = 1.0f;
-            else {
-                v3 qv = Rn.z.xv - Rn.z.xs, rv = S.xv - Rn.z.xs;
-                float cq = std::fabs(dot(normalize(qv), Rn.z.ns));
-                float cr = std::fabs(dot(normalize(rv), Rn.z.ns));
-                J = (cr / cq) * (length(qv) / length(rv));
-            }
-            J = std::min(std::max(J, 0.1f), 10.0f);
+            float J = std::min(std::max(gi_jacobian(Rn.z, S), 0.1f), 10.0f);
             float padj = luminance(Rn.z.lo) / J;
             uint32_t m0 = Rs.m;  // mergeReservoir, :266-275
-            update_res(finalize(combine(is, 2)), Rs, Rn.z, (padj * Rn.ucw) * (float)Rn.m);
+            update_res(finalize(combine(is, 2)), Rs, Rn.z,
+                       mut(OR_MUT_GI_MERGE_NO_M) ? padj * Rn.ucw : (padj * Rn.ucw) * (float)Rn.m);
             Rs.m = m0 + Rn.m;
             Q.push_back(Rn);
         }
         uint32_t Z = 0;
         for (const Reservoir &r : Q)
-            if (luminance(r.z.lo) > 0.0f) Z += r.m;
+            if (luminance(r.z.lo) > 0.0f || mut(OR_MUT_GI_Z_ALL)) Z += r.m;
         float ph = luminance(Rs.z.lo);
         Rs.ucw = ph == 0.0f ? 0.0f : Rs.wsum / ((float)Z * ph);
         res_store(&sp[id * 20], Rs);
@@ -1610,7 +1651,7 @@ static int render_impl(or_renderer *rr, or_scene *s, const or_camera *cam, const
     for (size_t id = 0; id < P; ++id) {
         Reservoir R = res_load(&sp[id * 20]);
         v3 dir = normalize(R.z.xs - ld3(&f.origin[(1 * P + id) * 3]));
-        float bp = dot(dir, R.z.nv) / PI_F;
+        float bp = mut(OR_MUT_GI_FINAL_COS) ? 1.0f / PI_F : dot(dir, R.z.nv) / PI_F;
         v3 L = ld3(&f.emis[id * 3]) + ((ld3(&f.refl[id * 3]) * R.z.lo) * R.ucw) * bp;
         st3(&fin[id * 3], L);
         float dv = R.ucw / 10.0f;

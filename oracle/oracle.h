@@ -69,6 +69,34 @@ int or_traverse_light_bvh(or_scene *s, const float *point, const float *normal, 
 float or_reverse_traverse_light_bvh(or_scene *s, const float *point, const float *normal,
                                     uint32_t inst, uint32_t bl_node);
 
+// nee_pdf.rs:281-337 for n rays: origin o, shading normal nrm, direction d (the sum of
+// computeNeePdf over every light triangle the ray crosses), the function frames use
+int or_nee_pdf(or_scene *s, uint64_t n, const float *o, const float *nrm, const float *d, float *out);
+// computeJacobianQToR (restir_spatial_resampling.rs:299-311) of two 20-word reservoirs
+float or_restir_gi_jacobian(const float *q_res, const float *r_res);
+
+// Planted mistakes, for the tests that show each pinning check fails on a plausible
+// error in the restatement (0 = the faithful restatement).
+enum {
+    OR_MUT_NONE = 0,
+    OR_MUT_NEE_AREA = 1,      // computeNeePdf: triangle area without the 1/2
+    OR_MUT_NEE_DIST = 2,      // computeNeePdf: d instead of d^2
+    OR_MUT_NEE_BARY = 3,      // computeNeePdf: raytrace's barycentric order (1-x-y, x, y)
+    OR_MUT_NEE_COS = 4,       // computeNeePdf: no cosine
+    OR_MUT_GI_UCW_NO_M = 5,   // temporal W = w_sum / p_hat (M left out)
+    OR_MUT_GI_MERGE_NO_M = 6, // spatial merge weight without M_n
+    OR_MUT_GI_Z_ALL = 7,      // spatial Z counts every reservoir (no p_hat > 0 filter)
+    OR_MUT_GI_FINAL_COS = 8,  // finalize without the cosine
+    OR_MUT_GI_JAC_SQ = 9,     // Jacobian with squared distances
+    OR_MUT_GI_JAC_SWAP = 10,  // Jacobian of r -> q instead of q -> r
+    OR_MUT_DI_AREA = 11,      // DI source pdf without the 1/area
+    OR_MUT_DI_COSY = 12,      // DI contribution without the light's cosine
+    OR_MUT_DI_UCW_NO_M = 13,  // DI W = w_sum / p_hat (M left out)
+    OR_MUT_DI_SPATIAL_M = 14, // DI spatial merge adds 1 to M instead of M_n
+};
+void or_set_mutation(int m);
+int or_get_mutation(void);
+
 typedef struct or_config {
     uint32_t width, height, num_bounces, rr_start_bounce;
     float rr_min_survival;

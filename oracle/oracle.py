@@ -43,6 +43,7 @@ def lib():
         L.or_nrc_hash.restype = C.c_uint32
         L.or_nrc_grads.restype = C.c_double
         L.or_nrc_train_step.restype = C.c_double
+        L.or_restir_gi_jacobian.restype = C.c_float
         _lib = L
     return _lib
 
@@ -51,6 +52,36 @@ def ptr(a):
     """Pointer argument that keeps `a` alive until the call returns (data_as holds a
     reference to the array, so temporaries built in the call expression stay valid)."""
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# planted mistakes (oracle.h OR_MUT_*): used only to show that a pinning check fails
+MUTATIONS = {"nee_area": 1, "nee_dist": 2, "nee_bary": 3, "nee_cos": 4, "gi_ucw_no_m": 5, "gi_merge_no_m": 6,
+             "gi_z_all": 7, "gi_final_cos": 8, "gi_jac_sq": 9, "gi_jac_swap": 10, "di_area": 11, "di_cosy": 12,
+             "di_ucw_no_m": 13, "di_spatial_m": 14}
+
+
+class mutation:
+    """Context manager planting one mistake in the restatement for the duration of a
+    block (always removed on exit)."""
+
+    def __init__(self, name):
+        self.m = MUTATIONS[name] if name else 0
+
+    def __enter__(self):
+        lib().or_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mutation(0)
+        return False
+
+
+def gi_jacobian(q_res, r_res):
+    """computeJacobianQToR of two 20-word reservoirs (q's sample reused at r)."""
+    q = np.ascontiguousarray(q_res, np.float32)
+    r = np.ascontiguousarray(r_res, np.float32)
+    assert q.size == 20 and r.size == 20
+    return lib().or_restir_gi_jacobian(ptr(q), ptr(r))
 
 
 def combine(h, k):
@@ -233,6 +264,15 @@ class Scene:
         self.L.or_trace_lights(self.h, ptr(np.asarray(o, np.float32)), ptr(np.asarray(d, np.float32)),
                                C.c_float(tmin), C.c_float(tmax), ptr(hits), cap, C.byref(n), ptr(uvs))
         return hits[:n.value], uvs[:n.value]
+
+    def nee_pdf(self, o, nrm, d):
+        """Per ray: the sum of computeNeePdf over every light triangle it crosses."""
+        o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(np.broadcast_to(np.asarray(nrm, np.float32), o.shape))
+        d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+        out = np.zeros(len(o), np.float32)
+        assert self.L.or_nee_pdf(self.h, C.c_uint64(len(o)), ptr(o), ptr(nrm), ptr(d), ptr(out)) == 0
+        return out
 
     def traverse_light_bvh(self, p, n, seed):
         inst, prim = C.c_uint32(), C.c_uint32()
