@@ -132,9 +132,12 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials,
                 const float *texture_luminances, uint32_t n_slots, int device, vrgi_ctx **out);
 int vrgi_destroy(vrgi_ctx *ctx);
 
-/* SceneUploader::upload_object (scene.rs:547-725). Thread-safe. n_verts == 0
- * yields the Empty handle (*out == NULL, status 0); n_verts % 3 != 0 is VRGI_E_INVALID.
- * Builds the object's light BL-BVH (build_bl_bvh over its emissive triangles). */
+/* SceneUploader::upload_object (scene.rs:547-725). Thread-safe: concurrent calls run
+ * side by side, each on its own stream and builder workspace (the reference uploads from
+ * a pool of threads, chunk_manager.rs:214-258); it must not race with vrgi_destroy.
+ * n_verts == 0 yields the Empty handle (*out == NULL, status 0); n_verts % 3 != 0 is
+ * VRGI_E_INVALID. Builds the object's light BL-BVH (build_bl_bvh over its emissive
+ * triangles). An object may be released after its context is destroyed. */
 int vrgi_upload_object(vrgi_ctx *ctx, const vrgi_vertex3d *verts, size_t n_verts,
                        vrgi_object **out);
 int vrgi_object_retain(vrgi_object *obj);
@@ -143,14 +146,21 @@ int vrgi_object_release(vrgi_object *obj);
 int vrgi_object_light_bvh(vrgi_object *obj, vrgi_bvh_node *dst, size_t cap, size_t *n_nodes);
 
 /* Scene::add_object / update_object / remove_object (scene.rs:150-225).
- * transform: InstanceData.transform, mat4x3 column-major (4 columns of xyz). */
+ * transform: InstanceData.transform, mat4x3 column-major (4 columns of xyz).
+ * add: an object uploaded through another context is VRGI_E_INVALID; a replaced object
+ * is kept until vrgi_dispose_old. update: unknown key is VRGI_E_INVALID (the reference
+ * panics, scene.rs:215). remove: removed objects are kept until vrgi_dispose_old. */
 int vrgi_scene_add(vrgi_ctx *ctx, uint32_t key, vrgi_object *obj, const float transform[12]);
 int vrgi_scene_update(vrgi_ctx *ctx, uint32_t key, const float transform[12]);
 int vrgi_scene_remove(vrgi_ctx *ctx, uint32_t key);
 /* Scene::dispose_old_objects (scene.rs:227). */
 int vrgi_dispose_old(vrgi_ctx *ctx);
 
-/* Renderer::rebuild(extent) (interactive_rendering.rs:818). */
+/* Renderer::rebuild(extent) (interactive_rendering.rs:818). Frame buffers and ReSTIR
+ * histories restart at the new extent (so does FP64 accumulation when it is on). A zero
+ * width or height marks the context minimised: vrgi_render then draws nothing and
+ * returns VRGI_OK with every field of its output zeroed until the next non-zero extent
+ * (interactive_rendering.rs:978-981). */
 int vrgi_resize(vrgi_ctx *ctx, uint32_t width, uint32_t height);
 
 /* Renderer::render (interactive_rendering.rs:952-1669): (re)builds the BVHs when

@@ -1,4 +1,4 @@
-"""GPU exposed-face mesher (chunk.rs:158-292) against the CPU mesher, vertex for vertex
+"""GPU exposed-face mesher (chunk.rs:158-292) against the oracle's mesher, vertex for vertex
 (the primitive order every BVH is built over), on both benchmark worlds and on a world
 with translucent blocks and edits."""
 import numpy as np
@@ -10,12 +10,16 @@ pytestmark = pytest.mark.gpu
 
 
 def _check_world(blocks, nt, tr, chunks):
+    """GPU mesher output == the oracle's restatement of chunk::gen_mesh, and == the
+    library's own CPU mesher (the caller-side chunk path), chunk by chunk."""
+    import oracle
     import vrgi
     with vrgi.Context(vrgi.config(16, 16), *vrgi.world_materials(0, 0)) as c:
         got = c.mesh_chunks(blocks, nt, tr, chunks)
     for (cx, cy, cz), g in zip(chunks, got):
-        ref = vrgi.gen_mesh_chunk(blocks, nt, tr, cx, cy, cz)
+        ref = oracle.gen_mesh_chunk(blocks, nt, tr, cx, cy, cz)
         assert g.tobytes() == ref.tobytes(), (cx, cy, cz)
+        assert vrgi.gen_mesh_chunk(blocks, nt, tr, cx, cy, cz).tobytes() == ref.tobytes(), (cx, cy, cz)
     return sum(len(g) for g in got)
 
 

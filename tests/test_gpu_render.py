@@ -250,3 +250,25 @@ def test_restir_di_parity(world0, scenes):
             assert gw.tobytes() == ref["di_weights"].tobytes()
             assert c.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
             assert (gl != 0xFFFFFFFF).mean() > 0.1
+
+
+@pytest.mark.parametrize("view", [1, 2])
+def test_debug_views_parity(world0, scenes, view):
+    """The postprocess views (postprocess.rs:39-88): 1 = ReSTIR radiance, 2 = the debug
+    buffer (spatial W / 10), as BGRA8 bit-exact against the oracle over 3 frames."""
+    import oracle
+    import vrgi
+    s, objs = scenes
+    W, H, B = 96, 64, 3
+    cam = _camera0()
+    r = oracle.Renderer(W, H, B)
+    with vrgi.Context(vrgi.config(W, H, B), world0["mats"], world0["lum"]) as c:
+        c.add_objects(objs)
+        for f in range(3):
+            say(f"debug view {view} frame {f}")
+            c.render(cam, nee_type=1, debug_view=view, spatial=5)
+            c.sync()
+            ref = r.render(s, cam, f, nee_type=1, debug_view=view, spatial=5)
+            img = c.read(vrgi.BUF_BGRA8, np.uint8)
+            assert img.tobytes() == ref["bgra8"].tobytes()
+            assert len(np.unique(img.reshape(-1, 4)[:, :3])) > 4

@@ -121,3 +121,66 @@ def test_voxels_in_batch(world0):
             a, b = _bufs(g), _bufs(r)
             for k in ("bvh", "light_bvh", "tl", "radiance", "restir", "temporal"):
                 assert a[k] == b[k], k
+
+
+def _oracle_chunks(blocks, nt, tr):
+    """chunk_objects with the oracle's mesher: (key, transform, vertices) per 32^3 chunk."""
+    import oracle
+    nx, ny, nz = (s // 32 for s in blocks.shape)
+    out = []
+    for cx in range(nx):
+        for cy in range(ny):
+            for cz in range(nz):
+                v = oracle.gen_mesh_chunk(blocks, nt, tr, cx, cy, cz)
+                xf = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 32 * cx, 32 * cy, 32 * cz], np.float32)
+                out.append(((cx * ny + cy) * nz + cz, xf, v))
+    return out
+
+
+@pytest.mark.parametrize("mode,cache", [(0, False), (1, False), (1, True)])
+def test_voxel_edits_vs_oracle(world0, mode, cache):
+    """The dynamic workload's path on the 64^3 world: random-walk toggles through the
+    device voxel world (re-mesh of the dirty chunks, light gather, rebuild), 4 bounces with
+    Russian roulette from bounce 2, NEE, ReSTIR GI or DI, with or without the learned
+    cache; every frame equals the oracle's render of the edited world (its chunks meshed by
+    the oracle's mesher), given the GPU's cache radiance in cache mode."""
+    import oracle
+    import vrgi
+    W, H, B, RR = 96, 64, 4, 2
+    frac, seed = 0.25, 3
+    blocks = world0["blocks"].copy()
+    nt = world0["n_types"]
+    walkers = np.array([[10, 40, 10], [31, 20, 33], [50, 30, 5], [40, 50, 60]], np.uint32)
+    cfg = vrgi.config(W, H, B, rr_start=RR, mode=mode, radius=30.0 if mode else 15.0)
+    r = oracle.Renderer(W, H, B, rr_start=RR, mode=mode, radius=30.0 if mode else 15.0)
+    cam = _cam()
+    with vrgi.Context(cfg, world0["mats"], world0["lum"]) as g:
+        g.voxels_load(blocks, nt, world0["tr"])
+        if cache:
+            g.cache_enable(box_lo=(0, 0, 0), box_hi=(0, 0, 0), train_fraction=frac, train_bounces=B, batch=2048,
+                           steps_per_frame=2, seed=seed, query_vertex=0)
+        for f in range(4):
+            if f > 0:
+                xyz, vals = vrgi.world_walk(blocks, nt, 2000 + f, walkers, 3000)
+                g.voxels_set(xyz, vals)
+            say(f"edited world vs oracle: mode {mode}, cache {cache}, frame {f}")
+            g.render(cam, nee_type=1, spatial=5)
+            g.sync()
+            s = oracle.Scene(world0["mats"], world0["lum"])
+            s.add_objects(_oracle_chunks(blocks, nt, world0["tr"]))
+            s.build()
+            if cache:
+                crad = g.read(vrgi.BUF_CACHE_RADIANCE).reshape(W * H, 3)
+                ref = r.render_cache(s, cam, f, crad, frac, oracle.frame_seed(seed, f, 5), nee_type=1, spatial=5,
+                                     query_vertex=0)
+            else:
+                ref = r.render(s, cam, f, nee_type=1, spatial=5)
+            assert g.read(vrgi.BUF_GEOM_BVH, np.uint8).tobytes() == s.geom_bvh().tobytes()
+            assert g.read(vrgi.BUF_RADIANCE).tobytes() == ref["l_out"][0].tobytes()
+            assert g.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
+            assert g.read(vrgi.BUF_BGRA8, np.uint8).tobytes() == ref["bgra8"].tobytes()
+            if mode == 1:
+                assert g.read(vrgi.BUF_DI_LIGHT, np.uint32).tobytes() == ref["di_light"].tobytes()
+                assert g.read(vrgi.BUF_DI_WEIGHTS).tobytes() == ref["di_weights"].tobytes()
+            else:
+                assert g.read(vrgi.BUF_TEMPORAL_RESERVOIR).tobytes() == ref["temporal"].tobytes()

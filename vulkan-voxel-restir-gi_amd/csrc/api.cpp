@@ -68,11 +68,29 @@ struct Counters {
 
 namespace {
 struct VxArenaPool;  // voxel commits' arenas for reuse (below)
+
+// The device and frame streams of a context, shared with its objects: an object may be
+// released after its context is destroyed (a handle dropped late), and then finds
+// `alive` false instead of a destroyed stream.
+struct CtxLife {
+    std::mutex mu;
+    int device = 0;
+    hipStream_t streams[2] = {nullptr, nullptr};  // frame stream, build stream
+    bool alive = true;
+};
+
+// One upload in flight: its own stream and builder workspace, so that uploads from
+// several threads run side by side (the reference uploads from a 15-thread pool,
+// chunk_manager.rs:214-258).
+struct UpSlot {
+    hipStream_t s = nullptr;
+    Builder b;
+};
 }  // namespace
 
 struct vrgi_object {
     std::atomic<int> refs{1};
-    vrgi_ctx *ctx = nullptr;
+    std::shared_ptr<CtxLife> life;  // the uploading context's device and streams
     uint32_t n_tris = 0;
     vrgi_vertex3d *d_verts = nullptr;
     uint32_t n_light = 0;
@@ -102,7 +120,11 @@ struct vrgi_ctx {
     std::vector<hipEvent_t> ev_bounce;  // trace (and sort) of bounce b done
     hipEvent_t ev_query = nullptr, ev_infer = nullptr, ev_train = nullptr, ev_accum = nullptr;
     hipEvent_t ev_fstart = nullptr, ev_snap = nullptr, ev_held = nullptr;
-    std::mutex up_mu;
+    std::shared_ptr<CtxLife> life = std::make_shared<CtxLife>();
+    std::mutex up_mu;                 // guards up_slots / up_idle
+    std::vector<UpSlot *> up_slots;   // every upload slot made so far
+    std::vector<UpSlot *> up_idle;    // the ones no upload holds
+    bool minimized = false;           // zero extent: render() draws nothing (interactive_rendering.rs:978-981)
     vrgi_config cfg{};
     uint32_t W = 0, H = 0, P = 0, B = 0;
     std::vector<vrgi_material> mats;
@@ -113,7 +135,7 @@ struct vrgi_ctx {
     std::vector<vrgi_object *> old;
     bool inst_dirty = true, geo_dirty = true;
 
-    Builder geo_builder, up_builder, tl_builder, api_builder;
+    Builder geo_builder, tl_builder, api_builder;
 
     // instances
     std::vector<vrgi_object *> inst_objs;
@@ -207,9 +229,10 @@ struct vrgi_ctx {
         DevBuf<uint32_t> counts;  // [0] training records, [1] query records, [2] caller records
     } cache;
 
-    // FP64 running sum of the radiance output (vrgi_accum_*)
+    // FP64 running sum of the radiance output (vrgi_accum_*) over accum_P pixels
     bool accum_on = false;
     uint64_t accum_frames = 0;
+    size_t accum_P = 0;
     DevBuf<double> accum;
 
     // device-resident voxel world (vrgi_voxels_*): grid, per-type data, pending edits
@@ -637,9 +660,13 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s) {
 }
 
 void free_object(vrgi_object *o) {
-    if (o->ctx) {
-        (void)hipSetDevice(o->ctx->device);
-        (void)hipStreamSynchronize(o->ctx->stream);
+    if (o->life) {
+        // frames (and the next frame's build) that may still read the object end first
+        std::lock_guard<std::mutex> lk(o->life->mu);
+        (void)hipSetDevice(o->life->device);
+        if (o->life->alive)
+            for (hipStream_t st : o->life->streams)
+                if (st) (void)hipStreamSynchronize(st);
     }
     if (!o->arena) {
         if (o->d_verts) (void)hipFree(o->d_verts);
@@ -812,7 +839,7 @@ void voxels_commit(vrgi_ctx *c) {
         vrgi_object *o = nullptr;
         if (faces[i]) {
             o = new vrgi_object();
-            o->ctx = c;
+            o->life = c->life;
             o->arena = A;
             o->n_tris = 2 * faces[i];
             o->d_verts = A->verts.p + voff[i];
@@ -866,6 +893,18 @@ vrgi_ctx::~vrgi_ctx() {
     if (h_qcount) (void)hipHostFree(h_qcount);
     for (auto &kv : objects) release(kv.second.first);
     for (vrgi_object *o : old) release(o);
+    {
+        // objects still held by the caller no longer wait on this context's streams
+        std::lock_guard<std::mutex> lk(life->mu);
+        life->alive = false;
+    }
+    for (UpSlot *u : up_slots) {
+        if (u->s) {
+            (void)hipStreamSynchronize(u->s);
+            (void)hipStreamDestroy(u->s);
+        }
+        delete u;
+    }
     if (stream) (void)hipStreamDestroy(stream);
     if (up_stream) (void)hipStreamDestroy(up_stream);
     if (side) (void)hipStreamSynchronize(side);
@@ -923,6 +962,9 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
         HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        c->life->device = device;
+        c->life->streams[0] = c->stream;
+        c->life->streams[1] = c->up_stream;
         for (int k = 0; k < VRGI_NEE_STREAMS; ++k) {
             HIP_CHECK(hipStreamCreateWithFlags(&c->nee_s[k], hipStreamNonBlocking));
             HIP_CHECK(hipEventCreateWithFlags(&c->ev_nee_done[k], hipEventDisableTiming));
@@ -978,11 +1020,32 @@ int vrgi_upload_object(vrgi_ctx *c, const vrgi_vertex3d *v, size_t n, vrgi_objec
     if (n / 3 > (1u << 30)) return fail(VRGI_E_LIMIT, "more than 2^30 triangles in one object");
     for (size_t i = 0; i < n; i += 3)
         if (v[i].t >= c->lum.size()) return fail(VRGI_E_INVALID, "material slot out of range");
-    std::lock_guard<std::mutex> lk(c->up_mu);
-    set_dev(c);
-    hipStream_t s = c->up_stream;
+    HIP_CHECK(hipSetDevice(c->device));
+    // an idle upload slot (a new one when every slot is busy): no lock is held while the
+    // upload runs, so uploads from several threads overlap on their own streams
+    UpSlot *slot = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(c->up_mu);
+        if (!c->up_idle.empty()) {
+            slot = c->up_idle.back();
+            c->up_idle.pop_back();
+        } else {
+            slot = new UpSlot();
+            c->up_slots.push_back(slot);
+            HIP_CHECK(hipStreamCreateWithFlags(&slot->s, hipStreamNonBlocking));
+        }
+    }
+    struct SlotBack {
+        vrgi_ctx *c;
+        UpSlot *u;
+        ~SlotBack() {
+            std::lock_guard<std::mutex> lk(c->up_mu);
+            c->up_idle.push_back(u);
+        }
+    } slot_back{c, slot};
+    hipStream_t s = slot->s;
     vrgi_object *o = new vrgi_object();
-    o->ctx = c;
+    o->life = c->life;
     try {
         o->n_tris = (uint32_t)(n / 3);
         dev_alloc((void **)&o->d_verts, n * sizeof(vrgi_vertex3d));
@@ -1028,7 +1091,7 @@ int vrgi_upload_object(vrgi_ctx *c, const vrgi_vertex3d *v, size_t n, vrgi_objec
             HIP_CHECK(hipMemcpyAsync(dids.p, ids.data(), m * 4, hipMemcpyHostToDevice, s));
             HIP_CHECK(hipMemcpyAsync(dtrees.p, trees, 12, hipMemcpyHostToDevice, s));
             launch_prims_from_tris(s, dtri.p, m, dprim9.p);
-            if (c->up_builder.build(s, dprim9.p, dpw.p, dids.p, m, 1, dtrees.p, dtrees.p + 1, dtrees.p + 2,
+            if (slot->b.build(s, dprim9.p, dpw.p, dids.p, m, 1, dtrees.p, dtrees.p + 1, dtrees.p + 2,
                                     trees + 2, o->d_light_bvh, ddepth.p, nullptr) != 0)
                 throw HipError("light build exceeded its workspace");
             dev_alloc((void **)&o->d_lpairs, (2 * (size_t)m - 1) * sizeof(LPair));
@@ -1092,7 +1155,7 @@ int vrgi_object_light_bvh(vrgi_object *o, vrgi_bvh_node *dst, size_t cap, size_t
     API_BEGIN
     if (!o || !n_nodes) return fail(VRGI_E_INVALID, "null argument");
     if (o->h_light_bvh.empty() && o->n_light && o->d_light_bvh) {  // an object of a voxel commit: read on demand
-        HIP_CHECK(hipSetDevice(o->ctx->device));
+        HIP_CHECK(hipSetDevice(o->life->device));
         o->h_light_bvh.resize(2 * (size_t)o->n_light - 1);
         HIP_CHECK(hipMemcpy(o->h_light_bvh.data(), o->d_light_bvh, o->h_light_bvh.size() * sizeof(vrgi_bvh_node),
                             hipMemcpyDeviceToHost));
@@ -1109,6 +1172,7 @@ int vrgi_object_light_bvh(vrgi_object *o, vrgi_bvh_node *dst, size_t cap, size_t
 int vrgi_scene_add(vrgi_ctx *c, uint32_t key, vrgi_object *o, const float xf[12]) {
     API_BEGIN
     if (!c || !valid_xf(xf)) return fail(VRGI_E_INVALID, "null context or non-finite transform");
+    if (o && o->life != c->life) return fail(VRGI_E_INVALID, "the object was uploaded through another context");
     auto it = c->objects.find(key);
     if (it != c->objects.end()) {
         if (it->second.first) c->old.push_back(it->second.first);
@@ -1151,7 +1215,9 @@ int vrgi_dispose_old(vrgi_ctx *c) {
     API_BEGIN
     if (!c) return fail(VRGI_E_INVALID, "null context");
     set_dev(c);
+    // the frames in flight and the next frame's build (on the build stream) end first
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->up_stream));
     for (vrgi_object *o : c->old) release(o);
     c->old.clear();
     return VRGI_OK;
@@ -1161,17 +1227,30 @@ int vrgi_dispose_old(vrgi_ctx *c) {
 int vrgi_resize(vrgi_ctx *c, uint32_t w, uint32_t h) {
     API_BEGIN
     if (!c) return fail(VRGI_E_INVALID, "null context");
-    if (w == 0 || h == 0) return VRGI_OK;  // zero extent: frames are skipped (interactive_rendering.rs:978-981)
-    if ((uint64_t)w * h > (1ull << 26)) return fail(VRGI_E_LIMIT, "extent above 2^26 pixels");
     if (c->in_batch) return fail(VRGI_E_STATE, "the scene is in a batch (its extent is fixed there)");
+    if (w == 0 || h == 0) {
+        // a minimised window: render() draws nothing until the next non-zero extent
+        // (interactive_rendering.rs:978-981); the buffers of the last extent are kept
+        c->minimized = true;
+        return VRGI_OK;
+    }
+    if ((uint64_t)w * h > (1ull << 26)) return fail(VRGI_E_LIMIT, "extent above 2^26 pixels");
     set_dev(c);
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->minimized = false;
     c->W = w;
     c->H = h;
     c->P = w * h;
     c->cfg.width = w;
     c->cfg.height = h;
     alloc_frame(c);
+    if (c->accum_on) {
+        // the FP64 sums restart at the new extent
+        c->accum.reserve((size_t)c->P * 3);
+        HIP_CHECK(hipMemsetAsync(c->accum.p, 0, (size_t)c->P * 24, c->stream));
+        c->accum_P = c->P;
+        c->accum_frames = 0;
+    }
     HIP_CHECK(hipStreamSynchronize(c->stream));
     return VRGI_OK;
     API_END
@@ -1182,6 +1261,10 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     if (!c || !cam || !prefs) return fail(VRGI_E_INVALID, "null argument");
     if (prefs->nee_type > 2) return fail(VRGI_E_INVALID, "nee_type must be 0, 1 or 2");
     if (prefs->restir_spatial_iterations > 20) return fail(VRGI_E_INVALID, "restir_spatial_iterations above 20");
+    if (c->minimized) {  // zero extent: no frame (interactive_rendering.rs:978-981)
+        if (out) std::memset(out, 0, sizeof(*out));
+        return VRGI_OK;
+    }
     set_dev(c);
     hipStream_t s = c->stream;
     const uint32_t P = c->P, B = c->B;
@@ -2193,6 +2276,7 @@ int vrgi_accum_enable(vrgi_ctx *c, int on) {
     if (c->accum_on) {
         c->accum.reserve((size_t)c->P * 3);
         HIP_CHECK(hipMemsetAsync(c->accum.p, 0, (size_t)c->P * 24, c->stream));
+        c->accum_P = c->P;
     }
     return VRGI_OK;
     API_END
@@ -2202,6 +2286,7 @@ int vrgi_accum_read(vrgi_ctx *c, double *mean, uint64_t *frames) {
     API_BEGIN
     if (!c || !mean) return fail(VRGI_E_INVALID, "null argument");
     if (!c->accum.p) return fail(VRGI_E_STATE, "accumulation was never enabled");
+    if (c->accum_P != c->P) return fail(VRGI_E_STATE, "the extent changed since accumulation was enabled");
     set_dev(c);
     HIP_CHECK(hipMemcpyAsync(mean, c->accum.p, (size_t)c->P * 24, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));

@@ -27,6 +27,7 @@ assert VERTEX_DTYPE.itemsize == 24 and NODE_DTYPE.itemsize == 40 and MATERIAL_DT
 CACHE_FREEZE = 1
 CACHE_WEIGHTS = 4 * 64 * 64 + 64 * 16
 
+VRGI_OK, VRGI_E_INVALID, VRGI_E_DEVICE, VRGI_E_STATE, VRGI_E_LIMIT = 0, -1, -2, -3, -4
 FLAG_REBUILD_EVERY_FRAME = 1
 FLAG_KEEP_BOUNCES = 2
 FLAG_SERIAL_NEE = 4
