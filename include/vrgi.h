@@ -308,6 +308,18 @@ int vrgi_build_bl_bvh(vrgi_ctx *ctx, const float *tri_xyz, const float *prim_lum
                       const uint32_t *prim_index_ids, uint64_t n, vrgi_bvh_node *nodes,
                       uint32_t *prim_order, float root_box[6], float *root_power);
 
+/* No counterpart in the reference: vrgi_build_bl_bvh for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, results and errors as vrgi_build_bl_bvh, for at most
+ * VRGI_NOALLOC_MAX_PRIMS primitives (more is VRGI_E_LIMIT); it calls no hipMalloc /
+ * hipFree. Its staging and builder workspace are one fixed array per GPU (about 18 MiB,
+ * held until the process ends) shared by every call on that GPU: calls that overlap on
+ * one GPU race. */
+#define VRGI_NOALLOC_MAX_PRIMS 65536u
+int vrgi_build_bl_bvh_noalloc(vrgi_ctx *ctx, const float *tri_xyz, const float *prim_luminances,
+                              const uint32_t *prim_index_ids, uint64_t n, vrgi_bvh_node *nodes,
+                              uint32_t *prim_order, float root_box[6], float *root_power);
+
 /* build_tl_bvh (build.rs:374-480): m instance boxes (aabbs: 6m = min xyz, max xyz),
  * transforms (12m, column-major mat4x3), powers (m), ids (m). nodes: 2m-1. */
 int vrgi_build_tl_bvh(vrgi_ctx *ctx, const float *transforms, const float *aabbs,

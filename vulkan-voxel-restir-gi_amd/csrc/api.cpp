@@ -879,6 +879,67 @@ bool valid_xf(const float *xf) {
     return true;
 }
 
+// ---------------------------------------------------------------- allocation-free build
+// vrgi_build_bl_bvh_noalloc works in `scratch`, one array per GPU that lives until the
+// process ends: the staging of its inputs and outputs (below) and the builder workspace
+// of up to VRGI_NOALLOC_MAX_PRIMS primitives after them. Host code uses the per-device
+// address hipGetSymbolAddress returns (noalloc_setup), never the array's name.
+constexpr size_t na_round(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NA_N = VRGI_NOALLOC_MAX_PRIMS;
+constexpr size_t NA_TRI = 0;                                   // 9 floats per triangle
+constexpr size_t NA_PRIM9 = NA_TRI + na_round(36 * NA_N);      // builder input planes
+constexpr size_t NA_LUM = NA_PRIM9 + na_round(36 * NA_N);      // powers
+constexpr size_t NA_IDS = NA_LUM + na_round(4 * NA_N);         // primitive ids
+constexpr size_t NA_ORDER = NA_IDS + na_round(4 * NA_N);       // primitive order out
+constexpr size_t NA_NODES = NA_ORDER + na_round(4 * NA_N);     // 2n - 1 nodes out
+constexpr size_t NA_TREES = NA_NODES + na_round(40 * (2 * NA_N - 1));  // tree begin / count / base, depth
+constexpr size_t NA_BUILDER = NA_TREES + 256;                  // builder workspace
+constexpr size_t NA_BUILDER_BYTES = 8u << 20;
+constexpr size_t NA_BYTES = NA_BUILDER + NA_BUILDER_BYTES;     // about 18 MiB
+
+__device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+struct NoAllocDev {
+    char *ptr = nullptr;  // this device's copy of `scratch`
+    Builder builder;      // bound to ptr + NA_BUILDER
+};
+constexpr int NA_MAX_DEVICES = 64;
+NoAllocDev g_na[NA_MAX_DEVICES];
+std::mutex g_na_mu;
+
+// once per device, with that device current (vrgi_create)
+void noalloc_setup(int device) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    NoAllocDev &d = g_na[device];
+    if (d.ptr) return;
+    void *p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (Builder::workspace_bytes(VRGI_NOALLOC_MAX_PRIMS, 1) > NA_BUILDER_BYTES)
+        throw HipError("allocation-free scratch too small for its builder workspace");
+    d.builder.bind((char *)p + NA_BUILDER, NA_BUILDER_BYTES, VRGI_NOALLOC_MAX_PRIMS, 1);
+    d.ptr = (char *)p;
+}
+
+// the (Aabb, f32) build_bl_bvh returns (build.rs:359-371), from the host nodes
+void bl_root(const float *tri, const float *lum, const vrgi_bvh_node *nodes, float *root_box, float *root_power) {
+    if (!root_box && !root_power) return;
+    float rb[6], rp;
+    if (nodes[0].left == VRGI_INVALID) {
+        for (int k = 0; k < 3; ++k) {
+            float a = tri[k] + 0.0f, b = tri[3 + k] + 0.0f, d = tri[6 + k] + 0.0f;
+            rb[k] = std::min(std::min(a, b), d);
+            rb[3 + k] = std::max(std::max(a, b), d);
+        }
+        rp = lum[0];
+    } else {
+        for (int k = 0; k < 3; ++k) { rb[k] = nodes[0].min[k]; rb[3 + k] = nodes[0].max[k]; }
+        rp = nodes[0].power;
+    }
+    if (root_box) std::memcpy(root_box, rb, 24);
+    if (root_power) *root_power = rp;
+}
+
 }  // namespace
 
 vrgi_ctx::~vrgi_ctx() {
@@ -990,6 +1051,7 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials, const fl
             HIP_CHECK(hipMemcpy(c->d_mats.p, materials, n_slots * sizeof(vrgi_material), hipMemcpyHostToDevice));
         c->d_counters.reserve(1);
         HIP_CHECK(hipMemset(c->d_counters.p, 0, sizeof(Counters)));
+        noalloc_setup(device);
         alloc_frame(c);
         rebuild_instances(c);
         HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1771,22 +1833,43 @@ int vrgi_build_bl_bvh(vrgi_ctx *c, const float *tri, const float *lum, const uin
     HIP_CHECK(hipMemcpyAsync(nodes, c->a_nodes.p, (2 * (size_t)m - 1) * 40, hipMemcpyDeviceToHost, s));
     if (order) HIP_CHECK(hipMemcpyAsync(order, c->a_u2.p, (size_t)m * 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    if (root_box || root_power) {  // build.rs:359-371
-        float rb[6], rp;
-        if (nodes[0].left == VRGI_INVALID) {
-            for (int k = 0; k < 3; ++k) {
-                float a = tri[k] + 0.0f, b = tri[3 + k] + 0.0f, d = tri[6 + k] + 0.0f;
-                rb[k] = std::min(std::min(a, b), d);
-                rb[3 + k] = std::max(std::max(a, b), d);
-            }
-            rp = lum[0];
-        } else {
-            for (int k = 0; k < 3; ++k) { rb[k] = nodes[0].min[k]; rb[3 + k] = nodes[0].max[k]; }
-            rp = nodes[0].power;
-        }
-        if (root_box) std::memcpy(root_box, rb, 24);
-        if (root_power) *root_power = rp;
-    }
+    bl_root(tri, lum, nodes, root_box, root_power);
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_build_bl_bvh_noalloc(vrgi_ctx *c, const float *tri, const float *lum, const uint32_t *ids, uint64_t n,
+                              vrgi_bvh_node *nodes, uint32_t *order, float root_box[6], float *root_power) {
+    API_BEGIN
+    if (!c || !tri || !lum || !ids || !nodes) return fail(VRGI_E_INVALID, "null argument");
+    if (n == 0) return fail(VRGI_E_INVALID, "build_bl_bvh needs at least one primitive");
+    if (n > VRGI_NOALLOC_MAX_PRIMS)
+        return fail(VRGI_E_LIMIT, "more than VRGI_NOALLOC_MAX_PRIMS primitives for the allocation-free build");
+    if (c->device < 0 || c->device >= NA_MAX_DEVICES || !g_na[c->device].ptr)
+        return fail(VRGI_E_STATE, "no allocation-free scratch on the context's device");
+    set_dev(c);
+    hipStream_t s = c->stream;
+    NoAllocDev &d = g_na[c->device];
+    const uint32_t m = (uint32_t)n;
+    float *d_tri = (float *)(d.ptr + NA_TRI), *d_prim9 = (float *)(d.ptr + NA_PRIM9);
+    float *d_lum = (float *)(d.ptr + NA_LUM);
+    uint32_t *d_ids = (uint32_t *)(d.ptr + NA_IDS), *d_order = (uint32_t *)(d.ptr + NA_ORDER);
+    uint32_t *d_trees = (uint32_t *)(d.ptr + NA_TREES);
+    vrgi_bvh_node *d_nodes = (vrgi_bvh_node *)(d.ptr + NA_NODES);
+    // tree 0 = [0, m) at node 0; the depth word after it starts at zero
+    const uint32_t trees[4] = {0, m, 0, 0};
+    HIP_CHECK(hipMemcpyAsync(d_tri, tri, 9 * (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_lum, lum, (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_ids, ids, (size_t)m * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_trees, trees, 16, hipMemcpyHostToDevice, s));
+    launch_prims_from_tris(s, d_tri, m, d_prim9);
+    if (d.builder.build(s, d_prim9, d_lum, d_ids, m, 1, d_trees, d_trees + 1, d_trees + 2, trees + 2, d_nodes,
+                        d_trees + 3, order ? d_order : nullptr) != 0)
+        return fail(VRGI_E_LIMIT, "build exceeded its workspace");
+    HIP_CHECK(hipMemcpyAsync(nodes, d_nodes, (2 * (size_t)m - 1) * 40, hipMemcpyDeviceToHost, s));
+    if (order) HIP_CHECK(hipMemcpyAsync(order, d_order, (size_t)m * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    bl_root(tri, lum, nodes, root_box, root_power);
     return VRGI_OK;
     API_END
 }

@@ -1162,53 +1162,97 @@ __global__ __launch_bounds__(64) void k_subtree(BuildArgs a, BuildState s) {
 }
 
 // ---------------------------------------------------------------- host driver
-void Builder::reserve(uint32_t n, uint32_t n_trees) {
-    if (n <= cap_n && n_trees <= cap_t) return;
-    const bool regrow = cap_n != 0;  // 25 % headroom when a workspace grows again
-    free_all();
-    cap_n = std::max(regrow ? n + n / 4 : n, 16u);
-    cap_t = std::max(regrow ? n_trees + n_trees / 4 : n_trees, 1u);
-    uint32_t open_cap = cap_n / SUB_MAX_DEFAULT + cap_t + 16;
-    uint32_t item_cap = cap_n / CHUNK + open_cap + 16;
-    auto A = [&](void **p, size_t bytes) { dev_alloc(p, bytes); };
-    A((void **)&perm, (size_t)cap_n * 4);
-    A((void **)&pc, (size_t)cap_n * 12);
-    A((void **)&box8, (size_t)cap_n * 32);
-    A((void **)&st.rank_tmp, (size_t)cap_n * 4);
-    A((void **)&st.pair, (size_t)cap_n * 4);
-    A((void **)&st.open, (size_t)open_cap * sizeof(OpenNode));
-    A((void **)&st.open_next, (size_t)open_cap * sizeof(OpenNode));
-    A((void **)&st.jobs, (size_t)(cap_n / 3 + cap_t + 16) * sizeof(OpenNode));
-    A((void **)&st.items, (size_t)item_cap * sizeof(Item));
-    A((void **)&st.item_true, (size_t)item_cap * 4);
-
-    A((void **)&st.item_beg, (size_t)open_cap * 4);
-    A((void **)&st.item_end, (size_t)open_cap * 4);
-    A((void **)&st.bins, (size_t)open_cap * BIN_WORDS * 4);
-    A((void **)&st.bounds, (size_t)open_cap * 6 * 4);
-    A((void **)&st.split, (size_t)open_cap * sizeof(SplitInfo));
-    A((void **)&st.level_nodes, (size_t)cap_n * 8 + 64);
-    A((void **)&counters, 64 * 4);
-    A((void **)&powered, (size_t)cap_t * 4);
-    level_cap = 65536;
-    A((void **)&level_start, (size_t)level_cap * 4);
+// The workspace regions of a build of up to n primitives in `trees` trees, in one
+// layout for both owners: hipMalloc'ed one by one (reserve) or carved, 256-byte
+// aligned, from a caller's region (bind). `carve` receives (pointer slot, bytes).
+template <class F>
+static void workspace_layout(uint32_t n, uint32_t trees, uint32_t &open_cap, uint32_t *&perm, float *&pc,
+                             float4 *&box8, uint32_t *&counters, uint32_t *&level_start, uint32_t *&powered,
+                             BuildState &st, F carve) {
+    open_cap = n / SUB_MAX_DEFAULT + trees + 16;
+    const uint32_t item_cap = n / CHUNK + open_cap + 16;
+    carve((void **)&perm, (size_t)n * 4);
+    carve((void **)&pc, (size_t)n * 12);
+    carve((void **)&box8, (size_t)n * 32);
+    carve((void **)&st.rank_tmp, (size_t)n * 4);
+    carve((void **)&st.pair, (size_t)n * 4);
+    carve((void **)&st.open, (size_t)open_cap * sizeof(OpenNode));
+    carve((void **)&st.open_next, (size_t)open_cap * sizeof(OpenNode));
+    carve((void **)&st.jobs, (size_t)(n / 3 + trees + 16) * sizeof(OpenNode));
+    carve((void **)&st.items, (size_t)item_cap * sizeof(Item));
+    carve((void **)&st.item_true, (size_t)item_cap * 4);
+    carve((void **)&st.item_beg, (size_t)open_cap * 4);
+    carve((void **)&st.item_end, (size_t)open_cap * 4);
+    carve((void **)&st.bins, (size_t)open_cap * BIN_WORDS * 4);
+    carve((void **)&st.bounds, (size_t)open_cap * 6 * 4);
+    carve((void **)&st.split, (size_t)open_cap * sizeof(SplitInfo));
+    carve((void **)&st.level_nodes, (size_t)n * 8 + 64);
+    carve((void **)&counters, 64 * 4);
+    carve((void **)&powered, (size_t)trees * 4);
+    carve((void **)&level_start, (size_t)BUILDER_LEVEL_CAP * 4);
     st.n_open = counters + 0;
     st.n_open_next = counters + 1;
     st.n_jobs = counters + 2;
     st.n_items = counters + 3;
     st.n_level_nodes = counters + 4;
-    open_capacity = open_cap;
+}
+
+size_t Builder::workspace_bytes(uint32_t n, uint32_t n_trees) {
+    n = std::max(n, 16u);
+    n_trees = std::max(n_trees, 1u);
+    size_t total = 0;
+    uint32_t oc;
+    uint32_t *pe, *co, *ls, *pw;
+    float *pc_;
+    float4 *b8;
+    BuildState st_;
+    workspace_layout(n, n_trees, oc, pe, pc_, b8, co, ls, pw, st_, [&](void **, size_t bytes) {
+        total += (bytes + 255) / 256 * 256;
+    });
+    return total;
+}
+
+void Builder::bind(void *ws, size_t bytes, uint32_t n_max, uint32_t trees_max) {
+    free_all();
+    n_max = std::max(n_max, 16u);
+    trees_max = std::max(trees_max, 1u);
+    if (workspace_bytes(n_max, trees_max) > bytes) throw HipError("builder workspace region too small");
+    char *cur = (char *)ws;
+    workspace_layout(n_max, trees_max, open_capacity, perm, pc, box8, counters, level_start, powered, st,
+                     [&](void **slot, size_t b) {
+                         *slot = cur;
+                         cur += (b + 255) / 256 * 256;
+                     });
+    cap_n = n_max;
+    cap_t = trees_max;
+    level_cap = BUILDER_LEVEL_CAP;
+    external = true;
+}
+
+void Builder::reserve(uint32_t n, uint32_t n_trees) {
+    if (n <= cap_n && n_trees <= cap_t) return;
+    if (external) throw HipError("build larger than the bound workspace");
+    const bool regrow = cap_n != 0;  // 25 % headroom when a workspace grows again
+    free_all();
+    cap_n = std::max(regrow ? n + n / 4 : n, 16u);
+    cap_t = std::max(regrow ? n_trees + n_trees / 4 : n_trees, 1u);
+    workspace_layout(cap_n, cap_t, open_capacity, perm, pc, box8, counters, level_start, powered, st,
+                     [](void **p, size_t bytes) { dev_alloc(p, bytes); });
+    level_cap = BUILDER_LEVEL_CAP;
 }
 
 void Builder::free_all() {
-    void *ptrs[] = {perm, pc, box8, st.rank_tmp, st.pair, st.open, st.open_next, st.jobs, st.items,
-                    st.item_true, st.item_beg, st.item_end, st.bins, st.bounds,
-                    st.split, st.level_nodes, counters, level_start, powered};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    if (!external) {
+        void *ptrs[] = {perm, pc, box8, st.rank_tmp, st.pair, st.open, st.open_next, st.jobs, st.items,
+                        st.item_true, st.item_beg, st.item_end, st.bins, st.bounds,
+                        st.split, st.level_nodes, counters, level_start, powered};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
     perm = nullptr; pc = nullptr; box8 = nullptr; counters = nullptr; level_start = nullptr; powered = nullptr;
     st = BuildState{};
     cap_n = cap_t = 0;
+    external = false;
 }
 
 Builder::~Builder() { free_all(); }

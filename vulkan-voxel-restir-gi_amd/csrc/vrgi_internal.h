@@ -80,10 +80,18 @@ struct BuildState {
     uint32_t *n_level_nodes = nullptr;
 };
 
-// Owns the builder's workspace (grown on demand, reused across builds).
+// top-phase levels a build may take (entries of its level list)
+static constexpr uint32_t BUILDER_LEVEL_CAP = 65536;
+
+// Owns the builder's workspace (grown on demand, reused across builds), or works in a
+// region the caller binds (no device allocation by the builder then).
 class Builder {
   public:
     ~Builder();
+    // bytes of the workspace of builds of up to n primitives in up to n_trees trees
+    static size_t workspace_bytes(uint32_t n, uint32_t n_trees);
+    // works in [ws, ws + bytes) from now on: builds above n_max / trees_max then fail
+    void bind(void *ws, size_t bytes, uint32_t n_max, uint32_t trees_max);
     // Queues a batched build on `stream`; synchronises the stream after each group of
     // top-phase levels (to see whether any node is still open): the first group has the
     // previous same-sized build's level count, later groups 1, 2, 4, 4, ... levels. prim_order_out (may be null) receives the
@@ -108,6 +116,7 @@ class Builder {
     uint32_t *powered = nullptr;      // per tree: some primitive has non-zero power
     uint32_t level_cap = 0;
     uint32_t prev_n = 0, prev_trees = 0, prev_levels = 0;  // last build's size and non-empty top levels
+    bool external = false;  // the workspace is a bound region, not owned
     BuildState st;
 };
 
