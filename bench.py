@@ -251,6 +251,8 @@ def main():
                     help="one BVH buffer set built on the frame stream (default: frame f + 1 builds beside frame f)")
     ap.add_argument("--serial-nee", action="store_true",
                     help="NEE pdf launches after all bounces on one stream (default: beside the next trace)")
+    ap.add_argument("--full-nee", action="store_true",
+                    help="NEE pdf of every continued ray (default: only where it can change the outputs)")
     ap.add_argument("--sort-bits", type=int, default=0,
                     help="Morton ray-queue ordering: origin bits per axis (0 = compaction order)")
     a = ap.parse_args()
@@ -270,7 +272,7 @@ def main():
     cams = [vrgi.camera_spline(1, k, 16) for k in range(16)]
     cfg = vrgi.config(W_PX, H_PX, BOUNCES, rr_start=RR_START, rr_min=0.1,
                       flags=vrgi.FLAG_REBUILD_EVERY_FRAME | (vrgi.FLAG_SERIAL_NEE if a.serial_nee else 0)
-                      | (vrgi.FLAG_SERIAL_BUILD if a.serial_build else 0),
+                      | (vrgi.FLAG_SERIAL_BUILD if a.serial_build else 0) | (vrgi.FLAG_FULL_NEE_PDF if a.full_nee else 0),
                       sort_bits=a.sort_bits)
     L = vrgi.lib()
     with vrgi.Context(cfg, mats, lum) as ctx:

@@ -89,6 +89,11 @@ typedef struct vrgi_config {
 #define VRGI_FLAG_SERIAL_BUILD 8u        /* one set of BVH buffers, built on the frame stream (default: two
                                            sets; frame f + 1 builds on a second stream while frame f's rays
                                            still read the other) */
+#define VRGI_FLAG_FULL_NEE_PDF 16u       /* compute the NEE pdf of every continued ray (the reference's
+                                           nee_pdf buffer). Default: only where it can change the frame's
+                                           outputs (the radiance the ray returns is non-zero, or the pdf
+                                           might not be finite); elsewhere the buffer holds 0. The outputs
+                                           are the same bits either way. */
 
 typedef struct vrgi_camera {
     float eye[3], front[3], right[3], up[3]; /* Camera::eye_front_right_up (camera.rs:115-122) */

@@ -72,9 +72,12 @@ def _camera0():
 NO_RR = 0xFFFFFFFF
 
 
-def _check_frame(c, ref, P, B, nee_type):
+def _check_frame(c, ref, P, B, nee_type, full_nee=False):
     """Every per-bounce buffer of the live bounces, the radiance, the reservoirs and the
-    image of the frame just rendered by context c equal the oracle's frame ref bit for bit."""
+    image of the frame just rendered by context c equal the oracle's frame ref bit for bit.
+    The NEE pdf buffer: bit for bit with VRGI_FLAG_FULL_NEE_PDF; by default (the lazy NEE
+    pdf) bit for bit on every ray whose next vertex returns radiance, and 0 or bit for bit
+    on the others, whose pdf cannot reach any output."""
     import vrgi
     plen = c.read(vrgi.BUF_PATH_LEN, np.uint32)
     assert np.array_equal(plen, ref["path_len"])
@@ -94,7 +97,15 @@ def _check_frame(c, ref, P, B, nee_type):
     if nee_type:
         cont = np.arange(B)[:, None] + 1 < plen[None, :]
         g = c.read(vrgi.BUF_BOUNCE_NEE_PDF).reshape(B, P)
-        assert g[cont].tobytes() == ref["nee_pdf"][cont].tobytes()
+        same = g.view(np.uint32) == ref["nee_pdf"].view(np.uint32)
+        if full_nee:
+            assert np.all(same[cont])
+        else:
+            lz = np.zeros((B, P), bool)
+            lz[:-1] = np.all(ref["l_out"][1:] == 0, axis=2)  # radiance of the next vertex is 0
+            assert np.all(same[cont & ~lz])
+            assert np.all((same | (g == 0))[cont & lz])
+            assert (~same[cont]).any() or not (ref["nee_pdf"][cont & lz] != 0).any()
         assert (ref["nee_pdf"] > 0).any() or nee_type == 2
     assert c.read(vrgi.BUF_RADIANCE).tobytes() == ref["l_out"][0].tobytes()
     assert c.read(vrgi.BUF_TEMPORAL_RESERVOIR).tobytes() == ref["temporal"].tobytes()
@@ -104,13 +115,15 @@ def _check_frame(c, ref, P, B, nee_type):
     return plen
 
 
-@pytest.mark.parametrize("nee_type,spatial,rr_start,sort_bits,serial", [
-    (0, 0, NO_RR, 0, False), (1, 5, NO_RR, 0, False), (2, 3, NO_RR, 0, False),
-    # NEE pdf launches after all bounces on the frame stream
-    (1, 5, NO_RR, 0, True), (1, 2, 1, 0, True),
+@pytest.mark.parametrize("nee_type,spatial,rr_start,sort_bits,serial,full", [
+    (0, 0, NO_RR, 0, False, False), (1, 5, NO_RR, 0, False, False), (2, 3, NO_RR, 0, False, False),
+    # the NEE pdf of every ray: beside the next bounce's trace, or after all bounces on the
+    # frame stream
+    (1, 5, NO_RR, 0, False, True), (1, 5, NO_RR, 0, True, True), (1, 2, 1, 0, True, True),
+    (1, 2, 1, 0, False, False),
     # Morton-ordered ray queues (with and without Russian roulette) give the same bits
-    (1, 5, NO_RR, 4, False), (1, 2, 1, 3, False), (0, 1, 1, 5, False)])
-def test_frames_parity(world0, scenes, nee_type, spatial, rr_start, sort_bits, serial):
+    (1, 5, NO_RR, 4, False, False), (1, 2, 1, 3, False, True), (0, 1, 1, 5, False, False)])
+def test_frames_parity(world0, scenes, nee_type, spatial, rr_start, sort_bits, serial, full):
     import oracle
     import vrgi
     s, objs = scenes
@@ -118,15 +131,17 @@ def test_frames_parity(world0, scenes, nee_type, spatial, rr_start, sort_bits, s
     cam = _camera0()
     r = oracle.Renderer(W, H, B, rr_start=rr_start)
     cfg = vrgi.config(W, H, B, rr_start=rr_start, sort_bits=sort_bits,
-                      flags=vrgi.FLAG_KEEP_BOUNCES | (vrgi.FLAG_SERIAL_NEE if serial else 0))
+                      flags=vrgi.FLAG_KEEP_BOUNCES | (vrgi.FLAG_SERIAL_NEE if serial else 0)
+                      | (vrgi.FLAG_FULL_NEE_PDF if full else 0))
     with vrgi.Context(cfg, world0["mats"], world0["lum"]) as c:
         c.add_objects(objs)
         for f in range(4):
-            say(f"frame {f} nee={nee_type} spatial={spatial} rr={rr_start} sort={sort_bits} serial={serial}")
+            say(f"frame {f} nee={nee_type} spatial={spatial} rr={rr_start} sort={sort_bits} serial={serial} "
+                f"full={full}")
             c.render(cam, nee_type=nee_type, spatial=spatial)
             c.sync()
             ref = r.render(s, cam, f, nee_type=nee_type, spatial=spatial)
-            _check_frame(c, ref, W * H, B, nee_type)
+            _check_frame(c, ref, W * H, B, nee_type, full_nee=full)
 
 
 def _cubes_crossed(o, d, lo, tmin=1e-4, tmax=1000.0):
@@ -168,7 +183,7 @@ def test_frames_parity_many_light_crossings(world0):
     W, H, B = 96, 64, 3
     cam = vrgi.camera_look([6.0, 48.0, 8.0], [40.0, 0.0, 44.0])
     r = oracle.Renderer(W, H, B)
-    cfg = vrgi.config(W, H, B, flags=vrgi.FLAG_KEEP_BOUNCES)
+    cfg = vrgi.config(W, H, B, flags=vrgi.FLAG_KEEP_BOUNCES | vrgi.FLAG_FULL_NEE_PDF)
     with vrgi.Context(cfg, world0["mats"], world0["lum"]) as c:
         c.add_objects(objs)
         for f in range(2):
@@ -176,7 +191,7 @@ def test_frames_parity_many_light_crossings(world0):
             c.render(cam, nee_type=1, spatial=5)
             c.sync()
             ref = r.render(s, cam, f, nee_type=1, spatial=5)
-            plen = _check_frame(c, ref, W * H, B, 1)
+            plen = _check_frame(c, ref, W * H, B, 1, full_nee=True)
     # the scene does what it is for: many pdf rays cross 3 or more emissive cubes, i.e.
     # at least 6 light triangles, and some 5 or more (10 triangles, above the 8 a ray
     # keeps for the walk pass)

@@ -185,6 +185,7 @@ struct vrgi_ctx {
         nee_oc[VRGI_NEE_STREAMS];
     DevBuf<float2> nee_uv[VRGI_NEE_STREAMS];
     DevBuf<float> f_radiance, f_restir, f_debug, f_temporal, f_spatial;
+    DevBuf<float> f_lrun;  // [P][3] radiance of the next vertex for the lazy NEE pdf
     DevBuf<uint8_t> f_bgra;
     DevBuf<float> di_T, di_hist, di_w;
     DevBuf<uint32_t> di_light;
@@ -423,6 +424,7 @@ void alloc_frame(vrgi_ctx *c) {
     }
     c->f_qcount.reserve(B + 2);
     c->f_radiance.reserve(P * 3);
+    c->f_lrun.reserve(P * 3);
     c->f_restir.reserve(P * 3);
     c->f_debug.reserve(P * 3);
     c->f_temporal.reserve(3 * P * 20);
@@ -1352,7 +1354,12 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     // NEE pdf of bounce b needs only bounce b's outgoing rays (slot b, queue b + 1) and
     // writes only its slot of the pdf, while the trace of bounce b + 1 reads slot b and
     // writes slot b + 1: the two run side by side
-    const bool nee_overlap = prefs->nee_type != 0 && !(c->cfg.flags & VRGI_FLAG_SERIAL_NEE);
+    // Lazy NEE pdf: the pdfs run after all bounces, from the last bounce back, each after
+    // the resolve step that gives the radiance its rays return, and only for rays where the
+    // value can reach the outputs (launch_nee_pdf_lazy). Not with the learned cache, whose
+    // training records take the pdf of every vertex.
+    const bool lazy_nee = prefs->nee_type != 0 && !c->cache.on && !(c->cfg.flags & VRGI_FLAG_FULL_NEE_PDF);
+    const bool nee_overlap = prefs->nee_type != 0 && !lazy_nee && !(c->cfg.flags & VRGI_FLAG_SERIAL_NEE);
     while (c->ev_bounce.size() < B) {
         hipEvent_t e;
         HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1514,28 +1521,6 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             rec_on(2 + B + b, ns);
         }
     }
-    ba.touch_nodes = ba.touch_tris = nullptr;
-    if (nee_overlap) {
-        for (uint32_t k = 0; k < VRGI_NEE_STREAMS && k + 1 < B; ++k) {
-            HIP_CHECK(hipEventRecord(c->ev_nee_done[k], c->nee_s[k]));
-            HIP_CHECK(hipStreamWaitEvent(s, c->ev_nee_done[k], 0));
-        }
-    } else {
-        for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
-            rec(3 + 2 * (size_t)B + b);
-            if (prefs->nee_type != 0) {
-                ba.bounce = b;
-                ba.queue_in = qin[b + 1];
-                const bool short_q = cache_on && b >= c->cache.cfg.query_vertex;
-                BounceArgs na = ba;
-                if (short_q) nee_passes(c, s, 0, na);
-                launch_nee_pdf(s, sv, na, P, short_q ? VRGI_SMALL_QUEUE_GRID : 0u);
-            }
-            rec(2 + B + b);
-        }
-    }
-    if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_infer, 0));  // cache radiance is ready
-    rec(2 + B + (B - 1));
     FrameArgs fa{};
     fa.W = c->W; fa.H = c->H; fa.P = P; fa.B = B; fa.frame = frame;
     fa.slot = (uint32_t)(c->frame_count % 3);
@@ -1570,6 +1555,39 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         fa.trec_count = tcnt_cur;
         fa.tcap = c->cache.tcap;
     }
+    ba.touch_nodes = ba.touch_tris = nullptr;
+    if (nee_overlap) {
+        for (uint32_t k = 0; k < VRGI_NEE_STREAMS && k + 1 < B; ++k) {
+            HIP_CHECK(hipEventRecord(c->ev_nee_done[k], c->nee_s[k]));
+            HIP_CHECK(hipStreamWaitEvent(s, c->ev_nee_done[k], 0));
+        }
+    } else if (lazy_nee) {
+        for (int b = (int)B - 2; b >= 0; --b) {  // interactive_rendering.rs:1206
+            launch_resolve_step(s, fa, (uint32_t)b + 1u, c->f_lrun.p);
+            rec(3 + 2 * (size_t)B + (size_t)b);
+            BounceArgs na = ba;
+            na.bounce = (uint32_t)b;
+            na.queue_in = qin[b + 1];
+            na.lrun = c->f_lrun.p;
+            launch_nee_pdf_lazy(s, sv, na, P);
+            rec(2 + B + (size_t)b);
+        }
+    } else {
+        for (uint32_t b = 0; b + 1 < B; ++b) {  // interactive_rendering.rs:1206
+            rec(3 + 2 * (size_t)B + b);
+            if (prefs->nee_type != 0) {
+                ba.bounce = b;
+                ba.queue_in = qin[b + 1];
+                const bool short_q = cache_on && b >= c->cache.cfg.query_vertex;
+                BounceArgs na = ba;
+                if (short_q) nee_passes(c, s, 0, na);
+                launch_nee_pdf(s, sv, na, P, short_q ? VRGI_SMALL_QUEUE_GRID : 0u);
+            }
+            rec(2 + B + b);
+        }
+    }
+    if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_infer, 0));  // cache radiance is ready
+    rec(2 + B + (B - 1));
     launch_accum_temporal(s, fa);
     if (cache_on && c->cache.cfg.fallback_bound > 0.0f) {
         // held-out error of this frame's records at the query vertex (the side stream has

@@ -45,6 +45,31 @@ __device__ __forceinline__ void update_res(uint32_t seed, Reservoir &r, const Sa
     if (float_construct(seed) < w / r.wsum) r.z = z;
 }
 
+// one step of outgoing_radiance.rs:58-94: L_b from L_{b+1} (np 0 past the path's end,
+// where the reference's nee_pdf is never written)
+__device__ __forceinline__ f3 radiance_step(const FrameArgs &fa, size_t bi, bool next_live, f3 L, float bsp) {
+    float bp = fa.pdf[bi], w = fa.mis[bi];
+    float np = (fa.nee != nullptr && next_live) ? fa.nee[bi] : 0.0f;
+    float q = np * w + (1.0f - w) * bsp;
+    float rf = bp / q;
+    f3 term = ((ld3(fa.refl + bi * 3) * L) * rf) * 1.0f;
+    return ld3(fa.emis + bi * 3) + term;
+}
+
+__global__ __launch_bounds__(256) void k_resolve_step(const FrameArgs fa, uint32_t k, float *lrun) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= fa.P) return;
+    const uint32_t plen = fa.path_len[id];
+    if (plen <= k) return;
+    const bool next_live = k + 1 < plen;
+    const f3 Ln = next_live ? ld3(lrun + (size_t)id * 3) : mk(0, 0, 0);
+    const float bsp = 1.0f / (2.0f * VRGI_PI_F);
+    st3(lrun + (size_t)id * 3, radiance_step(fa, (size_t)k * fa.P + id, next_live, Ln, bsp));
+}
+void launch_resolve_step(hipStream_t s, const FrameArgs &fa, uint32_t k, float *lrun) {
+    k_resolve_step<<<(fa.P + 255) / 256, 256, 0, s>>>(fa, k, lrun);
+}
+
 // outgoing_radiance (backward recurrence) + initial sample + temporal update. Called by
 // every thread of the block (a block-wide record allocation when the cache is on).
 __device__ __forceinline__ void accum_temporal_px(const FrameArgs &fa, uint32_t id) {

@@ -419,9 +419,19 @@ __device__ float reverse_traverse_anc(const SceneView &sv, f3 p, f3 n, uint32_t 
         const bool tl = (e & 0x40000000u) != 0u;
         const uint32_t ni = e & 0x3FFFFFFFu;
         const LPair r = ld_pair((tl ? sv.tlpairs : blp) + ni);
-        const PairImp q = (tl || lin) ? pair_importance_t(p, n, tl ? zero : off, r, VRGI_EPS_NEE)
-                                      : pair_importance(p, n, I.xf, r, VRGI_EPS_NEE);
-        prob *= (e & 0x80000000u) ? q.rn : q.ln;
+        const f3 la = mk(r.a.x, r.a.y, r.a.z), lb = mk(r.b.x, r.b.y, r.b.z);
+        const f3 ra = mk(r.c.x, r.c.y, r.c.z), rb = mk(r.d.x, r.d.y, r.d.z);
+        float li, ri;
+        if (tl || lin) {
+            const f3 t = tl ? zero : off;
+            li = box_importance_t(p, n, t, la, lb, r.a.w, VRGI_EPS_NEE);
+            ri = box_importance_t(p, n, t, ra, rb, r.b.w, VRGI_EPS_NEE);
+        } else {
+            li = box_importance(p, n, I.xf, la, lb, r.a.w, VRGI_EPS_NEE);
+            ri = box_importance(p, n, I.xf, ra, rb, r.b.w, VRGI_EPS_NEE);
+        }
+        // only the walked side's share: ln = li / tot or rn = ri / tot (nee_pdf.rs:201-215)
+        prob *= ((e & 0x80000000u) ? ri : li) / (li + ri);
     }
     return prob;
 }
@@ -627,9 +637,64 @@ __global__ __launch_bounds__(64) void k_trace_shade_gs(const SceneView sv, const
         trace_shade_slice<STACK, false, CACHE>(sv, ba, stack, base + threadIdx.x, count);
 }
 
+// true when the ray crosses some light triangle in [VRGI_EPS_NEE, tmax] (the first one
+// found ends the walk); `stk` is the lane's LDS stack column
+__device__ __forceinline__ bool light_crossed(const SceneView &sv, const RayPre &r, float tmax, uint32_t *stk) {
+    int sp = 0;
+    uint32_t ref = sv.n_ltri == 1 ? 0x80000000u | sv.lroot_prim : 0u;
+    bool have = true;
+    while (have) {
+        if (ref & 0x80000000u) {
+            const Tri t = sv.ltri[ref & 0x7FFFFFFFu];
+            float tt, u, v;
+            if (tri_hit(r, mk(t.v0.x, t.v0.y, t.v0.z), mk(t.v1.x, t.v1.y, t.v1.z), mk(t.v2.x, t.v2.y, t.v2.z),
+                        VRGI_EPS_NEE, tmax, &tt, &u, &v))
+                return true;
+            have = false;
+        } else {
+            const Wide4 *w4 = sv.lwide4 + ref;
+            const float4 lx = w4->lox, ly = w4->loy, lz = w4->loz, hx = w4->hix, hy = w4->hiy, hz = w4->hiz;
+            const uint4 rf = w4->ref;
+            uint32_t m = 0;
+            m |= (rf.x != VRGI_INVALID && box_hit(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, VRGI_EPS_NEE, tmax)) ? 1u : 0u;
+            m |= (rf.y != VRGI_INVALID && box_hit(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, VRGI_EPS_NEE, tmax)) ? 2u : 0u;
+            m |= (rf.z != VRGI_INVALID && box_hit(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, VRGI_EPS_NEE, tmax)) ? 4u : 0u;
+            m |= (rf.w != VRGI_INVALID && box_hit(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, VRGI_EPS_NEE, tmax)) ? 8u : 0u;
+            have = m != 0u;
+            if (m & 8u && m != 8u) stk[(sp++) * 64] = rf.w;
+            if (m & 4u && (m & 3u)) stk[(sp++) * 64] = rf.z;
+            if (m & 2u && (m & 1u)) stk[(sp++) * 64] = rf.y;
+            ref = (m & 1u) ? rf.x : (m & 2u) ? rf.y : (m & 4u) ? rf.z : rf.w;
+        }
+        if (!have && sp > 0) { ref = stk[(--sp) * 64]; have = true; }
+    }
+    return false;
+}
+
+// Whether the NEE pdf of a ray can reach the frame's outputs (the lazy mode). It enters
+// the recurrence only as L_b = E_b + ((R_b * L_{b+1}) * (bsdf_pdf / q)) with
+// q = np * w + (1 - w) / (2 pi), w in (0, 0.3]: when L_{b+1} is zero (either sign) and R_b
+// finite, the product is a zero whose sign does not depend on np, unless q is 0, NaN or
+// infinite. q is finite and positive whenever np is a finite non-negative number, and np
+// is one when the direction is above the tangent plane (cs = n . d > 0, so d^2 / (cs A) is
+// positive) and every crossed light triangle lies at least 0.01 above that plane: every
+// ancestor box of such a triangle then has a corner at height >= 0.01 > 1e-4 plus any
+// rounding of the corner test, so the walked side's importance is positive and no level
+// of the reverse walk divides 0 by 0. The crossing at parameter t is t * cs above the
+// plane, so it suffices that no light triangle is crossed for t <= 0.01 / cs.
+__device__ __forceinline__ bool nee_pdf_needed(const SceneView &sv, const BounceArgs &ba, size_t bi, uint32_t px,
+                                               f3 o, f3 d, uint32_t *stk) {
+    const f3 ln = ld3(ba.lrun + (size_t)px * 3);
+    const f3 R = ld3(ba.refl + bi * 3);
+    if (!zero3(ln) || !(isfinite(R.x) && isfinite(R.y) && isfinite(R.z))) return true;
+    const float cs = dot3(ld3(ba.normal + bi * 3), d);
+    if (!(cs > 0.0f)) return true;
+    return light_crossed(sv, ray_pre(o, d), gmin(0.01f / cs, 1000.0f), stk);
+}
+
 // NEE pdf of bounce b (nee_pdf.rs:281-337) for the rays alive at bounce b+1
 constexpr int NEE_DEFER = 4;  // crossed light triangles kept per lane before their pdfs are summed
-template <int STACK, bool COUNT_RAYS = true>
+template <int STACK, bool COUNT_RAYS = true, bool LAZY = false>
 __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArgs &ba, uint32_t *stack, uint32_t i,
                                             uint32_t count) {
     const uint32_t b = ba.bounce;  // pdf of bounce b, rays of bounce b + 1
@@ -639,7 +704,9 @@ __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArg
     float w = ba.mis[bi];
     f3 d = ld3(ba.dir + bi * 3);
     float np = 0.0f;
-    const bool traced = !(len3(d) == 0.0f || w == 0.0f) && sv.n_ltri > 0;
+    bool traced = !(len3(d) == 0.0f || w == 0.0f) && sv.n_ltri > 0;
+    if (LAZY && traced)
+        traced = nee_pdf_needed(sv, ba, bi, px, ld3(ba.origin + bi * 3), d, stack + (threadIdx.x & 63u));
     if (COUNT_RAYS) {
         uint64_t bal = __ballot(traced);
         if ((threadIdx.x & 63) == (uint32_t)(__ffsll((unsigned long long)bal) - 1))
@@ -714,6 +781,11 @@ template <int STACK>
 __global__ __launch_bounds__(64) void k_nee_pdf(const SceneView sv, const BounceArgs ba) {
     __shared__ uint32_t stack[STACK * 64];
     nee_pdf_ray<STACK>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, ba.qcount[ba.bounce + 1]);
+}
+template <int STACK>
+__global__ __launch_bounds__(64) void k_nee_pdf_lazy(const SceneView sv, const BounceArgs ba) {
+    __shared__ uint32_t stack[STACK * 64];
+    nee_pdf_ray<STACK, true, true>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, ba.qcount[ba.bounce + 1]);
 }
 template <int STACK>
 __global__ __launch_bounds__(64) void k_nee_pdf_gs(const SceneView sv, const BounceArgs ba) {
@@ -1192,6 +1264,12 @@ __global__ void k_popcount(const uint32_t *bits, size_t n_words, unsigned long l
 }
 void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsigned long long *out) {
     if (n_words) k_popcount<<<(unsigned)((n_words + 255) / 256), 256, 0, s>>>(bits, n_words, out);
+}
+void launch_nee_pdf_lazy(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
+    if (!max_rays) return;
+    with_stack(3 * ((sv.ldepth + 1) / 2), [&](auto S) {
+        k_nee_pdf_lazy<decltype(S)::value><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+    });
 }
 void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays, uint32_t grid_cap) {
     if (!max_rays) return;

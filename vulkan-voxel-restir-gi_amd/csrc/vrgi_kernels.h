@@ -77,6 +77,9 @@ struct BounceArgs {
     uint32_t *nee_cnt, *nee_lid;
     float2 *nee_uv;
     uint32_t *nee_over, *nee_over_count;
+    // lazy NEE pdf (launch_nee_pdf_lazy): [P][3] outgoing radiance of the next vertex,
+    // L_{b+1}, left by launch_resolve_step(b + 1)
+    const float *lrun;
 };
 
 struct FlattenArgs {
@@ -415,6 +418,16 @@ void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba
                         uint32_t grid_cap = 0);
 void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays,
                     uint32_t grid_cap = 0);
+// The NEE pdf of bounce b computed only where its value can reach the frame's outputs:
+// rays whose next vertex returns radiance (ba.lrun non-zero), or whose pdf might not be
+// a finite non-negative number. The others get 0: their q differs from the reference's,
+// but L_b = E_b + ((R_b * 0) * (bsdf_pdf / q)) does not. Needs launch_resolve_step(b + 1)
+// first.
+void launch_nee_pdf_lazy(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays);
+// One bounce k of the outgoing-radiance recurrence (outgoing_radiance.rs:58-94) for every
+// pixel whose path reached vertex k: lrun = E_k + ((R_k * lrun) * (bsdf_pdf_k / q_k)),
+// lrun taken as 0 past the path's end; the arithmetic of the full resolve.
+void launch_resolve_step(hipStream_t s, const FrameArgs &fa, uint32_t k, float *lrun);
 void launch_accum_temporal(hipStream_t s, const FrameArgs &fa);
 void launch_spatial_finalize(hipStream_t s, const FrameArgs &fa);
 void launch_postprocess(hipStream_t s, const FrameArgs &fa);

@@ -32,6 +32,7 @@ FLAG_REBUILD_EVERY_FRAME = 1
 FLAG_KEEP_BOUNCES = 2
 FLAG_SERIAL_NEE = 4
 FLAG_SERIAL_BUILD = 8
+FLAG_FULL_NEE_PDF = 16
 
 
 class Config(C.Structure):
