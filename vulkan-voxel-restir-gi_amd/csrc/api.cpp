@@ -186,6 +186,7 @@ struct vrgi_ctx {
     DevBuf<float2> nee_uv[VRGI_NEE_STREAMS];
     DevBuf<float> f_radiance, f_restir, f_debug, f_temporal, f_spatial;
     DevBuf<float> f_lrun;  // [P][3] radiance of the next vertex for the lazy NEE pdf
+    DevBuf<uint32_t> f_nlist, f_ncount;  // lazy NEE pdf: rays listed for the full pdf, count per bounce
     DevBuf<uint8_t> f_bgra;
     DevBuf<float> di_T, di_hist, di_w;
     DevBuf<uint32_t> di_light;
@@ -425,6 +426,8 @@ void alloc_frame(vrgi_ctx *c) {
     c->f_qcount.reserve(B + 2);
     c->f_radiance.reserve(P * 3);
     c->f_lrun.reserve(P * 3);
+    c->f_nlist.reserve(P);
+    c->f_ncount.reserve(16);
     c->f_restir.reserve(P * 3);
     c->f_debug.reserve(P * 3);
     c->f_temporal.reserve(3 * P * 20);
@@ -1562,6 +1565,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             HIP_CHECK(hipStreamWaitEvent(s, c->ev_nee_done[k], 0));
         }
     } else if (lazy_nee) {
+        HIP_CHECK(hipMemsetAsync(c->f_ncount.p, 0, (size_t)B * 4, s));
         for (int b = (int)B - 2; b >= 0; --b) {  // interactive_rendering.rs:1206
             launch_resolve_step(s, fa, (uint32_t)b + 1u, c->f_lrun.p);
             rec(3 + 2 * (size_t)B + (size_t)b);
@@ -1569,7 +1573,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             na.bounce = (uint32_t)b;
             na.queue_in = qin[b + 1];
             na.lrun = c->f_lrun.p;
-            launch_nee_pdf_lazy(s, sv, na, P);
+            launch_nee_pdf_lazy(s, sv, na, P, c->f_nlist.p, c->f_ncount.p + b);
             rec(2 + B + (size_t)b);
         }
     } else {

@@ -694,7 +694,7 @@ __device__ __forceinline__ bool nee_pdf_needed(const SceneView &sv, const Bounce
 
 // NEE pdf of bounce b (nee_pdf.rs:281-337) for the rays alive at bounce b+1
 constexpr int NEE_DEFER = 4;  // crossed light triangles kept per lane before their pdfs are summed
-template <int STACK, bool COUNT_RAYS = true, bool LAZY = false>
+template <int STACK, bool COUNT_RAYS = true>
 __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArgs &ba, uint32_t *stack, uint32_t i,
                                             uint32_t count) {
     const uint32_t b = ba.bounce;  // pdf of bounce b, rays of bounce b + 1
@@ -704,9 +704,7 @@ __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArg
     float w = ba.mis[bi];
     f3 d = ld3(ba.dir + bi * 3);
     float np = 0.0f;
-    bool traced = !(len3(d) == 0.0f || w == 0.0f) && sv.n_ltri > 0;
-    if (LAZY && traced)
-        traced = nee_pdf_needed(sv, ba, bi, px, ld3(ba.origin + bi * 3), d, stack + (threadIdx.x & 63u));
+    const bool traced = !(len3(d) == 0.0f || w == 0.0f) && sv.n_ltri > 0;
     if (COUNT_RAYS) {
         uint64_t bal = __ballot(traced);
         if ((threadIdx.x & 63) == (uint32_t)(__ffsll((unsigned long long)bal) - 1))
@@ -782,10 +780,47 @@ __global__ __launch_bounds__(64) void k_nee_pdf(const SceneView sv, const Bounce
     __shared__ uint32_t stack[STACK * 64];
     nee_pdf_ray<STACK>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, ba.qcount[ba.bounce + 1]);
 }
+// Lazy NEE pdf, pass 1: per ray of the bounce's queue, 0 for the rays whose pdf cannot
+// reach the outputs (or that trace none); the others are appended to `list` (queue slot
+// indices, ballot + one atomic per wave) for pass 2
 template <int STACK>
-__global__ __launch_bounds__(64) void k_nee_pdf_lazy(const SceneView sv, const BounceArgs ba) {
+__global__ __launch_bounds__(64) void k_nee_select(const SceneView sv, const BounceArgs ba, uint32_t *list,
+                                                   uint32_t *list_count) {
     __shared__ uint32_t stack[STACK * 64];
-    nee_pdf_ray<STACK, true, true>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, ba.qcount[ba.bounce + 1]);
+    const uint32_t b = ba.bounce;
+    const uint32_t count = ba.qcount[b + 1];
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (blockIdx.x * 64u >= count) return;
+    bool need = false;
+    if (i < count) {
+        const uint32_t px = ba.queue_in[i];
+        const size_t bi = (size_t)b * ba.P + px;
+        const float w = ba.mis[bi];
+        const f3 d = ld3(ba.dir + bi * 3);
+        need = !(len3(d) == 0.0f || w == 0.0f) && sv.n_ltri > 0 &&
+               nee_pdf_needed(sv, ba, bi, px, ld3(ba.origin + bi * 3), d, stack + (threadIdx.x & 63u));
+        if (!need) ba.nee[bi] = 0.0f;
+    }
+    const uint64_t bal = __ballot(need);
+    if (bal) {
+        const uint32_t lane = threadIdx.x & 63u;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(list_count, (uint32_t)__popcll(bal));
+        base = __shfl(base, 0, 64);
+        if (need) list[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
+    }
+}
+// pass 2: the full pdf of the listed rays (a capped grid striding over the list), each
+// ray exactly as nee_pdf_ray computes it
+template <int STACK>
+__global__ __launch_bounds__(64) void k_nee_listed(const SceneView sv, const BounceArgs ba, const uint32_t *list,
+                                                   const uint32_t *list_count) {
+    __shared__ uint32_t stack[STACK * 64];
+    const uint32_t n = *list_count, count = ba.qcount[ba.bounce + 1];
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t k = base + threadIdx.x;
+        nee_pdf_ray<STACK>(sv, ba, stack, k < n ? list[k] : count, count);
+    }
 }
 template <int STACK>
 __global__ __launch_bounds__(64) void k_nee_pdf_gs(const SceneView sv, const BounceArgs ba) {
@@ -1265,10 +1300,14 @@ __global__ void k_popcount(const uint32_t *bits, size_t n_words, unsigned long l
 void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsigned long long *out) {
     if (n_words) k_popcount<<<(unsigned)((n_words + 255) / 256), 256, 0, s>>>(bits, n_words, out);
 }
-void launch_nee_pdf_lazy(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays) {
+void launch_nee_pdf_lazy(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays, uint32_t *list,
+                         uint32_t *list_count) {
     if (!max_rays) return;
     with_stack(3 * ((sv.ldepth + 1) / 2), [&](auto S) {
-        k_nee_pdf_lazy<decltype(S)::value><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba);
+        constexpr int K = decltype(S)::value;
+        k_nee_select<K><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba, list, list_count);
+        const uint32_t grid = std::min<uint32_t>((max_rays + 63) / 64, VRGI_NEE_LIST_GRID);
+        k_nee_listed<K><<<grid, 64, 0, s>>>(sv, ba, list, list_count);
     });
 }
 void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays, uint32_t grid_cap) {

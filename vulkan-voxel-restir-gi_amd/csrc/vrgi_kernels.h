@@ -422,8 +422,11 @@ void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, ui
 // rays whose next vertex returns radiance (ba.lrun non-zero), or whose pdf might not be
 // a finite non-negative number. The others get 0: their q differs from the reference's,
 // but L_b = E_b + ((R_b * 0) * (bsdf_pdf / q)) does not. Needs launch_resolve_step(b + 1)
-// first.
-void launch_nee_pdf_lazy(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays);
+// first. Pass 1 writes the 0s and lists the other rays (list: max_rays entries,
+// *list_count zeroed before), pass 2 computes their pdf at full lane occupancy.
+#define VRGI_NEE_LIST_GRID 4096  // blocks of the pass striding over the listed rays
+void launch_nee_pdf_lazy(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays, uint32_t *list,
+                         uint32_t *list_count);
 // One bounce k of the outgoing-radiance recurrence (outgoing_radiance.rs:58-94) for every
 // pixel whose path reached vertex k: lrun = E_k + ((R_k * lrun) * (bsdf_pdf_k / q_k)),
 // lrun taken as 0 past the path's end; the arithmetic of the full resolve.
