@@ -50,7 +50,8 @@ class ProfileOut(C.Structure):
     _fields_ = [("frames", C.c_uint64), ("build_ms", C.c_double), ("trace_ms", C.c_double * 16),
                 ("nee_ms", C.c_double * 16), ("resolve_ms", C.c_double), ("frame_ms", C.c_double),
                 ("rays", C.c_uint64 * 16), ("touched_bytes", C.c_uint64 * 16), ("build_prims", C.c_uint64),
-                ("build_leaf_depth_sum", C.c_uint64), ("build_nodes", C.c_uint64)]
+                ("build_leaf_depth_sum", C.c_uint64), ("build_nodes", C.c_uint64),
+                ("nee_listed", C.c_uint64 * 16)]
 
 
 def light_structure_bytes(n_ltri):
@@ -65,18 +66,30 @@ def kernel_rooflines(prof, touch_bytes, n_ltri, B):
     """Per-launch algorithmic bytes and average launch time of the per-ray kernels.
     k_trace_shade (one launch per bounce): (48 + 150) B per ray -- ray in 24-32 B and hit
     out 16 B, shading outputs and material / triangle reads 150 B -- plus the unique node
-    and triangle bytes the launch touches (counted on the GPU). k_nee_pdf (one launch per
-    bounce but the last): per ray 48 B (queue id, origin, direction, normal, MIS weight,
-    pdf out) plus the light structures."""
+    and triangle bytes the launch touches (counted on the GPU).
+    NEE pdf of a bounce (the lazy form: k_nee_select over the bounce's queue, then
+    k_nee_listed over the rays it lists): 76 B per queued ray (queue id, MIS weight,
+    direction, next-vertex radiance, reflectivity, normal, origin; pdf 0 or list entry out)
+    plus 52 B per listed ray (list entry, queue id, MIS weight, direction, origin, normal,
+    pdf out) plus the light structures; with every pdf computed (--full-nee, one k_nee_pdf
+    launch) 48 B per ray plus the light structures."""
     f = max(prof.frames, 1)
     rays = [prof.rays[b] / f for b in range(B)]
+    listed = [prof.nee_listed[b] / f for b in range(B - 1)]
+    lazy = any(listed)
     trace_ms = [prof.trace_ms[b] / f for b in range(B)]
     nee_ms = [prof.nee_ms[b] / f for b in range(B - 1)]
     trace_bytes = [198 * rays[b] + touch_bytes[b] for b in range(B)]
-    nee_bytes = [48 * rays[b + 1] + light_structure_bytes(n_ltri) for b in range(B - 1)]
+    if lazy:
+        nee_bytes = [76 * rays[b + 1] + 52 * listed[b] + light_structure_bytes(n_ltri) for b in range(B - 1)]
+    else:
+        nee_bytes = [48 * rays[b + 1] + light_structure_bytes(n_ltri) for b in range(B - 1)]
+    nee = {"launches": B - 1, "bytes": sum(nee_bytes) / (B - 1), "ms": sum(nee_ms) / (B - 1)}
+    if lazy:
+        nee["listed_fraction"] = round(sum(listed) / max(sum(rays[1:]), 1), 4)
     return {
         "k_trace_shade": {"launches": B, "bytes": sum(trace_bytes) / B, "ms": sum(trace_ms) / B},
-        "k_nee_pdf": {"launches": B - 1, "bytes": sum(nee_bytes) / (B - 1), "ms": sum(nee_ms) / (B - 1)},
+        ("k_nee_select + k_nee_listed" if lazy else "k_nee_pdf"): nee,
     }
 
 
@@ -109,7 +122,16 @@ def cpu_baseline(objs, mats, lum, cams):
     temporal + 5 spatial; OpenMP over pixels). ms/frame = build + frame."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or usable)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     s = oracle.Scene(mats, lum)
     s.add_objects(objs)
     t0 = time.perf_counter()
@@ -121,8 +143,10 @@ def cpu_baseline(objs, mats, lum, cams):
     t_frame = time.perf_counter() - t0
     ms = 1000.0 * (t_build + t_frame)
     return {"value": round(ms, 1), "unit": "ms/frame", "cores": threads, "kind": "port",
-            "sample": f"oracle: 1 full geometry + light BVH build (1 thread, {t_build:.2f} s) + 1 full 1920x1080 "
-                      f"frame with NEE and ReSTIR GI ({t_frame:.2f} s, {threads} threads)"}
+            "nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model,
+            "sample": f"oracle: 1 full geometry + light BVH build (1 thread: one SAH tree over the world's "
+                      f"triangles, {t_build:.2f} s) + 1 full 1920x1080 frame with NEE and ReSTIR GI "
+                      f"({t_frame:.2f} s, {threads} OpenMP threads)"}
 
 
 def batched_bench(steps):
@@ -316,16 +340,20 @@ def main():
     dom = max(kr, key=lambda k: kr[k]["ms"] * kr[k]["launches"])
     kd = kr[dom]
     achieved = kd["bytes"] / (kd["ms"] * 1e-3) / 1e9 if kd["ms"] > 0 else 0.0
+    # FETCH_SIZE + WRITE_SIZE per launch of the same command under rocprofv3 --pmc (one
+    # counter per pass, scaled by a plain copy): L2-miss traffic to MALL / HBM
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get(dom, {}).get("hbm_bytes_per_launch")
+        traffic = json.load(open(tfile)).get(dom, {}).get("l2_miss_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this command",
                 "alg_bytes_per_launch": int(kd["bytes"]), "avg_launch_ms": round(kd["ms"], 4),
                 "launches_per_frame": kd["launches"]}
     others = {k: {"alg_bytes_per_launch": int(v["bytes"]), "avg_launch_ms": round(v["ms"], 4),
-                  "achieved_GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 2) if v["ms"] > 0 else 0.0}
+                  "achieved_GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 2) if v["ms"] > 0 else 0.0,
+                  **({"listed_fraction": v["listed_fraction"]} if "listed_fraction" in v else {})}
               for k, v in kr.items()}
     if prof.build_ms > 0:
         bb = build_alg_bytes(prof)

@@ -197,6 +197,7 @@ typedef struct vrgi_profile_out {
     uint64_t build_prims;        /* primitives of the last geometry build (both trees) */
     uint64_t build_leaf_depth_sum; /* sum over leaves of their depth, last build */
     uint64_t build_nodes;        /* nodes written by the last build */
+    uint64_t nee_listed[16];     /* lazy NEE pdf: rays whose full pdf was computed, per bounce */
 } vrgi_profile_out;
 int vrgi_profile(vrgi_ctx *ctx, uint32_t flags);
 int vrgi_profile_read(vrgi_ctx *ctx, vrgi_profile_out *out, int reset);

@@ -202,7 +202,7 @@ struct vrgi_ctx {
     std::vector<hipEvent_t> events[2];
     bool pend[2] = {false, false}, pend_built[2] = {false, false};
     uint32_t pend_P[2] = {0, 0};
-    uint32_t *h_qcount = nullptr;  // pinned, 2 x (B + 2) queue counts
+    uint32_t *h_qcount = nullptr;  // pinned, 2 x (2B + 2): queue counts (B + 2), lazy NEE list counts (B)
     DevBuf<uint32_t> touch_bits;
     DevBuf<unsigned long long> touch_count;
     DevBuf<unsigned long long> depth_sum_copy;
@@ -553,12 +553,13 @@ void harvest_frame(vrgi_ctx *c, uint32_t k) {
         return (double)ms;
     };
     vrgi_profile_out &p = c->prof;
-    const uint32_t *q = c->h_qcount + (size_t)k * (B + 2);
+    const uint32_t *q = c->h_qcount + (size_t)k * (2 * B + 2);
     if (c->pend_built[k]) p.build_ms += el(3 + 3 * B, 4 + 3 * B);
     for (size_t b = 0; b < B && b < 16; ++b) {
         p.trace_ms[b] += el(b == 0 ? 1 : 1 + b, 2 + b);
         if (b + 1 < B) p.nee_ms[b] += el(3 + 2 * B + b, 2 + B + b);
         p.rays[b] += b == 0 ? c->pend_P[k] : q[b];
+        if (b + 1 < B) p.nee_listed[b] += q[B + 2 + b];
     }
     p.resolve_ms += el(2 + B + (B - 1), 2 + 2 * B);
     p.frame_ms += el(0, 2 + 2 * B);
@@ -1639,9 +1640,11 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     HIP_CHECK(hipEventRecord(c->G().ev_free, s));
     c->G().freed_once = true;
     if (stages) {
-        if (!c->h_qcount) HIP_CHECK(hipHostMalloc((void **)&c->h_qcount, 2 * ((size_t)B + 2) * 4, hipHostMallocDefault));
-        HIP_CHECK(hipMemcpyAsync(c->h_qcount + (size_t)par * (B + 2), c->f_qcount.p, ((size_t)B + 2) * 4,
-                                 hipMemcpyDeviceToHost, s));
+        if (!c->h_qcount) HIP_CHECK(hipHostMalloc((void **)&c->h_qcount, 2 * (2 * (size_t)B + 2) * 4, hipHostMallocDefault));
+        uint32_t *hq = c->h_qcount + (size_t)par * (2 * B + 2);
+        HIP_CHECK(hipMemcpyAsync(hq, c->f_qcount.p, ((size_t)B + 2) * 4, hipMemcpyDeviceToHost, s));
+        if (lazy_nee) HIP_CHECK(hipMemcpyAsync(hq + B + 2, c->f_ncount.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
+        else std::memset(hq + B + 2, 0, (size_t)B * 4);
     }
     rec(2 + 2 * (size_t)B);
     if (stages) {

@@ -816,7 +816,8 @@ template <int STACK>
 __global__ __launch_bounds__(64) void k_nee_listed(const SceneView sv, const BounceArgs ba, const uint32_t *list,
                                                    const uint32_t *list_count) {
     __shared__ uint32_t stack[STACK * 64];
-    const uint32_t n = *list_count, count = ba.qcount[ba.bounce + 1];
+    const uint32_t count = ba.qcount[ba.bounce + 1];
+    const uint32_t n = min(*list_count, count);  // each queued ray is listed at most once
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t k = base + threadIdx.x;
         nee_pdf_ray<STACK>(sv, ba, stack, k < n ? list[k] : count, count);

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvrgi.so")
+# VRGI_LIB: another build of the same sources (the sanitizer test's host-ASan build)
+LIB_PATH = os.environ.get("VRGI_LIB") or os.path.join(HERE, "libvrgi.so")
 
 VERTEX_DTYPE = np.dtype([("position", "<f4", 3), ("t", "<u4"), ("uv", "<f4", 2)])
 NODE_DTYPE = np.dtype([("left", "<u4"), ("right", "<u4"), ("min", "<f4", 3), ("power", "<f4"),
@@ -47,7 +48,8 @@ class ProfileOut(C.Structure):
     _fields_ = [("frames", C.c_uint64), ("build_ms", C.c_double), ("trace_ms", C.c_double * 16),
                 ("nee_ms", C.c_double * 16), ("resolve_ms", C.c_double), ("frame_ms", C.c_double),
                 ("rays", C.c_uint64 * 16), ("touched_bytes", C.c_uint64 * 16), ("build_prims", C.c_uint64),
-                ("build_leaf_depth_sum", C.c_uint64), ("build_nodes", C.c_uint64)]
+                ("build_leaf_depth_sum", C.c_uint64), ("build_nodes", C.c_uint64),
+                ("nee_listed", C.c_uint64 * 16)]
 
 
 PROFILE_STAGES, PROFILE_TOUCH = 1, 2
