@@ -102,7 +102,11 @@ typedef struct vrgi_camera {
 /* RenderingPreferences (camera.rs:37-56). */
 typedef struct vrgi_prefs {
     uint32_t nee_type;                  /* 0 none, 1 NEE every bounce, 2 NEE at bounce 1 only */
-    uint32_t debug_view;                /* 0 path tracer, 1 ReSTIR, 2 debug */
+    uint32_t debug_view;                /* 0 path tracer, 1 ReSTIR, 2 debug (postprocess.rs:39-88);
+                                           3 ReSTIR tonemapped (x / (1 + x) per channel); 4 the path
+                                           tracer's running mean over the frames since the camera, the
+                                           scene, the extent or the view last changed, tonemapped
+                                           (no reference counterpart; not in a batch) */
     uint32_t restir_spatial_iterations; /* spatial neighbours, 0..20 */
 } vrgi_prefs;
 

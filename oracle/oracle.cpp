@@ -551,6 +551,7 @@ struct or_scene {
     std::vector<BvhNode> lgeom;               // BVH over world light triangles
     std::vector<uint32_t> lgeom_order;
     std::vector<uint32_t> lgeom_rank;         // leaf position of each light id
+    uint64_t version = 0;                     // bumped by every add / remove
 };
 
 namespace {
@@ -1017,6 +1018,12 @@ struct or_renderer {
     Cfg c;
     std::vector<float> temporal;  // [3][P][20]
     std::vector<float> di_hist;   // [2][P][8] ReSTIR DI history: light, y xyz, W, M, -, -
+    // view 4 (accumulated path tracer): running sum, frames in it, and what it was taken with
+    std::vector<float> view_acc;  // [P][3]
+    uint32_t view_n = 0, view_prev = 0;
+    float view_cam[12] = {0};
+    const or_scene *view_scene = nullptr;
+    uint64_t view_version = 0;
 };
 
 namespace {
@@ -1423,10 +1430,12 @@ int or_scene_add(or_scene *s, uint32_t key, const void *verts, uint64_t n, const
     std::memcpy(in.xf, xf, 48);
     if (n > 0) in.obj = upload(s, (const Vertex3D *)verts, n);
     s->objects[key] = in;
+    s->version += 1;
     return 0;
 }
 int or_scene_remove(or_scene *s, uint32_t key) {
     s->objects.erase(key);
+    s->version += 1;
     return 0;
 }
 int or_scene_build(or_scene *s) {
@@ -1660,11 +1669,27 @@ static int render_impl(or_renderer *rr, or_scene *s, const or_camera *cam, const
     std::vector<uint32_t> di_light(P, 0xFFFFFFFFu);
     std::vector<float> di_w(P * 4, 0.0f);
     if (c.mode == 1) restir_di(rr, s, prefs, frame, DiInputs{f.hit, f.origin, f.normal, f.emis}, fin, di_light, di_w);
+    // view 4: running mean of the path-traced radiance over the frames since the camera,
+    // the scene or the view last changed (restarted on any of them)
+    const uint32_t view = prefs->debug_view;
+    if (view == 4) {
+        const bool reset = rr->view_n == 0 || rr->view_prev != 4 || std::memcmp(rr->view_cam, cam, 48) != 0 ||
+                           rr->view_scene != s || rr->view_version != s->version;
+        if (rr->view_acc.size() != P * 3) rr->view_acc.assign(P * 3, 0.0f);
+        rr->view_n = reset ? 1u : rr->view_n + 1u;
+        for (size_t k = 0; k < P * 3; ++k) rr->view_acc[k] = reset ? f.lout[k] : rr->view_acc[k] + f.lout[k];
+        std::memcpy(rr->view_cam, cam, 48);
+        rr->view_scene = s;
+        rr->view_version = s->version;
+    }
+    rr->view_prev = view;
     if (out->bgra8) {
         for (size_t id = 0; id < P; ++id) {
-            v3 col = prefs->debug_view == 0 ? ld3(&f.lout[id * 3])
-                     : prefs->debug_view == 1 ? ld3(&fin[id * 3]) : ld3(&dbg[id * 3]);
+            v3 col = view == 0 ? ld3(&f.lout[id * 3])
+                     : (view == 1 || view == 3) ? ld3(&fin[id * 3])
+                     : view == 4 ? ld3(&rr->view_acc[id * 3]) / (float)rr->view_n : ld3(&dbg[id * 3]);
             col = col / (float)(1 * 1);
+            if (view >= 3) col = V(col.x / (1.0f + col.x), col.y / (1.0f + col.y), col.z / (1.0f + col.z));
             auto cv = [](float v) -> uint8_t {
                 float x = v * 255.0f;
                 if (!(x > 0.0f)) return 0;

@@ -5,6 +5,8 @@ t within 1e-6 relative (the tests require bit equality, which is stronger). Whol
 frames of the 64^3 world: path lengths, hit ids and every per-bounce buffer of the live
 bounces bit-exact, radiance / ReSTIR reservoirs / final image bit-exact, over three
 frames so that the 3-slot temporal history is exercised."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -287,3 +289,41 @@ def test_debug_views_parity(world0, scenes, view):
             img = c.read(vrgi.BUF_BGRA8, np.uint8)
             assert img.tobytes() == ref["bgra8"].tobytes()
             assert len(np.unique(img.reshape(-1, 4)[:, :3])) > 4
+
+
+def test_tonemap_and_accumulate_views(world0):
+    """View 3 (ReSTIR radiance, Reinhard per channel) and view 4 (the path tracer's
+    running mean since the camera, the scene or the view last changed, tonemapped):
+    BGRA8 bit-exact against the oracle through restarts by a view change, a camera move
+    and a scene edit."""
+    import oracle
+    import vrgi
+    objs = world0["objs"]
+    s = oracle.Scene(world0["mats"], world0["lum"])
+    s.add_objects(objs)
+    s.build()
+    W, H, B = 64, 48, 3
+    cam0 = _camera0()
+    cam1 = vrgi.camera_look([10.0, 60.0, 6.0], [36.0, 0.0, 40.0])
+    r = oracle.Renderer(W, H, B)
+    lit_key = next(k for k, xf, v in objs if len(v))
+    steps = [(4, cam0, None), (4, cam0, None), (4, cam0, None), (3, cam0, None), (4, cam0, None), (4, cam0, None),
+             (4, cam1, None), (4, cam1, "edit"), (4, cam1, None)]
+    with vrgi.Context(vrgi.config(W, H, B), world0["mats"], world0["lum"]) as c:
+        c.add_objects(objs)
+        imgs = []
+        for f, (view, cam, edit) in enumerate(steps):
+            if edit:
+                c.remove(lit_key)
+                s.remove(lit_key)
+                s.build()
+            say(f"view {view} frame {f}{' after an edit' if edit else ''}")
+            c.render(cam, nee_type=1, debug_view=view, spatial=5)
+            c.sync()
+            ref = r.render(s, cam, f, nee_type=1, debug_view=view, spatial=5)
+            img = c.read(vrgi.BUF_BGRA8, np.uint8)
+            assert img.tobytes() == ref["bgra8"].tobytes(), f
+            imgs.append(img)
+        assert imgs[2].tobytes() != imgs[0].tobytes()  # the mean moved on
+        bad = vrgi.Prefs(1, 5, 0)
+        assert vrgi.lib().vrgi_render(c.h, ctypes.byref(cam0), ctypes.byref(bad), None) == vrgi.VRGI_E_INVALID

@@ -125,6 +125,11 @@ struct vrgi_ctx {
     std::vector<UpSlot *> up_slots;   // every upload slot made so far
     std::vector<UpSlot *> up_idle;    // the ones no upload holds
     bool minimized = false;           // zero extent: render() draws nothing (interactive_rendering.rs:978-981)
+    // view 4 (accumulated path tracer): running sum, frames in it, and its restart state
+    DevBuf<float> f_view_acc;
+    uint32_t view_n = 0, view_prev = 0;
+    vrgi_camera view_cam{};
+    bool view_dirty = true;  // the scene or the extent changed since the sum started
     vrgi_config cfg{};
     uint32_t W = 0, H = 0, P = 0, B = 0;
     std::vector<vrgi_material> mats;
@@ -426,6 +431,8 @@ void alloc_frame(vrgi_ctx *c) {
     c->f_qcount.reserve(B + 2);
     c->f_radiance.reserve(P * 3);
     c->f_lrun.reserve(P * 3);
+    c->f_view_acc.reserve(P * 3);
+    c->view_dirty = true;
     c->f_nlist.reserve(P);
     c->f_ncount.reserve(16);
     c->f_restir.reserve(P * 3);
@@ -1249,6 +1256,7 @@ int vrgi_scene_add(vrgi_ctx *c, uint32_t key, vrgi_object *o, const float xf[12]
     if (o) o->refs.fetch_add(1);
     c->objects[key] = {o, std::vector<float>(xf, xf + 12)};
     c->inst_dirty = c->geo_dirty = true;
+    c->view_dirty = true;
     return VRGI_OK;
     API_END
 }
@@ -1261,6 +1269,7 @@ int vrgi_scene_update(vrgi_ctx *c, uint32_t key, const float xf[12]) {
     if (!it->second.first) return VRGI_OK;  // Some(None): no-op
     it->second.second.assign(xf, xf + 12);
     c->inst_dirty = c->geo_dirty = true;
+    c->view_dirty = true;
     return VRGI_OK;
     API_END
 }
@@ -1274,6 +1283,7 @@ int vrgi_scene_remove(vrgi_ctx *c, uint32_t key) {
         c->old.push_back(it->second.first);
         c->inst_dirty = c->geo_dirty = true;
     }
+    c->view_dirty = true;
     c->objects.erase(it);
     return VRGI_OK;
     API_END
@@ -1329,6 +1339,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     if (!c || !cam || !prefs) return fail(VRGI_E_INVALID, "null argument");
     if (prefs->nee_type > 2) return fail(VRGI_E_INVALID, "nee_type must be 0, 1 or 2");
     if (prefs->restir_spatial_iterations > 20) return fail(VRGI_E_INVALID, "restir_spatial_iterations above 20");
+    if (prefs->debug_view > 4) return fail(VRGI_E_INVALID, "debug_view must be in [0, 4]");
     if (c->minimized) {  // zero extent: no frame (interactive_rendering.rs:978-981)
         if (out) std::memset(out, 0, sizeof(*out));
         return VRGI_OK;
@@ -1533,6 +1544,16 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     fa.spatial_iters = prefs->restir_spatial_iterations;
     fa.radius = c->cfg.restir_spatial_radius;
     fa.debug_view = prefs->debug_view;
+    if (prefs->debug_view == 4) {
+        const bool restart = c->view_n == 0 || c->view_prev != 4 || c->view_dirty ||
+                             std::memcmp(&c->view_cam, cam, sizeof(vrgi_camera)) != 0;
+        c->view_n = restart ? 1u : c->view_n + 1u;
+        c->view_cam = *cam;
+        c->view_dirty = false;
+        fa.view_acc = c->f_view_acc.p;
+        fa.view_n = c->view_n;
+    }
+    c->view_prev = prefs->debug_view;
     fa.origin = c->f_origin.p; fa.normal = c->f_normal.p; fa.emis = c->f_emis.p; fa.refl = c->f_refl.p;
     fa.mis = c->f_mis.p; fa.pdf = c->f_pdf.p; fa.nee = prefs->nee_type ? c->f_nee.p : nullptr;
     fa.path_len = c->f_plen.p;
@@ -2312,6 +2333,7 @@ int vrgi_voxels_load(vrgi_ctx *c, const uint16_t *blocks, const uint32_t dims[3]
     HIP_CHECK(hipMemsetAsync(V.dirty.p, 1, nch, c->stream));
     V.on = true;
     V.pending = true;
+    c->view_dirty = true;
     voxels_commit(c);
     return VRGI_OK;
     API_END
@@ -2336,6 +2358,7 @@ int vrgi_voxels_set(vrgi_ctx *c, uint64_t n, const uint32_t *xyz, const uint16_t
     HIP_CHECK(hipMemcpyAsync(V.edit_val.p, blocks, (size_t)2 * n, hipMemcpyHostToDevice, c->stream));
     launch_vx_set(c->stream, V.blocks.p, V.dims, n, V.edit_xyz.p, V.edit_val.p, V.dirty.p);
     V.pending = true;
+    c->view_dirty = true;
     return VRGI_OK;
     API_END
 }
@@ -2781,6 +2804,7 @@ int vrgi_batch_render(vrgi_batch *bt, const vrgi_camera *cams, const vrgi_prefs 
     API_BEGIN
     if (!bt || !cams || !prefs) return fail(VRGI_E_INVALID, "null argument");
     if (prefs->nee_type > 2) return fail(VRGI_E_INVALID, "nee_type must be 0, 1 or 2");
+    if (prefs->debug_view > 3) return fail(VRGI_E_INVALID, "debug_view must be in [0, 3] in a batch");
     if (prefs->restir_spatial_iterations > 20) return fail(VRGI_E_INVALID, "restir_spatial_iterations above 20");
     const uint32_t n = (uint32_t)bt->sc.size();
     for (vrgi_ctx *c : bt->sc)

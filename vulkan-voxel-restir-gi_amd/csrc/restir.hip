@@ -229,8 +229,19 @@ __global__ __launch_bounds__(256) void k_spatial_finalize(const FrameArgs fa) {
 // out-of-range conversion undefined)
 __device__ __forceinline__ void postprocess_px(const FrameArgs &fa, uint32_t id) {
     if (id >= fa.P) return;
-    const float *src = fa.debug_view == 0 ? fa.radiance : (fa.debug_view == 1 ? fa.restir : fa.debug);
-    f3 c = ld3(src + (size_t)id * 3) / (float)(1 * 1);
+    const uint32_t v = fa.debug_view;
+    f3 c;
+    if (v == 4) {  // running mean of the path-traced radiance since the last restart
+        const f3 L = ld3(fa.radiance + (size_t)id * 3);
+        const f3 a = fa.view_n == 1u ? L : ld3(fa.view_acc + (size_t)id * 3) + L;
+        st3(fa.view_acc + (size_t)id * 3, a);
+        c = a / (float)fa.view_n;
+    } else {
+        const float *src = v == 0 ? fa.radiance : ((v == 1 || v == 3) ? fa.restir : fa.debug);
+        c = ld3(src + (size_t)id * 3);
+    }
+    c = c / (float)(1 * 1);
+    if (v >= 3) c = mk(c.x / (1.0f + c.x), c.y / (1.0f + c.y), c.z / (1.0f + c.z));  // Reinhard, per channel
     auto cv = [](float v) -> uint32_t {
         float x = v * 255.0f;
         if (!(x > 0.0f)) return 0u;

@@ -103,7 +103,9 @@ struct FrameArgs {
     uint32_t tseed, sseed;
     uint32_t spatial_iters;
     float radius;
-    uint32_t debug_view;
+    uint32_t debug_view;  // 0 path tracer, 1 ReSTIR, 2 debug, 3 ReSTIR tonemapped, 4 accumulated path tracer
+    float *view_acc;      // view 4: [P][3] running sum of the path-traced radiance
+    uint32_t view_n;      // view 4: frames in the sum, this one included (1: restart)
     const float *origin, *normal, *emis, *refl, *mis, *pdf, *nee;
     const uint32_t *path_len;
     float *radiance;   // [P][3]
