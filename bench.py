@@ -7,8 +7,11 @@ ReSTIR GI temporal + spatial (5 neighbours), a full geometry + light-triangle BV
 rebuild every frame, camera on a 16-frame seeded Catmull-Rom spline.
 
 Second part of the metric (field "learned_cache"): time to equal RMSE of the learned
-radiance cache against the plain path tracer on the same scene at 1920x1080, static
-camera, both against an FP64 reference of the plain 8-bounce tracer at 256 spp.
+radiance cache against the plain path tracer on the same scene at 1920x1080: the cache
+trains online along the 16-frame camera spline (looped twice, every frame's time counted),
+its single frames at four spline poses are compared with the plain tracer's, then both
+accumulate 32 frames at pose 0; references are the plain 8-bounce tracer in FP64 (1024 spp
+at pose 0, 256 spp at the probe poses), their own noise measured and removed.
 
 Third part (field "batched", not the headline): 64 independent seeded scenes of uneven
 size (sides 32..160 voxels, 128^2..512^2 pixels), 2 spp, 3 bounces, NEE, ReSTIR DI (32
@@ -362,10 +365,10 @@ def main():
     cache = None
     if not a.no_cache_study:
         import convergence
-        log("learned cache vs plain: time to equal RMSE (1920x1080, FP64 reference 256 spp)")
-        cache = convergence.study(objs, mats, lum, cams[0], W_PX, H_PX, bounces=8, rr_start=RR_START, ref_spp=256,
-                                  frames=32, warmup=32, nee=NEE, spatial=SPATIAL, cache_kw=dict(query_vertex=0),
-                                  log=log)
+        log("learned cache vs plain: time to equal RMSE (1920x1080, spline camera, FP64 references)")
+        cache = convergence.study(objs, mats, lum, cams, W_PX, H_PX, bounces=8, rr_start=RR_START, ref_spp=1024,
+                                  frames=32, warmup=32, nee=NEE, spatial=SPATIAL, probe_poses=(0, 4, 8, 12),
+                                  probe_ref_spp=256, cache_kw=dict(query_vertex=0), log=log)
     batched = None
     if not a.no_batch:
         batched = batched_bench(a.steps)

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define VRGI_ABI_VERSION 1u
+#define VRGI_ABI_VERSION 2u
 
 /* status codes */
 #define VRGI_OK 0
@@ -270,9 +270,18 @@ typedef struct vrgi_cache_config {
                                    exceeds this traces full paths next frame; 0 = off (0) */
     uint32_t fallback_tile;     /* tile side in pixels (16) */
     float fallback_ema;         /* weight of a frame in the per-tile means (0.1) */
+    uint32_t precision;         /* VRGI_CACHE_BF16 (default) or VRGI_CACHE_FP16: weights, activations
+                                   and gradients on the matrix cores, fp32 accumulation */
 } vrgi_cache_config;
 
 #define VRGI_CACHE_FREEZE 1u  /* query only: render() takes no training steps */
+#define VRGI_CACHE_DYNAMIC_SCALE 2u /* dynamic loss scaling (always on in fp16): a step whose scaled
+                                      gradients overflow the 16-bit format is skipped and the scale
+                                      halved; the scale doubles after VRGI_CACHE_SCALE_INTERVAL good
+                                      steps in a row (up to 2^24) */
+#define VRGI_CACHE_SCALE_INTERVAL 1000u
+#define VRGI_CACHE_BF16 0u
+#define VRGI_CACHE_FP16 1u
 #define VRGI_CACHE_WEIGHTS (4 * 64 * 64 + 64 * 16) /* floats: layers 1-4 [k][n] 64x64, layer 5 [k][n] 64x16 */
 
 typedef struct vrgi_cache_info {
@@ -282,6 +291,9 @@ typedef struct vrgi_cache_info {
     uint64_t table_floats;      /* levels x 2^log2_table x feats */
     uint64_t steps;             /* Adam steps taken */
     float last_loss;            /* mean loss of the last frame's steps */
+    float loss_scale;           /* the current loss scale */
+    uint32_t precision;         /* VRGI_CACHE_BF16 / VRGI_CACHE_FP16 */
+    uint64_t skipped_steps;     /* steps skipped by dynamic loss scaling (overflow) */
 } vrgi_cache_info;
 
 void vrgi_cache_config_default(vrgi_cache_config *cfg);

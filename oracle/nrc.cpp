@@ -33,6 +33,28 @@ double bf16(double x) {  // double -> float -> bfloat16, round to nearest even
     return (double)f;
 }
 
+// double -> float -> IEEE binary16, round to nearest even, overflow to infinity
+double fp16(double x) {
+    const float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t sign = u & 0x80000000u, a = u & 0x7FFFFFFFu;
+    if (a >= 0x7F800000u) return (double)f;                                  // inf, NaN
+    const float af = std::fabs(f);
+    if (af >= 65520.0f) return sign ? -HUGE_VAL : HUGE_VAL;                  // rounds past 65504
+    if (af < 6.103515625e-05f) {                                             // subnormal: steps of 2^-24
+        const double q = std::nearbyint((double)af * 16777216.0) / 16777216.0;
+        return sign ? -q : q;
+    }
+    const uint32_t lsb = 1u << 13, rem = a & (lsb - 1u);                     // 23 -> 10 mantissa bits
+    uint32_t base = a & ~(lsb - 1u);
+    if (rem > lsb / 2 || (rem == lsb / 2 && (base & lsb))) base += lsb;
+    const uint32_t r = sign | base;
+    float out;
+    std::memcpy(&out, &r, 4);
+    return (double)out;
+}
+
 size_t w_off(int layer) { return (size_t)layer * IN * WID; }  // layer 0..4
 int w_cols(int layer) { return layer < HID ? WID : OUTP; }
 
@@ -45,10 +67,14 @@ struct or_nrc {
     std::vector<double> w, table, m_w, v_w, m_t, v_t;
     uint64_t step = 0;
     size_t T() const { return (size_t)1 << c.log2_T; }
-    double rnd(double x) const { return c.emulate_bf16 ? bf16(x) : x; }
+    // the rounding of the GPU's 16-bit points: 1 bf16, 2 fp16, 0 none
+    double rnd(double x) const { return c.emulate_bf16 == 1 ? bf16(x) : c.emulate_bf16 == 2 ? fp16(x) : x; }
+    bool emu() const { return c.emulate_bf16 != 0; }
 };
 
 extern "C" {
+
+double or_nrc_round16(double x, uint32_t mode) { return mode == 1 ? bf16(x) : mode == 2 ? fp16(x) : x; }
 
 uint32_t or_nrc_hash(uint32_t x, uint32_t y, uint32_t z, uint32_t log2_T) {
     uint32_t h = (x * 1u) ^ (y * 2654435761u) ^ (z * 805459861u);
@@ -147,14 +173,14 @@ void forward_one(const or_nrc *m, const double *x, double h[HID + 1][IN], double
             double s = 0.0;
             for (int k = 0; k < IN; ++k) s += h[l][k] * m->rnd(W[(size_t)k * WID + nn]);
             s = s > 0.0 ? s : 0.0;
-            h[l + 1][nn] = m->c.emulate_bf16 ? bf16(s) : s;
+            h[l + 1][nn] = m->emu() ? m->rnd(s) : s;
         }
     }
     const double *W5 = m->w.data() + w_off(HID);
     for (int nn = 0; nn < 3; ++nn) {
         double s = 0.0;
         for (int k = 0; k < WID; ++k) s += h[HID][k] * m->rnd(W5[(size_t)k * OUTP + nn]);
-        y[nn] = m->c.emulate_bf16 ? (double)(float)s : s;
+        y[nn] = m->emu() ? (double)(float)s : s;
     }
 }
 
@@ -177,7 +203,7 @@ double grads_impl(or_nrc *m, uint64_t n, const float *pos, const float *dir, con
             double den = m->c.loss == 0 ? y[c] * y[c] + 0.01 : 1.0;
             loss += e * e / den / (3.0 * n);
             double gg = S * 2.0 * e / den / (3.0 * n);
-            go[c] = m->c.emulate_bf16 ? bf16(gg) : gg;
+            go[c] = m->emu() ? m->rnd(gg) : gg;
         }
         // layer 5
         double *G5 = gw + w_off(HID);
@@ -187,7 +213,7 @@ double grads_impl(or_nrc *m, uint64_t n, const float *pos, const float *dir, con
             double s = 0.0;
             for (int c = 0; c < 3; ++c) s += m->rnd(W5[(size_t)k * OUTP + c]) * go[c];
             s = h[HID][k] > 0.0 ? s : 0.0;
-            g[k] = m->c.emulate_bf16 ? bf16(s) : s;
+            g[k] = m->emu() ? m->rnd(s) : s;
         }
         // hidden layers 4..1: g = gradient w.r.t. the output of layer l (post-mask)
         for (int l = HID - 1; l >= 0; --l) {
@@ -200,9 +226,9 @@ double grads_impl(or_nrc *m, uint64_t n, const float *pos, const float *dir, con
                 for (int nn = 0; nn < WID; ++nn) s += m->rnd(W[(size_t)k * WID + nn]) * g[nn];
                 if (l > 0) {
                     s = h[l][k] > 0.0 ? s : 0.0;
-                    gn[k] = m->c.emulate_bf16 ? bf16(s) : s;
+                    gn[k] = m->emu() ? m->rnd(s) : s;
                 } else {
-                    gn[k] = m->c.emulate_bf16 ? (double)(float)s : s;  // input gradient, fp32
+                    gn[k] = m->emu() ? (double)(float)s : s;  // input gradient, fp32
                 }
             }
             std::memcpy(g, gn, sizeof(g));

@@ -27,7 +27,7 @@ typedef struct or_nrc_cfg {
     float lr, beta1, beta2, eps; // Adam
     float loss_scale;      // multiplies the output gradient; divided out before Adam
     uint32_t loss;         // 0: relative L2 (pred - y)^2 / (sg(pred)^2 + 0.01); 1: plain L2
-    uint32_t emulate_bf16; // 1: round at the GPU's bf16 points
+    uint32_t emulate_bf16; // round at the GPU's 16-bit points: 1 bf16, 2 fp16 (IEEE binary16), 0 none
 } or_nrc_cfg;
 
 typedef struct or_nrc or_nrc;
@@ -35,6 +35,8 @@ typedef struct or_nrc or_nrc;
 or_nrc *or_nrc_create(const or_nrc_cfg *cfg, const float *weights, const float *table);
 void or_nrc_destroy(or_nrc *m);
 uint32_t or_nrc_hash(uint32_t x, uint32_t y, uint32_t z, uint32_t log2_T);
+// the 16-bit rounding of emulate mode `mode` (1 bf16, 2 fp16) of a double
+double or_nrc_round16(double x, uint32_t mode);
 void or_nrc_sh16(const double *v, double *out16);
 // n samples, each: position (3), direction (3), normal (3), albedo (3)
 void or_nrc_encode(or_nrc *m, uint64_t n, const float *pos, const float *dir, const float *nrm,

@@ -45,6 +45,7 @@ def lib():
         L.or_nrc_grads.restype = C.c_double
         L.or_nrc_train_step.restype = C.c_double
         L.or_restir_gi_jacobian.restype = C.c_float
+        L.or_nrc_round16.restype = C.c_double
         _lib = L
     return _lib
 
@@ -386,6 +387,10 @@ def nrc_hash(x, y, z, log2_T):
     return lib().or_nrc_hash(C.c_uint32(x), C.c_uint32(y), C.c_uint32(z), C.c_uint32(log2_T))
 
 
+def nrc_round16(x, mode):
+    return lib().or_nrc_round16(C.c_double(x), C.c_uint32(mode))
+
+
 def nrc_sh16(v):
     out = np.zeros(16, np.float64)
     lib().or_nrc_sh16(ptr(np.ascontiguousarray(v, np.float64)), ptr(out))
@@ -396,11 +401,11 @@ class Nrc:
     """FP64 learned radiance cache (encoding, MLP, relative L2 loss, backprop, Adam)."""
 
     def __init__(self, levels, feats, log2_T, res, box_lo, box_hi, weights, table, lr=1e-2, beta1=0.9,
-                 beta2=0.99, eps=1e-10, loss_scale=128.0, loss=0, emulate_bf16=False):
+                 beta2=0.99, eps=1e-10, loss_scale=128.0, loss=0, emulate_bf16=False, emulate_fp16=False):
         self.L = lib()
         r = (C.c_uint32 * 16)(*list(res))
         self.cfg = NrcConfig(levels, feats, log2_T, r, (C.c_float * 3)(*box_lo), (C.c_float * 3)(*box_hi),
-                             lr, beta1, beta2, eps, loss_scale, loss, int(emulate_bf16))
+                             lr, beta1, beta2, eps, loss_scale, loss, 2 if emulate_fp16 else int(bool(emulate_bf16)))
         self.n_table = levels * (1 << log2_T) * feats
         w = np.ascontiguousarray(weights, np.float32)
         t = np.ascontiguousarray(table, np.float32)

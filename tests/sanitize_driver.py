@@ -74,7 +74,7 @@ def _world0():
 def run_host():
     import vrgi
     L = vrgi.lib()
-    assert L.vrgi_abi_version() == 1
+    assert L.vrgi_abi_version() == 2
     cfg = vrgi.Config()
     L.vrgi_config_reference(C.byref(cfg), 64, 48)
     # worlds, materials, translucency, the CPU mesher, edits

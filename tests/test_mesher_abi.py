@@ -86,7 +86,7 @@ def test_abi_argument_errors_without_device():
     cfg = vrgi.Config()
     L.vrgi_config_reference(ctypes.byref(cfg), 1920, 1080)
     assert (cfg.num_bounces, cfg.restir_spatial_radius, cfg.rr_start_bounce) == (3, 15.0, 0xFFFFFFFF)
-    assert L.vrgi_abi_version() == 1
+    assert L.vrgi_abi_version() == 2
     h = ctypes.c_void_p()
     assert L.vrgi_create(None, None, None, 0, 0, ctypes.byref(h)) == -1
     assert b"null" in L.vrgi_last_error()

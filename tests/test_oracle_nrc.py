@@ -206,3 +206,25 @@ def test_bf16_emulation_close_to_fp64():
     inputs = _samples(64, rng)
     y64, y16 = net.forward(*inputs), em.forward(*inputs)
     assert np.abs(y16 - y64).max() <= 0.05 * np.abs(y64).max()
+
+
+def test_fp16_rounding_matches_numpy():
+    """The oracle's binary16 emulation (double -> float -> half, round to nearest even,
+    overflow to infinity, subnormals) against numpy's float16 conversion, an independent
+    implementation, on special values and 20000 random magnitudes over the whole range;
+    bf16 against its bit-level definition (round the float's upper 16 bits, ties to even)."""
+    import oracle
+    special = [0.0, -0.0, 1.0, -1.0, 1 + 2 ** -11, 1 + 3 * 2 ** -11, 65504.0, 65519.99, 65520.0, -70000.0,
+               2 ** -24, 2 ** -25, 1.5 * 2 ** -25, 2 ** -14, 2 ** -14 - 2 ** -25, 6e-8, 1e-3, 3.14159, 1e5]
+    rng = np.random.default_rng(0)
+    vals = special + list(rng.choice([-1, 1], 20000) * np.exp(rng.uniform(np.log(1e-9), np.log(1e6), 20000)))
+    for v in vals:
+        with np.errstate(over="ignore"):
+            want = float(np.float16(np.float32(v)))
+        got = oracle.nrc_round16(v, 2)
+        assert (np.isinf(want) and got == want) or got == want, (v, got, want)
+    for v in vals[:2000]:
+        u = int(np.array([v], np.float32).view(np.uint32)[0])
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+        want = float(np.array([u], np.uint32).view(np.float32)[0])
+        assert oracle.nrc_round16(v, 1) == want, v

@@ -234,6 +234,7 @@ struct vrgi_ctx {
         DevBuf<float> t_acc, t_ema, hpred;
         DevBuf<uint8_t> t_fb;
         DevBuf<uint32_t> counts;  // [0] training records, [1] query records, [2] caller records
+        DevBuf<float> scale_state;  // dynamic loss scaling: scale, good steps, overflow flag, skipped steps
     } cache;
 
     // FP64 running sum of the radiance output (vrgi_accum_*) over accum_P pixels
@@ -317,6 +318,7 @@ NrcNet cache_net(const vrgi_ctx *c) {
     n.table = c->cache.table.p;
     n.wt = c->cache.wt.p;
     n.wb = c->cache.wb.p;
+    n.prec = c->cache.cfg.precision;
     return n;
 }
 
@@ -335,6 +337,10 @@ void cache_box_from_scene(vrgi_ctx *c) {
         for (int k = 0; k < 3; ++k) { C.lo[k] = 0.0f; C.hi[k] = 1.0f; }
     }
     C.box_ready = true;
+}
+
+bool cache_dynamic_scale(const vrgi_cache_config &cfg) {
+    return cfg.precision == VRGI_CACHE_FP16 || (cfg.flags & VRGI_CACHE_DYNAMIC_SCALE);
 }
 
 // one training step over records rec[k + i * steps] (at most `batch`): gradients only, or
@@ -356,9 +362,13 @@ void cache_step(vrgi_ctx *c, hipStream_t s, const float *rec, const uint32_t *co
     tr.gw = C.gw.p;
     tr.gtable = C.gtable.p;
     tr.loss_out = C.loss.p;
-    if (!apply) {
+    if (!apply) {  // gradients only (a test hook): at the fixed configured scale
         launch_nrc_grads(s, cache_net(c), tr);
         return;
+    }
+    if (cache_dynamic_scale(C.cfg)) {
+        tr.scale_state = C.scale_state.p;
+        tr.scale_interval = VRGI_CACHE_SCALE_INTERVAL;
     }
     C.steps += 1;
     NrcAdam ad{};
@@ -2011,7 +2021,11 @@ void cache_clear_state(vrgi_ctx *c) {
     HIP_CHECK(hipMemsetAsync(C.gtable.p, 0, C.n_table * 4, s));
     HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, s));
     C.steps = 0;
-    launch_nrc_pack(s, C.w32.p, C.wt.p, C.wb.p);
+    const float st[4] = {C.cfg.loss_scale, 0.0f, 0.0f, 0.0f};  // scale, good steps, overflow flag, skipped
+    C.scale_state.reserve(4);
+    HIP_CHECK(hipMemcpyAsync(C.scale_state.p, st, 16, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));  // st is released on return
+    launch_nrc_pack(s, C.w32.p, C.wt.p, C.wb.p, C.cfg.precision);
 }
 
 void cache_planes(vrgi_ctx *c, uint32_t batch) {
@@ -2075,6 +2089,7 @@ int vrgi_cache_enable(vrgi_ctx *c, const vrgi_cache_config *cfg) {
         return fail(VRGI_E_INVALID, "batch must be a multiple of 16 in [16, 2^20]");
     if (cfg->steps_per_frame > 64) return fail(VRGI_E_INVALID, "steps_per_frame above 64");
     if (cfg->query_vertex > 1) return fail(VRGI_E_INVALID, "query_vertex must be 0 or 1");
+    if (cfg->precision > VRGI_CACHE_FP16) return fail(VRGI_E_INVALID, "precision must be VRGI_CACHE_BF16 or VRGI_CACHE_FP16");
     if (!(cfg->fallback_bound >= 0.0f) || cfg->fallback_tile == 0 || cfg->fallback_tile > 4096 ||
         !(cfg->fallback_ema > 0.0f && cfg->fallback_ema <= 1.0f))
         return fail(VRGI_E_INVALID, "fallback_bound >= 0, fallback_tile in [1, 4096], fallback_ema in (0, 1]");
@@ -2152,9 +2167,14 @@ int vrgi_cache_info_get(vrgi_ctx *c, vrgi_cache_info *out) {
     for (int k = 0; k < 3; ++k) { out->box_lo[k] = C.lo[k]; out->box_hi[k] = C.hi[k]; }
     out->table_floats = C.n_table;
     out->steps = C.steps;
-    float l = 0.0f;
+    out->precision = C.cfg.precision;
+    float l = 0.0f, st[4] = {C.cfg.loss_scale, 0.0f, 0.0f, 0.0f};
     HIP_CHECK(hipMemcpyAsync(&l, C.loss.p, 4, hipMemcpyDeviceToHost, c->stream));
+    if (cache_dynamic_scale(C.cfg) && C.scale_state.p)
+        HIP_CHECK(hipMemcpyAsync(st, C.scale_state.p, 16, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    out->loss_scale = st[0];
+    out->skipped_steps = (uint64_t)st[3];
     out->last_loss = C.cfg.steps_per_frame ? l / (float)C.cfg.steps_per_frame : l;
     return VRGI_OK;
     API_END

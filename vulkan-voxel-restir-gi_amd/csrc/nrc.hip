@@ -6,14 +6,20 @@
 // mod T), real spherical harmonics l < 4 of the ray direction and of the normal,
 // albedo, a constant 1 (the first layer's bias), 4 zeros.
 //
-// MLP on v_mfma_f32_16x16x16_bf16 (fp32 accumulation). A wave works on blocks of 16
-// samples in the transposed form Y^T = W^T X^T: the A operand is 16 output neurons x 16
-// inputs of the weights ([n][k] bf16, 8-byte loads), the B operand 16 inputs x 16
-// samples. The accumulator of output tile t (lane l: neurons 4(l/16)+j, sample l%16) is,
-// after ReLU and rounding to bf16, exactly the B operand of k-step t of the next layer,
-// so activations never leave the registers. Lane (s = l%16, g = l/16) therefore owns the
+// MLP on the matrix cores, fp32 accumulation, in bf16 or fp16 (the precision P of every
+// kernel below): gfx950's v_mfma_f32_16x16x32_{bf16,f16} for the layers' products,
+// v_mfma_f32_16x16x16_{bf16,f16} where the contraction is 16 long. A wave works on
+// blocks of 16 samples in the transposed form Y^T = W^T X^T: the A operand is 16 output
+// neurons x inputs of the weights ([n][k]), the B operand inputs x 16 samples. The
+// accumulator of output tile t (lane l: neurons 4(l/16)+j, sample l%16) is, after ReLU
+// and rounding to 16 bits, the B operand of the next layer for inputs 16t + 4g + j, so
+// activations never leave the registers. Lane (s = l%16, g = l/16) therefore owns the
 // input features 16t + 4g + j, i.e. feature groups q = 4t + g of four: groups 0-5 are
 // grid levels 2q and 2q+1, 6-9 direction SH, 10-13 normal SH, 14 albedo + 1, 15 zeros.
+// A 32-long step pairs tiles 2u and 2u + 1: lane group g brings inputs 32u + 4g + j and
+// 32u + 16 + 4g + j (j < 4) as its 8 elements, in B and, from the same [n][k] rows, in A;
+// the instruction pairs A's and B's elements by (lane group, element), so the product is
+// the same sum over k whatever order the hardware gives the 32 elements.
 //
 // Backward: the same trick with the weights in [k][n] layout (gradient w.r.t. the
 // inputs of a layer, ReLU mask from the stored bf16 activations). Layer inputs and
@@ -23,8 +29,12 @@
 
 namespace vrgi {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));  // four 16-bit values (bf16 or fp16 bits)
+typedef short h16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16v4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16v8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
     uint32_t u = __float_as_uint(f);
@@ -32,14 +42,58 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// the 16-bit format of the network: P = 0 bf16, P = 1 fp16 (IEEE binary16, round to
+// nearest even, overflow to infinity)
+template <int P> struct Prec;
+template <> struct Prec<0> {
+    static __device__ __forceinline__ uint16_t cvt(float f) { return f2bf(f); }
+    static __device__ __forceinline__ float back(uint16_t h) { return bf2f(h); }
+    static __device__ __forceinline__ f32x4 mma16(bf16x4 a, bf16x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ f32x4 mma32(h16x8 a, h16x8 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bfv8, a), __builtin_bit_cast(bfv8, b), c,
+                                                       0, 0, 0);
+    }
+};
+template <> struct Prec<1> {
+    static __device__ __forceinline__ uint16_t cvt(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
+    static __device__ __forceinline__ float back(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+    static __device__ __forceinline__ f32x4 mma16(bf16x4 a, bf16x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16v4, a), __builtin_bit_cast(f16v4, b), c,
+                                                     0, 0, 0);
+    }
+    static __device__ __forceinline__ f32x4 mma32(h16x8 a, h16x8 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16v8, a), __builtin_bit_cast(f16v8, b), c,
+                                                      0, 0, 0);
+    }
+};
+template <int P>
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
     bf16x4 r;
-    r[0] = (short)f2bf(a); r[1] = (short)f2bf(b); r[2] = (short)f2bf(c); r[3] = (short)f2bf(d);
+    r[0] = (short)Prec<P>::cvt(a); r[1] = (short)Prec<P>::cvt(b);
+    r[2] = (short)Prec<P>::cvt(c); r[3] = (short)Prec<P>::cvt(d);
+    return r;
+}
+// packs and records whether a value did not survive the rounding (overflow, NaN)
+template <int P>
+__device__ __forceinline__ bf16x4 pack4c(float a, float b, float c, float d, bool &bad) {
+    bf16x4 r = pack4<P>(a, b, c, d);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bad |= !isfinite(Prec<P>::back((uint16_t)r[i]));
     return r;
 }
 __device__ __forceinline__ bf16x4 ld_bf4(const uint16_t *p) { return *(const bf16x4 *)p; }
-__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ h16x8 cat8(bf16x4 lo, bf16x4 hi) {
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+// the two 32-long steps of a 64-long contraction along a [.][64] row at w (lane group g)
+template <int P>
+__device__ __forceinline__ f32x4 dot64(const uint16_t *w, uint32_t g, const bf16x4 x[4], f32x4 acc) {
+    acc = Prec<P>::mma32(cat8(ld_bf4(w + 4 * g), ld_bf4(w + 16 + 4 * g)), cat8(x[0], x[1]), acc);
+    acc = Prec<P>::mma32(cat8(ld_bf4(w + 32 + 4 * g), ld_bf4(w + 48 + 4 * g)), cat8(x[2], x[3]), acc);
+    return acc;
 }
 
 // -------------------------------------------------------------------- encoding
@@ -146,37 +200,36 @@ __device__ __forceinline__ f32x4 feature_group(const NrcNet &net, const Sample &
 }
 
 // encoding of one sample as the four B operands of layer 1 (lane group g)
+template <int P>
 __device__ __forceinline__ void encode(const NrcNet &net, const Sample &s, uint32_t g, bool live, bf16x4 x[4]) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         f32x4 v = live ? feature_group(net, s, 4 * t + g) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        x[t] = pack4(v[0], v[1], v[2], v[3]);
+        x[t] = pack4<P>(v[0], v[1], v[2], v[3]);
     }
 }
 
-// one hidden layer forward: h[nt] = bf16(relu(sum_t Wt[nt-tile][t-step] x[t]))
+// one hidden layer forward: h[nt] = round(relu(sum_k Wt[16 nt + row][k] x[k]))
+template <int P>
 __device__ __forceinline__ void layer_fwd(const uint16_t *wt, const bf16x4 x[4], bf16x4 h[4], uint32_t lane) {
     const uint32_t row = lane & 15u, g = lane >> 4;
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc = mfma(ld_bf4(wt + (16 * nt + row) * 64 + 16 * t + 4 * g), x[t], acc);
-        h[nt] = pack4(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
+        f32x4 acc = dot64<P>(wt + (16 * nt + row) * 64, g, x, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+        h[nt] = pack4<P>(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f), fmaxf(acc[3], 0.0f));
     }
 }
+template <int P>
 __device__ __forceinline__ f32x4 layer_out(const uint16_t *wt5, const bf16x4 x[4], uint32_t lane) {
     const uint32_t row = lane & 15u, g = lane >> 4;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc = mfma(ld_bf4(wt5 + row * 64 + 16 * t + 4 * g), x[t], acc);
-    return acc;  // lanes 0..15: outputs 0..3 of sample lane
+    return dot64<P>(wt5 + row * 64, g, x, f32x4{0.0f, 0.0f, 0.0f, 0.0f});  // lanes 0..15: outputs 0..3
 }
 
 // -------------------------------------------------------------------- inference
 // records: 16 floats (position, direction, normal, albedo, target, key); out: 3 floats
 // per record; when out_px is set, also out_px[3 * key + c] (negative outputs clamp to 0)
 // `count` may exceed the buffer's capacity `max_n` (producers count the records they drop)
+template <int P>
 __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float *rec, const uint32_t *count,
                                                    uint32_t max_n, float *out, float *out_px) {
     const uint32_t lane = threadIdx.x & 63u, s = lane & 15u, g = lane >> 4;
@@ -190,14 +243,14 @@ __global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float *rec,
         Sample sm{};
         if (live) sm = load_sample(rec + (size_t)i * 16);
         bf16x4 x[4], h[4];
-        encode(net, sm, g, live, x);
+        encode<P>(net, sm, g, live, x);
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
-            layer_fwd(net.wt + l * 4096, x, h, lane);
+            layer_fwd<P>(net.wt + l * 4096, x, h, lane);
 #pragma unroll
             for (int t = 0; t < 4; ++t) x[t] = h[t];
         }
-        f32x4 y = layer_out(net.wt + 4 * 4096, x, lane);
+        f32x4 y = layer_out<P>(net.wt + 4 * 4096, x, lane);
         if (g == 0 && live) {
             if (out) {
                 out[3 * (size_t)i] = y[0];
@@ -223,8 +276,13 @@ __device__ __forceinline__ uint32_t batch_size(uint32_t count, uint32_t cap, uin
     return b < max_batch ? b : max_batch;
 }
 
+template <int P>
 __global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
     const uint32_t lane = threadIdx.x & 63u, s = lane & 15u, g = lane >> 4;
+    // dynamic loss scaling: the scale of this step, and whether a scaled gradient or the
+    // output did not fit the 16-bit format (the step is then skipped)
+    const float lscale = tr.scale_state ? tr.scale_state[0] : tr.loss_scale;
+    bool bad = false;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t bsz = batch_size(*tr.count, tr.rec_cap, tr.step_k, tr.steps, tr.max_batch);
@@ -244,10 +302,10 @@ __global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
             tgt[0] = r[12]; tgt[1] = r[13]; tgt[2] = r[14];
         }
         bf16x4 a[5][4];  // inputs of layers 1..5
-        encode(net, sm, g, live, a[0]);
+        encode<P>(net, sm, g, live, a[0]);
 #pragma unroll
-        for (int l = 0; l < 4; ++l) layer_fwd(net.wt + l * 4096, a[l], a[l + 1], lane);
-        f32x4 y = layer_out(net.wt + 4 * 4096, a[4], lane);
+        for (int l = 0; l < 4; ++l) layer_fwd<P>(net.wt + l * 4096, a[l], a[l + 1], lane);
+        f32x4 y = layer_out<P>(net.wt + 4 * 4096, a[4], lane);
         // output gradient (relative L2, denominator not differentiated), scaled
         float go[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (g == 0 && live) {
@@ -256,11 +314,12 @@ __global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
                 float e = y[c] - tgt[c];
                 float den = tr.loss == 0 ? y[c] * y[c] + 0.01f : 1.0f;
                 loss += e * e / den * inv;
-                go[c] = tr.loss_scale * 2.0f * e / den * inv;
+                go[c] = lscale * 2.0f * e / den * inv;
+                bad |= !isfinite(y[c]);
             }
         }
         bf16x4 gcur[4];
-        gcur[0] = pack4(go[0], go[1], go[2], go[3]);
+        gcur[0] = pack4c<P>(go[0], go[1], go[2], go[3], bad);
         // save layer-5 input and output gradient planes
         {
             uint16_t *X = tr.xplane + (size_t)4 * 64 * B;
@@ -279,11 +338,11 @@ __global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
                 f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                acc = mfma(ld_bf4(wb5 + (16 * mt + s) * 16 + 4 * g), gcur[0], acc);
+                acc = Prec<P>::mma16(ld_bf4(wb5 + (16 * mt + s) * 16 + 4 * g), gcur[0], acc);
                 float v[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = bf2f((uint16_t)a[4][mt][r]) > 0.0f ? acc[r] : 0.0f;
-                gn[mt] = pack4(v[0], v[1], v[2], v[3]);
+                for (int r = 0; r < 4; ++r) v[r] = Prec<P>::back((uint16_t)a[4][mt][r]) > 0.0f ? acc[r] : 0.0f;
+                gn[mt] = pack4c<P>(v[0], v[1], v[2], v[3], bad);
             }
         }
         // hidden layers 4..1: gn = gradient w.r.t. the output of layer l + 1
@@ -302,18 +361,17 @@ __global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
             const uint16_t *wb = net.wb + l * 4096;
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                gin[mt] = dot64<P>(wb + (16 * mt + s) * 64, g, gn, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt) acc = mfma(ld_bf4(wb + (16 * mt + s) * 64 + 16 * nt + 4 * g), gn[nt], acc);
-                gin[mt] = acc;
+                for (int r = 0; r < 4; ++r) bad |= !isfinite(gin[mt][r]);
             }
             if (l > 0) {
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt) {
                     float v[4];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) v[r] = bf2f((uint16_t)a[l][mt][r]) > 0.0f ? gin[mt][r] : 0.0f;
-                    gn[mt] = pack4(v[0], v[1], v[2], v[3]);
+                    for (int r = 0; r < 4; ++r) v[r] = Prec<P>::back((uint16_t)a[l][mt][r]) > 0.0f ? gin[mt][r] : 0.0f;
+                    gn[mt] = pack4c<P>(v[0], v[1], v[2], v[3], bad);
                 }
             }
         }
@@ -340,9 +398,11 @@ __global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
     // batch loss (unscaled)
     for (int off = 32; off >= 1; off >>= 1) loss += __shfl_xor(loss, off, 64);
     if (lane == 0 && loss != 0.0f) atomicAdd(tr.loss_out, loss);
+    if (tr.scale_state && __ballot(bad) && lane == 0) atomicOr((uint32_t *)(tr.scale_state + 2), 1u);
 }
 
 // dW of layer `layer` = X G^T over the batch (split K): one wave per (tile, K chunk)
+template <int P>
 __global__ __launch_bounds__(64) void k_nrc_dw(NrcTrain tr, uint32_t chunk) {
     const uint32_t lane = threadIdx.x, s = lane & 15u, g = lane >> 4;
     const uint32_t layer = blockIdx.z;
@@ -359,7 +419,7 @@ __global__ __launch_bounds__(64) void k_nrc_dw(NrcTrain tr, uint32_t chunk) {
     const uint16_t *X = tr.xplane + (size_t)layer * 64 * B + (size_t)(16 * mt + s) * B;
     const uint16_t *G = tr.gplane + (size_t)layer * 64 * B + (size_t)(16 * nt + s) * B;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (uint32_t i = k0; i < k1; i += 16) acc = mfma(ld_bf4(X + i + 4 * g), ld_bf4(G + i + 4 * g), acc);
+    for (uint32_t i = k0; i < k1; i += 16) acc = Prec<P>::mma16(ld_bf4(X + i + 4 * g), ld_bf4(G + i + 4 * g), acc);
     // D[k = 16 mt + 4 g + r][n = 16 nt + s]
     const uint32_t ncols = layer < 4 ? 64u : 16u;
     float *gw = tr.gw + (size_t)layer * 4096;
@@ -370,11 +430,17 @@ __global__ __launch_bounds__(64) void k_nrc_dw(NrcTrain tr, uint32_t chunk) {
     }
 }
 
-// Adam (fp32): gradients carry the loss scale; they are cleared after use
-__global__ __launch_bounds__(256) void k_nrc_adam(float *p, float *gr, float *m, float *v, size_t n, NrcAdam a) {
+// Adam (fp32): gradients carry the loss scale; they are cleared after use. With dynamic
+// loss scaling (scale_state set) the scale is the step's own, and a step whose gradients
+// overflowed is skipped (gradients cleared, parameters and moments unchanged).
+__global__ __launch_bounds__(256) void k_nrc_adam(float *p, float *gr, float *m, float *v, size_t n, NrcAdam a,
+                                                  const float *scale_state) {
+    const bool skip = scale_state && *(const uint32_t *)(scale_state + 2) != 0u;
+    const float inv_scale = scale_state ? 1.0f / scale_state[0] : a.inv_scale;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const float gg = gr[i] * a.inv_scale;
+        const float gg = gr[i] * inv_scale;
         gr[i] = 0.0f;
+        if (skip) continue;
         const float mm = a.beta1 * m[i] + (1.0f - a.beta1) * gg;
         const float vv = a.beta2 * v[i] + (1.0f - a.beta2) * gg * gg;
         m[i] = mm;
@@ -384,7 +450,31 @@ __global__ __launch_bounds__(256) void k_nrc_adam(float *p, float *gr, float *m,
     }
 }
 
-// fp32 master weights -> bf16 in both matrix-core layouts
+// dynamic loss scaling after a step (state: scale, good steps in a row, overflow flag,
+// skipped steps): halve the scale (not below 1) and count the step as skipped when it
+// overflowed, double it (up to 2^24) after `interval` good steps in a row
+__global__ void k_nrc_scale_update(float *st, uint32_t interval) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t *flag = (uint32_t *)(st + 2);
+    float scale = st[0], good = st[1];
+    if (*flag) {
+        scale = fmaxf(scale * 0.5f, 1.0f);
+        good = 0.0f;
+        *flag = 0u;
+        st[3] += 1.0f;
+    } else {
+        good += 1.0f;
+        if (good >= (float)interval) {
+            scale = fminf(scale * 2.0f, 16777216.0f);
+            good = 0.0f;
+        }
+    }
+    st[0] = scale;
+    st[1] = good;
+}
+
+// fp32 master weights -> 16-bit in both matrix-core layouts
+template <int P>
 __global__ __launch_bounds__(256) void k_nrc_pack(const float *w, uint16_t *wt, uint16_t *wb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NRC_N_WEIGHTS) return;
@@ -392,7 +482,7 @@ __global__ __launch_bounds__(256) void k_nrc_pack(const float *w, uint16_t *wt, 
     const uint32_t ncols = layer < 4 ? 64u : 16u;
     const uint32_t k = r / ncols, n = r % ncols;  // w is [k][n]
     if (layer == 4 && k >= 64) return;
-    const uint16_t h = f2bf(w[i]);
+    const uint16_t h = Prec<P>::cvt(w[i]);
     wb[i] = h;                                  // [k][n]
     wt[layer * 4096 + n * 64 + k] = h;          // [n][k]
 }
@@ -445,38 +535,45 @@ void launch_tile_update(hipStream_t s, float *acc, float *ema, uint8_t *fallback
     if (n_tiles) k_tile_update<<<(n_tiles + 255) / 256, 256, 0, s>>>(acc, ema, fallback, n_tiles, alpha, bound);
 }
 
+template <class F>
+static void with_prec(uint32_t prec, F f) {
+    if (prec == 1) f(std::integral_constant<int, 1>());
+    else f(std::integral_constant<int, 0>());
+}
+
 void launch_nrc_infer(hipStream_t s, const NrcNet &net, const float *rec, const uint32_t *count, uint32_t max_n,
                       float *out, float *out_px) {
     uint32_t blocks16 = (max_n + 15) / 16;
     uint32_t grid = std::min<uint32_t>((blocks16 + 3) / 4, 4096u);
-    if (grid) k_nrc_infer<<<grid, 256, 0, s>>>(net, rec, count, max_n, out, out_px);
-}
-
-void launch_nrc_train_step(hipStream_t s, const NrcNet &net, const NrcTrain &tr, float *w32, float *mw, float *vw,
-                           float *table, float *mt, float *vt, size_t n_table, const NrcAdam &adam) {
-    uint32_t blocks16 = (tr.max_batch + 15) / 16;
-    uint32_t grid = std::min<uint32_t>((blocks16 + 3) / 4, 2048u);
-    k_nrc_fwd_bwd<<<grid, 256, 0, s>>>(net, tr);
-    const uint32_t chunk = 1024;
-    dim3 g((tr.max_batch + 15 + chunk - 1) / chunk, 16, 5);
-    k_nrc_dw<<<g, 64, 0, s>>>(tr, chunk);
-    k_nrc_adam<<<(NRC_N_WEIGHTS + 255) / 256, 256, 0, s>>>(w32, tr.gw, mw, vw, NRC_N_WEIGHTS, adam);
-    k_nrc_adam<<<4096, 256, 0, s>>>(table, tr.gtable, mt, vt, n_table, adam);
-    k_nrc_pack<<<(NRC_N_WEIGHTS + 255) / 256, 256, 0, s>>>(w32, const_cast<uint16_t *>(net.wt),
-                                                          const_cast<uint16_t *>(net.wb));
+    if (!grid) return;
+    with_prec(net.prec, [&](auto Pc) { k_nrc_infer<decltype(Pc)::value><<<grid, 256, 0, s>>>(net, rec, count, max_n, out, out_px); });
 }
 
 void launch_nrc_grads(hipStream_t s, const NrcNet &net, const NrcTrain &tr) {
     uint32_t blocks16 = (tr.max_batch + 15) / 16;
     uint32_t grid = std::min<uint32_t>((blocks16 + 3) / 4, 2048u);
-    k_nrc_fwd_bwd<<<grid, 256, 0, s>>>(net, tr);
     const uint32_t chunk = 1024;
     dim3 g((tr.max_batch + 15 + chunk - 1) / chunk, 16, 5);
-    k_nrc_dw<<<g, 64, 0, s>>>(tr, chunk);
+    with_prec(net.prec, [&](auto Pc) {
+        constexpr int P = decltype(Pc)::value;
+        k_nrc_fwd_bwd<P><<<grid, 256, 0, s>>>(net, tr);
+        k_nrc_dw<P><<<g, 64, 0, s>>>(tr, chunk);
+    });
 }
 
-void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb) {
-    k_nrc_pack<<<(NRC_N_WEIGHTS + 255) / 256, 256, 0, s>>>(w32, wt, wb);
+void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb, uint32_t prec) {
+    with_prec(prec, [&](auto Pc) {
+        k_nrc_pack<decltype(Pc)::value><<<(NRC_N_WEIGHTS + 255) / 256, 256, 0, s>>>(w32, wt, wb);
+    });
+}
+
+void launch_nrc_train_step(hipStream_t s, const NrcNet &net, const NrcTrain &tr, float *w32, float *mw, float *vw,
+                           float *table, float *mt, float *vt, size_t n_table, const NrcAdam &adam) {
+    launch_nrc_grads(s, net, tr);
+    k_nrc_adam<<<(NRC_N_WEIGHTS + 255) / 256, 256, 0, s>>>(w32, tr.gw, mw, vw, NRC_N_WEIGHTS, adam, tr.scale_state);
+    k_nrc_adam<<<4096, 256, 0, s>>>(table, tr.gtable, mt, vt, n_table, adam, tr.scale_state);
+    if (tr.scale_state) k_nrc_scale_update<<<1, 64, 0, s>>>(tr.scale_state, tr.scale_interval);
+    launch_nrc_pack(s, w32, const_cast<uint16_t *>(net.wt), const_cast<uint16_t *>(net.wb), net.prec);
 }
 
 void launch_nrc_zero(hipStream_t s, float *p, size_t n) {

@@ -456,8 +456,9 @@ struct NrcNet {
     uint32_t res[16];
     float lo[3], hi[3];       // box mapped to the unit cube
     const float *table;       // levels x 2^log2_T x 2, fp32
-    const uint16_t *wt;       // bf16 [n][k] per layer (forward A operand)
-    const uint16_t *wb;       // bf16 [k][n] per layer (backward A operand)
+    const uint16_t *wt;       // 16-bit [n][k] per layer (forward A operand)
+    const uint16_t *wb;       // 16-bit [k][n] per layer (backward A operand)
+    uint32_t prec;            // 0 bf16, 1 fp16
 };
 // training records: 16 floats = position, direction, normal, albedo, target, key
 struct NrcTrain {
@@ -468,9 +469,14 @@ struct NrcTrain {
     uint32_t max_batch;       // multiple of 16; row stride of the planes
     uint32_t loss;            // 0 relative L2, 1 plain L2
     float loss_scale;
-    uint16_t *xplane, *gplane;  // [5][64][max_batch] bf16: layer inputs, output gradients
+    uint16_t *xplane, *gplane;  // [5][64][max_batch] 16-bit: layer inputs, output gradients
     float *gw, *gtable;         // gradient accumulators (scaled)
     float *loss_out;
+    // dynamic loss scaling (null: the fixed loss_scale): scale, good steps in a row,
+    // overflow flag (u32 bits), skipped steps; the scale doubles after scale_interval
+    // good steps in a row
+    float *scale_state;
+    uint32_t scale_interval;
 };
 struct NrcAdam {
     float lr, beta1, beta2, eps;
@@ -482,7 +488,7 @@ void launch_nrc_infer(hipStream_t s, const NrcNet &net, const float *rec, const 
 void launch_nrc_train_step(hipStream_t s, const NrcNet &net, const NrcTrain &tr, float *w32, float *mw, float *vw,
                            float *table, float *mt, float *vt, size_t n_table, const NrcAdam &adam);
 void launch_nrc_grads(hipStream_t s, const NrcNet &net, const NrcTrain &tr);
-void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb);
+void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb, uint32_t prec);
 void launch_nrc_zero(hipStream_t s, float *p, size_t n);
 // held-out error per screen tile: records of vertex `vertex` against the cache's output
 void launch_tile_stats(hipStream_t s, const float *rec, const uint32_t *count, uint32_t cap, const float *pred,

@@ -26,6 +26,9 @@ assert VERTEX_DTYPE.itemsize == 24 and NODE_DTYPE.itemsize == 40 and MATERIAL_DT
  BUF_HIT_PRIM, BUF_HIT_T, BUF_DI_LIGHT, BUF_DI_WEIGHTS, BUF_BOUNCE_ALBEDO, BUF_CACHE_RADIANCE,
  BUF_CACHE_QUERIES, BUF_CACHE_TRAIN_RECORDS, BUF_CACHE_FALLBACK) = range(27)
 CACHE_FREEZE = 1
+CACHE_DYNAMIC_SCALE = 2
+CACHE_BF16, CACHE_FP16 = 0, 1
+ABI_VERSION = 2
 CACHE_WEIGHTS = 4 * 64 * 64 + 64 * 16
 
 VRGI_OK, VRGI_E_INVALID, VRGI_E_DEVICE, VRGI_E_STATE, VRGI_E_LIMIT = 0, -1, -2, -3, -4
@@ -62,13 +65,14 @@ class CacheConfig(C.Structure):
                 ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("loss_scale", C.c_float), ("seed", C.c_uint32), ("flags", C.c_uint32),
                 ("query_vertex", C.c_uint32), ("fallback_bound", C.c_float), ("fallback_tile", C.c_uint32),
-                ("fallback_ema", C.c_float)]
+                ("fallback_ema", C.c_float), ("precision", C.c_uint32)]
 
 
 class CacheInfo(C.Structure):
     _fields_ = [("levels", C.c_uint32), ("feats", C.c_uint32), ("log2_table", C.c_uint32),
                 ("res", C.c_uint32 * 16), ("box_lo", C.c_float * 3), ("box_hi", C.c_float * 3),
-                ("table_floats", C.c_uint64), ("steps", C.c_uint64), ("last_loss", C.c_float)]
+                ("table_floats", C.c_uint64), ("steps", C.c_uint64), ("last_loss", C.c_float),
+                ("loss_scale", C.c_float), ("precision", C.c_uint32), ("skipped_steps", C.c_uint64)]
 
 
 class Camera(C.Structure):
