@@ -229,6 +229,8 @@ struct vrgi_ctx {
         DevBuf<float> table_snap;
         DevBuf<uint16_t> wt_snap;
         DevBuf<float> trec, qrec, crad, hrec, hout, albedo, d0;
+        DevBuf<float> trec_sorted;                // the previous frame's records in Morton order
+        DevBuf<uint32_t> sort_hist, sort_kr;
         // per-tile held-out error and fallback flags
         uint32_t tiles_x = 0, tiles_y = 0;
         DevBuf<float> t_acc, t_ema, hpred;
@@ -1419,22 +1421,30 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         HIP_CHECK(hipMemsetAsync(c->cache.counts.p + 1, 0, 4, s));
         HIP_CHECK(hipMemsetAsync(tcnt_cur, 0, 4, s));
     }
-    if (cache_train) {
-        // the frame's training steps on the previous frame's records, on the build stream
-        // (idle once this frame's BVHs are built), after a snapshot of the network for this
-        // frame's inference; they run beside the frame's rays and inference
+    // the frame's training steps on the previous frame's records, on the build stream
+    // (idle once this frame's BVHs are built), after a snapshot of the network for this
+    // frame's inference; they run beside the frame's rays and inference. They start from
+    // the frame's start (ev_fstart) but are queued on the host after the first bounce's
+    // launch, so that their ~30 launches do not delay the frame's first rays.
+    if (cache_train) HIP_CHECK(hipEventRecord(c->ev_fstart, s));
+    auto enqueue_training = [&]() {
         auto &C = c->cache;
         hipStream_t ts = c->up_stream;
-        HIP_CHECK(hipEventRecord(c->ev_fstart, s));
         HIP_CHECK(hipStreamWaitEvent(ts, c->ev_fstart, 0));
         HIP_CHECK(hipMemcpyAsync(C.table_snap.p, C.table.p, C.n_table * sizeof(float), hipMemcpyDeviceToDevice, ts));
         HIP_CHECK(hipMemcpyAsync(C.wt_snap.p, C.wt.p, NRC_N_WEIGHTS * sizeof(uint16_t), hipMemcpyDeviceToDevice, ts));
         HIP_CHECK(hipEventRecord(c->ev_snap, ts));
         HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, ts));
+        C.trec_sorted.reserve((size_t)C.tcap * 16);
+        C.sort_hist.reserve(NRC_SORT_BUCKETS);
+        C.sort_kr.reserve((size_t)2 * C.tcap);
+        RecBox box;
+        for (int k = 0; k < 3; ++k) { box.lo[k] = C.lo[k]; box.hi[k] = C.hi[k]; }
+        launch_rec_sort(ts, trec_prev, tcnt_prev, C.tcap, box, C.sort_hist.p, C.sort_kr.p, C.trec_sorted.p);
         for (uint32_t k = 0; k < C.cfg.steps_per_frame; ++k)
-            cache_step(c, ts, trec_prev, tcnt_prev, C.tcap, k, C.cfg.steps_per_frame, C.cfg.batch, true);
+            cache_step(c, ts, C.trec_sorted.p, tcnt_prev, C.tcap, k, C.cfg.steps_per_frame, C.cfg.batch, true);
         HIP_CHECK(hipEventRecord(c->ev_train, ts));
-    }
+    };
     if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) {
         size_t n = (size_t)B * P;
         HIP_CHECK(hipMemsetAsync(c->f_origin.p, 0, n * 12, s));
@@ -1497,6 +1507,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         const uint32_t small_q = cache_on && b > c->cache.cfg.query_vertex ? VRGI_SMALL_QUEUE_GRID : 0u;
         launch_trace_shade(s, sv, ba, P, small_q);
         qin[b + 1] = ba.queue_out;
+        if (cache_train && b == 0) enqueue_training();  // before any wait on ev_snap
         if (cache_on && b == c->cache.cfg.query_vertex) {
             // side stream: cache radiance of this bounce's queries, by the network as the
             // frame started (the snapshot while the training steps run)

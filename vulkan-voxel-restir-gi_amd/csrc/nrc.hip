@@ -276,6 +276,31 @@ __device__ __forceinline__ uint32_t batch_size(uint32_t count, uint32_t cap, uin
     return b < max_batch ? b : max_batch;
 }
 
+// Adds (v0, v1) at tbl[2 idx], tbl[2 idx + 1] for each of the 16 lanes of a row (one
+// level, one corner, 16 samples). Runs of neighbouring lanes with the same idx are summed
+// first (a segmented prefix sum over the row) and only a run's last lane adds the sum:
+// with records in Morton order the samples of a row share the cells of the coarse levels,
+// and the adds, which go past the L2 to where the XCDs' copies of a line meet, are the
+// cost of the training step. idx 0xFFFFFFFF marks a lane without a sample.
+__device__ __forceinline__ void seg_atomic2(float *tbl, uint32_t idx, float v0, float v1, uint32_t s) {
+    const uint32_t prev = __shfl_up(idx, 1, 16), next = __shfl_down(idx, 1, 16);
+    int head = (s == 0 || prev != idx) ? 1 : 0;  // set: the summed window reaches the run's head
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+        const float a0 = __shfl_up(v0, d, 16), a1 = __shfl_up(v1, d, 16);
+        const int hp = __shfl_up(head, d, 16);
+        if (!head && s >= (uint32_t)d) {
+            v0 += a0;
+            v1 += a1;
+            head = hp;
+        }
+    }
+    if ((s == 15 || next != idx) && idx != 0xFFFFFFFFu) {
+        atomicAdd(tbl + (size_t)idx * 2, v0);
+        atomicAdd(tbl + (size_t)idx * 2 + 1, v1);
+    }
+}
+
 template <int P>
 __global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
     const uint32_t lane = threadIdx.x & 63u, s = lane & 15u, g = lane >> 4;
@@ -376,21 +401,17 @@ __global__ __launch_bounds__(256) void k_nrc_fwd_bwd(NrcNet net, NrcTrain tr) {
             }
         }
         // grid gradient: gin[mt][r] = d(loss S)/d(input 16 mt + 4 g + r); grid inputs < 24
-        if (live) {
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                const uint32_t q = 4 * mt + g;  // feature group
-                if (q < 6) {
+        for (int mt = 0; mt < 2; ++mt) {
+            const uint32_t q = 4 * mt + g;  // feature group (uniform over the lanes of a row)
+            if (q < 6) {
 #pragma unroll
-                    for (int h2 = 0; h2 < 2; ++h2) {
-                        GridCorners c = grid_corners(net, sm.p, 2 * q + h2);
-                        const float g0 = gin[mt][2 * h2], g1 = gin[mt][2 * h2 + 1];
+                for (int h2 = 0; h2 < 2; ++h2) {
+                    GridCorners c = grid_corners(net, sm.p, 2 * q + h2);
+                    const float g0 = gin[mt][2 * h2], g1 = gin[mt][2 * h2 + 1];
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            atomicAdd(tr.gtable + (size_t)c.idx[k] * 2, c.w[k] * g0);
-                            atomicAdd(tr.gtable + (size_t)c.idx[k] * 2 + 1, c.w[k] * g1);
-                        }
-                    }
+                    for (int k = 0; k < 8; ++k)
+                        seg_atomic2(tr.gtable, live ? c.idx[k] : 0xFFFFFFFFu, c.w[k] * g0, c.w[k] * g1, s);
                 }
             }
         }
@@ -533,6 +554,80 @@ void launch_tile_stats(hipStream_t s, const float *rec, const uint32_t *count, u
 void launch_tile_update(hipStream_t s, float *acc, float *ema, uint8_t *fallback, uint32_t n_tiles, float alpha,
                         float bound) {
     if (n_tiles) k_tile_update<<<(n_tiles + 255) / 256, 256, 0, s>>>(acc, ema, fallback, n_tiles, alpha, bound);
+}
+
+// ---------------------------------------------------------------- record ordering
+// The frame's training records in Morton order of their position (32 cells per axis of
+// the cache box): a counting sort (bucket sizes with the lane's rank inside its bucket,
+// one-block exclusive scan, scatter). Each training step then takes every steps-th record
+// of that order, an even sample of the whole scene whose neighbouring samples share the
+// coarse grid cells (see seg_atomic2).
+__device__ __forceinline__ uint32_t spread3_5(uint32_t v) {  // bit i -> bit 3i, 5 bits
+    v &= 31u;
+    v = (v | (v << 8)) & 0x0000f00fu;
+    v = (v | (v << 4)) & 0x000c30c3u;
+    v = (v | (v << 2)) & 0x00249249u;
+    return v;
+}
+__global__ __launch_bounds__(256) void k_rec_clear(uint32_t *hist) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < NRC_SORT_BUCKETS) hist[i] = 0u;
+}
+__global__ __launch_bounds__(256) void k_rec_hist(const float *rec, const uint32_t *count, uint32_t cap,
+                                                  RecBox box, uint32_t *hist, uint32_t *key_rank) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = min(*count, cap);
+    if (i >= n) return;
+    uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float t = (rec[(size_t)i * 16 + a] - box.lo[a]) / (box.hi[a] - box.lo[a]) * 32.0f;
+        t = t == t ? fminf(fmaxf(t, 0.0f), 31.0f) : 0.0f;
+        q[a] = (uint32_t)t;
+    }
+    const uint32_t key = spread3_5(q[0]) | (spread3_5(q[1]) << 1) | (spread3_5(q[2]) << 2);
+    const uint32_t rank = atomicAdd(hist + key, 1u);
+    key_rank[2 * (size_t)i] = key;
+    key_rank[2 * (size_t)i + 1] = rank;
+}
+// exclusive scan of the bucket sizes in place: one block of 1024 threads, 32 buckets each
+__global__ __launch_bounds__(1024) void k_rec_scan(uint32_t *hist) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x, per = NRC_SORT_BUCKETS / 1024u;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k) sum += hist[t * per + k];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t c = hist[t * per + k];
+        hist[t * per + k] = run;
+        run += c;
+    }
+}
+__global__ __launch_bounds__(256) void k_rec_scatter(const float *rec, const uint32_t *count, uint32_t cap,
+                                                     const uint32_t *start, const uint32_t *key_rank, float *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = min(*count, cap);
+    if (i >= n) return;
+    const uint32_t dst = start[key_rank[2 * (size_t)i]] + key_rank[2 * (size_t)i + 1];
+    const float4 *a = (const float4 *)(rec + (size_t)i * 16);
+    float4 *b = (float4 *)(out + (size_t)dst * 16);
+    b[0] = a[0]; b[1] = a[1]; b[2] = a[2]; b[3] = a[3];
+}
+void launch_rec_sort(hipStream_t s, const float *rec, const uint32_t *count, uint32_t cap, const RecBox &box,
+                     uint32_t *hist, uint32_t *key_rank, float *out) {
+    if (!cap) return;
+    k_rec_clear<<<NRC_SORT_BUCKETS / 256, 256, 0, s>>>(hist);
+    k_rec_hist<<<(cap + 255) / 256, 256, 0, s>>>(rec, count, cap, box, hist, key_rank);
+    k_rec_scan<<<1, 1024, 0, s>>>(hist);
+    k_rec_scatter<<<(cap + 255) / 256, 256, 0, s>>>(rec, count, cap, hist, key_rank, out);
 }
 
 template <class F>

@@ -490,6 +490,14 @@ void launch_nrc_train_step(hipStream_t s, const NrcNet &net, const NrcTrain &tr,
 void launch_nrc_grads(hipStream_t s, const NrcNet &net, const NrcTrain &tr);
 void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb, uint32_t prec);
 void launch_nrc_zero(hipStream_t s, float *p, size_t n);
+// training records in Morton order of their position in the cache box; hist:
+// NRC_SORT_BUCKETS words, key_rank: 2 words per record, out: cap records
+static constexpr uint32_t NRC_SORT_BUCKETS = 32768;
+struct RecBox {
+    float lo[3], hi[3];
+};
+void launch_rec_sort(hipStream_t s, const float *rec, const uint32_t *count, uint32_t cap, const RecBox &box,
+                     uint32_t *hist, uint32_t *key_rank, float *out);
 // held-out error per screen tile: records of vertex `vertex` against the cache's output
 void launch_tile_stats(hipStream_t s, const float *rec, const uint32_t *count, uint32_t cap, const float *pred,
                        uint32_t vertex, uint32_t W, uint32_t tile, uint32_t tiles_x, float *acc);
