@@ -21,9 +21,11 @@ same frames through 64 separate contexts (one render call per scene and pass).
 
 Fourth part (field "large_dynamic", not the headline): the 512x256x512 world (~5.6M
 exposed faces, ~40k emissive voxels) as a device-resident voxel world at 3840x2160, 4
-bounces, NEE, ReSTIR DI, learned cache; every frame 1% of the voxels are toggled by 64
-seeded random walkers, the dirty chunks re-meshed on the GPU and every BVH rebuilt;
-beside it the same frames without the cache.
+bounces, NEE, ReSTIR DI, learned cache in fp16 with dynamic loss scaling; 32 frames, each
+after 1% of the voxels are toggled by 64 seeded random walkers, the dirty chunks re-meshed
+on the GPU and every BVH rebuilt; beside it the same frames without the cache, both
+against FP64 references at four of the frames and on the final world (time to equal
+RMSE, convergence.dynamic_study).
 
 Prints ONE JSON line (rank 0). Progress goes to stderr.
   python bench.py [--gpus 1] [--steps 16] [--warmup 3] [--no-cpu-baseline] [--no-cache-study] [--no-batch]
@@ -207,11 +209,15 @@ def batched_bench(steps):
             "graph": bool(info.graph), "scenes_separately_ms": round(seq_ms, 3)}
 
 
-def large_dynamic_bench(frames=8):
-    """The large dynamic workload: ms per frame including the upload of the frame's voxel
-    edits, the re-mesh of the dirty chunks, the light and geometry rebuilds and the 4K
-    frame. The edits themselves (64 seeded random walkers) are generated beforehand."""
+def large_dynamic_bench(frames=32):
+    """The large dynamic workload: 32 frames of the 4K edited world, plain and with the
+    fp16 learned cache, ms per frame including the upload of the frame's voxel edits, the
+    re-mesh of the dirty chunks, the light and geometry rebuilds and the frame; the error
+    of both against FP64 references at four of the frames and the time to equal RMSE
+    (convergence.dynamic_study). The edits themselves (64 seeded random walkers) are
+    generated beforehand: they are the caller's side (a game's edits)."""
     import vrgi
+    import convergence
     log("large dynamic workload: 512x256x512 world")
     blocks0, nt = vrgi.world(4, 0)
     mats, lum = vrgi.world_materials(4, 0)
@@ -220,45 +226,28 @@ def large_dynamic_bench(frames=8):
     steps = nvox // 100  # 1% of the voxels per frame
     dx, dy, dz = blocks0.shape
     cam = vrgi.camera_look([0.1 * dx, 0.9 * dy, 0.1 * dz], [0.6 * dx, 0.15 * dy, 0.6 * dz])
-    out = {"workload": "512x256x512 terrain+400 boxes as a device voxel world, 3840x2160, 1 spp, 4 bounces, "
-                       "RR from bounce 2, NEE, ReSTIR DI 32 candidates + temporal + 5 spatial; 1% of the voxels "
-                       "toggled per frame by 64 seeded random walkers; dirty chunks re-meshed, all BVHs rebuilt",
-           "voxels_toggled_per_frame": steps, "frames": frames}
-    # the edits of every frame, generated before the timed frames: the seeded walkers are
-    # the caller's side (a game's edits), about 80 ms of host time per frame here, which
-    # would otherwise hide the library's own work
     blocks = blocks0.copy()
     rng = np.random.default_rng(4)
     walkers = np.stack([rng.integers(0, dx, 64), rng.integers(dy // 4, dy // 2, 64), rng.integers(0, dz, 64)],
                        1).astype(np.uint32)
     edits = [vrgi.world_walk(blocks, nt, 9000 + f, walkers, steps) for f in range(frames)]
-    for cache in (True, False):
-        blocks = blocks0.copy()
-        cfg = vrgi.config(3840, 2160, 4, rr_start=RR_START, rr_min=0.1, mode=1, radius=30.0, di_candidates=32)
-        with vrgi.Context(cfg, mats, lum) as ctx:
-            ctx.voxels_load(blocks, nt, tr)
-            if cache:
-                ctx.cache_enable(train_bounces=4, query_vertex=0)
-            ctx.render(cam, nee_type=NEE, spatial=SPATIAL)
-            ctx.sync()
-            st0 = ctx.stats(reset=True)
-            writes = 0
-            t0 = time.perf_counter()
-            for f in range(frames):
-                xyz, vals = edits[f]
-                writes += len(vals)
-                ctx.voxels_set(xyz, vals)
-                ctx.render(cam, nee_type=NEE, spatial=SPATIAL)
-            ctx.sync()
-            dt = time.perf_counter() - t0
-            st = ctx.stats(reset=True)
-            _, commits, remeshed = ctx.voxels_read()
-        key = "with_cache" if cache else "plain"
-        out[key] = {"ms_per_frame": round(1000.0 * dt / frames, 2), "mrays_per_s": round(st.rays_closest / dt / 1e6, 2),
-                    "distinct_voxel_writes_per_frame": int(writes // frames),
-                    "chunks_remeshed_per_frame": round(remeshed / max(commits, 1), 1),
-                    "triangles": int(st.bvh_nodes + 1) // 2 if st.bvh_nodes else 0}
-        log(f"large dynamic ({key}): {out[key]}")
+    out = {"workload": "512x256x512 terrain+400 boxes as a device voxel world, 3840x2160, 1 spp, 4 bounces, "
+                       "RR from bounce 2, NEE, ReSTIR DI 32 candidates + temporal + 5 spatial; 1% of the voxels "
+                       "toggled per frame by 64 seeded random walkers; dirty chunks re-meshed, all BVHs rebuilt; "
+                       "learned cache in fp16 with dynamic loss scaling",
+           "voxels_toggled_per_frame": steps,
+           "distinct_voxel_writes_per_frame": int(np.mean([len(v) for _, v in edits]))}
+    r = convergence.dynamic_study(blocks0, nt, tr, mats, lum, cam, 3840, 2160, edits, bounces=4, rr_start=RR_START,
+                                  mode=1, radius=30.0, di_candidates=32, nee=NEE, spatial=SPATIAL,
+                                  probe_frames=(7, 15, 23, 31), probe_ref_spp=64, ref_spp=256, static_frames=16,
+                                  cache_kw=dict(precision=vrgi.CACHE_FP16), log=log)
+    out.update(r)
+    # a refit keeps a tree's topology and needs the same triangles; a frame whose edits
+    # re-meshed no chunk could take one
+    out["frames_with_unchanged_triangles"] = int(sum(1 for k in r["chunks_remeshed_each"] if k == 0))
+    out["bvh_update"] = "rebuild (every frame re-meshes chunks: the triangle set changes)"
+    log(f"large dynamic: plain {r['plain_ms_per_frame']} ms, cache {r['cache_ms_per_frame']} ms, "
+        f"probes {r['probes']}, static ratios {r['static_time_to_equal_rmse_ratio']}")
     return out
 
 

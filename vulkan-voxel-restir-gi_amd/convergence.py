@@ -176,3 +176,122 @@ def study(objs, mats, lum, cams, W, H, bounces=8, rr_start=2, ref_spp=1024, fram
         "reference_mean_luminance": round(float(np.mean(ref @ LUM)), 5),
     })
     return out
+
+
+def dynamic_study(blocks, n_types, translucent, mats, lum, cam, W, H, edits, bounces=4, rr_start=2, mode=1,
+                  radius=30.0, di_candidates=32, nee=1, spatial=5, probe_frames=(7, 15, 23, 31), probe_ref_spp=64,
+                  ref_spp=256, static_frames=16, cache_kw=None, log=print):
+    """Time to equal RMSE on a world edited every frame (the large dynamic workload).
+
+    Two device voxel worlds, plain and with the learned cache, take the same voxel writes
+    (`edits`, one (xyz, values) pair per frame) and render one frame after each, with a
+    static camera; every frame's GPU time (voxel upload, re-mesh of the dirty chunks, BVH
+    rebuilds, the frame, the cache's inference and training) comes from HIP events. A
+    third world, the reference, follows the edits only at the probe frames: there the
+    plain tracer (same estimator as the plain world) accumulates `probe_ref_spp` frames in
+    FP64 in two halves on seeds disjoint from the measured frames, its own noise is
+    measured and removed as in study(), and each world's single frame is compared with it.
+    `plain_spp_to_match` is the sample count at which the plain tracer's error, falling as
+    1/sqrt(spp) on that world, reaches the cache's; the dynamic time ratio is the cache's
+    frame time over plain_spp_to_match plain frames.
+
+    After the last edit both worlds accumulate `static_frames` frames of the final world
+    (running FP64 means) against a `ref_spp` reference, and the time to equal RMSE is
+    measured as in study(): the cache's first frame whose running-mean error reaches the
+    plain tracer's error after k frames, divided by the plain time of k frames. Both
+    worlds rendered the same edited frames before, so neither side counts them."""
+    frames = len(edits)
+    kw = dict(rr_start=rr_start, rr_min=0.1, mode=mode, radius=radius, di_candidates=di_candidates)
+    cfg = vrgi.config(W, H, bounces, **kw)
+    pr = [f for f in probe_frames if f < frames]
+    out = {"resolution": f"{W}x{H}", "bounces": bounces, "frames": frames, "probe_frames": pr,
+           "probe_reference_spp": probe_ref_spp, "reference_spp": ref_spp, "static_frames": static_frames}
+    ctxs = [vrgi.Context(cfg, mats, lum) for _ in range(3)]
+    try:
+        plain, cache, refc = ctxs
+        for c in ctxs:
+            c.voxels_load(blocks, n_types, translucent)
+        cache.render(cam, nee_type=nee, spatial=spatial)  # the cache box comes from the scene
+        cache.sync()
+        ckw = dict(train_bounces=bounces, query_vertex=0)
+        ckw.update(cache_kw or {})
+        cache.cache_enable(**ckw)
+        info = cache.cache_info()
+        out["cache_precision"] = "fp16" if info.precision == vrgi.CACHE_FP16 else "bf16"
+        for c in (plain, cache):
+            c.frame_set(0)
+            c.profile(vrgi.PROFILE_STAGES)
+            _frame_ms(c)
+        ms = {"plain": [], "cache": []}
+        remeshed, tris, probes, pending = [], [], [], []
+        _, r0 = plain.voxels_counts()
+        for f, (xyz, vals) in enumerate(edits):
+            pending.append((xyz, vals))
+            for name, c in (("plain", plain), ("cache", cache)):
+                c.voxels_set(xyz, vals)
+                c.render(cam, nee_type=nee, spatial=spatial)
+                c.sync()
+                ms[name].append(_frame_ms(c))
+            _, r1 = plain.voxels_counts()
+            remeshed.append(r1 - r0)
+            r0 = r1
+            tris.append(int(plain.stats().bvh_nodes + 1) // 2)
+            if f in pr:
+                imgs = {n: c.read(vrgi.BUF_RADIANCE).reshape(-1, 3).astype(np.float64) for n, c in
+                        (("plain", plain), ("cache", cache))}
+                for xyz_p, vals_p in pending:
+                    refc.voxels_set(xyz_p, vals_p)
+                pending = []
+                log(f"dynamic study: reference at frame {f}, {probe_ref_spp} spp")
+                ref, var = _reference(refc, cam, probe_ref_spp, (1 << 24) + f * (1 << 16), nee, spatial)
+                e = {n: _rm(_mse(np.where(np.isfinite(v), v, 0.0), ref), var)[1] for n, v in imgs.items()}
+                spp = (e["plain"] / e["cache"]) ** 2 if e["cache"] > 0 else None
+                probes.append({"frame": f, "plain_rmse": round(e["plain"], 5), "cache_rmse": round(e["cache"], 5),
+                               "reference_rmse": round(float(np.sqrt(var)), 5),
+                               "plain_spp_to_match": round(spp, 2) if spp else None})
+        for xyz_p, vals_p in pending:
+            refc.voxels_set(xyz_p, vals_p)
+        log(f"dynamic study: final-world reference, {ref_spp} spp")
+        ref, var = _reference(refc, cam, ref_spp, 1 << 25, nee, spatial)
+        run = {}
+        for name, c in (("plain", plain), ("cache", cache)):
+            c.accum_enable(True)
+            t, e = [], []
+            for k in range(static_frames):
+                c.render(cam, nee_type=nee, spatial=spatial)
+                c.sync()
+                t.append(_frame_ms(c))
+                m, _ = c.accum_read()
+                e.append(_rm(_mse(m, ref), var)[1])
+            run[name] = (np.cumsum(t), e)
+            c.profile(0)
+        info = cache.cache_info()
+    finally:
+        for c in ctxs:
+            c.close()
+    (tp, ep), (tc, ec) = run["plain"], run["cache"]
+    ratios = {}
+    for k in (1, 4, 16):
+        if k > static_frames:
+            continue
+        hit = [j for j in range(static_frames) if ec[j] <= ep[k - 1]]
+        ratios[str(k)] = round(float(tc[hit[0]] / tp[k - 1]), 4) if hit else None
+    pm, cm = float(np.mean(ms["plain"])), float(np.mean(ms["cache"]))
+    keys = [k for k in (1, 2, 4, 8, 16) if k <= static_frames]
+    out.update({
+        "plain_ms_per_frame": round(pm, 3), "cache_ms_per_frame": round(cm, 3),
+        "plain_ms_per_frame_each": [round(v, 2) for v in ms["plain"]],
+        "cache_ms_per_frame_each": [round(v, 2) for v in ms["cache"]],
+        "chunks_remeshed_each": remeshed,
+        "triangles_each": tris,
+        "probes": probes,
+        "dynamic_time_ratio": (round(float(np.mean([cm / (q["plain_spp_to_match"] * pm) for q in probes
+                                                    if q["plain_spp_to_match"]])), 4) if probes else None),
+        "final_reference_rmse": round(float(np.sqrt(var)), 5),
+        "static_plain_rmse": {str(k): round(ep[k - 1], 5) for k in keys},
+        "static_cache_rmse": {str(k): round(ec[k - 1], 5) for k in keys},
+        "static_time_to_equal_rmse_ratio": ratios,
+        "cache_train_steps": int(info.steps), "cache_skipped_steps": int(info.skipped_steps),
+        "cache_loss_scale": float(info.loss_scale),
+    })
+    return out

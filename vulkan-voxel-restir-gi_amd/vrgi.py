@@ -328,6 +328,12 @@ class Context:
         check(self.L.vrgi_voxels_read(self.h, ptr(out), C.byref(commits), C.byref(remeshed)))
         return out, commits.value, remeshed.value
 
+    def voxels_counts(self):
+        """(commits, chunks re-meshed) so far, without reading the grid back."""
+        commits, remeshed = C.c_uint64(), C.c_uint64()
+        check(self.L.vrgi_voxels_read(self.h, None, C.byref(commits), C.byref(remeshed)))
+        return commits.value, remeshed.value
+
     # ---- GPU mesher
     def mesh_chunks(self, blocks, n_types, translucent, chunks):
         """chunk::gen_mesh on the GPU for chunks [(cx, cy, cz), ...]: list of vertex arrays."""
