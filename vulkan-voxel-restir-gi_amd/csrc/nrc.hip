@@ -590,13 +590,13 @@ __global__ __launch_bounds__(256) void k_rec_hist(const float *rec, const uint32
     key_rank[2 * (size_t)i] = key;
     key_rank[2 * (size_t)i + 1] = rank;
 }
-// exclusive scan of the bucket sizes in place: one block of 1024 threads, 32 buckets each
+// exclusive scan of the bucket sizes in place, per block of 1024 buckets (one bucket per
+// thread, coalesced); the block's total goes to hist[NRC_SORT_BUCKETS + block]
 __global__ __launch_bounds__(1024) void k_rec_scan(uint32_t *hist) {
     __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x, per = NRC_SORT_BUCKETS / 1024u;
-    uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k) sum += hist[t * per + k];
-    part[t] = sum;
+    const uint32_t t = threadIdx.x, i = blockIdx.x * 1024u + t;
+    const uint32_t c = hist[i];
+    part[t] = c;
     __syncthreads();
     for (uint32_t d = 1; d < 1024; d <<= 1) {
         const uint32_t v = t >= d ? part[t - d] : 0u;
@@ -604,19 +604,17 @@ __global__ __launch_bounds__(1024) void k_rec_scan(uint32_t *hist) {
         part[t] += v;
         __syncthreads();
     }
-    uint32_t run = part[t] - sum;
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t c = hist[t * per + k];
-        hist[t * per + k] = run;
-        run += c;
-    }
+    hist[i] = part[t] - c;
+    if (t == 1023) hist[NRC_SORT_BUCKETS + blockIdx.x] = part[t];
 }
 __global__ __launch_bounds__(256) void k_rec_scatter(const float *rec, const uint32_t *count, uint32_t cap,
                                                      const uint32_t *start, const uint32_t *key_rank, float *out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = min(*count, cap);
     if (i >= n) return;
-    const uint32_t dst = start[key_rank[2 * (size_t)i]] + key_rank[2 * (size_t)i + 1];
+    const uint32_t key = key_rank[2 * (size_t)i];
+    uint32_t dst = start[key] + key_rank[2 * (size_t)i + 1];
+    for (uint32_t blk = 0; blk < (key >> 10); ++blk) dst += start[NRC_SORT_BUCKETS + blk];  // earlier blocks
     const float4 *a = (const float4 *)(rec + (size_t)i * 16);
     float4 *b = (float4 *)(out + (size_t)dst * 16);
     b[0] = a[0]; b[1] = a[1]; b[2] = a[2]; b[3] = a[3];
@@ -626,7 +624,7 @@ void launch_rec_sort(hipStream_t s, const float *rec, const uint32_t *count, uin
     if (!cap) return;
     k_rec_clear<<<NRC_SORT_BUCKETS / 256, 256, 0, s>>>(hist);
     k_rec_hist<<<(cap + 255) / 256, 256, 0, s>>>(rec, count, cap, box, hist, key_rank);
-    k_rec_scan<<<1, 1024, 0, s>>>(hist);
+    k_rec_scan<<<NRC_SORT_BUCKETS / 1024, 1024, 0, s>>>(hist);
     k_rec_scatter<<<(cap + 255) / 256, 256, 0, s>>>(rec, count, cap, hist, key_rank, out);
 }
 

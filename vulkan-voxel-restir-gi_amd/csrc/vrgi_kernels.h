@@ -491,7 +491,8 @@ void launch_nrc_grads(hipStream_t s, const NrcNet &net, const NrcTrain &tr);
 void launch_nrc_pack(hipStream_t s, const float *w32, uint16_t *wt, uint16_t *wb, uint32_t prec);
 void launch_nrc_zero(hipStream_t s, float *p, size_t n);
 // training records in Morton order of their position in the cache box; hist:
-// NRC_SORT_BUCKETS words, key_rank: 2 words per record, out: cap records
+// NRC_SORT_BUCKETS + NRC_SORT_BUCKETS / 1024 words, key_rank: 2 words per record, out:
+// cap records
 static constexpr uint32_t NRC_SORT_BUCKETS = 32768;
 struct RecBox {
     float lo[3], hi[3];
