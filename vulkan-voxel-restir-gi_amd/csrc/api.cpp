@@ -677,7 +677,7 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s) {
     if (nt) c->G().geo_depth = dep[t++];
     if (nl) c->G().lgeo_depth = dep[t++];
     if (geom_stack_need(c->G().geo_depth) + 2 > VRGI_MAX_STACK || 3 * ((c->G().lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
-        throw std::runtime_error("BVH deeper than the traversal stack");
+        throw std::runtime_error("BVH deeper than the traversal stack (geometry depth " + std::to_string(c->G().geo_depth) + ", light-geometry depth " + std::to_string(c->G().lgeo_depth) + ")");
     launch_wide(s, c->G().d_nodes.p, gnodes, c->G().d_gwide.p);
     if (VRGI_GEOM4) launch_wide4(s, c->G().d_nodes.p, gnodes, c->G().d_gwide4.p);
     launch_wide(s, c->G().d_nodes.p + gnodes, lnodes, c->G().d_lwide.p);
@@ -2612,7 +2612,9 @@ void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
         if (p.gt >= 0) G.geo_depth = dep[p.gt];
         if (p.lt >= 0) G.lgeo_depth = dep[p.lt];
         if (geom_stack_need(G.geo_depth) + 2 > VRGI_MAX_STACK || 3 * ((G.lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
-            throw std::runtime_error("BVH deeper than the traversal stack");
+            throw std::runtime_error("BVH deeper than the traversal stack (geometry depth " +
+                                     std::to_string(G.geo_depth) + ", light-geometry depth " +
+                                     std::to_string(G.lgeo_depth) + ")");
         if (p.nt) launch_wide(s, bt->nodes.p + p.gbase, 2 * p.nt - 1, G.d_gwide.p);
         if (p.nl) {
             launch_wide(s, bt->nodes.p + p.lbase, 2 * p.nl - 1, G.d_lwide.p);

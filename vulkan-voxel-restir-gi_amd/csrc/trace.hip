@@ -939,7 +939,8 @@ __global__ __launch_bounds__(64) void k_nee_walk(const SceneView sv, const Bounc
 template <int STACK>
 __global__ __launch_bounds__(64) void k_nee_overflow(const SceneView sv, const BounceArgs ba) {
     __shared__ uint32_t stack[STACK * 64];
-    const uint32_t n = *ba.nee_over_count, count = ba.qcount[ba.bounce + 1];
+    const uint32_t count = ba.qcount[ba.bounce + 1];
+    const uint32_t n = min(*ba.nee_over_count, count);  // each queued ray is listed at most once
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t k = base + threadIdx.x;
         nee_pdf_ray<STACK, false>(sv, ba, stack, k < n ? ba.nee_over[k] : count, count);
