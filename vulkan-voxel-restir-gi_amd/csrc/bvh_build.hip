@@ -117,6 +117,10 @@ __device__ __forceinline__ void emit_pair(const BuildArgs &a, uint32_t base, uin
     emit_leaf(a, base + 2 * r + 2, idx, p1);
 }
 
+// a node for the middle phase (k_mid): more than SUB_MAX and at most MID_MAX primitives
+// of a tree whose leaves all have zero power (the middle phase writes power +0)
+__device__ __forceinline__ bool mid_node(const BuildArgs &a, uint32_t count, uint32_t tree);
+
 // ---------------------------------------------------------------- init
 __global__ void k_init(BuildArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,6 +159,9 @@ __global__ void k_roots(BuildArgs a, BuildState s) {
         if (n <= (uint32_t)a.sub_max) {
             uint32_t j = atomicAdd(s.n_jobs, 1u);
             ((OpenNode *)s.jobs)[j] = o;
+        } else if (mid_node(a, n, t)) {
+            uint32_t j = atomicAdd(s.n_mid, 1u);
+            ((OpenNode *)s.mid)[j] = o;
         } else {
             uint32_t j = atomicAdd(s.n_open_next, 1u);
             ((OpenNode *)s.open_next)[j] = o;
@@ -717,7 +724,7 @@ __global__ void k_children(BuildArgs a, BuildState s, uint32_t lvl, uint32_t *po
                 splits += 2;
             } else {
                 ch[side] = OpenNode{cf, cc, cr, ci, on.node, on.tree, on.depth + 1, 0};
-                kind[side] = cc <= (uint32_t)a.sub_max ? 1u : 2u;
+                kind[side] = cc <= (uint32_t)a.sub_max ? 1u : (mid_node(a, cc, on.tree) ? 3u : 2u);
             }
         }
     }
@@ -733,6 +740,8 @@ __global__ void k_children(BuildArgs a, BuildState s, uint32_t lvl, uint32_t *po
         if (kind[side] == 1u) ((OpenNode *)s.jobs)[j] = ch[side];
         const uint32_t q = wave_append(s.n_open_next, kind[side] == 2u);
         if (kind[side] == 2u) ((OpenNode *)s.open_next)[q] = ch[side];
+        const uint32_t m = wave_append(s.n_mid, kind[side] == 3u);
+        if (kind[side] == 3u) ((OpenNode *)s.mid)[m] = ch[side];
     }
     // depth per tree: one atomic per (wave, tree present in the wave)
     bool pend = live;
@@ -1179,6 +1188,7 @@ static void workspace_layout(uint32_t n, uint32_t trees, uint32_t &open_cap, uin
     carve((void **)&st.open, (size_t)open_cap * sizeof(OpenNode));
     carve((void **)&st.open_next, (size_t)open_cap * sizeof(OpenNode));
     carve((void **)&st.jobs, (size_t)(n / 3 + trees + 16) * sizeof(OpenNode));
+    carve((void **)&st.mid, (size_t)open_cap * sizeof(OpenNode));
     carve((void **)&st.items, (size_t)item_cap * sizeof(Item));
     carve((void **)&st.item_true, (size_t)item_cap * 4);
     carve((void **)&st.item_beg, (size_t)open_cap * 4);
@@ -1195,6 +1205,7 @@ static void workspace_layout(uint32_t n, uint32_t trees, uint32_t &open_cap, uin
     st.n_jobs = counters + 2;
     st.n_items = counters + 3;
     st.n_level_nodes = counters + 4;
+    st.n_mid = counters + 7;
 }
 
 size_t Builder::workspace_bytes(uint32_t n, uint32_t n_trees) {
@@ -1256,6 +1267,304 @@ void Builder::free_all() {
 }
 
 Builder::~Builder() { free_all(); }
+
+// ---------------------------------------------------------------- middle phase
+// Nodes of zero-power trees holding SUB_MAX < n <= MID_MAX primitives are built by one
+// workgroup each, with the node's primitive order and centroids resident in LDS, instead
+// of by the top phase's per-level kernels over global memory. Level by level, the
+// workgroup splits all of the node's ranges larger than SUB_MAX at once, with the top
+// phase's arithmetic per range (centroid bounds, 32 bins per axis over the primitive
+// boxes, the SAH sweep, the in-place partition of build.rs:145-149, children), every
+// thread covering positions of the node and each position knowing its range; ranges of
+// at most SUB_MAX primitives become k_subtree jobs. A wave's 64 consecutive positions
+// meet at most two ranges (a range holds more than SUB_MAX), so bounds and bins are
+// reduced per range across the wave before their LDS updates.
+#ifndef VRGI_MID
+#define VRGI_MID 1
+#endif
+static constexpr int MID_MAX = CHUNK;  // a node of the middle phase is one top-phase item
+static constexpr int MID_THREADS = 256;
+static constexpr int MID_LIST = MID_MAX / 129 + 1;  // ranges > SUB_MAX of one level are disjoint
+static constexpr int MID_BATCH = 8;                 // ranges binned together (LDS for their bins)
+static_assert(SUB_MAX_DEFAULT == 128, "MID_LIST counts ranges of more than 128 primitives");
+static_assert(MID_MAX <= 32767 && MID_LIST < 255, "16-bit positions, 8-bit range ids");
+static_assert(MID_MAX % MID_THREADS == 0, "every thread owns MID_MAX / MID_THREADS positions");
+__device__ __forceinline__ bool mid_node(const BuildArgs &a, uint32_t count, uint32_t tree) {
+    return VRGI_MID && count > (uint32_t)a.sub_max && count <= (uint32_t)MID_MAX && a.powered[tree] == 0u;
+}
+
+struct MidRange {
+    uint32_t first, count, rank, node, depth, parent;  // first: position in the node
+};
+
+// min / max over the lanes whose range id equals the lane's own, for the (at most two)
+// distinct ids of the wave; `apply(id, values)` runs on one lane per id
+template <class F>
+__device__ __forceinline__ void by_range(uint32_t rid, bool live, F apply) {
+    uint64_t todo = __ballot(live);
+    while (todo) {
+        const int lead = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)rid, lead);
+        const bool mine = live && rid == r;
+        apply(r, mine, lead);
+        todo &= ~__ballot(mine);
+    }
+}
+
+__global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) {
+    constexpr int PER = MID_MAX / MID_THREADS;
+    __shared__ uint32_t s_perm[MID_MAX];
+    __shared__ float s_c[3][MID_MAX];       // centroids, permuted with s_perm
+    __shared__ uint8_t s_rid[MID_MAX];      // range of each position at this level (0xFF: none)
+    __shared__ uint16_t s_cj[MID_MAX];      // exclusive count of trues over the node | predicate bit 15
+    __shared__ uint16_t s_pair[MID_MAX];    // right-region trues of a range, by rank from the right
+    __shared__ int s_bins[MID_BATCH][BIN_WORDS];
+    __shared__ int s_bnd[MID_LIST][6];      // centroid bound keys per range
+    __shared__ float s_lo[MID_LIST][3], s_sc[MID_LIST][3];
+    __shared__ uint32_t s_dim[MID_LIST], s_T[MID_LIST], s_P0[MID_LIST];
+    __shared__ float s_split[MID_LIST];
+    __shared__ int s_box[MID_LIST][6];
+    __shared__ MidRange s_r[2][MID_LIST];
+    __shared__ uint32_t s_n[2];
+    __shared__ uint32_t s_wsum[MID_THREADS / 64];
+    const OpenNode on = ((const OpenNode *)s.mid)[blockIdx.x];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t base = a.tree_base[on.tree];
+    const uint32_t c = on.count;
+    for (uint32_t k = tid; k < MID_MAX; k += MID_THREADS) {
+        if (k < c) {
+            s_perm[k] = a.perm[on.first + k];
+            for (int q = 0; q < 3; ++q) s_c[q][k] = a.pc[q * a.n + on.first + k];
+        }
+        s_rid[k] = k < c ? 0 : 0xFF;
+    }
+    if (tid == 0) {
+        s_r[0][0] = MidRange{0u, c, on.rank, on.node, on.depth, on.parent};
+        s_n[0] = 1;
+        s_n[1] = 0;
+    }
+    __syncthreads();
+    const float pad = 0.0001f;
+    unsigned long long splits = 0;
+    uint32_t maxdepth = 0;
+    for (uint32_t cur = 0; s_n[cur] != 0u; cur ^= 1u) {
+        const uint32_t nr = s_n[cur];
+        const MidRange *R = s_r[cur];
+        // ---- centroid bounds (build.rs:60-66)
+        for (uint32_t k = tid; k < nr * 6; k += MID_THREADS) s_bnd[k / 6][k % 6] = (k % 6) < 3 ? KEY_MIN_ID : KEY_MAX_ID;
+        __syncthreads();
+        for (int i = 0; i < PER; ++i) {
+            const uint32_t k = i * MID_THREADS + tid;
+            const uint32_t rid = s_rid[k];
+            const bool live = rid != 0xFFu;
+            int key[3] = {0, 0, 0};
+            if (live)
+                for (int d = 0; d < 3; ++d) key[d] = fkey(s_c[d][k]);
+            by_range(rid, live, [&](uint32_t r, bool mine, int lead) {
+                for (int d = 0; d < 3; ++d) {
+                    const int mn = wave_min(mine ? key[d] : KEY_MIN_ID), mx = wave_max(mine ? key[d] : KEY_MAX_ID);
+                    if ((int)lane == lead) { atomicMin(&s_bnd[r][d], mn); atomicMax(&s_bnd[r][3 + d], mx); }
+                }
+            });
+        }
+        __syncthreads();
+        if (tid < nr * 3) {
+            const uint32_t r = tid / 3, d = tid % 3;
+            const float lo = funkey(s_bnd[r][d]);
+            s_lo[r][d] = lo;
+            s_sc[r][d] = (float)BINS / (funkey(s_bnd[r][3 + d]) - lo);
+        }
+        // ---- bins (build.rs:68-85) and the SAH sweep (build.rs:87-127), MID_BATCH ranges at a time
+        for (uint32_t rb = 0; rb < nr; rb += MID_BATCH) {
+            const uint32_t nb = min((uint32_t)MID_BATCH, nr - rb);
+            for (uint32_t k = tid; k < nb * BIN_WORDS; k += MID_THREADS) {
+                const uint32_t w = (k % BIN_WORDS) % 7;
+                s_bins[k / BIN_WORDS][k % BIN_WORDS] = w == 0 ? 0 : (w <= 3 ? KEY_MIN_ID : KEY_MAX_ID);
+            }
+            __syncthreads();
+            // the thread's PER primitive records are fetched together (one memory latency)
+            float4 rb0[PER], rb1[PER];
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const uint32_t k = i * MID_THREADS + tid;
+                const uint32_t rid = s_rid[k];
+                rb0[i] = rb1[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (rid != 0xFFu && rid >= rb && rid < rb + nb) {
+                    const uint32_t p = s_perm[k];
+                    rb0[i] = a.box8[2 * p];
+                    rb1[i] = a.box8[2 * p + 1];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const uint32_t k = i * MID_THREADS + tid;
+                const uint32_t rid = s_rid[k];
+                const bool live = rid != 0xFFu && rid >= rb && rid < rb + nb;
+                int bk[6] = {KEY_MIN_ID, KEY_MIN_ID, KEY_MIN_ID, KEY_MAX_ID, KEY_MAX_ID, KEY_MAX_ID};
+                uint32_t bb[3] = {0u, 0u, 0u};
+                if (live) {
+                    const float4 b0 = rb0[i], b1 = rb1[i];
+                    bk[0] = fkey(b0.x); bk[1] = fkey(b0.y); bk[2] = fkey(b0.z);
+                    bk[3] = fkey(b0.w); bk[4] = fkey(b1.x); bk[5] = fkey(b1.y);
+                    for (int d = 0; d < 3; ++d) bb[d] = bin_index(s_c[d][k], s_lo[rid][d], s_sc[rid][d]);
+                }
+                by_range(rid, live, [&](uint32_t r, bool mine, int lead) {
+                    int *bins = s_bins[r - rb];
+                    for (int d = 0; d < 3; ++d) {
+                        const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)bb[d], lead);
+                        if (__ballot(mine && bb[d] != lb) == 0) {  // one bin for the range's lanes
+                            const int cnt = (int)__popcll(__ballot(mine));  // outside the one-lane branch
+                            const int m0 = wave_min(mine ? bk[0] : KEY_MIN_ID), m1 = wave_min(mine ? bk[1] : KEY_MIN_ID);
+                            const int m2 = wave_min(mine ? bk[2] : KEY_MIN_ID), x0 = wave_max(mine ? bk[3] : KEY_MAX_ID);
+                            const int x1 = wave_max(mine ? bk[4] : KEY_MAX_ID), x2 = wave_max(mine ? bk[5] : KEY_MAX_ID);
+                            if ((int)lane == lead) {
+                                int *bin = bins + (d * BINS + lb) * 7;
+                                atomicAdd(bin, cnt);
+                                atomicMin(bin + 1, m0); atomicMin(bin + 2, m1); atomicMin(bin + 3, m2);
+                                atomicMax(bin + 4, x0); atomicMax(bin + 5, x1); atomicMax(bin + 6, x2);
+                            }
+                        } else if (mine) {
+                            int *bin = bins + (d * BINS + bb[d]) * 7;
+                            atomicAdd(bin, 1);
+                            atomicMin(bin + 1, bk[0]); atomicMin(bin + 2, bk[1]); atomicMin(bin + 3, bk[2]);
+                            atomicMax(bin + 4, bk[3]); atomicMax(bin + 5, bk[4]); atomicMax(bin + 6, bk[5]);
+                        }
+                    }
+                });
+            }
+            __syncthreads();
+            for (uint32_t r = rb + wave; r < rb + nb; r += MID_THREADS / 64) {  // one wave per range
+                uint32_t dim;
+                float split;
+                int box[6];
+                sah_sweep_wave(s_bins[r - rb], s_bnd[r], &dim, &split, box);
+                if (lane == 0) {
+                    s_dim[r] = dim;
+                    s_split[r] = split;
+                    for (int q = 0; q < 6; ++q) s_box[r][q] = box[q];
+                }
+            }
+            __syncthreads();
+        }
+        // ---- partition predicate c[dim] < split (build.rs:148): exclusive count of trues
+        // over the node's positions (thread t owns positions PER t .. PER t + PER - 1)
+        {
+            uint32_t flags = 0, cnt = 0;
+            for (int i = 0; i < PER; ++i) {
+                const uint32_t k = tid * PER + i;
+                const uint32_t rid = s_rid[k];
+                const bool p = rid != 0xFFu && s_c[s_dim[rid]][k] < s_split[rid];
+                flags |= (p ? 1u : 0u) << i;
+                cnt += p ? 1u : 0u;
+            }
+            uint32_t inc = cnt;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t u = __shfl_up(inc, off, 64);
+                if (lane >= (uint32_t)off) inc += u;
+            }
+            if (lane == 63) s_wsum[wave] = inc;
+            __syncthreads();
+            uint32_t run = inc - cnt;
+            for (uint32_t w = 0; w < wave; ++w) run += s_wsum[w];
+            for (int i = 0; i < PER; ++i) {
+                const uint32_t k = tid * PER + i;
+                const bool p = (flags >> i) & 1u;
+                s_cj[k] = (uint16_t)(run | (p ? 0x8000u : 0u));
+                run += p ? 1u : 0u;
+            }
+        }
+        __syncthreads();
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < MID_THREADS / 64; ++w) total += s_wsum[w];
+        if (tid < nr) {
+            const uint32_t f = R[tid].first, e = f + R[tid].count;
+            const uint32_t p0 = s_cj[f] & 0x7FFFu, p1 = e < c ? (s_cj[e] & 0x7FFFu) : total;
+            s_P0[tid] = p0;
+            s_T[tid] = p1 - p0;
+        }
+        __syncthreads();
+        // right-region trues publish their position; then the r-th false of [0, T) trades
+        // places with the r-th true of [T, n) counted from the right (ranges whose T is 0 or
+        // n keep their order: the half split of build.rs:152)
+        for (int i = 0; i < PER; ++i) {
+            const uint32_t k = i * MID_THREADS + tid;
+            const uint32_t rid = s_rid[k];
+            if (rid == 0xFFu) continue;
+            const MidRange &g = R[rid];
+            const uint32_t T = s_T[rid];
+            if (T == 0 || T == g.count) continue;
+            const uint32_t cj = s_cj[k];
+            const uint32_t j = k - g.first;
+            if ((cj & 0x8000u) && j >= T) s_pair[g.first + (T - 1 - ((cj & 0x7FFFu) - s_P0[rid]))] = (uint16_t)j;
+        }
+        __syncthreads();
+        for (int i = 0; i < PER; ++i) {
+            const uint32_t k = i * MID_THREADS + tid;
+            const uint32_t rid = s_rid[k];
+            if (rid == 0xFFu) continue;
+            const MidRange &g = R[rid];
+            const uint32_t T = s_T[rid];
+            const uint32_t j = k - g.first;
+            if (T == 0 || T == g.count || j >= T) continue;
+            const uint32_t cj = s_cj[k];
+            if (cj & 0x8000u) continue;
+            const uint32_t e1 = g.first + s_pair[g.first + (j - ((cj & 0x7FFFu) - s_P0[rid]))];
+            const uint32_t tp = s_perm[k]; s_perm[k] = s_perm[e1]; s_perm[e1] = tp;
+            for (int q = 0; q < 3; ++q) { const float f = s_c[q][k]; s_c[q][k] = s_c[q][e1]; s_c[q][e1] = f; }
+        }
+        __syncthreads();
+        // ---- node records and children, one thread per range
+        if (tid < nr) {
+            const MidRange g = R[tid];
+            const uint32_t T = s_T[tid];
+            const bool fb = T == 0 || T == g.count;
+            const uint32_t nl = fb ? g.count / 2 : T, nrt = g.count - nl;
+            const uint32_t li = 2 * g.rank + 1, ri = 2 * g.rank + 2;
+            float mn[3], mx[3];
+            for (int k = 0; k < 3; ++k) { mn[k] = funkey(s_box[tid][k]) - pad; mx[k] = funkey(s_box[tid][3 + k]) + pad; }
+            write_node(a.nodes + base + g.node, li, ri, mn, 0.0f, mx, g.parent);
+            splits += g.count;
+            maxdepth = max(maxdepth, g.depth + 1);
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t cf = side ? g.first + nl : g.first;
+                const uint32_t cc = side ? nrt : nl;
+                const uint32_t cr = side ? g.rank + nl : g.rank + 1;
+                const uint32_t ci = side ? ri : li;
+                if (cc == 1) {
+                    emit_leaf(a, base + ci, g.node, s_perm[cf]);
+                } else if (cc == 2) {
+                    emit_pair(a, base, ci, cr, g.node, s_perm[cf], s_perm[cf + 1]);
+                    splits += 2;
+                    maxdepth = max(maxdepth, g.depth + 2);
+                } else if (cc <= (uint32_t)a.sub_max) {
+                    const uint32_t jb = atomicAdd(s.n_jobs, 1u);
+                    ((OpenNode *)s.jobs)[jb] = OpenNode{on.first + cf, cc, cr, ci, g.node, on.tree, g.depth + 1, 0};
+                } else {
+                    const uint32_t q = atomicAdd(&s_n[cur ^ 1u], 1u);
+                    s_r[cur ^ 1u][q] = MidRange{cf, cc, cr, ci, g.depth + 1, g.node};
+                }
+            }
+        }
+        __syncthreads();
+        // ---- range ids of the next level
+        for (uint32_t k = tid; k < MID_MAX; k += MID_THREADS) s_rid[k] = 0xFF;
+        if (tid == 0) s_n[cur] = 0;
+        __syncthreads();
+        const uint32_t nn = s_n[cur ^ 1u];
+        for (uint32_t q = 0; q < nn; ++q) {
+            const MidRange g = s_r[cur ^ 1u][q];
+            for (uint32_t k = tid; k < g.count; k += MID_THREADS) s_rid[g.first + k] = (uint8_t)q;
+        }
+        __syncthreads();
+    }
+    // the jobs read the node's final primitive order from global memory
+    for (uint32_t k = tid; k < c; k += MID_THREADS) {
+        a.perm[on.first + k] = s_perm[k];
+        for (int q = 0; q < 3; ++q) a.pc[q * a.n + on.first + k] = s_c[q][k];
+    }
+    if (maxdepth) atomicMax(a.depth + on.tree, maxdepth);
+    if (splits) atomicAdd(a.split_count_sum, splits);
+}
 
 // start of a top-phase level: the open list of the previous level's children becomes
 // current, and the level's first entry in the level-node list is recorded
@@ -1340,7 +1649,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
             k_part_b<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
             k_children<<<std::min((nb + 127) / 128, GRID_CAP_NODES / 2), 128, 0, stream>>>(a, st, lvl, counters + 6);
         }
-        HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 7 * 4, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 8 * 4, hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         if (h_cnt[1] == 0) break;  // no node left open for the next level
         group = first_group ? 1u : std::min(group * 2, 4u);
@@ -1352,7 +1661,13 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     std::swap(st.open, st.open_next);
     k_advance<<<1, 64, 0, stream>>>(st, level_start, lvl);  // level_start[lvl] = end of the list
     const uint32_t n_levels = std::min(lvl, h_cnt[6]);  // levels past h_cnt[6] held no powered node
-    const uint32_t n_jobs = h_cnt[2];
+    uint32_t n_jobs = h_cnt[2];
+    if (h_cnt[7]) {  // the middle phase adds jobs: their count is read back
+        k_mid<<<h_cnt[7], MID_THREADS, 0, stream>>>(a, st);
+        HIP_CHECK(hipMemcpyAsync(h_cnt + 2, counters + 2, 4, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        n_jobs = h_cnt[2];
+    }
     if (n_jobs) {
         k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL, false><<<n_jobs, 64, 0, stream>>>(a, st);
         k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL, true><<<n_jobs, 64, 0, stream>>>(a, st);

@@ -78,6 +78,8 @@ struct BuildState {
     uint32_t *n_jobs = nullptr;
     uint32_t *n_items = nullptr;
     uint32_t *n_level_nodes = nullptr;
+    void *mid = nullptr;          // nodes for the middle phase (one workgroup each)
+    uint32_t *n_mid = nullptr;
 };
 
 // top-phase levels a build may take (entries of its level list)
