@@ -167,6 +167,10 @@ struct vrgi_ctx {
         DevBuf<Wide4> d_lwide4, d_gwide4;
         DevBuf<uint32_t> d_lanc, d_lanc_len;  // per light triangle: ancestors for the NEE pdf
         uint32_t geo_depth = 0, lgeo_depth = 0;
+        DevBuf<uint8_t> d_lgrid;  // light grid of the lazy NEE pdf (SceneView::lgrid)
+        float lg_lo[3] = {0, 0, 0}, lg_inv = 1.0f;
+        uint32_t lg_dim[3] = {0, 0, 0};
+        bool lg_on = false;
         hipEvent_t ev_free = nullptr;  // recorded after the last frame that read the set
         bool freed_once = false;
     } geo[2];
@@ -191,7 +195,9 @@ struct vrgi_ctx {
     DevBuf<float2> nee_uv[VRGI_NEE_STREAMS];
     DevBuf<float> f_radiance, f_restir, f_debug, f_temporal, f_spatial;
     DevBuf<float> f_lrun;  // [P][3] radiance of the next vertex for the lazy NEE pdf
-    DevBuf<uint32_t> f_nlist, f_ncount;  // lazy NEE pdf: rays listed for the full pdf, count per bounce
+    // lazy NEE pdf: rays listed for the full pdf, rays listed for the near-crossing walk;
+    // counts per bounce (full pdf at [b], walk at [16 + b])
+    DevBuf<uint32_t> f_nlist, f_mlist, f_ncount;
     DevBuf<uint8_t> f_bgra;
     DevBuf<float> di_T, di_hist, di_w;
     DevBuf<uint32_t> di_light;
@@ -309,7 +315,39 @@ SceneView scene_view(vrgi_ctx *c) {
     sv.lanc_len = c->G().d_lanc_len.p;
     sv.mats = c->d_mats.p;
     sv.n_mats = (uint32_t)c->mats.size();
+    if (c->G().lg_on) {
+        sv.lgrid = c->G().d_lgrid.p;
+        for (int k = 0; k < 3; ++k) { sv.lg_lo[k] = c->G().lg_lo[k]; sv.lg_dim[k] = c->G().lg_dim[k]; }
+        sv.lg_inv = c->G().lg_inv;
+    }
     return sv;
+}
+
+// the light grid of the set just built, over the light-geometry root box `mn`..`mx`:
+// cells of side 1 (larger when the box would need more than 2^25 cells), every cell within
+// 1 + one side of a light triangle's box marked
+void build_light_grid(vrgi_ctx *c, hipStream_t s, const float mn[3], const float mx[3]) {
+    auto &G = c->G();
+    G.lg_on = false;
+    if (!c->n_ltri) return;
+    double side = 1.0;
+    for (int it = 0; it < 8; ++it) {
+        const double m = 1.0 + side;
+        double cells = 1.0;
+        for (int k = 0; k < 3; ++k) cells *= std::ceil(((double)mx[k] - mn[k] + 2.0 * m + 2.0 * side) / side) + 1.0;
+        if (!(cells <= (double)(1u << 25))) { side *= 1.5; continue; }
+        const float margin = (float)m;
+        for (int k = 0; k < 3; ++k) {
+            G.lg_lo[k] = (float)(mn[k] - m - side);
+            G.lg_dim[k] = (uint32_t)std::ceil(((double)mx[k] - mn[k] + 2.0 * m + 2.0 * side) / side) + 1u;
+        }
+        G.lg_inv = (float)(1.0 / side);
+        const size_t n = (size_t)G.lg_dim[0] * G.lg_dim[1] * G.lg_dim[2];
+        G.d_lgrid.reserve((n + 3) / 4 * 4);
+        launch_light_grid(s, G.d_ltri.p, c->n_ltri, G.d_lgrid.p, G.lg_lo, G.lg_inv, G.lg_dim, margin);
+        G.lg_on = true;
+        return;
+    }
 }
 
 NrcNet cache_net(const vrgi_ctx *c) {
@@ -446,7 +484,8 @@ void alloc_frame(vrgi_ctx *c) {
     c->f_view_acc.reserve(P * 3);
     c->view_dirty = true;
     c->f_nlist.reserve(P);
-    c->f_ncount.reserve(16);
+    c->f_mlist.reserve(P);
+    c->f_ncount.reserve(32);
     c->f_restir.reserve(P * 3);
     c->f_debug.reserve(P * 3);
     c->f_temporal.reserve(3 * P * 20);
@@ -626,6 +665,7 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s) {
     c->d_trees.reserve(6);
     c->G().d_depth.reserve(2);
     c->G().geo_depth = c->G().lgeo_depth = 0;
+    c->G().lg_on = false;
     if (N == 0) return;
     FlattenArgs f{};
     f.inst = c->d_inst.p;
@@ -666,9 +706,12 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s) {
         throw HipError("geometry build exceeded its workspace");
     uint32_t dep[2] = {0, 0};
     unsigned long long S = 0;
+    vrgi_bvh_node lroot{};
     HIP_CHECK(hipMemcpyAsync(dep, c->G().d_depth.p, nt_trees * 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(&S, c->geo_builder.leaf_depth_sum(), 8, hipMemcpyDeviceToHost, s));
+    if (nl) HIP_CHECK(hipMemcpyAsync(&lroot, c->G().d_nodes.p + gnodes, sizeof(lroot), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    build_light_grid(c, s, lroot.min, lroot.max);
     // structural figures of the build (profile output)
     c->prof.build_leaf_depth_sum = S;
     c->prof.build_prims = (uint64_t)nt + nl;
@@ -1608,7 +1651,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             HIP_CHECK(hipStreamWaitEvent(s, c->ev_nee_done[k], 0));
         }
     } else if (lazy_nee) {
-        HIP_CHECK(hipMemsetAsync(c->f_ncount.p, 0, (size_t)B * 4, s));
+        HIP_CHECK(hipMemsetAsync(c->f_ncount.p, 0, 32 * 4, s));
         for (int b = (int)B - 2; b >= 0; --b) {  // interactive_rendering.rs:1206
             launch_resolve_step(s, fa, (uint32_t)b + 1u, c->f_lrun.p);
             rec(3 + 2 * (size_t)B + (size_t)b);
@@ -1616,7 +1659,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             na.bounce = (uint32_t)b;
             na.queue_in = qin[b + 1];
             na.lrun = c->f_lrun.p;
-            launch_nee_pdf_lazy(s, sv, na, P, c->f_nlist.p, c->f_ncount.p + b);
+            launch_nee_pdf_lazy(s, sv, na, P, c->f_nlist.p, c->f_ncount.p + b, c->f_mlist.p, c->f_ncount.p + 16 + b);
             rec(2 + B + (size_t)b);
         }
     } else {
@@ -2550,6 +2593,7 @@ void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
         vrgi_ctx *c = p.c;
         auto &G = c->G();
         G.geo_depth = G.lgeo_depth = 0;
+        G.lg_on = false;  // a batched build makes no light grid
         c->ext_gnodes = c->ext_lnodes = nullptr;
     }
     if (N == 0) return;

@@ -671,6 +671,22 @@ __device__ __forceinline__ bool light_crossed(const SceneView &sv, const RayPre 
     return false;
 }
 
+// No light triangle lies within distance 1 of p: p's cell of the light grid is clear (the
+// grid marks every cell within 1 + one cell side of a light triangle's box, a full cell
+// more than the distance, so the rounding of the cell index cannot miss a marked cell).
+// A point outside the grid is farther than that from every light triangle.
+__device__ __forceinline__ bool lgrid_clear(const SceneView &sv, f3 p) {
+    const float x = (p.x - sv.lg_lo[0]) * sv.lg_inv, y = (p.y - sv.lg_lo[1]) * sv.lg_inv,
+                z = (p.z - sv.lg_lo[2]) * sv.lg_inv;
+    if (!(x == x && y == y && z == z)) return false;  // NaN: take the walk
+    if (x < 0.0f || y < 0.0f || z < 0.0f || x >= (float)sv.lg_dim[0] || y >= (float)sv.lg_dim[1] ||
+        z >= (float)sv.lg_dim[2])
+        return true;
+    const uint32_t ix = min((uint32_t)x, sv.lg_dim[0] - 1u), iy = min((uint32_t)y, sv.lg_dim[1] - 1u),
+                   iz = min((uint32_t)z, sv.lg_dim[2] - 1u);
+    return sv.lgrid[((size_t)iz * sv.lg_dim[1] + iy) * sv.lg_dim[0] + ix] == 0u;
+}
+
 // Whether the NEE pdf of a ray can reach the frame's outputs (the lazy mode). It enters
 // the recurrence only as L_b = E_b + ((R_b * L_{b+1}) * (bsdf_pdf / q)) with
 // q = np * w + (1 - w) / (2 pi), w in (0, 0.3]: when L_{b+1} is zero (either sign) and R_b
@@ -682,24 +698,30 @@ __device__ __forceinline__ bool light_crossed(const SceneView &sv, const RayPre 
 // rounding of the corner test, so the walked side's importance is positive and no level
 // of the reverse walk divides 0 by 0. The crossing at parameter t is t * cs above the
 // plane, so it suffices that no light triangle is crossed for t <= 0.01 / cs.
-__device__ __forceinline__ bool nee_pdf_needed(const SceneView &sv, const BounceArgs &ba, size_t bi, uint32_t px,
-                                               f3 o, f3 d, uint32_t *stk) {
+// 0: the pdf cannot reach the outputs; 1: it can; 2: only if a light triangle is
+// crossed for t <= *tmax (light_crossed decides)
+__device__ __forceinline__ int nee_pdf_class(const SceneView &sv, const BounceArgs &ba, size_t bi, uint32_t px,
+                                             f3 o, f3 d, float *tmax) {
     const f3 ln = ld3(ba.lrun + (size_t)px * 3);
     const f3 R = ld3(ba.refl + bi * 3);
-    if (!zero3(ln) || !(isfinite(R.x) && isfinite(R.y) && isfinite(R.z))) return true;
+    if (!zero3(ln) || !(isfinite(R.x) && isfinite(R.y) && isfinite(R.z))) return 1;
     const float cs = dot3(ld3(ba.normal + bi * 3), d);
-    if (!(cs > 0.0f)) return true;
-    return light_crossed(sv, ray_pre(o, d), gmin(0.01f / cs, 1000.0f), stk);
+    if (!(cs > 0.0f)) return 1;
+    *tmax = gmin(0.01f / cs, 1000.0f);
+    // d is a unit vector, so a crossing for t <= tmax <= 1 lies within distance 1 of o
+    if (sv.lgrid && *tmax <= 1.0f && lgrid_clear(sv, o)) return 0;
+    return 2;
 }
 
 // NEE pdf of bounce b (nee_pdf.rs:281-337) for the rays alive at bounce b+1
 constexpr int NEE_DEFER = 4;  // crossed light triangles kept per lane before their pdfs are summed
-template <int STACK, bool COUNT_RAYS = true>
+// i: the ray's slot in the queue of bounce b + 1 (BY_PX: its pixel)
+template <int STACK, bool COUNT_RAYS = true, bool BY_PX = false>
 __device__ __forceinline__ void nee_pdf_ray(const SceneView &sv, const BounceArgs &ba, uint32_t *stack, uint32_t i,
                                             uint32_t count) {
     const uint32_t b = ba.bounce;  // pdf of bounce b, rays of bounce b + 1
     if (i >= count) return;
-    const uint32_t px = ba.queue_in[i];
+    const uint32_t px = BY_PX ? i : ba.queue_in[i];
     const size_t bi = (size_t)b * ba.P + px;
     float w = ba.mis[bi];
     f3 d = ld3(ba.dir + bi * 3);
@@ -780,34 +802,89 @@ __global__ __launch_bounds__(64) void k_nee_pdf(const SceneView sv, const Bounce
     __shared__ uint32_t stack[STACK * 64];
     nee_pdf_ray<STACK>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, ba.qcount[ba.bounce + 1]);
 }
-// Lazy NEE pdf, pass 1: per ray of the bounce's queue, 0 for the rays whose pdf cannot
-// reach the outputs (or that trace none); the others are appended to `list` (queue slot
-// indices, ballot + one atomic per wave) for pass 2
-template <int STACK>
-__global__ __launch_bounds__(64) void k_nee_select(const SceneView sv, const BounceArgs ba, uint32_t *list,
-                                                   uint32_t *list_count) {
-    __shared__ uint32_t stack[STACK * 64];
+// slot in a list for every lane with `take`: one atomic per wave (all 64 lanes call it)
+__device__ __forceinline__ void list_append(uint32_t *list, uint32_t *count, bool take, uint32_t v) {
+    const uint64_t bal = __ballot(take);
+    if (!bal) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(bal));
+    base = __shfl(base, 0, 64);
+    if (take) list[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = v;
+}
+// Lazy NEE pdf, pass 1: per ray of the bounce's queue (slot i), 0 for the rays whose pdf
+// cannot reach the outputs (or that trace none); the rays whose pdf can are appended to
+// `list` for the full pdf, those for which it depends on a light triangle crossed near the
+// origin to `mlist` (queue slots). No walk runs here, so the pass only streams the rays'
+// state; the appends take one counter update per block of NEE_SELECT_THREADS rays, since
+// updates of one address by every wave of so short a pass queue up behind each other.
+constexpr int NEE_SELECT_THREADS = 1024;
+__global__ __launch_bounds__(NEE_SELECT_THREADS) void k_nee_select(const SceneView sv, const BounceArgs ba,
+                                                                   uint32_t *list, uint32_t *list_count,
+                                                                   uint32_t *mlist, uint32_t *mcount) {
+    constexpr int WAVES = NEE_SELECT_THREADS / 64;
+    __shared__ uint32_t s_cnt[2][WAVES];
+    __shared__ uint32_t s_base[2];
     const uint32_t b = ba.bounce;
     const uint32_t count = ba.qcount[b + 1];
-    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
-    if (blockIdx.x * 64u >= count) return;
-    bool need = false;
+    if (blockIdx.x * NEE_SELECT_THREADS >= count) return;  // uniform over the block
+    const uint32_t i = blockIdx.x * NEE_SELECT_THREADS + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    int cls = 0;
     if (i < count) {
         const uint32_t px = ba.queue_in[i];
         const size_t bi = (size_t)b * ba.P + px;
-        const float w = ba.mis[bi];
+        const float wm = ba.mis[bi];
         const f3 d = ld3(ba.dir + bi * 3);
-        need = !(len3(d) == 0.0f || w == 0.0f) && sv.n_ltri > 0 &&
-               nee_pdf_needed(sv, ba, bi, px, ld3(ba.origin + bi * 3), d, stack + (threadIdx.x & 63u));
-        if (!need) ba.nee[bi] = 0.0f;
+        float tmax;
+        if (!(len3(d) == 0.0f || wm == 0.0f) && sv.n_ltri > 0)
+            cls = nee_pdf_class(sv, ba, bi, px, ld3(ba.origin + bi * 3), d, &tmax);
+        if (cls == 0) ba.nee[bi] = 0.0f;
     }
-    const uint64_t bal = __ballot(need);
-    if (bal) {
-        const uint32_t lane = threadIdx.x & 63u;
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(list_count, (uint32_t)__popcll(bal));
-        base = __shfl(base, 0, 64);
-        if (need) list[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
+    const uint64_t b1 = __ballot(cls == 1), b2 = __ballot(cls == 2);
+    if (lane == 0) {
+        s_cnt[0][w] = (uint32_t)__popcll(b1);
+        s_cnt[1][w] = (uint32_t)__popcll(b2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t t = 0;
+        for (int k = 0; k < WAVES; ++k) t += s_cnt[threadIdx.x][k];
+        s_base[threadIdx.x] = t ? atomicAdd(threadIdx.x ? mcount : list_count, t) : 0u;
+    }
+    __syncthreads();
+    if (cls != 0) {
+        const uint32_t q = cls - 1;
+        uint32_t off = s_base[q];
+        for (uint32_t k = 0; k < w; ++k) off += s_cnt[q][k];
+        const uint64_t m = q ? b2 : b1;
+        (q ? mlist : list)[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    }
+}
+// pass 1b: the rays of `mlist`, whose pdf matters only if a light triangle is crossed for
+// t <= 0.01 / cos (a striding pass over the list): those go to `list`, the others get 0
+template <int STACK>
+__global__ __launch_bounds__(64) void k_nee_near(const SceneView sv, const BounceArgs ba, const uint32_t *mlist,
+                                                 const uint32_t *mcount, uint32_t *list, uint32_t *list_count) {
+    __shared__ uint32_t stack[STACK * 64];
+    const uint32_t b = ba.bounce;
+    const uint32_t count = ba.qcount[b + 1];
+    const uint32_t n = min(*mcount, count);  // each queued ray is listed at most once
+    for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+        const uint32_t k = base + threadIdx.x;
+        bool need = false;
+        uint32_t i = 0;
+        if (k < n) {
+            i = mlist[k];
+            const uint32_t px = ba.queue_in[i];
+            const size_t bi = (size_t)b * ba.P + px;
+            const f3 d = ld3(ba.dir + bi * 3);
+            const float cs = dot3(ld3(ba.normal + bi * 3), d);
+            need = light_crossed(sv, ray_pre(ld3(ba.origin + bi * 3), d), gmin(0.01f / cs, 1000.0f),
+                                 stack + (threadIdx.x & 63u));
+            if (!need) ba.nee[bi] = 0.0f;
+        }
+        list_append(list, list_count, need, i);
     }
 }
 // pass 2: the full pdf of the listed rays (a capped grid striding over the list), each
@@ -816,11 +893,10 @@ template <int STACK>
 __global__ __launch_bounds__(64) void k_nee_listed(const SceneView sv, const BounceArgs ba, const uint32_t *list,
                                                    const uint32_t *list_count) {
     __shared__ uint32_t stack[STACK * 64];
-    const uint32_t count = ba.qcount[ba.bounce + 1];
-    const uint32_t n = min(*list_count, count);  // each queued ray is listed at most once
+    const uint32_t n = min(*list_count, ba.qcount[ba.bounce + 1]);  // each queued ray is listed at most once
     for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
         const uint32_t k = base + threadIdx.x;
-        nee_pdf_ray<STACK>(sv, ba, stack, k < n ? list[k] : count, count);
+        nee_pdf_ray<STACK>(sv, ba, stack, k < n ? list[k] : ba.qcount[ba.bounce + 1], ba.qcount[ba.bounce + 1]);
     }
 }
 template <int STACK>
@@ -1288,6 +1364,40 @@ void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba
     });
 }
 
+__global__ __launch_bounds__(256) void k_lgrid_clear(uint32_t *g, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        g[i] = 0u;
+}
+__global__ __launch_bounds__(256) void k_lgrid_mark(const Tri *ltri, uint32_t nl, uint8_t *grid, float lo0,
+                                                    float lo1, float lo2, float inv, uint32_t d0, uint32_t d1,
+                                                    uint32_t d2, float margin) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const Tri t = ltri[i];
+    const float lo[3] = {lo0, lo1, lo2};
+    const uint32_t dim[3] = {d0, d1, d2};
+    const float v[3][3] = {{t.v0.x, t.v0.y, t.v0.z}, {t.v1.x, t.v1.y, t.v1.z}, {t.v2.x, t.v2.y, t.v2.z}};
+    uint32_t c0[3], c1[3];
+    for (int a = 0; a < 3; ++a) {
+        const float mn = fminf(fminf(v[0][a], v[1][a]), v[2][a]) - margin;
+        const float mx = fmaxf(fmaxf(v[0][a], v[1][a]), v[2][a]) + margin;
+        const float f0 = floorf((mn - lo[a]) * inv), f1 = floorf((mx - lo[a]) * inv);
+        c0[a] = f0 <= 0.0f ? 0u : (uint32_t)fminf(f0, (float)(dim[a] - 1u));
+        c1[a] = f1 <= 0.0f ? 0u : (uint32_t)fminf(f1, (float)(dim[a] - 1u));
+    }
+    for (uint32_t z = c0[2]; z <= c1[2]; ++z)
+        for (uint32_t y = c0[1]; y <= c1[1]; ++y)
+            for (uint32_t x = c0[0]; x <= c1[0]; ++x) grid[((size_t)z * d1 + y) * d0 + x] = 1u;
+}
+void launch_light_grid(hipStream_t s, const Tri *ltri, uint32_t nl, uint8_t *grid, const float lo[3], float inv,
+                       const uint32_t dim[3], float margin) {
+    const size_t cells = (size_t)dim[0] * dim[1] * dim[2];
+    const size_t words = (cells + 3) / 4;
+    k_lgrid_clear<<<(uint32_t)std::min<size_t>((words + 255) / 256, 8192), 256, 0, s>>>((uint32_t *)grid, words);
+    if (nl)
+        k_lgrid_mark<<<(nl + 255) / 256, 256, 0, s>>>(ltri, nl, grid, lo[0], lo[1], lo[2], inv, dim[0], dim[1], dim[2],
+                                                      margin);
+}
 void launch_wide4(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, Wide4 *wide4) {
     if (n_nodes > 1) k_wide4<<<(n_nodes + 255) / 256, 256, 0, s>>>(nodes, n_nodes, wide4);
 }
@@ -1303,12 +1413,14 @@ void launch_popcount(hipStream_t s, const uint32_t *bits, size_t n_words, unsign
     if (n_words) k_popcount<<<(unsigned)((n_words + 255) / 256), 256, 0, s>>>(bits, n_words, out);
 }
 void launch_nee_pdf_lazy(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays, uint32_t *list,
-                         uint32_t *list_count) {
+                         uint32_t *list_count, uint32_t *mlist, uint32_t *mcount) {
     if (!max_rays) return;
+    k_nee_select<<<(max_rays + NEE_SELECT_THREADS - 1) / NEE_SELECT_THREADS, NEE_SELECT_THREADS, 0, s>>>(
+        sv, ba, list, list_count, mlist, mcount);
     with_stack(3 * ((sv.ldepth + 1) / 2), [&](auto S) {
         constexpr int K = decltype(S)::value;
-        k_nee_select<K><<<(max_rays + 63) / 64, 64, 0, s>>>(sv, ba, list, list_count);
         const uint32_t grid = std::min<uint32_t>((max_rays + 63) / 64, VRGI_NEE_LIST_GRID);
+        k_nee_near<K><<<grid, 64, 0, s>>>(sv, ba, mlist, mcount, list, list_count);
         k_nee_listed<K><<<grid, 64, 0, s>>>(sv, ba, list, list_count);
     });
 }

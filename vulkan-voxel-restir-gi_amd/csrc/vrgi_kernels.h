@@ -43,6 +43,11 @@ struct SceneView {
     const uint32_t *lanc_len;
     const vrgi_material *mats;
     uint32_t n_mats;
+    // cells of a grid over the light triangles, set where some light triangle's box comes
+    // within LGRID_MARGIN of the cell (null: no grid); cell of p = floor((p - lg_lo) * lg_inv)
+    const uint8_t *lgrid;
+    float lg_lo[3], lg_inv;
+    uint32_t lg_dim[3];
 };
 
 struct BounceArgs {
@@ -411,6 +416,10 @@ void launch_flatten(hipStream_t s, const FlattenArgs &f);
 void launch_flatten_lights(hipStream_t s, const FlattenArgs &f);
 void launch_wide(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, WideNode *wide);
 void launch_wide4(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, Wide4 *wide4);
+// the light grid of SceneView (cells of side 1 / inv, dim[0] x dim[1] x dim[2], x fastest):
+// cleared, then every cell within `margin` of a light triangle's box set
+void launch_light_grid(hipStream_t s, const Tri *ltri, uint32_t nl, uint8_t *grid, const float lo[3], float inv,
+                       const uint32_t dim[3], float margin);
 void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, const float *d,
                           uint32_t n, float tmin, float tmax, uint32_t *hit, float *t, float *uv);
 // grid_cap > 0: a grid of at most grid_cap blocks striding over the queue (for queues
@@ -424,11 +433,14 @@ void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, ui
 // rays whose next vertex returns radiance (ba.lrun non-zero), or whose pdf might not be
 // a finite non-negative number. The others get 0: their q differs from the reference's,
 // but L_b = E_b + ((R_b * 0) * (bsdf_pdf / q)) does not. Needs launch_resolve_step(b + 1)
-// first. Pass 1 writes the 0s and lists the other rays (list: max_rays entries,
-// *list_count zeroed before), pass 2 computes their pdf at full lane occupancy.
-#define VRGI_NEE_LIST_GRID 4096  // blocks of the pass striding over the listed rays
+// first. Pass 1 writes the 0s and lists the other rays: those whose pdf matters (list),
+// and those for which that depends on a light triangle crossed near the origin (mlist,
+// decided by a walk over the light triangles in pass 1b, which lists them or writes 0);
+// pass 2 computes the listed rays' pdf at full lane occupancy. list, mlist: max_rays
+// entries each; *list_count, *mcount zeroed before.
+#define VRGI_NEE_LIST_GRID 4096  // blocks of the passes striding over a list
 void launch_nee_pdf_lazy(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays, uint32_t *list,
-                         uint32_t *list_count);
+                         uint32_t *list_count, uint32_t *mlist, uint32_t *mcount);
 // One bounce k of the outgoing-radiance recurrence (outgoing_radiance.rs:58-94) for every
 // pixel whose path reached vertex k: lrun = E_k + ((R_k * lrun) * (bsdf_pdf_k / q_k)),
 // lrun taken as 0 past the path's end; the arithmetic of the full resolve.
