@@ -1327,6 +1327,11 @@ __global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) 
     __shared__ MidRange s_r[2][MID_LIST];
     __shared__ uint32_t s_n[2];
     __shared__ uint32_t s_wsum[MID_THREADS / 64];
+    // the node's subtree jobs, appended to the global list at the end with one counter
+    // update (a job holds >= 3 primitives: at most MID_MAX / 3 of them)
+    __shared__ uint4 s_job[MID_MAX / 3 + 1];  // first | count << 16, rank, node, parent
+    __shared__ uint16_t s_jdep[MID_MAX / 3 + 1];
+    __shared__ uint32_t s_nj, s_jbase;
     const OpenNode on = ((const OpenNode *)s.mid)[blockIdx.x];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t base = a.tree_base[on.tree];
@@ -1342,6 +1347,7 @@ __global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) 
         s_r[0][0] = MidRange{0u, c, on.rank, on.node, on.depth, on.parent};
         s_n[0] = 1;
         s_n[1] = 0;
+        s_nj = 0;
     }
     __syncthreads();
     const float pad = 0.0001f;
@@ -1537,8 +1543,9 @@ __global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) 
                     splits += 2;
                     maxdepth = max(maxdepth, g.depth + 2);
                 } else if (cc <= (uint32_t)a.sub_max) {
-                    const uint32_t jb = atomicAdd(s.n_jobs, 1u);
-                    ((OpenNode *)s.jobs)[jb] = OpenNode{on.first + cf, cc, cr, ci, g.node, on.tree, g.depth + 1, 0};
+                    const uint32_t jb = atomicAdd(&s_nj, 1u);
+                    s_job[jb] = make_uint4(cf | (cc << 16), cr, ci, g.node);
+                    s_jdep[jb] = (uint16_t)(g.depth + 1);
                 } else {
                     const uint32_t q = atomicAdd(&s_n[cur ^ 1u], 1u);
                     s_r[cur ^ 1u][q] = MidRange{cf, cc, cr, ci, g.depth + 1, g.node};
@@ -1561,6 +1568,13 @@ __global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) 
     for (uint32_t k = tid; k < c; k += MID_THREADS) {
         a.perm[on.first + k] = s_perm[k];
         for (int q = 0; q < 3; ++q) a.pc[q * a.n + on.first + k] = s_c[q][k];
+    }
+    if (tid == 0) s_jbase = s_nj ? atomicAdd(s.n_jobs, s_nj) : 0u;
+    __syncthreads();
+    for (uint32_t k = tid; k < s_nj; k += MID_THREADS) {
+        const uint4 j = s_job[k];
+        ((OpenNode *)s.jobs)[s_jbase + k] =
+            OpenNode{on.first + (j.x & 0xFFFFu), j.x >> 16, j.y, j.z, j.w, on.tree, s_jdep[k], 0};
     }
     if (maxdepth) atomicMax(a.depth + on.tree, maxdepth);
     if (splits) atomicAdd(a.split_count_sum, splits);
