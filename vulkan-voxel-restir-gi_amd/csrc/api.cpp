@@ -1426,8 +1426,10 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     // writes slot b + 1: the two run side by side
     // Lazy NEE pdf: the pdfs run after all bounces, from the last bounce back, each after
     // the resolve step that gives the radiance its rays return, and only for rays where the
-    // value can reach the outputs (launch_nee_pdf_lazy). Not with the learned cache, whose
-    // training records take the pdf of every vertex.
+    // value can reach the outputs (launch_nee_pdf_lazy). Not with the learned cache: its
+    // training paths return radiance from most of their vertices, so few pdfs could be
+    // left out, and their passes, which otherwise run beside the next bounce, would add
+    // to the end of the frame (measured: cache frame 2.8 -> 3.7 ms).
     const bool lazy_nee = prefs->nee_type != 0 && !c->cache.on && !(c->cfg.flags & VRGI_FLAG_FULL_NEE_PDF);
     const bool nee_overlap = prefs->nee_type != 0 && !lazy_nee && !(c->cfg.flags & VRGI_FLAG_SERIAL_NEE);
     while (c->ev_bounce.size() < B) {
@@ -1470,13 +1472,22 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     // the frame's start (ev_fstart) but are queued on the host after the first bounce's
     // launch, so that their ~30 launches do not delay the frame's first rays.
     if (cache_train) HIP_CHECK(hipEventRecord(c->ev_fstart, s));
-    auto enqueue_training = [&]() {
+    // the snapshot is queued with the first bounce (the inference waits for it), the rest
+    // of the training after the second bounce's launch, so that the host's ~30 calls do
+    // not hold back the launches of the frame's ray chain
+    auto enqueue_snapshot = [&]() {
         auto &C = c->cache;
         hipStream_t ts = c->up_stream;
         HIP_CHECK(hipStreamWaitEvent(ts, c->ev_fstart, 0));
         HIP_CHECK(hipMemcpyAsync(C.table_snap.p, C.table.p, C.n_table * sizeof(float), hipMemcpyDeviceToDevice, ts));
         HIP_CHECK(hipMemcpyAsync(C.wt_snap.p, C.wt.p, NRC_N_WEIGHTS * sizeof(uint16_t), hipMemcpyDeviceToDevice, ts));
         HIP_CHECK(hipEventRecord(c->ev_snap, ts));
+    };
+    bool training_queued = false;
+    auto enqueue_training = [&]() {
+        auto &C = c->cache;
+        hipStream_t ts = c->up_stream;
+        training_queued = true;
         HIP_CHECK(hipMemsetAsync(C.loss.p, 0, 4, ts));
         C.trec_sorted.reserve((size_t)C.tcap * 16);
         C.sort_hist.reserve(NRC_SORT_BUCKETS + NRC_SORT_BUCKETS / 1024);
@@ -1550,7 +1561,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         const uint32_t small_q = cache_on && b > c->cache.cfg.query_vertex ? VRGI_SMALL_QUEUE_GRID : 0u;
         launch_trace_shade(s, sv, ba, P, small_q);
         qin[b + 1] = ba.queue_out;
-        if (cache_train && b == 0) enqueue_training();  // before any wait on ev_snap
+        if (cache_train && b == 0) enqueue_snapshot();  // before any wait on ev_snap
+        if (cache_train && b == 1) enqueue_training();
         if (cache_on && b == c->cache.cfg.query_vertex) {
             // side stream: cache radiance of this bounce's queries, by the network as the
             // frame started (the snapshot while the training steps run)
@@ -1600,6 +1612,7 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             rec_on(2 + B + b, ns);
         }
     }
+    if (cache_train && !training_queued) enqueue_training();
     FrameArgs fa{};
     fa.W = c->W; fa.H = c->H; fa.P = P; fa.B = B; fa.frame = frame;
     fa.slot = (uint32_t)(c->frame_count % 3);

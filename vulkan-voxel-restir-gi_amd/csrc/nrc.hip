@@ -226,42 +226,98 @@ __device__ __forceinline__ f32x4 layer_out(const uint16_t *wt5, const bf16x4 x[4
 }
 
 // -------------------------------------------------------------------- inference
+// The five layers' 16-bit [n][k] weights in LDS, one row of 64 per output neuron padded
+// to WROW halves: rows 34 words apart put the 16 rows one ds_read_b64 touches in distinct
+// bank pairs. Row r of the packed weights (l * 64 + n, the output layer's rows 256 + n)
+// is row r here.
+static constexpr int WROW = 68;
+static constexpr int WROWS = 4 * 64 + 16;
+__device__ __forceinline__ void weights_to_lds(const uint16_t *wt, uint16_t *sw) {
+    for (uint32_t i = threadIdx.x; i < NRC_N_WEIGHTS / 4; i += blockDim.x) {
+        const uint32_t h = 4u * i;
+        *(bf16x4 *)(sw + (h / 64u) * WROW + h % 64u) = *(const bf16x4 *)(wt + h);
+    }
+    __syncthreads();
+}
+
 // records: 16 floats (position, direction, normal, albedo, target, key); out: 3 floats
 // per record; when out_px is set, also out_px[3 * key + c] (negative outputs clamp to 0)
-// `count` may exceed the buffer's capacity `max_n` (producers count the records they drop)
+// `count` may exceed the buffer's capacity `max_n` (producers count the records they drop).
+// A wave takes 16 * INFER_NB records per iteration as INFER_NB blocks of 16 (the lane
+// layout of the file comment within each block), so that every weight operand read from
+// LDS serves INFER_NB MFMAs; each output is the same two 32-long steps in the same order
+// as in training.
+#ifndef VRGI_INFER_NB
+#define VRGI_INFER_NB 4
+#endif
+#ifndef VRGI_INFER_WAVES
+#define VRGI_INFER_WAVES 3
+#endif
+static constexpr int INFER_NB = VRGI_INFER_NB;
 template <int P>
-__global__ __launch_bounds__(256) void k_nrc_infer(NrcNet net, const float *rec, const uint32_t *count,
+__global__ __launch_bounds__(256, VRGI_INFER_WAVES) void k_nrc_infer(NrcNet net, const float *rec, const uint32_t *count,
                                                    uint32_t max_n, float *out, float *out_px) {
+    __shared__ __align__(16) uint16_t sw[WROWS * WROW];
+    weights_to_lds(net.wt, sw);
     const uint32_t lane = threadIdx.x & 63u, s = lane & 15u, g = lane >> 4;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t n = min(*count, max_n);
-    const uint32_t n_blocks = (n + 15u) / 16u;
-    for (uint32_t blk = wave; blk < n_blocks; blk += n_waves) {
-        const uint32_t i = blk * 16u + s;
-        const bool live = i < n;
-        Sample sm{};
-        if (live) sm = load_sample(rec + (size_t)i * 16);
-        bf16x4 x[4], h[4];
-        encode<P>(net, sm, g, live, x);
+    const uint32_t n_groups = (n + 16u * INFER_NB - 1u) / (16u * INFER_NB);
+    for (uint32_t grp = wave; grp < n_groups; grp += n_waves) {
+        // the weights are read from LDS where they are used, not kept in registers across
+        // iterations (a compiler barrier: the LDS reads stay inside the loop)
+        asm volatile("" ::: "memory");
+        bf16x4 x[INFER_NB][4];
+#pragma unroll
+        for (int j = 0; j < INFER_NB; ++j) {
+            const uint32_t i = grp * (16u * INFER_NB) + 16u * j + s;
+            const bool live = i < n;
+            Sample sm{};
+            if (live) sm = load_sample(rec + (size_t)i * 16);
+            encode<P>(net, sm, g, live, x[j]);
+        }
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
-            layer_fwd<P>(net.wt + l * 4096, x, h, lane);
+            bf16x4 h[INFER_NB][4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) x[t] = h[t];
-        }
-        f32x4 y = layer_out<P>(net.wt + 4 * 4096, x, lane);
-        if (g == 0 && live) {
-            if (out) {
-                out[3 * (size_t)i] = y[0];
-                out[3 * (size_t)i + 1] = y[1];
-                out[3 * (size_t)i + 2] = y[2];
+            for (int nt = 0; nt < 4; ++nt) {
+                const uint16_t *w = sw + (l * 64 + 16 * nt + s) * WROW;
+                const h16x8 a0 = cat8(ld_bf4(w + 4 * g), ld_bf4(w + 16 + 4 * g));
+                const h16x8 a1 = cat8(ld_bf4(w + 32 + 4 * g), ld_bf4(w + 48 + 4 * g));
+#pragma unroll
+                for (int j = 0; j < INFER_NB; ++j) {
+                    f32x4 acc = Prec<P>::mma32(a0, cat8(x[j][0], x[j][1]), f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+                    acc = Prec<P>::mma32(a1, cat8(x[j][2], x[j][3]), acc);
+                    h[j][nt] = pack4<P>(fmaxf(acc[0], 0.0f), fmaxf(acc[1], 0.0f), fmaxf(acc[2], 0.0f),
+                                        fmaxf(acc[3], 0.0f));
+                }
             }
-            if (out_px) {
-                uint32_t px = __float_as_uint(rec[(size_t)i * 16 + 15]) >> 4;
-                out_px[3 * (size_t)px] = fmaxf(y[0], 0.0f);
-                out_px[3 * (size_t)px + 1] = fmaxf(y[1], 0.0f);
-                out_px[3 * (size_t)px + 2] = fmaxf(y[2], 0.0f);
+#pragma unroll
+            for (int j = 0; j < INFER_NB; ++j)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x[j][t] = h[j][t];
+        }
+        const uint16_t *w5 = sw + (256 + s) * WROW;
+        const h16x8 a0 = cat8(ld_bf4(w5 + 4 * g), ld_bf4(w5 + 16 + 4 * g));
+        const h16x8 a1 = cat8(ld_bf4(w5 + 32 + 4 * g), ld_bf4(w5 + 48 + 4 * g));
+#pragma unroll
+        for (int j = 0; j < INFER_NB; ++j) {
+            f32x4 y = Prec<P>::mma32(a0, cat8(x[j][0], x[j][1]), f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            y = Prec<P>::mma32(a1, cat8(x[j][2], x[j][3]), y);  // lanes 0..15: outputs 0..3
+            const uint32_t i = grp * (16u * INFER_NB) + 16u * j + s;
+            if (g == 0 && i < n) {
+                if (out) {
+                    out[3 * (size_t)i] = y[0];
+                    out[3 * (size_t)i + 1] = y[1];
+                    out[3 * (size_t)i + 2] = y[2];
+                }
+                if (out_px) {
+                    uint32_t px = __float_as_uint(rec[(size_t)i * 16 + 15]) >> 4;
+                    out_px[3 * (size_t)px] = fmaxf(y[0], 0.0f);
+                    out_px[3 * (size_t)px + 1] = fmaxf(y[1], 0.0f);
+                    out_px[3 * (size_t)px + 2] = fmaxf(y[2], 0.0f);
+                }
             }
         }
     }
@@ -590,22 +646,30 @@ __global__ __launch_bounds__(256) void k_rec_hist(const float *rec, const uint32
     key_rank[2 * (size_t)i] = key;
     key_rank[2 * (size_t)i + 1] = rank;
 }
-// exclusive scan of the bucket sizes in place, per block of 1024 buckets (one bucket per
-// thread, coalesced); the block's total goes to hist[NRC_SORT_BUCKETS + block]
-__global__ __launch_bounds__(1024) void k_rec_scan(uint32_t *hist) {
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x, i = blockIdx.x * 1024u + t;
-    const uint32_t c = hist[i];
-    part[t] = c;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1) {
-        const uint32_t v = t >= d ? part[t - d] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+// exclusive scan of the bucket sizes in place, per block of 1024 buckets (four consecutive
+// buckets per thread of 256, a wave scan of the threads' sums, then the waves' totals);
+// the block's total goes to hist[NRC_SORT_BUCKETS + block]. 256-thread blocks: the scan
+// runs beside the frame's ray kernels, where 1024-thread blocks wait for a CU to free
+// 16 wave slots at once.
+__global__ __launch_bounds__(256) void k_rec_scan(uint32_t *hist) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    uint4 *p = (uint4 *)(hist + (size_t)blockIdx.x * 1024u) + t;
+    const uint4 c = *p;
+    const uint32_t own = c.x + c.y + c.z + c.w;
+    uint32_t inc = own;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += u;
     }
-    hist[i] = part[t] - c;
-    if (t == 1023) hist[NRC_SORT_BUCKETS + blockIdx.x] = part[t];
+    if (lane == 63u) wsum[wv] = inc;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < wv; ++k) base += wsum[k];
+    const uint32_t e = base + inc - own;  // exclusive prefix of this thread's first bucket
+    *p = make_uint4(e, e + c.x, e + c.x + c.y, e + c.x + c.y + c.z);
+    if (t == 255u) hist[NRC_SORT_BUCKETS + blockIdx.x] = e + own;
 }
 __global__ __launch_bounds__(256) void k_rec_scatter(const float *rec, const uint32_t *count, uint32_t cap,
                                                      const uint32_t *start, const uint32_t *key_rank, float *out) {
@@ -624,7 +688,7 @@ void launch_rec_sort(hipStream_t s, const float *rec, const uint32_t *count, uin
     if (!cap) return;
     k_rec_clear<<<NRC_SORT_BUCKETS / 256, 256, 0, s>>>(hist);
     k_rec_hist<<<(cap + 255) / 256, 256, 0, s>>>(rec, count, cap, box, hist, key_rank);
-    k_rec_scan<<<NRC_SORT_BUCKETS / 1024, 1024, 0, s>>>(hist);
+    k_rec_scan<<<NRC_SORT_BUCKETS / 1024, 256, 0, s>>>(hist);
     k_rec_scatter<<<(cap + 255) / 256, 256, 0, s>>>(rec, count, cap, hist, key_rank, out);
 }
 
@@ -636,8 +700,9 @@ static void with_prec(uint32_t prec, F f) {
 
 void launch_nrc_infer(hipStream_t s, const NrcNet &net, const float *rec, const uint32_t *count, uint32_t max_n,
                       float *out, float *out_px) {
-    uint32_t blocks16 = (max_n + 15) / 16;
-    uint32_t grid = std::min<uint32_t>((blocks16 + 3) / 4, 4096u);
+    // one wave per 16 * INFER_NB records, up to one residency round (4 blocks per CU)
+    uint32_t groups = (max_n + 16 * INFER_NB - 1) / (16 * INFER_NB);
+    uint32_t grid = std::min<uint32_t>((groups + 3) / 4, 1024u);
     if (!grid) return;
     with_prec(net.prec, [&](auto Pc) { k_nrc_infer<decltype(Pc)::value><<<grid, 256, 0, s>>>(net, rec, count, max_n, out, out_px); });
 }
