@@ -1546,6 +1546,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     }
     // queue of the rays of each bounce: compaction order, or Morton order when enabled
     std::vector<const uint32_t *> qin(B + 1, nullptr);
+    // a cache frame querying at vertex 0: bounce 0 in two parts (BounceArgs::b0_part)
+    const bool split_b0 = cache_on && c->cache.cfg.query_vertex == 0 && VRGI_CACHE_SPLIT_B0;
     const bool sort_rays = c->cfg.ray_sort_bits != 0 && c->n_tris >= 2;
     for (uint32_t b = 0; b < B; ++b) {
         ba.bounce = b;
@@ -1559,7 +1561,24 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
         }
         // past the cache's query vertex only training paths (and fallback tiles) go on
         const uint32_t small_q = cache_on && b > c->cache.cfg.query_vertex ? VRGI_SMALL_QUEUE_GRID : 0u;
-        launch_trace_shade(s, sv, ba, P, small_q);
+        if (b == 0 && split_b0) {
+            // the full-path pixels' first bounce starts the frame's ray chain on this stream;
+            // the other pixels' (which end at vertex 0 with a cache query) run beside it on
+            // the side stream, followed there by the inference
+            launch_full_list(s, ba, c->f_queue.p, c->f_qcount.p);
+            HIP_CHECK(hipEventRecord(c->ev_query, s));
+            HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_query, 0));
+            BounceArgs bb = ba;
+            bb.b0_part = 2;
+            launch_trace_shade(c->side, sv, bb, P, 0);
+            ba.b0_part = 1;
+            ba.queue_in = c->f_queue.p;
+            launch_trace_shade(s, sv, ba, P, VRGI_SMALL_QUEUE_GRID);
+            ba.b0_part = 0;
+            ba.queue_in = nullptr;
+        } else {
+            launch_trace_shade(s, sv, ba, P, small_q);
+        }
         qin[b + 1] = ba.queue_out;
         if (cache_train && b == 0) enqueue_snapshot();  // before any wait on ev_snap
         if (cache_train && b == 1) enqueue_training();
@@ -1567,8 +1586,10 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             // side stream: cache radiance of this bounce's queries, by the network as the
             // frame started (the snapshot while the training steps run)
             auto &C = c->cache;
-            HIP_CHECK(hipEventRecord(c->ev_query, s));
-            HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_query, 0));
+            if (!split_b0) {  // split: the queries were written on the side stream
+                HIP_CHECK(hipEventRecord(c->ev_query, s));
+                HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_query, 0));
+            }
             NrcNet net = cache_net(c);
             if (cache_train) {
                 HIP_CHECK(hipStreamWaitEvent(c->side, c->ev_snap, 0));
@@ -1689,9 +1710,25 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
             rec(2 + B + b);
         }
     }
-    if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_infer, 0));  // cache radiance is ready
-    rec(2 + B + (B - 1));
-    launch_accum_temporal(s, fa);
+    if (split_b0) {
+        // pixels that end at vertex 0 with a cache query: after the inference on the side
+        // stream, beside the training paths' bounces; the full-path pixels here
+        FrameArgs f1 = fa;
+        f1.part = 1;
+        launch_accum_temporal(c->side, f1);
+        HIP_CHECK(hipEventRecord(c->ev_infer, c->side));
+        rec(2 + B + (B - 1));
+        FrameArgs f2 = fa;
+        f2.part = 2;
+        f2.full_list = c->f_queue.p;
+        f2.full_count = c->f_qcount.p;
+        launch_accum_temporal(s, f2);
+        HIP_CHECK(hipStreamWaitEvent(s, c->ev_infer, 0));  // every pixel's temporal reservoir
+    } else {
+        if (cache_on) HIP_CHECK(hipStreamWaitEvent(s, c->ev_infer, 0));  // cache radiance is ready
+        rec(2 + B + (B - 1));
+        launch_accum_temporal(s, fa);
+    }
     if (cache_on && c->cache.cfg.fallback_bound > 0.0f) {
         // held-out error of this frame's records at the query vertex (the side stream has
         // finished its training steps, none of them on these records), per screen tile

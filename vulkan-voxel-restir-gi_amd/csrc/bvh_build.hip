@@ -176,9 +176,15 @@ __global__ void k_roots(BuildArgs a, BuildState s) {
 // It also starts the level: the open list of the previous level's children becomes
 // current, the level's first entry in the level-node list is recorded and, when the level
 // holds open nodes, the count of non-empty levels is raised to include it.
-__global__ void k_items(BuildState s, uint32_t *level_start, uint32_t lvl, uint32_t *levels_used) {
-    __shared__ uint32_t s_sum[1024];
+// One block of ITEMS_THREADS: the build runs beside the frame's ray kernels, and a block
+// of 1024 threads waits there until one CU has 16 wave slots free at once.
+static constexpr uint32_t ITEMS_THREADS = 256;
+__global__ __launch_bounds__(ITEMS_THREADS) void k_items(BuildState s, uint32_t *level_start, uint32_t lvl,
+                                                         uint32_t *levels_used) {
+    constexpr uint32_t NT = ITEMS_THREADS, NW = NT / 64;
+    __shared__ uint32_t s_sum[NT];
     __shared__ uint32_t s_nopen;
+    __shared__ uint32_t s_wave[NW];
     if (threadIdx.x == 0) {
         const uint32_t n = *s.n_open_next;
         s_nopen = n;
@@ -191,12 +197,11 @@ __global__ void k_items(BuildState s, uint32_t *level_start, uint32_t lvl, uint3
     const OpenNode *open = (const OpenNode *)s.open;
     const uint32_t n_open = s_nopen;
     uint32_t carry = 0;
-    __shared__ uint32_t s_wave[16];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    for (uint32_t base = 0; base < n_open; base += 1024) {
+    for (uint32_t base = 0; base < n_open; base += NT) {
         uint32_t i = base + threadIdx.x;
         uint32_t c = i < n_open ? (open[i].count + CHUNK - 1) / CHUNK : 0;
-        // inclusive scan: within each wave, then over the 16 wave totals
+        // inclusive scan: within each wave, then over the wave totals
         uint32_t inc = c;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t u = __shfl_up(inc, off, 64);
@@ -204,26 +209,18 @@ __global__ void k_items(BuildState s, uint32_t *level_start, uint32_t lvl, uint3
         }
         if (lane == 63) s_wave[w] = inc;
         __syncthreads();
-        if (w == 0) {
-            const uint32_t v = lane < 16 ? s_wave[lane] : 0u;
-            uint32_t wi = v;
-            for (int off = 1; off < 16; off <<= 1) {
-                const uint32_t u = __shfl_up(wi, off, 64);
-                if (lane >= (uint32_t)off) wi += u;
-            }
-            if (lane < 16) s_wave[lane] = wi;
-        }
-        __syncthreads();
-        s_sum[threadIdx.x] = (w ? s_wave[w - 1] : 0u) + inc;
+        uint32_t before = 0;
+        for (uint32_t k = 0; k < w; ++k) before += s_wave[k];
+        s_sum[threadIdx.x] = before + inc;
         __syncthreads();
         uint32_t excl = carry + s_sum[threadIdx.x] - c;
         if (i < n_open) {
             s.item_beg[i] = excl;
             s.item_end[i] = excl + c;
         }
-        const uint32_t total = s_sum[1023];
-        for (uint32_t j = threadIdx.x; j < total; j += 1024) {
-            uint32_t lo = 0, hi = 1023;  // first t with s_sum[t] > j
+        const uint32_t total = s_sum[NT - 1];
+        for (uint32_t j = threadIdx.x; j < total; j += NT) {
+            uint32_t lo = 0, hi = NT - 1;  // first t with s_sum[t] > j
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (s_sum[mid] > j) hi = mid;
@@ -1651,7 +1648,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
             if (lvl + 2 >= level_cap) return -5;
             std::swap(st.open, st.open_next);
             const uint32_t nb = bound(lvl);
-            k_items<<<1, 1024, 0, stream>>>(st, level_start, lvl, counters + 5);
+            k_items<<<1, ITEMS_THREADS, 0, stream>>>(st, level_start, lvl, counters + 5);
             const uint32_t node_grid = std::min(nb, GRID_CAP_NODES);
             const uint32_t item_grid = std::min(n / CHUNK + nb + 1, GRID_CAP_ITEMS);
             k_clear_node_data<<<node_grid, 256, 0, stream>>>(st);

@@ -10,6 +10,8 @@
 // these kernels substitute without reading them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "devcommon.h"
 #include "vrgi_kernels.h"
 
@@ -169,7 +171,19 @@ __device__ __forceinline__ void accum_temporal_px(const FrameArgs &fa, uint32_t 
     res_store(slot, R);
 }
 __global__ __launch_bounds__(256) void k_accum_temporal(const FrameArgs fa) {
-    accum_temporal_px(fa, blockIdx.x * blockDim.x + threadIdx.x);
+    if (fa.part == 2) {  // the listed full-path pixels; blocks stride (uniformly per block)
+        const uint32_t n = min(*fa.full_count, fa.P);
+        for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+            const uint32_t i = base + threadIdx.x;
+            accum_temporal_px(fa, i < n ? fa.full_list[i] : 0xFFFFFFFFu);
+        }
+        return;
+    }
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fa.part == 1 && id < fa.P &&
+        full_path_px(fa.train_seed, fa.train_frac, fa.fallback, id, fa.W, fa.tile, fa.tiles_x))
+        id = 0xFFFFFFFFu;  // left to part 2
+    accum_temporal_px(fa, id);
 }
 
 // restir_spatial_resampling.rs:316-406 + restir_finalize.rs:121-141
@@ -301,7 +315,8 @@ void launch_batch_pixels_post(hipStream_t s, const BatchLaunch &L) {
 }
 
 void launch_accum_temporal(hipStream_t s, const FrameArgs &fa) {
-    k_accum_temporal<<<(fa.P + 255) / 256, 256, 0, s>>>(fa);
+    const uint32_t grid = (fa.P + 255) / 256;
+    k_accum_temporal<<<fa.part == 2 ? std::min(grid, 2048u) : grid, 256, 0, s>>>(fa);
 }
 void launch_spatial_finalize(hipStream_t s, const FrameArgs &fa) {
     k_spatial_finalize<<<(fa.P + 255) / 256, 256, 0, s>>>(fa);

@@ -464,6 +464,10 @@ __device__ __forceinline__ IcS local_cs(f3 a, f3 b, f3 c) {  // raytrace.rs:319-
 }
 
 // one wave-sized slice of the bounce's rays (all 64 lanes call it: compaction ballots)
+__device__ __forceinline__ bool full_path(const BounceArgs &ba, uint32_t px) {
+    return full_path_px(ba.train_seed, ba.train_frac, ba.fallback, px, ba.W, ba.tile, ba.tiles_x);
+}
+
 template <int STACK, bool COUNT, bool CACHE>
 __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const BounceArgs &ba, uint32_t *stack,
                                                   uint32_t i, uint32_t count) {
@@ -471,10 +475,12 @@ __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const Bou
     bool active = i < count;
     uint32_t px = 0;
     f3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+    if (CACHE && active && b == 0 && ba.b0_part == 2)  // the full-path pixels run in part 1
+        active = !full_path(ba, i);
     if (active) {
         if (b == 0) {
             // raygen.rs:84-111 (jitter is multiplied by 0.00)
-            px = i;
+            px = (CACHE && ba.b0_part == 1) ? ba.queue_in[i] : i;
             uint32_t x = px % ba.W, y = px / ba.W;
             float uvx = 2.0f * (float)x / (float)ba.W - 1.0f;
             float uvy = 2.0f * (float)y / (float)ba.H - 1.0f;
@@ -494,8 +500,7 @@ __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const Bou
     bool train = false;
     uint32_t cap = ba.B;  // bounces this pixel's path may take
     if (CACHE) {
-        train = mm_finalizef(mm_combine(ba.train_seed, px)) < ba.train_frac ||
-                in_fallback(ba.fallback, px, ba.W, ba.tile, ba.tiles_x);
+        train = full_path(ba, px);
         cap = train ? ba.B : ba.cache_vertex + 1u;
         if (active && b == 0) st3(ba.d0 + (size_t)px * 3, d);
     }
@@ -621,7 +626,7 @@ __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const Bou
 template <int STACK, bool COUNT, bool CACHE>
 __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const BounceArgs ba) {
     __shared__ uint32_t stack[STACK * 64];
-    const uint32_t count = ba.bounce == 0 ? ba.P : ba.qcount[ba.bounce];
+    const uint32_t count = ba.bounce == 0 ? (CACHE && ba.b0_part == 1 ? ba.qcount[0] : ba.P) : ba.qcount[ba.bounce];
     if (blockIdx.x * 64u >= count) return;
     trace_shade_slice<STACK, COUNT, CACHE>(sv, ba, stack, blockIdx.x * 64u + threadIdx.x, count);
 }
@@ -632,7 +637,7 @@ __global__ __launch_bounds__(64) void k_trace_shade(const SceneView sv, const Bo
 template <int STACK, bool CACHE>
 __global__ __launch_bounds__(64) void k_trace_shade_gs(const SceneView sv, const BounceArgs ba) {
     __shared__ uint32_t stack[STACK * 64];
-    const uint32_t count = ba.bounce == 0 ? ba.P : ba.qcount[ba.bounce];
+    const uint32_t count = ba.bounce == 0 ? (CACHE && ba.b0_part == 1 ? ba.qcount[0] : ba.P) : ba.qcount[ba.bounce];
     for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u)
         trace_shade_slice<STACK, false, CACHE>(sv, ba, stack, base + threadIdx.x, count);
 }
@@ -1362,6 +1367,44 @@ void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba
         else if (cnt) k_trace_shade<K, true, false><<<grid, 64, 0, s>>>(sv, ba);
         else k_trace_shade<K, false, false><<<grid, 64, 0, s>>>(sv, ba);
     });
+}
+
+// the full-path pixels of a cache frame: FULL_PER_THREAD pixels per thread and one counter
+// update per block (every wave updating one address queues the updates up behind each
+// other: 0.32 ms at 1080p)
+constexpr uint32_t FULL_PER_THREAD = 16;
+__global__ __launch_bounds__(256) void k_full_list(const BounceArgs ba, uint32_t *list, uint32_t *count) {
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    const uint32_t first = blockIdx.x * 256u * FULL_PER_THREAD + t;
+    uint32_t take = 0;  // bit k: pixel first + 256 k
+#pragma unroll
+    for (uint32_t k = 0; k < FULL_PER_THREAD; ++k) {
+        const uint32_t px = first + 256u * k;
+        if (px < ba.P && full_path(ba, px)) take |= 1u << k;
+    }
+    const uint32_t own = (uint32_t)__popc(take);
+    uint32_t inc = own;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += u;
+    }
+    if (lane == 63u) s_wave[wv] = inc;
+    __syncthreads();
+    if (t == 0) {
+        const uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        s_base = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos = s_base + inc - own;
+    for (uint32_t k = 0; k < wv; ++k) pos += s_wave[k];
+    for (uint32_t k = 0; k < FULL_PER_THREAD; ++k)
+        if (take >> k & 1u) list[pos++] = first + 256u * k;
+}
+void launch_full_list(hipStream_t s, const BounceArgs &ba, uint32_t *list, uint32_t *count) {
+    k_full_list<<<(ba.P + 256 * FULL_PER_THREAD - 1) / (256 * FULL_PER_THREAD), 256, 0, s>>>(ba, list, count);
 }
 
 __global__ __launch_bounds__(256) void k_lgrid_clear(uint32_t *g, size_t words) {

@@ -8,6 +8,9 @@ namespace vrgi {
 
 #define VRGI_MAX_STACK 160  // traversal stack entries per lane (LDS); deeper trees are rejected
 #define VRGI_SMALL_QUEUE_GRID 2048  // blocks of a striding ray launch over a queue expected to be short
+#ifndef VRGI_CACHE_SPLIT_B0
+#define VRGI_CACHE_SPLIT_B0 1  // cache frames querying at vertex 0 split bounce 0 (BounceArgs::b0_part)
+#endif
 #ifndef VRGI_NEE_STREAMS
 #define VRGI_NEE_STREAMS 4  // streams the NEE pdf launches of a frame rotate over
 #endif
@@ -76,6 +79,10 @@ struct BounceArgs {
     float *albedo;          // [B][P][3] material reflectivity at each hit
     float *qrec;            // [P][16] query records
     uint32_t *qrec_count;
+    // bounce 0 of a cache frame in two launches, so that the training paths' bounces run
+    // beside the other pixels' first hits: 0 every pixel; 1 the full-path pixels listed in
+    // queue_in (qcount[0] of them, launch_full_list); 2 every other pixel
+    uint32_t b0_part;
     // NEE pdf in three passes (nee_rec set): per queue slot i, the crossed light
     // triangles (at most VRGI_NEE_RECS: lid plane [k][P], barycentric pair plane [k][P]),
     // their count (VRGI_NEE_RECS + 1: more, the ray is on the overflow list)
@@ -129,6 +136,10 @@ struct FrameArgs {
     float *trec;                       // training records [tcap][16]
     uint32_t *trec_count;
     uint32_t tcap;
+    // k_accum_temporal over part of the pixels: 0 all; 1 all but the full-path pixels;
+    // 2 the full-path pixels of full_list[0 .. *full_count) (see BounceArgs::b0_part)
+    uint32_t part;
+    const uint32_t *full_list, *full_count;
 };
 
 // ReSTIR DI (benchmark mode, restir_mode = 1)
@@ -425,6 +436,9 @@ void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, co
 // grid_cap > 0: a grid of at most grid_cap blocks striding over the queue (for queues
 // expected to be far shorter than max_rays). NEE pdf with ba.nee_cnt set: enumeration,
 // walk and overflow passes instead of one kernel.
+// the pixels that trace full paths in a cache frame (training pixels, fallback tiles), in
+// list[0 .. *count): the full-path pixels of bounce 0 with b0_part = 1
+void launch_full_list(hipStream_t s, const BounceArgs &ba, uint32_t *list, uint32_t *count);
 void launch_trace_shade(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays,
                         uint32_t grid_cap = 0);
 void launch_nee_pdf(hipStream_t s, const SceneView &sv, const BounceArgs &ba, uint32_t max_rays,
@@ -519,6 +533,11 @@ void launch_tile_update(hipStream_t s, float *acc, float *ema, uint8_t *fallback
 // a pixel of a fallback tile traces a full path, like a training pixel
 __device__ __forceinline__ bool in_fallback(const uint8_t *fb, uint32_t px, uint32_t W, uint32_t tile, uint32_t tx) {
     return fb && fb[((px / W) / tile) * tx + (px % W) / tile];
+}
+// a cache frame's pixel that traces a full path: a training pixel or one of a fallback tile
+__device__ __forceinline__ bool full_path_px(uint32_t train_seed, float train_frac, const uint8_t *fb, uint32_t px,
+                                             uint32_t W, uint32_t tile, uint32_t tx) {
+    return mm_finalizef(mm_combine(train_seed, px)) < train_frac || in_fallback(fb, px, W, tile, tx);
 }
 
 // ---------------------------------------------------------------- batched scenes
