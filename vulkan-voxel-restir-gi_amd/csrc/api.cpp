@@ -139,6 +139,19 @@ struct vrgi_ctx {
     std::map<uint32_t, std::pair<vrgi_object *, std::vector<float>>> objects;
     std::vector<vrgi_object *> old;
     bool inst_dirty = true, geo_dirty = true;
+    // results of the last geometry build made with readbacks, which a rebuild of the
+    // unchanged geometry repeats (the builder replays that build's launch sequence): tree
+    // depths, leaf-depth sum and light root box; cleared whenever the geometry changes
+    struct GeoPlan {
+        bool valid = false;
+        uint32_t dep[2] = {0, 0};
+        unsigned long long S = 0;
+        vrgi_bvh_node lroot{};
+    } geo_plan;
+    DevBuf<uint32_t> d_check;     // replayed builds whose counts left their plan (sticky)
+    uint32_t *h_check = nullptr;  // pinned copy, read at the next replay
+    hipEvent_t ev_check = nullptr;
+    bool check_pending = false;
 
     Builder geo_builder, tl_builder, api_builder;
 
@@ -283,6 +296,12 @@ struct vrgi_ctx {
 };
 
 namespace {
+
+// the scene's geometry changed: the next frame rebuilds it, with readbacks (no replay)
+void mark_geo_dirty(vrgi_ctx *c) {
+    c->geo_dirty = true;
+    c->geo_plan.valid = false;
+}
 
 // current device of the context; a scene in a batch first waits for the batch's queued work
 void set_dev(vrgi_ctx *c) {
@@ -597,7 +616,7 @@ void rebuild_instances(vrgi_ctx *c) {
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
-void build_geometry_on(vrgi_ctx *c, hipStream_t s);
+void build_geometry_on(vrgi_ctx *c, hipStream_t s, bool replay = false);
 
 // adds a rendered frame's stage times (events of parity k) to the profile output
 void harvest_frame(vrgi_ctx *c, uint32_t k) {
@@ -627,7 +646,7 @@ void harvest_frame(vrgi_ctx *c, uint32_t k) {
 
 // world triangles + world light triangles, geometry and light-geometry BVHs in one
 // batched build, traversal copies
-void rebuild_geometry(vrgi_ctx *c, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+void rebuild_geometry(vrgi_ctx *c, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, bool replay = false) {
     const bool pipe = !(c->cfg.flags & VRGI_FLAG_SERIAL_BUILD);
     if (pipe) {
         // the other set; the frame that read it last must have ended (its buffers may
@@ -637,7 +656,7 @@ void rebuild_geometry(vrgi_ctx *c, hipEvent_t e0 = nullptr, hipEvent_t e1 = null
     }
     hipStream_t s = pipe ? c->up_stream : c->stream;
     if (e0) HIP_CHECK(hipEventRecord(e0, s));
-    build_geometry_on(c, s);
+    build_geometry_on(c, s, replay);
     if (e1) HIP_CHECK(hipEventRecord(e1, s));
     if (pipe) {
         HIP_CHECK(hipEventRecord(c->ev_geo_ready, s));
@@ -645,9 +664,22 @@ void rebuild_geometry(vrgi_ctx *c, hipEvent_t e0 = nullptr, hipEvent_t e1 = null
     }
 }
 
-void build_geometry_on(vrgi_ctx *c, hipStream_t s) {
+// With `replay`, a rebuild of geometry unchanged since the last build: when that build's
+// plan is at hand, the builder repeats its launch sequence and the host reads nothing back
+// (depths, leaf-depth sum and light root box are the plan's), so the frame's launches are
+// queued without waiting for the build.
+void build_geometry_on(vrgi_ctx *c, hipStream_t s, bool replay) {
     c->ext_gnodes = c->ext_lnodes = nullptr;
     uint32_t nt = c->n_tris, nl = c->n_ltri, N = nt + nl;
+    const uint32_t n_trees = (nt ? 1u : 0u) + (nl ? 1u : 0u);
+    const BuildPlan &bp = c->geo_builder.plan();
+    const bool rep = replay && c->geo_plan.valid && bp.valid && bp.n == N && bp.trees == n_trees;
+    if (rep && c->check_pending) {  // the previous replay's check (its copy ended long ago)
+        HIP_CHECK(hipEventSynchronize(c->ev_check));
+        c->check_pending = false;
+        if (*c->h_check)
+            throw std::runtime_error("a replayed geometry build left its plan (the geometry changed unnoticed)");
+    }
     c->G().d_tri.reserve(nt);
     c->G().d_tri_inst.reserve(nt);
     c->G().d_ltri.reserve(nl);
@@ -699,18 +731,38 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s) {
     if (nt) { beg[nt_trees] = 0; cnt[nt_trees] = nt; base[nt_trees] = 0; ++nt_trees; }
     if (nl) { beg[nt_trees] = nt; cnt[nt_trees] = nl; base[nt_trees] = gnodes; ++nt_trees; }
     for (uint32_t t = 0; t < nt_trees; ++t) { trees[t] = beg[t]; trees[2 + t] = cnt[t]; trees[4 + t] = base[t]; }
-    HIP_CHECK(hipMemcpyAsync(c->d_trees.p, trees, 24, hipMemcpyHostToDevice, s));
+    if (!rep) HIP_CHECK(hipMemcpyAsync(c->d_trees.p, trees, 24, hipMemcpyHostToDevice, s));  // else unchanged
+    if (rep && !c->h_check) {
+        c->d_check.reserve(1);
+        HIP_CHECK(hipMemsetAsync(c->d_check.p, 0, 4, s));
+        HIP_CHECK(hipHostMalloc((void **)&c->h_check, 4, hipHostMallocDefault));
+        *c->h_check = 0;
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_check, hipEventDisableTiming));
+    }
     if (c->geo_builder.build(s, c->d_prim9.p, c->d_power.p, c->d_ids.p, N, nt_trees, c->d_trees.p,
                              c->d_trees.p + 2, c->d_trees.p + 4, base, c->G().d_nodes.p, c->G().d_depth.p,
-                             nullptr) != 0)
+                             nullptr, rep, rep ? c->d_check.p : nullptr) != 0)
         throw HipError("geometry build exceeded its workspace");
     uint32_t dep[2] = {0, 0};
     unsigned long long S = 0;
     vrgi_bvh_node lroot{};
-    HIP_CHECK(hipMemcpyAsync(dep, c->G().d_depth.p, nt_trees * 4, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(&S, c->geo_builder.leaf_depth_sum(), 8, hipMemcpyDeviceToHost, s));
-    if (nl) HIP_CHECK(hipMemcpyAsync(&lroot, c->G().d_nodes.p + gnodes, sizeof(lroot), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    if (rep) {
+        std::memcpy(dep, c->geo_plan.dep, sizeof(dep));
+        S = c->geo_plan.S;
+        lroot = c->geo_plan.lroot;
+        HIP_CHECK(hipMemcpyAsync(c->h_check, c->d_check.p, 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(c->ev_check, s));
+        c->check_pending = true;
+    } else {
+        HIP_CHECK(hipMemcpyAsync(dep, c->G().d_depth.p, nt_trees * 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(&S, c->geo_builder.leaf_depth_sum(), 8, hipMemcpyDeviceToHost, s));
+        if (nl) HIP_CHECK(hipMemcpyAsync(&lroot, c->G().d_nodes.p + gnodes, sizeof(lroot), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        std::memcpy(c->geo_plan.dep, dep, sizeof(dep));
+        c->geo_plan.S = S;
+        c->geo_plan.lroot = lroot;
+        c->geo_plan.valid = true;
+    }
     build_light_grid(c, s, lroot.min, lroot.max);
     // structural figures of the build (profile output)
     c->prof.build_leaf_depth_sum = S;
@@ -937,7 +989,8 @@ void voxels_commit(vrgi_ctx *c) {
     }
     V.commits += 1;
     V.chunks_remeshed += n;
-    c->inst_dirty = c->geo_dirty = true;
+    c->inst_dirty = true;
+    mark_geo_dirty(c);
 }
 
 bool valid_xf(const float *xf) {
@@ -1020,6 +1073,8 @@ vrgi_ctx::~vrgi_ctx() {
         if (geo[k].ev_free) (void)hipEventDestroy(geo[k].ev_free);
     if (ev_geo_ready) (void)hipEventDestroy(ev_geo_ready);
     if (h_qcount) (void)hipHostFree(h_qcount);
+    if (h_check) (void)hipHostFree(h_check);
+    if (ev_check) (void)hipEventDestroy(ev_check);
     for (auto &kv : objects) release(kv.second.first);
     for (vrgi_object *o : old) release(o);
     {
@@ -1310,7 +1365,8 @@ int vrgi_scene_add(vrgi_ctx *c, uint32_t key, vrgi_object *o, const float xf[12]
     }
     if (o) o->refs.fetch_add(1);
     c->objects[key] = {o, std::vector<float>(xf, xf + 12)};
-    c->inst_dirty = c->geo_dirty = true;
+    c->inst_dirty = true;
+    mark_geo_dirty(c);
     c->view_dirty = true;
     return VRGI_OK;
     API_END
@@ -1323,7 +1379,8 @@ int vrgi_scene_update(vrgi_ctx *c, uint32_t key, const float xf[12]) {
     if (it == c->objects.end()) return fail(VRGI_E_INVALID, "object with key does not exist (scene.rs:215)");
     if (!it->second.first) return VRGI_OK;  // Some(None): no-op
     it->second.second.assign(xf, xf + 12);
-    c->inst_dirty = c->geo_dirty = true;
+    c->inst_dirty = true;
+    mark_geo_dirty(c);
     c->view_dirty = true;
     return VRGI_OK;
     API_END
@@ -1336,7 +1393,8 @@ int vrgi_scene_remove(vrgi_ctx *c, uint32_t key) {
     if (it == c->objects.end()) return VRGI_OK;
     if (it->second.first) {
         c->old.push_back(it->second.first);
-        c->inst_dirty = c->geo_dirty = true;
+        c->inst_dirty = true;
+    mark_geo_dirty(c);
     }
     c->view_dirty = true;
     c->objects.erase(it);
@@ -1443,11 +1501,11 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     if (c->inst_dirty) {
         rebuild_instances(c);
         c->inst_dirty = false;
-        c->geo_dirty = true;
+        mark_geo_dirty(c);
     }
     if (c->geo_dirty || (c->cfg.flags & VRGI_FLAG_REBUILD_EVERY_FRAME)) {
         rebuild_geometry(c, stages ? c->events[par][3 + 3 * (size_t)B] : nullptr,
-                         stages ? c->events[par][4 + 3 * (size_t)B] : nullptr);
+                         stages ? c->events[par][4 + 3 * (size_t)B] : nullptr, !c->geo_dirty);
         c->geo_dirty = false;
         built = true;
     }
@@ -2076,7 +2134,7 @@ int vrgi_trace_closest(vrgi_ctx *c, const float *o, const float *d, uint64_t n, 
     if (c->inst_dirty) {
         rebuild_instances(c);
         c->inst_dirty = false;
-        c->geo_dirty = true;
+        mark_geo_dirty(c);
     }
     if (c->geo_dirty) {
         rebuild_geometry(c);
@@ -2837,7 +2895,7 @@ vrgi_batch::~vrgi_batch() {
     for (vrgi_ctx *c : sc) {
         // the canonical nodes lived in this batch's build arrays; the next render rebuilds
         c->ext_gnodes = c->ext_lnodes = nullptr;
-        c->geo_dirty = true;
+        mark_geo_dirty(c);
         c->in_batch = false;
         c->batch_stream = nullptr;
     }
@@ -2953,7 +3011,7 @@ int vrgi_batch_render(vrgi_batch *bt, const vrgi_camera *cams, const vrgi_prefs 
         if (c->inst_dirty) {
             rebuild_instances(c);
             c->inst_dirty = false;
-            c->geo_dirty = true;
+            mark_geo_dirty(c);
         }
         HIP_CHECK(hipStreamSynchronize(c->up_stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));

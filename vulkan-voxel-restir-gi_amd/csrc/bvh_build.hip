@@ -1599,12 +1599,30 @@ __global__ void k_power_pairs(vrgi_bvh_node *nodes, const uint32_t *pairs, const
     }
 }
 
+// clears the counters and the per-tree power flags (a kernel, not a memset: a build
+// replayed under stream capture may be recorded into a graph)
+__global__ __launch_bounds__(256) void k_reset(uint32_t *counters, uint32_t *powered, uint32_t n_trees) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 64) counters[i] = 0u;
+    if (i < n_trees) powered[i] = 0u;
+}
+// a replayed build's counts against its plan (counters after k_advance: [0] nodes open
+// past the last level, [2] jobs, [5] non-empty levels, [6] power levels, [7] middle nodes)
+__global__ void k_plan_check(const uint32_t *counters, BuildPlan p, uint32_t *check) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const bool bad = counters[0] != 0u || counters[2] != p.n_jobs || counters[5] != p.levels ||
+                     min(p.levels, counters[6]) != p.power_levels || counters[7] != p.n_mid;
+    if (bad) atomicOr(check, 1u);
+}
+
 int Builder::build(hipStream_t stream, const float *prim9, const float *power, const uint32_t *ids,
                    uint32_t n, uint32_t n_trees, const uint32_t *d_tree_beg,
                    const uint32_t *d_tree_n, const uint32_t *d_tree_base,
                    const uint32_t *h_tree_base, vrgi_bvh_node *nodes, uint32_t *d_depth,
-                   uint32_t *prim_order_out) {
+                   uint32_t *prim_order_out, bool replay, uint32_t *check) {
     if (n == 0 || n_trees == 0) return 0;
+    replay = replay && last_plan.valid && last_plan.n == n && last_plan.trees == n_trees;
+    if (!replay) last_plan.valid = false;
     reserve(n, n_trees);
     BuildArgs a{};
     a.prim9 = prim9;
@@ -1624,8 +1642,7 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     a.powered = powered;
     a.split_count_sum = (unsigned long long *)(counters + 8);
     (void)h_tree_base;
-    HIP_CHECK(hipMemsetAsync(counters, 0, 64 * 4, stream));
-    HIP_CHECK(hipMemsetAsync(powered, 0, (size_t)n_trees * 4, stream));
+    k_reset<<<(std::max(n_trees, 64u) + 255) / 256, 256, 0, stream>>>(counters, powered, n_trees);
     k_init<<<(n + 255) / 256, 256, 0, stream>>>(a);
     k_roots<<<(n_trees + 255) / 256, 256, 0, stream>>>(a, st);
     // Top phase, level-synchronous. Level l holds at most n_trees * 2^l open nodes, so the
@@ -1643,22 +1660,46 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     };
     uint32_t h_cnt[8];
     uint32_t lvl = 0;
+    auto queue_level = [&]() {
+        std::swap(st.open, st.open_next);
+        const uint32_t nb = bound(lvl);
+        k_items<<<1, ITEMS_THREADS, 0, stream>>>(st, level_start, lvl, counters + 5);
+        const uint32_t node_grid = std::min(nb, GRID_CAP_NODES);
+        const uint32_t item_grid = std::min(n / CHUNK + nb + 1, GRID_CAP_ITEMS);
+        k_clear_node_data<<<node_grid, 256, 0, stream>>>(st);
+        k_bounds<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+        k_bin<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+        if (!VRGI_SPLIT_IN_COUNT) k_split<<<node_grid, 64, 0, stream>>>(st);
+        k_count<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+        k_part_a<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+        k_part_b<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
+        k_children<<<std::min((nb + 127) / 128, GRID_CAP_NODES / 2), 128, 0, stream>>>(a, st, lvl, counters + 6);
+    };
+    if (replay) {
+        // the plan's launch sequence: its levels, middle nodes and jobs, no readback
+        const BuildPlan p = last_plan;
+        for (lvl = 0; lvl < p.levels; ++lvl) queue_level();
+        std::swap(st.open, st.open_next);
+        k_advance<<<1, 64, 0, stream>>>(st, level_start, lvl);
+        if (p.n_mid) k_mid<<<p.n_mid, MID_THREADS, 0, stream>>>(a, st);
+        if (p.n_jobs) {
+            k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL, false><<<p.n_jobs, 64, 0, stream>>>(a, st);
+            k_subtree<SUB_MAX_DEFAULT, VRGI_SUB_SMALL, true><<<p.n_jobs, 64, 0, stream>>>(a, st);
+        }
+        for (int l = (int)p.power_levels - 1; l >= 0; --l) {
+            const uint32_t nb = bound((uint32_t)l);
+            k_power_pairs<<<std::min((nb + 255) / 256, GRID_CAP_NODES), 256, 0, stream>>>(nodes, st.level_nodes,
+                                                                                         level_start, (uint32_t)l);
+        }
+        if (check) k_plan_check<<<1, 64, 0, stream>>>(counters, p, check);
+        if (prim_order_out && prim_order_out != perm)
+            HIP_CHECK(hipMemcpyAsync(prim_order_out, perm, (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
+        return 0;
+    }
     for (;;) {
         for (uint32_t j = 0; j < group; ++j, ++lvl) {
             if (lvl + 2 >= level_cap) return -5;
-            std::swap(st.open, st.open_next);
-            const uint32_t nb = bound(lvl);
-            k_items<<<1, ITEMS_THREADS, 0, stream>>>(st, level_start, lvl, counters + 5);
-            const uint32_t node_grid = std::min(nb, GRID_CAP_NODES);
-            const uint32_t item_grid = std::min(n / CHUNK + nb + 1, GRID_CAP_ITEMS);
-            k_clear_node_data<<<node_grid, 256, 0, stream>>>(st);
-            k_bounds<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
-            k_bin<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
-            if (!VRGI_SPLIT_IN_COUNT) k_split<<<node_grid, 64, 0, stream>>>(st);
-            k_count<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
-            k_part_a<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
-            k_part_b<<<item_grid, TOP_THREADS, 0, stream>>>(a, st);
-            k_children<<<std::min((nb + 127) / 128, GRID_CAP_NODES / 2), 128, 0, stream>>>(a, st, lvl, counters + 6);
+            queue_level();
         }
         HIP_CHECK(hipMemcpyAsync(h_cnt, counters, 8 * 4, hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
@@ -1691,6 +1732,13 @@ int Builder::build(hipStream_t stream, const float *prim9, const float *power, c
     }
     if (prim_order_out && prim_order_out != perm)
         HIP_CHECK(hipMemcpyAsync(prim_order_out, perm, (size_t)n * 4, hipMemcpyDeviceToDevice, stream));
+    last_plan.n = n;
+    last_plan.trees = n_trees;
+    last_plan.levels = h_cnt[5];
+    last_plan.power_levels = n_levels;
+    last_plan.n_mid = h_cnt[7];
+    last_plan.n_jobs = n_jobs;
+    last_plan.valid = true;
     return 0;
 }
 

@@ -87,6 +87,15 @@ static constexpr uint32_t BUILDER_LEVEL_CAP = 65536;
 
 // Owns the builder's workspace (grown on demand, reused across builds), or works in a
 // region the caller binds (no device allocation by the builder then).
+// the counts of a build, which a build of the same input repeats exactly
+struct BuildPlan {
+    uint32_t n = 0, trees = 0;
+    uint32_t levels = 0;        // top-phase levels holding open nodes
+    uint32_t power_levels = 0;  // of those, levels holding nodes of trees with non-zero power
+    uint32_t n_mid = 0, n_jobs = 0;
+    bool valid = false;
+};
+
 class Builder {
   public:
     ~Builder();
@@ -98,10 +107,15 @@ class Builder {
     // top-phase levels (to see whether any node is still open): the first group has the
     // previous same-sized build's level count, later groups 1, 2, 4, 4, ... levels. prim_order_out (may be null) receives the
     // final primitive index per leaf position (global over all trees).
+    // With `replay` set, the caller states that the input equals that of the last build
+    // made without it (the plan()); the build then repeats that build's launch sequence
+    // with no readback and no host synchronisation (so a stream capture can record it), and
+    // a check on the device ORs 1 into *check when a count differs from the plan.
     int build(hipStream_t stream, const float *prim9, const float *power, const uint32_t *ids,
               uint32_t n, uint32_t n_trees, const uint32_t *d_tree_beg, const uint32_t *d_tree_n,
               const uint32_t *d_tree_base, const uint32_t *h_tree_base, vrgi_bvh_node *nodes,
-              uint32_t *d_depth, uint32_t *prim_order_out);
+              uint32_t *d_depth, uint32_t *prim_order_out, bool replay = false, uint32_t *check = nullptr);
+    const BuildPlan &plan() const { return last_plan; }
     const uint32_t *prim_order() const { return perm; }
     // device address of the last build's sum over leaves of their depth
     const unsigned long long *leaf_depth_sum() const { return (const unsigned long long *)(counters + 8); }
@@ -118,6 +132,7 @@ class Builder {
     uint32_t *powered = nullptr;      // per tree: some primitive has non-zero power
     uint32_t level_cap = 0;
     uint32_t prev_n = 0, prev_trees = 0, prev_levels = 0;  // last build's size and non-empty top levels
+    BuildPlan last_plan;  // counts of the last build made without replay
     bool external = false;  // the workspace is a bound region, not owned
     BuildState st;
 };
