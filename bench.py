@@ -204,9 +204,13 @@ def batched_bench(steps):
             "scenes": n, "triangles": int(info.triangles), "light_triangles": int(info.light_triangles),
             "pixels": int(info.pixels), "ms_per_batched_frame": round(ms, 3),
             "mrays_per_s": round(info.rays / dt / 1e6, 2),
-            "last_frame_build_ms": round(float(info.build_ms), 3), "last_frame_passes_ms": round(float(info.passes_ms), 3),
             "rays_per_block": int(info.rays_per_block), "pixels_per_block": int(info.pixels_per_block),
-            "graph": bool(info.graph), "scenes_separately_ms": round(seq_ms, 3)}
+            "graph": {0: "none", 1: "one graph per pass", 2: "one graph per batched frame (builds and passes)"}[
+                int(info.graph)],
+            **({"last_frame_ms": round(float(info.passes_ms), 3)} if info.build_ms < 0 else
+               {"last_frame_build_ms": round(float(info.build_ms), 3),
+                "last_frame_passes_ms": round(float(info.passes_ms), 3)}),
+            "scenes_separately_ms": round(seq_ms, 3)}
 
 
 def large_dynamic_bench(frames=32):

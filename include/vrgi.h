@@ -391,10 +391,12 @@ typedef struct vrgi_batch_config {
 typedef struct vrgi_batch_info {
     uint32_t n_scenes, rays_per_block, pixels_per_block;
     uint32_t tuned;             /* 1 once the autotuner has fixed both sizes */
-    uint32_t graph;             /* 1 when passes replay a captured graph */
+    uint32_t graph;             /* 1: passes replay captured graphs; 2: the last batched frame
+                                   ran as one graph (its builds replayed, then every pass) */
     uint32_t pad;
     uint64_t triangles, light_triangles, pixels;
-    float build_ms, passes_ms;  /* GPU time of the last batched frame's build and passes */
+    float build_ms, passes_ms;  /* GPU time of the last batched frame's build and passes (graph
+                                   2 with builds: build_ms -1, passes_ms the whole frame) */
     uint64_t rays;              /* closest-hit rays since the previous vrgi_batch_info_get */
 } vrgi_batch_info;
 void vrgi_batch_config_default(vrgi_batch_config *cfg);

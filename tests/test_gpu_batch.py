@@ -88,7 +88,7 @@ def test_batch_matches_scenes_alone(world0, mode):
                 want = (outs[2] + outs[3]) * np.float32(0.5)
                 assert bt.mean(k).tobytes() == want.tobytes(), f"scene {k} mean"
             info = bt.info()
-            assert info.n_scenes == len(ctxs) and info.graph == 1
+            assert info.n_scenes == len(ctxs) and info.graph == 2  # static scenes: each batched frame one graph of its passes
             assert info.rays > 0
     finally:
         _close(ctxs)
@@ -143,25 +143,68 @@ def test_batch_tiles_graph_and_tuning_agree(world0):
         assert r == runs[0]
 
 
-def test_batch_rebuild_every_frame(world0):
+@pytest.mark.parametrize("spp,frames", [(1, 3), (2, 4)])
+def test_batch_rebuild_every_frame(world0, spp, frames):
     """Scenes that rebuild every frame: the batched multi-tree build gives each scene the
-    canonical BVHs its own build gives, and the frames still agree."""
+    canonical BVHs its own build gives, and the frames still agree. The first batched frame
+    builds with readbacks; the later ones replay that build inside one graph with every
+    pass of the frame (info.graph 2)."""
     import vrgi
     flags = vrgi.FLAG_REBUILD_EVERY_FRAME
-    say("batched rebuilds")
-    ref = _alone(world0, 0, 3, flags=flags)
+    say(f"batched rebuilds, spp {spp}, {frames} batched frames")
+    ref = _alone(world0, 0, spp * frames, flags=flags)
     ctxs, cams = _scenes(world0, flags=flags)
     try:
-        with vrgi.Batch(ctxs, spp=1, rays_per_block=64, pixels_per_block=128) as bt:
-            for f in range(3):
+        with vrgi.Batch(ctxs, spp=spp, rays_per_block=64, pixels_per_block=128) as bt:
+            for f in range(frames):
                 bt.render(cams, nee_type=1, spatial=5)
             bt.sync()
+            info = bt.info()
+            assert info.graph == 2 and info.build_ms < 0
             for k, c in enumerate(ctxs):
                 got = _state(c, bvh=True)
                 for key in got:
                     assert got[key] == ref[k][0][key], f"scene {k} {key}"
     finally:
         _close(ctxs)
+
+
+def test_batch_frame_graph_after_edit(world0):
+    """A scene's geometry changes between batched frames (an instance moves): that frame
+    builds with readbacks again, the next ones replay the new build in a new frame graph,
+    and every scene still equals the same scene rendered alone through the same edit."""
+    import vrgi
+    flags = vrgi.FLAG_REBUILD_EVERY_FRAME
+    say("batched frame graph across a scene edit")
+    iso = [1, 0, 0, 0, 1, 0, 0, 0, 1, 3.0, 0.0, -2.0]
+    key = vrgi.batch_scene(1)[0][0][0]  # an object of scene 1
+    states = []
+    for batched in (False, True):
+        ctxs, cams = _scenes(world0, flags=flags)
+        try:
+            if batched:
+                with vrgi.Batch(ctxs, spp=1, rays_per_block=64, pixels_per_block=128) as bt:
+                    for f in range(5):
+                        if f == 2:
+                            ctxs[1].update(key, iso)
+                        bt.render(cams, nee_type=1, spatial=5)
+                    bt.sync()
+                    assert bt.info().graph == 2
+                    states.append([_state(c, bvh=True) for c in ctxs])
+            else:
+                for f in range(5):
+                    if f == 2:
+                        ctxs[1].update(key, iso)
+                    for c, cam in zip(ctxs, cams):
+                        c.render(cam, nee_type=1, spatial=5)
+                for c in ctxs:
+                    c.sync()
+                states.append([_state(c, bvh=True) for c in ctxs])
+        finally:
+            _close(ctxs)
+    for k in range(len(states[0])):
+        for key in states[0][k]:
+            assert states[1][k][key] == states[0][k][key], f"scene {k} {key}"
 
 
 def test_batch_rejects_unsupported(world0):

@@ -2662,6 +2662,26 @@ struct vrgi_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<uint32_t> graph_key;
+    // Frame graph: the replayed builds and every pass of a batched frame in one graph,
+    // with the passes' arguments in spp consecutive regions of f_args (one copy per frame)
+    static constexpr uint32_t FRAME_GRAPH_SPP = 8;  // larger spp: one graph per pass
+    DevBuf<uint8_t> f_args;
+    uint8_t *h_fargs[RING] = {};
+    hipGraph_t fgraph = nullptr;
+    hipGraphExec_t fexec = nullptr;
+    std::vector<uint32_t> fgraph_key;
+    // plan of the last batched build made with readbacks, which a rebuild of the same
+    // scenes' unchanged geometry replays: the scenes (and their geometry buffer sets), the
+    // trees' depths; gen counts such builds (graphs holding a replay key on it)
+    bool plan_valid = false;
+    std::vector<vrgi_ctx *> plan_list;
+    std::vector<uint32_t> plan_gi, plan_dep;
+    uint32_t plan_gen = 0;
+    DevBuf<uint32_t> d_check;
+    uint32_t *h_check = nullptr;
+    hipEvent_t ev_check = nullptr;
+    bool check_pending = false;
+    bool last_frame_graph = false;  // the last batched frame ran as one graph
     // tiles: fixed, or tuned over the first batched frames
     uint32_t wpb = 0, pb = 0;
     int tune_step = 0;  // 0..2: rays per block 64/128/256; 3..5: pixels per block 64/128/256
@@ -2677,8 +2697,37 @@ namespace {
 
 const uint32_t BATCH_TILES[3] = {64, 128, 256};
 
-// one multi-tree build over the geometry and light-geometry trees of the listed scenes
-void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
+// the device word of replayed builds' plan checks and its pinned copy (set up before any
+// capture: no allocation may be recorded into a graph)
+void batch_check_init(vrgi_batch *bt) {
+    if (bt->h_check) return;
+    bt->d_check.reserve(1);
+    HIP_CHECK(hipMemsetAsync(bt->d_check.p, 0, 4, bt->stream));
+    HIP_CHECK(hipHostMalloc((void **)&bt->h_check, 4, hipHostMallocDefault));
+    *bt->h_check = 0;
+    HIP_CHECK(hipEventCreateWithFlags(&bt->ev_check, hipEventDisableTiming));
+}
+// reads the previous replay's check (its copy is queued behind that frame)
+void batch_check_read(vrgi_batch *bt) {
+    if (!bt->check_pending) return;
+    HIP_CHECK(hipEventSynchronize(bt->ev_check));
+    bt->check_pending = false;
+    if (*bt->h_check) throw std::runtime_error("a replayed batched build left its plan (a scene's geometry changed unnoticed)");
+}
+
+// whether the listed scenes' build can replay the batch's plan: the same scenes, in the
+// same geometry buffer sets, none of whose geometry changed since that build
+bool batch_replay_ok(const vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
+    if (!bt->plan_valid || list != bt->plan_list) return false;
+    for (size_t k = 0; k < list.size(); ++k)
+        if (list[k]->geo_dirty || list[k]->gi != bt->plan_gi[k]) return false;
+    return true;
+}
+
+// One multi-tree build over the geometry and light-geometry trees of the listed scenes.
+// replay (batch_replay_ok): the plan's build repeated with no readback, host state set
+// from the plan (and, with launch unset, only that: the launches are in a frame graph).
+void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list, bool replay = false, bool launch = true) {
     hipStream_t s = bt->stream;
     struct Part {
         vrgi_ctx *c;
@@ -2705,6 +2754,7 @@ void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
         c->ext_gnodes = c->ext_lnodes = nullptr;
     }
     if (N == 0) return;
+    if (replay && bt->plan_dep.size() != nT) throw std::logic_error("batched build replay without its plan");
     bt->prim9.reserve((size_t)9 * N);
     bt->power.reserve(N);
     bt->ids.reserve(N);
@@ -2724,6 +2774,7 @@ void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
         G.d_lwide4.reserve(p.nl > 1 ? p.nl - 1 : 1);
         G.d_lanc.reserve((size_t)p.nl * VRGI_LANC_MAX);
         G.d_lanc_len.reserve(p.nl);
+        if (!launch) continue;
         FlattenArgs f{};
         f.inst = c->d_inst.p;
         f.n_inst = c->n_inst;
@@ -2749,15 +2800,33 @@ void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
         launch_light_ancestry(s, G.d_ltri_inst.p, G.d_ltri_node.p, c->d_inst.p, c->d_tl.p, p.nl, G.d_lanc.p,
                               G.d_lanc_len.p);
     }
-    std::vector<uint32_t> tr(3 * (size_t)nT);
-    for (uint32_t t = 0; t < nT; ++t) { tr[t] = beg[t]; tr[nT + t] = cnt[t]; tr[2 * nT + t] = base[t]; }
-    HIP_CHECK(hipMemcpyAsync(bt->trees.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice, s));
-    if (bt->builder.build(s, bt->prim9.p, bt->power.p, bt->ids.p, (uint32_t)N, nT, bt->trees.p, bt->trees.p + nT,
-                          bt->trees.p + 2 * nT, base.data(), bt->nodes.p, bt->depth.p, nullptr) != 0)
-        throw HipError("batched build exceeded its workspace");
     std::vector<uint32_t> dep(nT);
-    HIP_CHECK(hipMemcpyAsync(dep.data(), bt->depth.p, nT * 4, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
+    if (replay) {
+        dep = bt->plan_dep;
+        if (launch) {  // the tree table is the plan's, still on the device
+            if (!bt->h_check) throw std::logic_error("batched build replay before its check was set up");
+            if (bt->builder.build(s, bt->prim9.p, bt->power.p, bt->ids.p, (uint32_t)N, nT, bt->trees.p,
+                                  bt->trees.p + nT, bt->trees.p + 2 * nT, base.data(), bt->nodes.p, bt->depth.p,
+                                  nullptr, true, bt->d_check.p) != 0)
+                throw HipError("batched build exceeded its workspace");
+        }
+    } else {
+        std::vector<uint32_t> tr(3 * (size_t)nT);
+        for (uint32_t t = 0; t < nT; ++t) { tr[t] = beg[t]; tr[nT + t] = cnt[t]; tr[2 * nT + t] = base[t]; }
+        HIP_CHECK(hipMemcpyAsync(bt->trees.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice, s));
+        if (bt->builder.build(s, bt->prim9.p, bt->power.p, bt->ids.p, (uint32_t)N, nT, bt->trees.p,
+                              bt->trees.p + nT, bt->trees.p + 2 * nT, base.data(), bt->nodes.p, bt->depth.p,
+                              nullptr) != 0)
+            throw HipError("batched build exceeded its workspace");
+        HIP_CHECK(hipMemcpyAsync(dep.data(), bt->depth.p, nT * 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        bt->plan_valid = bt->builder.plan().valid;
+        bt->plan_list = list;
+        bt->plan_gi.clear();
+        for (vrgi_ctx *c : list) bt->plan_gi.push_back(c->gi);
+        bt->plan_dep = dep;
+        ++bt->plan_gen;
+    }
     for (Part &p : parts) {
         vrgi_ctx *c = p.c;
         auto &G = c->G();
@@ -2767,10 +2836,12 @@ void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list) {
             throw std::runtime_error("BVH deeper than the traversal stack (geometry depth " +
                                      std::to_string(G.geo_depth) + ", light-geometry depth " +
                                      std::to_string(G.lgeo_depth) + ")");
-        if (p.nt) launch_wide(s, bt->nodes.p + p.gbase, 2 * p.nt - 1, G.d_gwide.p);
-        if (p.nl) {
-            launch_wide(s, bt->nodes.p + p.lbase, 2 * p.nl - 1, G.d_lwide.p);
-            launch_wide4(s, bt->nodes.p + p.lbase, 2 * p.nl - 1, G.d_lwide4.p);
+        if (launch) {
+            if (p.nt) launch_wide(s, bt->nodes.p + p.gbase, 2 * p.nt - 1, G.d_gwide.p);
+            if (p.nl) {
+                launch_wide(s, bt->nodes.p + p.lbase, 2 * p.nl - 1, G.d_lwide.p);
+                launch_wide4(s, bt->nodes.p + p.lbase, 2 * p.nl - 1, G.d_lwide4.p);
+            }
         }
         c->ext_gnodes = bt->nodes.p + p.gbase;
         c->ext_lnodes = bt->nodes.p + p.lbase;
@@ -2886,10 +2957,15 @@ vrgi_batch::~vrgi_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (fexec) (void)hipGraphExecDestroy(fexec);
+    if (fgraph) (void)hipGraphDestroy(fgraph);
     for (int k = 0; k < RING; ++k) {
         if (ev_args[k]) (void)hipEventDestroy(ev_args[k]);
         if (h_args[k]) (void)hipHostFree(h_args[k]);
+        if (h_fargs[k]) (void)hipHostFree(h_fargs[k]);
     }
+    if (h_check) (void)hipHostFree(h_check);
+    if (ev_check) (void)hipEventDestroy(ev_check);
     for (hipEvent_t e : {e_b0, e_b1, e_end})
         if (e) (void)hipEventDestroy(e);
     for (vrgi_ctx *c : sc) {
@@ -3020,8 +3096,19 @@ int vrgi_batch_render(vrgi_batch *bt, const vrgi_camera *cams, const vrgi_prefs 
     std::vector<vrgi_ctx *> list;
     for (vrgi_ctx *c : bt->sc)
         if (c->geo_dirty || (c->cfg.flags & VRGI_FLAG_REBUILD_EVERY_FRAME)) list.push_back(c);
-    if (!list.empty()) batch_build(bt, list);
+    // Rebuilds of unchanged geometry replay the plan of the last batched build; once the
+    // tiles are tuned, such a frame (or one without builds) runs as one graph holding the
+    // builds and every pass.
+    const bool replay = !list.empty() && batch_replay_ok(bt, list);
+    const bool frame_graph = (list.empty() || replay) && !(bt->cfg.flags & VRGI_BATCH_NO_GRAPH) && bt->tuned &&
+                             bt->cfg.spp <= vrgi_batch::FRAME_GRAPH_SPP;
+    if (replay) {
+        batch_check_init(bt);
+        batch_check_read(bt);
+    }
+    if (!list.empty()) batch_build(bt, list, replay, !frame_graph);  // frame graph: host state only here
     HIP_CHECK(hipEventRecord(bt->e_b1, s));
+    bt->last_frame_graph = frame_graph && !list.empty();
     uint32_t gneed = 0, lneed = 0;
     for (vrgi_ctx *c : bt->sc) {
         gneed = std::max(gneed, geom_stack_need(c->G().geo_depth));
@@ -3058,7 +3145,71 @@ int vrgi_batch_render(vrgi_batch *bt, const vrgi_camera *cams, const vrgi_prefs 
     const bool use_graph = !(bt->cfg.flags & VRGI_BATCH_NO_GRAPH) && !tuning;
     std::vector<uint32_t> key = {cls(gneed), cls(lneed), wpb, pb, L.ray_blocks, L.px_blocks, (uint32_t)L.nee,
                                  (uint32_t)L.di, bt->B};
-    for (uint32_t pass = 0; pass < bt->cfg.spp; ++pass) {
+    if (frame_graph) {
+        const uint32_t spp = bt->cfg.spp;
+        const size_t fbytes = (size_t)spp * bt->args_bytes;
+        if (bt->f_args.cap < fbytes) {  // new buffers: any graph holding the old ones goes
+            HIP_CHECK(hipStreamSynchronize(s));
+            bt->f_args.reserve(fbytes);
+            for (int k = 0; k < vrgi_batch::RING; ++k) {
+                if (bt->h_fargs[k]) HIP_CHECK(hipHostFree(bt->h_fargs[k]));
+                HIP_CHECK(hipHostMalloc((void **)&bt->h_fargs[k], fbytes, hipHostMallocDefault));
+                std::memset(bt->h_fargs[k], 0, fbytes);
+            }
+            bt->fgraph_key.clear();
+        }
+        const uint32_t r = bt->ring;
+        bt->ring = (r + 1) % vrgi_batch::RING;
+        if (bt->ev_used[r]) HIP_CHECK(hipEventSynchronize(bt->ev_args[r]));
+        uint8_t *h = bt->h_fargs[r];
+        for (uint32_t pass = 0; pass < spp; ++pass) {
+            uint8_t *hp = h + (size_t)pass * bt->args_bytes;
+            for (uint32_t k = 0; k < n; ++k) batch_fill(bt, hp, k, cams[k], *prefs);
+            BatchPass bp{};
+            bp.first = pass == 0;
+            bp.last = pass + 1 == spp;
+            bp.inv_spp = 1.0f / (float)spp;
+            *(BatchPass *)(hp + bt->off_pass) = bp;
+            for (vrgi_ctx *c : bt->sc) {
+                c->have_frame = true;
+                ++c->frame_count;
+            }
+        }
+        HIP_CHECK(hipMemcpyAsync(bt->f_args.p, h, fbytes, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipEventRecord(bt->ev_args[r], s));
+        bt->ev_used[r] = true;
+        std::vector<uint32_t> fkey = key;
+        fkey.insert(fkey.end(), {spp, (uint32_t)list.size(), replay ? bt->plan_gen : 0u});
+        if (!bt->fexec || bt->fgraph_key != fkey) {
+            if (bt->fexec) HIP_CHECK(hipGraphExecDestroy(bt->fexec));
+            if (bt->fgraph) HIP_CHECK(hipGraphDestroy(bt->fgraph));
+            bt->fexec = nullptr;
+            bt->fgraph = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            if (replay) batch_build(bt, list, true, true);
+            for (uint32_t pass = 0; pass < spp; ++pass) {
+                BatchLaunch Lp = L;
+                uint8_t *dp = bt->f_args.p + (size_t)pass * bt->args_bytes;
+                Lp.svs = (const SceneView *)(dp + bt->off_sv);
+                Lp.tba = (const BounceArgs *)(dp + bt->off_tba);
+                Lp.nba = (const BounceArgs *)(dp + bt->off_nba);
+                Lp.fas = (const FrameArgs *)(dp + bt->off_fa);
+                Lp.das = (const DiArgs *)(dp + bt->off_da);
+                Lp.pass = (const BatchPass *)(dp + bt->off_pass);
+                launch_batch_pass(s, Lp);
+            }
+            HIP_CHECK(hipStreamEndCapture(s, &bt->fgraph));
+            HIP_CHECK(hipGraphInstantiate(&bt->fexec, bt->fgraph, nullptr, nullptr, 0));
+            bt->fgraph_key = fkey;
+        }
+        HIP_CHECK(hipGraphLaunch(bt->fexec, s));
+        if (replay) {
+            HIP_CHECK(hipMemcpyAsync(bt->h_check, bt->d_check.p, 4, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipEventRecord(bt->ev_check, s));
+            bt->check_pending = true;
+        }
+    }
+    for (uint32_t pass = 0; pass < (frame_graph ? 0u : bt->cfg.spp); ++pass) {
         const uint32_t r = bt->ring;
         bt->ring = (r + 1) % vrgi_batch::RING;
         if (bt->ev_used[r]) HIP_CHECK(hipEventSynchronize(bt->ev_args[r]));
@@ -3092,6 +3243,11 @@ int vrgi_batch_render(vrgi_batch *bt, const vrgi_camera *cams, const vrgi_prefs 
             c->have_frame = true;
             ++c->frame_count;
         }
+    }
+    if (replay && !frame_graph) {
+        HIP_CHECK(hipMemcpyAsync(bt->h_check, bt->d_check.p, 4, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(bt->ev_check, s));
+        bt->check_pending = true;
     }
     HIP_CHECK(hipEventRecord(bt->e_end, s));
     bt->have_times = true;
@@ -3150,7 +3306,7 @@ int vrgi_batch_info_get(vrgi_batch *bt, vrgi_batch_info *info) {
     info->rays_per_block = bt->wpb * 64;
     info->pixels_per_block = bt->pb;
     info->tuned = bt->tuned ? 1u : 0u;
-    info->graph = bt->exec ? 1u : 0u;
+    info->graph = bt->fexec && !bt->fgraph_key.empty() ? 2u : bt->exec ? 1u : 0u;
     for (vrgi_ctx *c : bt->sc) {
         info->triangles += c->n_tris;
         info->light_triangles += c->n_ltri;
@@ -3160,6 +3316,10 @@ int vrgi_batch_info_get(vrgi_batch *bt, vrgi_batch_info *info) {
     if (bt->have_times) {
         HIP_CHECK(hipEventElapsedTime(&info->build_ms, bt->e_b0, bt->e_b1));
         HIP_CHECK(hipEventElapsedTime(&info->passes_ms, bt->e_b1, bt->e_end));
+        if (bt->last_frame_graph) {  // the builds ran inside the frame's graph
+            info->build_ms = -1.0f;
+            HIP_CHECK(hipEventElapsedTime(&info->passes_ms, bt->e_b0, bt->e_end));
+        }
     }
     uint64_t rays = 0;
     for (vrgi_ctx *c : bt->sc) {
