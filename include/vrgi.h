@@ -141,6 +141,21 @@ int vrgi_create(const vrgi_config *cfg, const vrgi_material *materials,
                 const float *texture_luminances, uint32_t n_slots, int device, vrgi_ctx **out);
 int vrgi_destroy(vrgi_ctx *ctx);
 
+/* The texture atlas of Renderer::new (interactive_rendering.rs:447, 735-760): for every
+ * texture slot t (the vertices' `t`, block.rs:116-119) three images, reflectivity
+ * (rgb, alpha = 1 - transmit probability), emissivity (rgb) and metallicity (r), each
+ * width x height RGBA8 in sRGB (R8G8B8A8_SRGB, interactive_rendering.rs:416), laid out
+ * [slot][3][height][width][4]. Shading then reads them at the hit's interpolated uv
+ * (raytrace.rs:437-481) with the reference's default sampler: nearest texel, clamp to
+ * edge, colour channels through the sRGB transfer function, alpha / 255. n_slots must
+ * cover the material table; rgba8 == NULL returns to the constant materials of
+ * vrgi_create. Not with ReSTIR DI (restir_mode 1) nor for a scene in a batch. */
+int vrgi_set_textures(vrgi_ctx *ctx, uint32_t n_slots, uint32_t width, uint32_t height, const uint8_t *rgba8);
+/* utils::get_texture_luminances (utils.rs:223-235) over such an atlas: per slot, the mean
+ * over its emissivity image of r + g + b in 8-bit units (the texture_luminances of
+ * vrgi_create). */
+int vrgi_texture_luminances(uint32_t n_slots, uint32_t width, uint32_t height, const uint8_t *rgba8, float *out);
+
 /* SceneUploader::upload_object (scene.rs:547-725). Thread-safe: concurrent calls run
  * side by side, each on its own stream and builder workspace (the reference uploads from
  * a pool of threads, chunk_manager.rs:214-258); it must not race with vrgi_destroy.

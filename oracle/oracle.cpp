@@ -534,6 +534,10 @@ struct Inst {
 struct or_scene {
     std::vector<Material> mats;
     std::vector<float> lum;
+    // texture atlas (or_scene_set_textures): linear RGBA per texel, [slot][3][h][w][4]
+    std::vector<float> tex;
+    uint32_t tex_w = 0, tex_h = 0, tex_slots = 0;
+    std::vector<float> tri_uv;                // per global id: uv of its 3 vertices (6 floats)
     std::map<uint32_t, Inst> objects;
     // built state
     std::vector<const Inst *> inst;           // instances = Some objects in key order
@@ -616,7 +620,7 @@ void scene_build(or_scene *s) {
         if (s->tl[k].left == 0xFFFFFFFFu && s->tl[k].right != 0xFFFFFFFFu)
             s->inst_tl_idx[s->tl[k].right] = (uint32_t)k;
     // world triangles in instance order
-    s->tri.clear(); s->tri_inst.clear(); s->tri_t.clear(); s->inst_tri_off.assign(ni + 1, 0);
+    s->tri.clear(); s->tri_inst.clear(); s->tri_t.clear(); s->tri_uv.clear(); s->inst_tri_off.assign(ni + 1, 0);
     s->ltri.clear(); s->ltri_inst.clear(); s->ltri_j.clear(); s->ltri_node.clear();
     for (size_t i = 0; i < ni; ++i) {
         const Object &o = *s->inst[i]->obj;
@@ -629,6 +633,10 @@ void scene_build(or_scene *s) {
             }
             s->tri_inst.push_back((uint32_t)i);
             s->tri_t.push_back(o.verts[3 * p].t);
+            for (int k = 0; k < 3; ++k) {
+                s->tri_uv.push_back(o.verts[3 * p + k].uv[0]);
+                s->tri_uv.push_back(o.verts[3 * p + k].uv[1]);
+            }
         }
         for (size_t j = 0; j < o.light_verts.size() / 3; ++j) {
             for (int k = 0; k < 3; ++k) {
@@ -875,6 +883,26 @@ inline IcS local_cs(v3 a, v3 b, v3 c) {
     return IcS{normalize(n), normalize(t), normalize(bt)};
 }
 
+// R8G8B8A8_SRGB read by a nearest, clamp-to-edge sampler (interactive_rendering.rs:416,
+// 744; vulkano 0.35.2 Sampler defaults): colour channels through the sRGB transfer
+// function (IEC 61966-2-1) evaluated in binary64 and rounded once, alpha as a / 255
+float srgb_decode(uint8_t v) {
+    const double c = (double)v / 255.0;
+    return (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+}
+// texel of normalised coordinate c over n texels: floor(c * n), clamped to the edge
+inline uint32_t texel_index(float c, uint32_t n) {
+    const float f = std::floor(c * (float)n);
+    return !(f >= 0.0f) ? 0u : (f >= (float)(n - 1) ? n - 1 : (uint32_t)f);
+}
+// image k (0 reflectivity, 1 emissivity, 2 metallicity) of texture slot t at uv
+// (raytrace.rs:465-467: atlas entry t * 3 + k)
+void tex_sample(const or_scene *s, uint32_t t, uint32_t k, const float uv[2], float out[4]) {
+    const uint32_t i = texel_index(uv[0], s->tex_w), j = texel_index(uv[1], s->tex_h);
+    const size_t at = (((size_t)t * 3 + k) * s->tex_h + j) * s->tex_w + i;
+    std::memcpy(out, &s->tex[at * 4], 16);
+}
+
 struct BounceOut {
     v3 origin, dir, normal, emis, refl;
     v3 alb = V(0, 0, 0);  // material reflectivity at the hit (learned-cache inputs)
@@ -912,14 +940,29 @@ BounceOut raytrace_px(const or_scene *s, const Cfg &c, uint32_t bounce, uint32_t
     v3 b3 = V((1.0f - h.u) - h.v, h.u, h.v);
     v3 t0 = tri_v(s->tri, h.prim, 0), t1 = tri_v(s->tri, h.prim, 1), t2 = tri_v(s->tri, h.prim, 2);
     IcS ics = local_cs(t0, t1, t2);
-    const Material &mt = s->mats[s->tri_t[h.prim]];
+    const uint32_t slot = s->tri_t[h.prim];
     v3 new_origin = (t0 * b3.x + t1 * b3.y) + t2 * b3.z;
     v3 new_dir;
-    v3 refl = V(mt.refl[0], mt.refl[1], mt.refl[2]);
+    // the slot's three textures at the hit (raytrace.rs:437-481), or its constant material
+    float t0v[4], t1v[4], t2v[4];
+    if (!s->tex.empty()) {
+        const float *q = &s->tri_uv[(size_t)h.prim * 6];
+        float uv[2];
+        for (int a = 0; a < 2; ++a) uv[a] = (q[a] * b3.x + q[2 + a] * b3.y) + q[4 + a] * b3.z;
+        tex_sample(s, slot, 0, uv, t0v);
+        tex_sample(s, slot, 1, uv, t1v);
+        tex_sample(s, slot, 2, uv, t2v);
+    } else {
+        const Material &mt = s->mats[slot];
+        std::memcpy(t0v, mt.refl, 16);
+        std::memcpy(t1v, mt.emis, 16);
+        std::memcpy(t2v, mt.metal, 16);
+    }
+    v3 refl = V(t0v[0], t0v[1], t0v[2]);
     o.alb = refl;
-    float alpha = mt.refl[3];
-    v3 emis = (1000.0f * V(mt.emis[0], mt.emis[1], mt.emis[2])) * -dot(dir, ics.n);
-    float metal = mt.metal[0];
+    float alpha = t0v[3];
+    v3 emis = (1000.0f * V(t1v[0], t1v[1], t1v[2])) * -dot(dir, ics.n);
+    float metal = t2v[0];
     float mis = 0.0f, pdf;
     float albedo_max;
     float u0 = finalizef(combine(seed, 0));
@@ -1423,6 +1466,33 @@ or_scene *or_scene_create(const void *materials, const float *lum, uint32_t n_sl
     return s;
 }
 void or_scene_destroy(or_scene *s) { delete s; }
+
+int or_scene_set_textures(or_scene *s, uint32_t n_slots, uint32_t w, uint32_t h, const uint8_t *rgba8) {
+    s->tex.clear();
+    s->tex_w = s->tex_h = s->tex_slots = 0;
+    if (!rgba8 || n_slots == 0) return 0;
+    if (w == 0 || h == 0 || n_slots < s->mats.size()) return -1;
+    const size_t texels = (size_t)n_slots * 3 * w * h;
+    s->tex.resize(texels * 4);
+    for (size_t q = 0; q < texels; ++q)
+        for (int c = 0; c < 4; ++c)
+            s->tex[q * 4 + c] = c < 3 ? srgb_decode(rgba8[q * 4 + c]) : (float)((double)rgba8[q * 4 + c] / 255.0);
+    s->tex_w = w;
+    s->tex_h = h;
+    s->tex_slots = n_slots;
+    return 0;
+}
+float or_srgb_decode(uint32_t v) { return srgb_decode((uint8_t)v); }
+uint32_t or_texel_index(float c, uint32_t n) { return texel_index(c, n); }
+// utils.rs:223-235: per slot, the mean over the emissivity image of r + g + b in u8 units
+void or_texture_luminances(uint32_t n_slots, uint32_t w, uint32_t h, const uint8_t *rgba8, float *out) {
+    for (uint32_t t = 0; t < n_slots; ++t) {
+        const uint8_t *e = rgba8 + ((size_t)t * 3 + 1) * w * h * 4;
+        float l = 0.0f;
+        for (size_t q = 0; q < (size_t)w * h; ++q) l += ((float)e[q * 4] + (float)e[q * 4 + 1]) + (float)e[q * 4 + 2];
+        out[t] = l / (float)(w * h);
+    }
+}
 
 int or_scene_add(or_scene *s, uint32_t key, const void *verts, uint64_t n, const float xf[12]) {
     if (n % 3 != 0) return -1;

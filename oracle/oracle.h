@@ -51,6 +51,12 @@ typedef struct or_scene or_scene;
 or_scene *or_scene_create(const void *materials /*vrgi_material[n]*/, const float *lum,
                           uint32_t n_slots);
 void or_scene_destroy(or_scene *s);
+// texture atlas: n_slots x 3 images (reflectivity, emissivity, metallicity) of w x h RGBA8
+// sRGB texels, [slot][3][h][w][4]; null clears it (constant materials)
+int or_scene_set_textures(or_scene *s, uint32_t n_slots, uint32_t w, uint32_t h, const uint8_t *rgba8);
+float or_srgb_decode(uint32_t v);
+uint32_t or_texel_index(float c, uint32_t n);
+void or_texture_luminances(uint32_t n_slots, uint32_t w, uint32_t h, const uint8_t *rgba8, float *out);
 int or_scene_add(or_scene *s, uint32_t key, const void *verts /*vrgi_vertex3d*/, uint64_t n,
                  const float xform[12]);
 int or_scene_remove(or_scene *s, uint32_t key);

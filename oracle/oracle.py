@@ -46,6 +46,9 @@ def lib():
         L.or_nrc_train_step.restype = C.c_double
         L.or_restir_gi_jacobian.restype = C.c_float
         L.or_nrc_round16.restype = C.c_double
+        L.or_srgb_decode.restype = C.c_float
+        L.or_texel_index.restype = C.c_uint32
+        L.or_texel_index.argtypes = [C.c_float, C.c_uint32]
         _lib = L
     return _lib
 
@@ -180,6 +183,25 @@ def frame_seed(seed, frame, salt):
     return L.or_murmur3_finalize(u(L.or_murmur3_combine(u(L.or_murmur3_combine(u(seed), u(salt))), u(frame))))
 
 
+def srgb_decode(v):
+    """Linear value of an 8-bit sRGB channel as the texture path reads it."""
+    return float(lib().or_srgb_decode(C.c_uint32(v)))
+
+
+def texel_index(c, n):
+    """Texel of normalised coordinate c over n texels (nearest, clamp to edge)."""
+    return int(lib().or_texel_index(c, n))
+
+
+def texture_luminances(rgba8):
+    """utils.rs:223-235 over an atlas [slots][3][h][w][4]."""
+    a = np.ascontiguousarray(rgba8, np.uint8)
+    out = np.zeros(a.shape[0], np.float32)
+    lib().or_texture_luminances(C.c_uint32(a.shape[0]), C.c_uint32(a.shape[3]), C.c_uint32(a.shape[2]), ptr(a),
+                                ptr(out))
+    return out
+
+
 class Frame(C.Structure):
     _fields_ = [(f, C.c_void_p) for f in FRAME_FIELDS]
 
@@ -212,6 +234,16 @@ class Scene:
 
     def remove(self, key):
         self.L.or_scene_remove(self.h, key)
+
+    def set_textures(self, rgba8):
+        """rgba8: [slots][3][h][w][4] uint8 (reflectivity, emissivity, metallicity images of
+        every texture slot, sRGB); None: constant materials."""
+        if rgba8 is None:
+            assert self.L.or_scene_set_textures(self.h, 0, 0, 0, None) == 0
+            return
+        a = np.ascontiguousarray(rgba8, np.uint8)
+        assert a.ndim == 5 and a.shape[1] == 3 and a.shape[4] == 4
+        assert self.L.or_scene_set_textures(self.h, a.shape[0], a.shape[3], a.shape[2], ptr(a)) == 0
 
     def build(self):
         self.L.or_scene_build(self.h)

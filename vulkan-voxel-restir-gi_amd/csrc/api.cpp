@@ -135,6 +135,10 @@ struct vrgi_ctx {
     std::vector<vrgi_material> mats;
     std::vector<float> lum;
     DevBuf<vrgi_material> d_mats;
+    // texture atlas (vrgi_set_textures): linear RGBA texels [slot][3][h][w]
+    DevBuf<float4> d_tex;
+    uint32_t tex_w = 0, tex_h = 0;
+    bool tex_on = false;
 
     std::map<uint32_t, std::pair<vrgi_object *, std::vector<float>>> objects;
     std::vector<vrgi_object *> old;
@@ -173,6 +177,7 @@ struct vrgi_ctx {
     // `up_stream` (unless VRGI_FLAG_SERIAL_BUILD)
     struct GeoSet {
         DevBuf<Tri> d_tri, d_ltri;
+        DevBuf<float2> d_tri_uv;  // texture atlas on: uv of each triangle's 3 vertices
         DevBuf<uint32_t> d_tri_inst, d_ltri_inst, d_ltri_node;
         DevBuf<uint32_t> d_depth;
         DevBuf<vrgi_bvh_node> d_nodes;
@@ -334,6 +339,12 @@ SceneView scene_view(vrgi_ctx *c) {
     sv.lanc_len = c->G().d_lanc_len.p;
     sv.mats = c->d_mats.p;
     sv.n_mats = (uint32_t)c->mats.size();
+    if (c->tex_on) {
+        sv.tex = c->d_tex.p;
+        sv.tri_uv = c->G().d_tri_uv.p;
+        sv.tex_w = c->tex_w;
+        sv.tex_h = c->tex_h;
+    }
     if (c->G().lg_on) {
         sv.lgrid = c->G().d_lgrid.p;
         for (int k = 0; k < 3; ++k) { sv.lg_lo[k] = c->G().lg_lo[k]; sv.lg_dim[k] = c->G().lg_dim[k]; }
@@ -713,6 +724,10 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s, bool replay) {
     f.power = c->d_power.p;
     f.ids = c->d_ids.p;
     f.prim_off = 0;
+    if (c->tex_on) {
+        c->G().d_tri_uv.reserve((size_t)3 * nt);
+        f.tri_uv = c->G().d_tri_uv.p;
+    }
     launch_flatten(s, f);
     FlattenArgs fl = f;
     fl.n_tris = nl;
@@ -1415,6 +1430,60 @@ int vrgi_dispose_old(vrgi_ctx *c) {
     API_END
 }
 
+// IEC 61966-2-1 decode of an 8-bit sRGB channel, in binary64 rounded once
+static float srgb_decode(uint8_t v) {
+    const double x = (double)v / 255.0;
+    return (float)(x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+}
+
+int vrgi_set_textures(vrgi_ctx *c, uint32_t n_slots, uint32_t w, uint32_t h, const uint8_t *rgba8) {
+    API_BEGIN
+    if (!c) return fail(VRGI_E_INVALID, "null context");
+    if (c->in_batch) return fail(VRGI_E_STATE, "the scene is in a batch (no texture atlas there)");
+    if (rgba8 && c->cfg.restir_mode == 1)
+        return fail(VRGI_E_STATE, "ReSTIR DI reads the constant materials: no texture atlas with restir_mode 1");
+    if (rgba8 && (n_slots < c->mats.size() || w == 0 || h == 0 || w > 4096 || h > 4096))
+        return fail(VRGI_E_INVALID, "the atlas needs a texture for every material slot, 1 <= width, height <= 4096");
+    if (rgba8 && (uint64_t)n_slots * 3 * w * h > (1ull << 28)) return fail(VRGI_E_LIMIT, "atlas above 2^28 texels");
+    set_dev(c);
+    // frames in flight may read the atlas or the uv of the current geometry
+    for (hipStream_t st : {c->stream, c->up_stream, c->side}) HIP_CHECK(hipStreamSynchronize(st));
+    for (hipStream_t st : c->nee_s) if (st) HIP_CHECK(hipStreamSynchronize(st));
+    if (!rgba8) {
+        c->tex_on = false;
+    } else {
+        float lut[256];
+        for (int v = 0; v < 256; ++v) lut[v] = srgb_decode((uint8_t)v);
+        const size_t texels = (size_t)n_slots * 3 * w * h;
+        std::vector<float4> t(texels);
+        for (size_t q = 0; q < texels; ++q)
+            t[q] = make_float4(lut[rgba8[4 * q]], lut[rgba8[4 * q + 1]], lut[rgba8[4 * q + 2]],
+                               (float)((double)rgba8[4 * q + 3] / 255.0));
+        c->d_tex.reserve(texels);
+        HIP_CHECK(hipMemcpy(c->d_tex.p, t.data(), texels * sizeof(float4), hipMemcpyHostToDevice));
+        c->tex_w = w;
+        c->tex_h = h;
+        c->tex_on = true;
+    }
+    mark_geo_dirty(c);  // the next build writes (or stops writing) the triangles' uv
+    c->view_dirty = true;
+    return VRGI_OK;
+    API_END
+}
+
+int vrgi_texture_luminances(uint32_t n_slots, uint32_t w, uint32_t h, const uint8_t *rgba8, float *out) {
+    API_BEGIN
+    if ((!rgba8 || !out) && n_slots) return fail(VRGI_E_INVALID, "null argument");
+    for (uint32_t t = 0; t < n_slots; ++t) {  // utils.rs:223-235
+        const uint8_t *e = rgba8 + ((size_t)t * 3 + 1) * w * h * 4;
+        float l = 0.0f;
+        for (size_t q = 0; q < (size_t)w * h; ++q) l += ((float)e[4 * q] + (float)e[4 * q + 1]) + (float)e[4 * q + 2];
+        out[t] = l / (float)(w * h);
+    }
+    return VRGI_OK;
+    API_END
+}
+
 int vrgi_resize(vrgi_ctx *c, uint32_t w, uint32_t h) {
     API_BEGIN
     if (!c) return fail(VRGI_E_INVALID, "null context");
@@ -1453,6 +1522,8 @@ int vrgi_render(vrgi_ctx *c, const vrgi_camera *cam, const vrgi_prefs *prefs, vr
     if (prefs->nee_type > 2) return fail(VRGI_E_INVALID, "nee_type must be 0, 1 or 2");
     if (prefs->restir_spatial_iterations > 20) return fail(VRGI_E_INVALID, "restir_spatial_iterations above 20");
     if (prefs->debug_view > 4) return fail(VRGI_E_INVALID, "debug_view must be in [0, 4]");
+    if (c->tex_on && c->cfg.restir_mode == 1)
+        return fail(VRGI_E_STATE, "ReSTIR DI reads the constant materials: no texture atlas with restir_mode 1");
     if (c->minimized) {  // zero extent: no frame (interactive_rendering.rs:978-981)
         if (out) std::memset(out, 0, sizeof(*out));
         return VRGI_OK;
@@ -2947,6 +3018,7 @@ const char *batch_scene_ok(const vrgi_ctx *c, const vrgi_ctx *first) {
     if (c->cfg.ray_sort_bits) return "ray sorting is not available in a batch";
     if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) return "VRGI_FLAG_KEEP_BOUNCES is not available in a batch";
     if (c->accum_on) return "FP64 accumulation is not available in a batch";
+    if (c->tex_on) return "a texture atlas is not available in a batch";
     return nullptr;
 }
 

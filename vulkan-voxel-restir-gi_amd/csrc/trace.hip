@@ -41,6 +41,8 @@ __global__ void k_flatten(const FlattenArgs f) {
     t.v2 = make_float4(w[2].x, w[2].y, w[2].z, 0.0f);
     f.tri[g] = t;
     f.tri_inst[g] = lo;
+    if (f.tri_uv)
+        for (int k = 0; k < 3; ++k) f.tri_uv[3 * (size_t)g + k] = make_float2(v[k].uv[0], v[k].uv[1]);
     write_prim9(f.prim9 + f.prim_off, f.prim_stride, g, w);
     f.power[f.prim_off + g] = 0.0f;
     f.ids[f.prim_off + g] = g;
@@ -464,6 +466,13 @@ __device__ __forceinline__ IcS local_cs(f3 a, f3 b, f3 c) {  // raytrace.rs:319-
 }
 
 // one wave-sized slice of the bounce's rays (all 64 lanes call it: compaction ballots)
+// texel of normalised texture coordinate c over n texels: nearest, clamp to edge
+// (the reference's default sampler, interactive_rendering.rs:744)
+__device__ __forceinline__ uint32_t texel_index(float c, uint32_t n) {
+    const float f = floorf(c * (float)n);
+    return !(f >= 0.0f) ? 0u : (f >= (float)(n - 1) ? n - 1 : (uint32_t)f);
+}
+
 __device__ __forceinline__ bool full_path(const BounceArgs &ba, uint32_t px) {
     return full_path_px(ba.train_seed, ba.train_frac, ba.fallback, px, ba.W, ba.tile, ba.tiles_x);
 }
@@ -523,13 +532,32 @@ __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const Bou
         uint32_t slot = __float_as_uint(tr.v0.w);
         f3 b3 = mk((1.0f - h.u) - h.v, h.u, h.v);
         IcS ics = local_cs(t0, t1, t2);
-        const vrgi_material mt = sv.mats[slot];
         out_o = (t0 * b3.x + t1 * b3.y) + t2 * b3.z;
-        refl = mk(mt.reflectivity[0], mt.reflectivity[1], mt.reflectivity[2]);
+        // the slot's reflectivity, emissivity and metallicity: its textures at the hit's uv
+        // (raytrace.rs:437-481) or its constant material
+        float4 m0, m1, m2;
+        if (sv.tex) {
+            const float2 q0 = sv.tri_uv[3 * (size_t)h.prim], q1 = sv.tri_uv[3 * (size_t)h.prim + 1],
+                         q2 = sv.tri_uv[3 * (size_t)h.prim + 2];
+            const float uvx = (q0.x * b3.x + q1.x * b3.y) + q2.x * b3.z;
+            const float uvy = (q0.y * b3.x + q1.y * b3.y) + q2.y * b3.z;
+            const size_t at = ((size_t)slot * 3 * sv.tex_h + texel_index(uvy, sv.tex_h)) * sv.tex_w +
+                              texel_index(uvx, sv.tex_w);
+            const size_t img = (size_t)sv.tex_h * sv.tex_w;
+            m0 = sv.tex[at];
+            m1 = sv.tex[at + img];
+            m2 = sv.tex[at + 2 * img];
+        } else {
+            const vrgi_material &mt = sv.mats[slot];
+            m0 = make_float4(mt.reflectivity[0], mt.reflectivity[1], mt.reflectivity[2], mt.reflectivity[3]);
+            m1 = make_float4(mt.emissivity[0], mt.emissivity[1], mt.emissivity[2], 0.0f);
+            m2 = make_float4(mt.metallicity[0], 0.0f, 0.0f, 0.0f);
+        }
+        refl = mk(m0.x, m0.y, m0.z);
         alb = refl;
-        float alpha = mt.reflectivity[3];
-        emis = (1000.0f * mk(mt.emissivity[0], mt.emissivity[1], mt.emissivity[2])) * -dot3(d, ics.n);
-        float metal = mt.metallicity[0];
+        float alpha = m0.w;
+        emis = (1000.0f * mk(m1.x, m1.y, m1.z)) * -dot3(d, ics.n);
+        float metal = m2.x;
         float albedo_max;
         float u0 = mm_finalizef(mm_combine(seed, 0));
         if (u0 < metal) {  // mirror

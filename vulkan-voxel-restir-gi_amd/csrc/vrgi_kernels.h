@@ -46,6 +46,11 @@ struct SceneView {
     const uint32_t *lanc_len;
     const vrgi_material *mats;
     uint32_t n_mats;
+    // texture atlas (null: the constant materials): linear RGBA texels [slot][3][h][w] and
+    // the uv of every triangle's three vertices
+    const float4 *tex;
+    const float2 *tri_uv;
+    uint32_t tex_w, tex_h;
     // cells of a grid over the light triangles, set where some light triangle's box comes
     // within LGRID_MARGIN of the cell (null: no grid); cell of p = floor((p - lg_lo) * lg_inv)
     const uint8_t *lgrid;
@@ -108,6 +113,7 @@ struct FlattenArgs {
     float *power;
     uint32_t *ids;
     uint32_t prim_stride, prim_off;
+    float2 *tri_uv;  // non-null: the uv of each triangle's 3 vertices (texture atlas on)
 };
 
 struct FrameArgs {

@@ -292,6 +292,16 @@ class Context:
     def remove(self, key):
         check(self.L.vrgi_scene_remove(self.h, C.c_uint32(key)))
 
+    def set_textures(self, rgba8):
+        """rgba8: [slots][3][h][w][4] uint8 sRGB atlas (None: constant materials)."""
+        if rgba8 is None:
+            check(self.L.vrgi_set_textures(self.h, 0, 0, 0, None))
+            return
+        a = np.ascontiguousarray(rgba8, np.uint8)
+        assert a.ndim == 5 and a.shape[1] == 3 and a.shape[4] == 4
+        check(self.L.vrgi_set_textures(self.h, C.c_uint32(a.shape[0]), C.c_uint32(a.shape[3]),
+                                       C.c_uint32(a.shape[2]), ptr(a)))
+
     def render(self, cam, nee_type=0, debug_view=0, spatial=0):
         p = Prefs(nee_type, debug_view, spatial)
         out = FrameOut()
