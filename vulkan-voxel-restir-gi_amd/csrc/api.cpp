@@ -152,7 +152,6 @@ struct vrgi_ctx {
         unsigned long long S = 0;
         vrgi_bvh_node lroot{};
     } geo_plan;
-    uint64_t geo_plan_gen = 0;    // counts the plans (graphs of replayed builds hold one)
     DevBuf<uint32_t> d_check;     // replayed builds whose counts left their plan (sticky)
     uint32_t *h_check = nullptr;  // pinned copy, read at the next replay
     hipEvent_t ev_check = nullptr;
@@ -187,15 +186,11 @@ struct vrgi_ctx {
         DevBuf<uint32_t> d_lanc, d_lanc_len;  // per light triangle: ancestors for the NEE pdf
         uint32_t geo_depth = 0, lgeo_depth = 0;
         DevBuf<uint8_t> d_lgrid;  // light grid of the lazy NEE pdf (SceneView::lgrid)
-        float lg_lo[3] = {0, 0, 0}, lg_inv = 1.0f, lg_margin = 0.0f;
+        float lg_lo[3] = {0, 0, 0}, lg_inv = 1.0f;
         uint32_t lg_dim[3] = {0, 0, 0};
         bool lg_on = false;
         hipEvent_t ev_free = nullptr;  // recorded after the last frame that read the set
         bool freed_once = false;
-        // every launch of a replayed build into this set, captured once per plan
-        hipGraph_t bgraph = nullptr;
-        hipGraphExec_t bgraph_exec = nullptr;
-        uint64_t bgraph_gen = 0;
     } geo[2];
     uint32_t gi = 0;  // the set the next frame reads
     // canonical nodes of the last build when a batched build made it (its combined array)
@@ -361,7 +356,7 @@ SceneView scene_view(vrgi_ctx *c) {
 // the light grid of the set just built, over the light-geometry root box `mn`..`mx`:
 // cells of side 1 (larger when the box would need more than 2^25 cells), every cell within
 // 1 + one side of a light triangle's box marked
-void light_grid_layout(vrgi_ctx *c, const float mn[3], const float mx[3]) {
+void build_light_grid(vrgi_ctx *c, hipStream_t s, const float mn[3], const float mx[3]) {
     auto &G = c->G();
     G.lg_on = false;
     if (!c->n_ltri) return;
@@ -371,7 +366,7 @@ void light_grid_layout(vrgi_ctx *c, const float mn[3], const float mx[3]) {
         double cells = 1.0;
         for (int k = 0; k < 3; ++k) cells *= std::ceil(((double)mx[k] - mn[k] + 2.0 * m + 2.0 * side) / side) + 1.0;
         if (!(cells <= (double)(1u << 25))) { side *= 1.5; continue; }
-        G.lg_margin = (float)m;
+        const float margin = (float)m;
         for (int k = 0; k < 3; ++k) {
             G.lg_lo[k] = (float)(mn[k] - m - side);
             G.lg_dim[k] = (uint32_t)std::ceil(((double)mx[k] - mn[k] + 2.0 * m + 2.0 * side) / side) + 1u;
@@ -379,13 +374,10 @@ void light_grid_layout(vrgi_ctx *c, const float mn[3], const float mx[3]) {
         G.lg_inv = (float)(1.0 / side);
         const size_t n = (size_t)G.lg_dim[0] * G.lg_dim[1] * G.lg_dim[2];
         G.d_lgrid.reserve((n + 3) / 4 * 4);
+        launch_light_grid(s, G.d_ltri.p, c->n_ltri, G.d_lgrid.p, G.lg_lo, G.lg_inv, G.lg_dim, margin);
         G.lg_on = true;
         return;
     }
-}
-void light_grid_launch(vrgi_ctx *c, hipStream_t s) {
-    auto &G = c->G();
-    if (G.lg_on) launch_light_grid(s, G.d_ltri.p, c->n_ltri, G.d_lgrid.p, G.lg_lo, G.lg_inv, G.lg_dim, G.lg_margin);
 }
 
 NrcNet cache_net(const vrgi_ctx *c) {
@@ -736,20 +728,25 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s, bool replay) {
         c->G().d_tri_uv.reserve((size_t)3 * nt);
         f.tri_uv = c->G().d_tri_uv.p;
     }
+    launch_flatten(s, f);
     FlattenArgs fl = f;
     fl.n_tris = nl;
     fl.tri = c->G().d_ltri.p;
     fl.tri_inst = c->G().d_ltri_inst.p;
     fl.light_node = c->G().d_ltri_node.p;
     fl.prim_off = nt;
+    launch_flatten_lights(s, fl);
     c->G().d_lanc.reserve((size_t)nl * VRGI_LANC_MAX);
     c->G().d_lanc_len.reserve(nl);
+    launch_light_ancestry(s, c->G().d_ltri_inst.p, c->G().d_ltri_node.p, c->d_inst.p, c->d_tl.p, nl, c->G().d_lanc.p,
+                          c->G().d_lanc_len.p);
     uint32_t trees[6];
     uint32_t nt_trees = 0;
     uint32_t beg[2], cnt[2], base[2];
     if (nt) { beg[nt_trees] = 0; cnt[nt_trees] = nt; base[nt_trees] = 0; ++nt_trees; }
     if (nl) { beg[nt_trees] = nt; cnt[nt_trees] = nl; base[nt_trees] = gnodes; ++nt_trees; }
     for (uint32_t t = 0; t < nt_trees; ++t) { trees[t] = beg[t]; trees[2 + t] = cnt[t]; trees[4 + t] = base[t]; }
+    if (!rep) HIP_CHECK(hipMemcpyAsync(c->d_trees.p, trees, 24, hipMemcpyHostToDevice, s));  // else unchanged
     if (rep && !c->h_check) {
         c->d_check.reserve(1);
         HIP_CHECK(hipMemsetAsync(c->d_check.p, 0, 4, s));
@@ -757,66 +754,21 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s, bool replay) {
         *c->h_check = 0;
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_check, hipEventDisableTiming));
     }
+    if (c->geo_builder.build(s, c->d_prim9.p, c->d_power.p, c->d_ids.p, N, nt_trees, c->d_trees.p,
+                             c->d_trees.p + 2, c->d_trees.p + 4, base, c->G().d_nodes.p, c->G().d_depth.p,
+                             nullptr, rep, rep ? c->d_check.p : nullptr) != 0)
+        throw HipError("geometry build exceeded its workspace");
     uint32_t dep[2] = {0, 0};
     unsigned long long S = 0;
     vrgi_bvh_node lroot{};
-    if (rep) {  // the plan's results, known before any launch
+    if (rep) {
         std::memcpy(dep, c->geo_plan.dep, sizeof(dep));
         S = c->geo_plan.S;
         lroot = c->geo_plan.lroot;
-        light_grid_layout(c, lroot.min, lroot.max);
-    }
-    auto stack_check = [&]() {
-        uint32_t t = 0;
-        if (nt) c->G().geo_depth = dep[t++];
-        if (nl) c->G().lgeo_depth = dep[t++];
-        if (geom_stack_need(c->G().geo_depth) + 2 > VRGI_MAX_STACK || 3 * ((c->G().lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
-            throw std::runtime_error("BVH deeper than the traversal stack (geometry depth " + std::to_string(c->G().geo_depth) + ", light-geometry depth " + std::to_string(c->G().lgeo_depth) + ")");
-    };
-    auto wide = [&]() {
-        launch_wide(s, c->G().d_nodes.p, gnodes, c->G().d_gwide.p);
-        if (VRGI_GEOM4) launch_wide4(s, c->G().d_nodes.p, gnodes, c->G().d_gwide4.p);
-        launch_wide(s, c->G().d_nodes.p + gnodes, lnodes, c->G().d_lwide.p);
-        launch_wide4(s, c->G().d_nodes.p + gnodes, lnodes, c->G().d_lwide4.p);
-    };
-    // flatten, light ancestry, the build (and, replayed, the light grid and traversal copies)
-    auto launches = [&]() {
-        launch_flatten(s, f);
-        launch_flatten_lights(s, fl);
-        launch_light_ancestry(s, c->G().d_ltri_inst.p, c->G().d_ltri_node.p, c->d_inst.p, c->d_tl.p, nl,
-                              c->G().d_lanc.p, c->G().d_lanc_len.p);
-        if (!rep) HIP_CHECK(hipMemcpyAsync(c->d_trees.p, trees, 24, hipMemcpyHostToDevice, s));  // else unchanged
-        if (c->geo_builder.build(s, c->d_prim9.p, c->d_power.p, c->d_ids.p, N, nt_trees, c->d_trees.p,
-                                 c->d_trees.p + 2, c->d_trees.p + 4, base, c->G().d_nodes.p, c->G().d_depth.p,
-                                 nullptr, rep, rep ? c->d_check.p : nullptr) != 0)
-            throw HipError("geometry build exceeded its workspace");
-        if (rep) {
-            light_grid_launch(c, s);
-            wide();
-        }
-    };
-    if (rep) {
-        // a replayed build is the same launch sequence every frame: one graph per set and
-        // plan, queued with one host call
-        stack_check();
-        auto &G = c->G();
-        if (!G.bgraph_exec || G.bgraph_gen != c->geo_plan_gen) {
-            if (G.bgraph_exec) HIP_CHECK(hipGraphExecDestroy(G.bgraph_exec));
-            if (G.bgraph) HIP_CHECK(hipGraphDestroy(G.bgraph));
-            G.bgraph_exec = nullptr;
-            G.bgraph = nullptr;
-            HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            launches();
-            HIP_CHECK(hipStreamEndCapture(s, &G.bgraph));
-            HIP_CHECK(hipGraphInstantiate(&G.bgraph_exec, G.bgraph, nullptr, nullptr, 0));
-            G.bgraph_gen = c->geo_plan_gen;
-        }
-        HIP_CHECK(hipGraphLaunch(G.bgraph_exec, s));
         HIP_CHECK(hipMemcpyAsync(c->h_check, c->d_check.p, 4, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipEventRecord(c->ev_check, s));
         c->check_pending = true;
     } else {
-        launches();
         HIP_CHECK(hipMemcpyAsync(dep, c->G().d_depth.p, nt_trees * 4, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipMemcpyAsync(&S, c->geo_builder.leaf_depth_sum(), 8, hipMemcpyDeviceToHost, s));
         if (nl) HIP_CHECK(hipMemcpyAsync(&lroot, c->G().d_nodes.p + gnodes, sizeof(lroot), hipMemcpyDeviceToHost, s));
@@ -825,16 +777,21 @@ void build_geometry_on(vrgi_ctx *c, hipStream_t s, bool replay) {
         c->geo_plan.S = S;
         c->geo_plan.lroot = lroot;
         c->geo_plan.valid = true;
-        ++c->geo_plan_gen;
-        light_grid_layout(c, lroot.min, lroot.max);
-        light_grid_launch(c, s);
-        stack_check();
-        wide();
     }
+    build_light_grid(c, s, lroot.min, lroot.max);
     // structural figures of the build (profile output)
     c->prof.build_leaf_depth_sum = S;
     c->prof.build_prims = (uint64_t)nt + nl;
     c->prof.build_nodes = (uint64_t)gnodes + lnodes;
+    uint32_t t = 0;
+    if (nt) c->G().geo_depth = dep[t++];
+    if (nl) c->G().lgeo_depth = dep[t++];
+    if (geom_stack_need(c->G().geo_depth) + 2 > VRGI_MAX_STACK || 3 * ((c->G().lgeo_depth + 1) / 2) + 2 > VRGI_MAX_STACK)
+        throw std::runtime_error("BVH deeper than the traversal stack (geometry depth " + std::to_string(c->G().geo_depth) + ", light-geometry depth " + std::to_string(c->G().lgeo_depth) + ")");
+    launch_wide(s, c->G().d_nodes.p, gnodes, c->G().d_gwide.p);
+    if (VRGI_GEOM4) launch_wide4(s, c->G().d_nodes.p, gnodes, c->G().d_gwide4.p);
+    launch_wide(s, c->G().d_nodes.p + gnodes, lnodes, c->G().d_lwide.p);
+    launch_wide4(s, c->G().d_nodes.p + gnodes, lnodes, c->G().d_lwide4.p);
 }
 
 void free_object(vrgi_object *o) {
@@ -1129,10 +1086,6 @@ vrgi_ctx::~vrgi_ctx() {
         for (hipEvent_t e : events[k]) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k)
         if (geo[k].ev_free) (void)hipEventDestroy(geo[k].ev_free);
-    for (int k = 0; k < 2; ++k) {
-        if (geo[k].bgraph_exec) (void)hipGraphExecDestroy(geo[k].bgraph_exec);
-        if (geo[k].bgraph) (void)hipGraphDestroy(geo[k].bgraph);
-    }
     if (ev_geo_ready) (void)hipEventDestroy(ev_geo_ready);
     if (h_qcount) (void)hipHostFree(h_qcount);
     if (h_check) (void)hipHostFree(h_check);
