@@ -67,13 +67,17 @@ def light_structure_bytes(n_ltri):
     return n_ltri * (64 + 48 + 8 + 80 + 128)
 
 
+NEE_LAZY = "k_nee_select + k_nee_near + k_nee_listed"
+
+
 def kernel_rooflines(prof, touch_bytes, n_ltri, B):
     """Per-launch algorithmic bytes and average launch time of the per-ray kernels.
     k_trace_shade (one launch per bounce): (48 + 150) B per ray -- ray in 24-32 B and hit
     out 16 B, shading outputs and material / triangle reads 150 B -- plus the unique node
     and triangle bytes the launch touches (counted on the GPU).
-    NEE pdf of a bounce (the lazy form: k_nee_select over the bounce's queue, then
-    k_nee_listed over the rays it lists): 76 B per queued ray (queue id, MIS weight,
+    NEE pdf of a bounce (the lazy form: k_nee_select over the bounce's queue, k_nee_near over
+    the rays whose pdf matters only through a light crossing near the origin, then
+    k_nee_listed over the rays listed for the full pdf): 76 B per queued ray (queue id, MIS weight,
     direction, next-vertex radiance, reflectivity, normal, origin; pdf 0 or list entry out)
     plus 52 B per listed ray (list entry, queue id, MIS weight, direction, origin, normal,
     pdf out) plus the light structures; with every pdf computed (--full-nee, one k_nee_pdf
@@ -94,7 +98,7 @@ def kernel_rooflines(prof, touch_bytes, n_ltri, B):
         nee["listed_fraction"] = round(sum(listed) / max(sum(rays[1:]), 1), 4)
     return {
         "k_trace_shade": {"launches": B, "bytes": sum(trace_bytes) / B, "ms": sum(trace_ms) / B},
-        ("k_nee_select + k_nee_listed" if lazy else "k_nee_pdf"): nee,
+        (NEE_LAZY if lazy else "k_nee_pdf"): nee,
     }
 
 
@@ -154,7 +158,7 @@ def cpu_baseline(objs, mats, lum, cams):
                       f"({t_frame:.2f} s, {threads} OpenMP threads)"}
 
 
-def batched_bench(steps):
+def batched_bench(steps, no_graph=False):
     """The 64-scene batched workload: ms per batched frame (2 passes of every scene, BVH
     rebuilds included), closest-hit Mrays/s, and the same 2 frames of every scene through its
     own context (64 render calls per pass)."""
@@ -182,7 +186,7 @@ def batched_bench(steps):
         for c in ctxs:
             c.sync()
         seq_ms = 1000.0 * (time.perf_counter() - t0)
-        with vrgi.Batch(ctxs, spp=2) as bt:
+        with vrgi.Batch(ctxs, spp=2, flags=vrgi.BATCH_NO_GRAPH if no_graph else 0) as bt:
             log("batched workload: autotuning and warm-up")
             for _ in range(8):
                 bt.render(cams, nee_type=NEE, spatial=SPATIAL)
@@ -266,6 +270,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (pmc counter scaling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched workload")
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="launch the batched frames' kernels directly instead of one graph per batched frame "
+                         "(for rocprofv3 kernel traces, which end the process inside such a graph)")
     ap.add_argument("--no-large", action="store_true", help="skip the large dynamic workload")
     ap.add_argument("--serial-build", action="store_true",
                     help="one BVH buffer set built on the frame stream (default: frame f + 1 builds beside frame f)")
@@ -366,7 +373,7 @@ def main():
                                   probe_ref_spp=256, cache_kw=dict(query_vertex=0), log=log)
     batched = None
     if not a.no_batch:
-        batched = batched_bench(a.steps)
+        batched = batched_bench(a.steps, a.batch_no_graph)
     large = None
     if not a.no_large:
         large = large_dynamic_bench()

@@ -59,7 +59,7 @@ def _cmds(kind):
         return [("trace", ["--kernel-trace", "--stats"], py + [BENCH] + FRAME_ARGS + ["--serial-build", "--serial-nee"])]
     if kind == "batched":
         return [("trace", ["--kernel-trace", "--stats"],
-                 py + [BENCH, "--no-cpu-baseline", "--no-cache-study", "--no-large"])]
+                 py + [BENCH, "--no-cpu-baseline", "--no-cache-study", "--no-large", "--batch-no-graph"])]
     if kind == "cache":
         return [("trace", ["--kernel-trace", "--stats"], py + [SELF, "workload", "cache"])]
     if kind == "counters":
@@ -161,8 +161,8 @@ def _frame_summary(kind, out, title):
             lines.append(f"  bench {k}: " + ", ".join(f"{a} {b}" for a, b in v.items()))
     # per-ray kernels over the timed frames, by launch index
     B = 4
-    per_ray = [("k_trace_shade<32, false, false>", B), ("k_nee_select<32>", B - 1), ("k_nee_listed<32>", B - 1),
-               ("k_nee_pdf<32>", B - 1)]
+    per_ray = [("k_trace_shade<32, false, false>", B), ("k_nee_select", B - 1), ("k_nee_near<32>", B - 1),
+               ("k_nee_listed<32>", B - 1), ("k_nee_pdf<32>", B - 1)]
     lines.append(f"rocprof kernel trace over the {steps} timed frames (frames {warm}..{warm + steps - 1}):")
     for name, per in per_ray:
         ds = [(e - s) / 1e3 for n, s, e, _ in rows if n == name]
@@ -198,12 +198,16 @@ def _batched_summary(out):
              "frame.", "Made by (files of this repository only):",
              "  python3 vulkan-voxel-restir-gi_amd/profile_summary.py collect batched OUT; "
              "python3 vulkan-voxel-restir-gi_amd/profile_summary.py summary batched OUT",
-             "which runs, from /tmp with TMPDIR=/tmp:", "  " + _cmdline(args, prog)]
+             "which runs, from /tmp with TMPDIR=/tmp:", "  " + _cmdline(args, prog),
+             "The batched frames' kernels are launched directly here (--batch-no-graph): under rocprofv3's kernel "
+             "trace the process ends inside the graph of a batched frame (builds and passes, ~1500 nodes); the "
+             "bench itself runs one graph per batched frame."]
     if bl and bl.get("batched"):
         b = bl["batched"]
+        parts = (f"last frame build {b['last_frame_build_ms']} ms, passes {b['last_frame_passes_ms']} ms"
+                 if "last_frame_build_ms" in b else f"last frame {b.get('last_frame_ms')} ms")
         lines.append(f"The same run's bench line: {b['ms_per_batched_frame']} ms per batched frame, "
-                     f"{b['mrays_per_s']} Mrays/s, last frame build {b['last_frame_build_ms']} ms, passes "
-                     f"{b['last_frame_passes_ms']} ms; the same frames through 64 contexts "
+                     f"{b['mrays_per_s']} Mrays/s, {parts}; the same frames through 64 contexts "
                      f"{b['scenes_separately_ms']} ms")
     if bi:
         w = rows[bi[0]:bi[-1] + 1]
@@ -295,7 +299,7 @@ def _counters_summary(out, traffic_file=None):
                      f"{f(r['occupancy'], '%6.1f')} {r['fetch'] / 1e6:9.2f} {r['write'] / 1e6:9.2f}")
     # the per-ray kernels against their algorithmic bytes
     ts = res.get("k_trace_shade<32, false, false>")
-    sel, lst = res.get("k_nee_select<32>"), res.get("k_nee_listed<32>")
+    sel, near, lst = res.get("k_nee_select"), res.get("k_nee_near<32>"), res.get("k_nee_listed<32>")
     traffic = {"_scale": {"fetch_bytes_per_KiB_reading": round(1024 * scale["FETCH_SIZE"], 2),
                           "write_bytes_per_KiB_reading": round(1024 * scale["WRITE_SIZE"], 2),
                           "calibration": "plain 256 MiB copy (bench.py --calibrate-copy)",
@@ -308,13 +312,14 @@ def _counters_summary(out, traffic_file=None):
         a = alg.get("k_trace_shade")
         lines.append(f"k_trace_shade: L2-miss traffic {t / 1e6:.1f} MB per launch" +
                      (f" against {a / 1e6:.1f} MB algorithmic ({t / a:.2f}x)" if a else ""))
-    if sel and lst:
-        t = sel["fetch"] + sel["write"] + lst["fetch"] + lst["write"]
-        traffic["k_nee_select + k_nee_listed"] = {"l2_miss_bytes_per_launch": int(t),
-                                                  "select_bytes": int(sel["fetch"] + sel["write"]),
-                                                  "listed_bytes": int(lst["fetch"] + lst["write"])}
-        a = alg.get("k_nee_select + k_nee_listed")
-        lines.append(f"k_nee_select + k_nee_listed (one NEE pdf of a bounce): L2-miss traffic {t / 1e6:.1f} MB" +
+    if sel and near and lst:
+        by = lambda r: r["fetch"] + r["write"]
+        t = by(sel) + by(near) + by(lst)
+        key = "k_nee_select + k_nee_near + k_nee_listed"
+        traffic[key] = {"l2_miss_bytes_per_launch": int(t), "select_bytes": int(by(sel)),
+                        "near_bytes": int(by(near)), "listed_bytes": int(by(lst))}
+        a = alg.get(key)
+        lines.append(f"{key} (one lazy NEE pdf of a bounce): L2-miss traffic {t / 1e6:.1f} MB" +
                      (f" against {a / 1e6:.1f} MB algorithmic ({t / a:.2f}x)" if a else ""))
     if traffic_file:
         with open(traffic_file, "w") as fo:
