@@ -465,7 +465,6 @@ __device__ __forceinline__ IcS local_cs(f3 a, f3 b, f3 c) {  // raytrace.rs:319-
     return r;
 }
 
-// one wave-sized slice of the bounce's rays (all 64 lanes call it: compaction ballots)
 // texel of normalised texture coordinate c over n texels: nearest, clamp to edge
 // (the reference's default sampler, interactive_rendering.rs:744)
 __device__ __forceinline__ uint32_t texel_index(float c, uint32_t n) {
@@ -477,6 +476,7 @@ __device__ __forceinline__ bool full_path(const BounceArgs &ba, uint32_t px) {
     return full_path_px(ba.train_seed, ba.train_frac, ba.fallback, px, ba.W, ba.tile, ba.tiles_x);
 }
 
+// one wave-sized slice of the bounce's rays (all 64 lanes call it: compaction ballots)
 template <int STACK, bool COUNT, bool CACHE>
 __device__ __forceinline__ void trace_shade_slice(const SceneView &sv, const BounceArgs &ba, uint32_t *stack,
                                                   uint32_t i, uint32_t count) {
