@@ -57,13 +57,17 @@ __device__ __forceinline__ void det_sincos(float xf, float *s, float *c) {
     const double q = floor(x * two_over_pi + 0.5);
     const double r = (x - q * pio2_hi) - q * pio2_lo;
     const double r2 = r * r;
+    // nested Horner on the reciprocals of 2k(2k+1) (sine) and (2k-1)2k (cosine), each
+    // rounded once at compile time (multiplications: no binary64 division in the loop)
+    constexpr double sinv[9] = {0.0, 1.0 / 6, 1.0 / 20, 1.0 / 42, 1.0 / 72, 1.0 / 110, 1.0 / 156, 1.0 / 210, 1.0 / 272};
+    constexpr double cinv[9] = {0.0, 1.0 / 2, 1.0 / 12, 1.0 / 30, 1.0 / 56, 1.0 / 90, 1.0 / 132, 1.0 / 182, 1.0 / 240};
     double ps = 1.0;
 #pragma unroll
-    for (int k = 8; k >= 1; --k) ps = 1.0 - r2 / (double)((2 * k) * (2 * k + 1)) * ps;
+    for (int k = 8; k >= 1; --k) ps = 1.0 - r2 * sinv[k] * ps;
     const double sr = r * ps;
     double pc = 1.0;
 #pragma unroll
-    for (int k = 8; k >= 1; --k) pc = 1.0 - r2 / (double)((2 * k - 1) * (2 * k)) * pc;
+    for (int k = 8; k >= 1; --k) pc = 1.0 - r2 * cinv[k] * pc;
     const double cr = pc;
     long qi = (long)q;
     int quad = (int)(((qi % 4) + 4) % 4);
