@@ -287,8 +287,8 @@ def main():
         calibrate_copy()
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the frame would shard by "
-              "image rows / scenes; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (a frame would shard by "
+              "image rows, a batch by scenes). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import vrgi
     log("generating the 256x128x256 world and meshing its chunks")

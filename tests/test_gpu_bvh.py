@@ -168,6 +168,16 @@ def test_full_size_invariants():
         c.add_objects(objs)
         c.trace_closest(np.zeros((1, 3), np.float32), np.ones((1, 3), np.float32))
         nodes = c.read(vrgi.BUF_GEOM_BVH, vrgi.NODE_DTYPE)
+        depth = c.stats().bvh_depth
+    # the depth the traversal stacks are sized by is the deepest leaf's (root 0)
+    level, front = 0, np.array([0])
+    while True:
+        inner_f = front[nodes["left"][front] != 0xFFFFFFFF]
+        if not len(inner_f):
+            break
+        front = np.concatenate([nodes["left"][inner_f], nodes["right"][inner_f]]).astype(np.int64)
+        level += 1
+    assert depth == level, (depth, level)
     n = (len(nodes) + 1) // 2
     tris = np.concatenate([v["position"].reshape(-1, 9) + np.tile(xf[9:], 3) for _, xf, v in objs if len(v)])
     assert len(tris) == n
