@@ -7,6 +7,9 @@
 namespace vrgi {
 
 #define VRGI_MAX_STACK 160  // traversal stack entries per lane (LDS); deeper trees are rejected
+#ifndef VRGI_PEND_LDS
+#define VRGI_PEND_LDS 1  // closest hit: pending triangles in the lane's top stack entries
+#endif
 #define VRGI_SMALL_QUEUE_GRID 2048  // blocks of a striding ray launch over a queue expected to be short
 #ifndef VRGI_CACHE_SPLIT_B0
 #define VRGI_CACHE_SPLIT_B0 1  // cache frames querying at vertex 0 split bounce 0 (BounceArgs::b0_part)
@@ -299,20 +302,31 @@ __device__ __forceinline__ Hit closest_hit2(const SceneView &sv, const RayPre &r
     uint32_t ref = 0;
     int sp = 0;
     bool walk = true;  // ref is an internal node still to visit
-    uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
     int np = 0;
+#if VRGI_PEND_LDS
+    // pending triangles in the lane's top stack entries (STACK - 1 down), below which the
+    // node stack (at most depth - 1 entries) never reaches
+    uint32_t *const pend = stk + (STACK - 1) * 64;
+    auto keep = [&](uint32_t prim) { pend[-64 * np++] = prim; };
+#else
+    uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
     auto keep = [&](uint32_t prim) {  // a shift register: no indexed private array
         p3 = p2; p2 = p1; p1 = p0; p0 = prim;
         ++np;
     };
+#endif
     for (;;) {
         // wave-uniform choice: test when no walking lane is without a pending triangle
         if (!__any(walk && np == 0)) {
             if (!__any(np > 0)) break;
 #pragma unroll 1
             for (; np > 0; --np) {
+#if VRGI_PEND_LDS
+                test(pend[-64 * (np - 1)]);
+#else
                 test(p0);
                 p0 = p1; p1 = p2; p2 = p3;
+#endif
             }
             continue;
         }
@@ -424,7 +438,11 @@ __device__ __forceinline__ Hit closest_hit(const SceneView &sv, const RayPre &r,
 #endif
 }
 // stack entries a closest-hit walk needs for a tree of binary depth d
-__host__ __device__ inline uint32_t geom_stack_need(uint32_t d) { return VRGI_GEOM4 ? 3 * ((d + 1) / 2) : d; }
+// LDS stack entries a closest-hit walk of a tree of depth d needs: the binary walk holds
+// at most d - 1 nodes and, with VRGI_PEND_LDS, up to 4 pending triangles at the top
+__host__ __device__ inline uint32_t geom_stack_need(uint32_t d) {
+    return VRGI_GEOM4 ? 3 * ((d + 1) / 2) : (VRGI_PEND_LDS ? d + 3 : d);
+}
 // bytes of one traversal node of the geometry tree (touch counting)
 #define VRGI_GEOM_NODE_BYTES (VRGI_GEOM4 ? 128ull : 64ull)
 
