@@ -1282,7 +1282,18 @@ Builder::~Builder() { free_all(); }
 static constexpr int MID_MAX = CHUNK;  // a node of the middle phase is one top-phase item
 static constexpr int MID_THREADS = 256;
 static constexpr int MID_LIST = MID_MAX / 129 + 1;  // ranges > SUB_MAX of one level are disjoint
-static constexpr int MID_BATCH = 8;                 // ranges binned together (LDS for their bins)
+// ranges binned together (LDS for their bins), and the node's subtree jobs staged in LDS
+// (the rest are appended to the global list one by one): with the partition scratch in
+// the bin area, a workgroup holds about 49 KB of LDS, three per CU
+#ifndef VRGI_MID_BATCH
+#define VRGI_MID_BATCH 4
+#endif
+static constexpr int MID_BATCH = VRGI_MID_BATCH;
+#ifndef VRGI_MID_JOBS_LDS
+#define VRGI_MID_JOBS_LDS 128
+#endif
+static constexpr int MID_JOBS_LDS = VRGI_MID_JOBS_LDS;
+static constexpr int MID_SCRATCH = MID_BATCH * BIN_WORDS > MID_MAX ? MID_BATCH * BIN_WORDS : MID_MAX;
 static_assert(SUB_MAX_DEFAULT == 128, "MID_LIST counts ranges of more than 128 primitives");
 static_assert(MID_MAX <= 32767 && MID_LIST < 255, "16-bit positions, 8-bit range ids");
 static_assert(MID_MAX % MID_THREADS == 0, "every thread owns MID_MAX / MID_THREADS positions");
@@ -1313,9 +1324,13 @@ __global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) 
     __shared__ uint32_t s_perm[MID_MAX];
     __shared__ float s_c[3][MID_MAX];       // centroids, permuted with s_perm
     __shared__ uint8_t s_rid[MID_MAX];      // range of each position at this level (0xFF: none)
-    __shared__ uint16_t s_cj[MID_MAX];      // exclusive count of trues over the node | predicate bit 15
-    __shared__ uint16_t s_pair[MID_MAX];    // right-region trues of a range, by rank from the right
-    __shared__ int s_bins[MID_BATCH][BIN_WORDS];
+    // bins of MID_BATCH ranges while binning and sweeping; then the partition's
+    // exclusive counts of trues (| predicate bit 15) and its right-region trues of a
+    // range, by rank from the right
+    __shared__ int s_scratch[MID_SCRATCH];
+    int (*const s_bins)[BIN_WORDS] = reinterpret_cast<int (*)[BIN_WORDS]>(s_scratch);
+    uint16_t *const s_cj = reinterpret_cast<uint16_t *>(s_scratch);
+    uint16_t *const s_pair = s_cj + MID_MAX;
     __shared__ int s_bnd[MID_LIST][6];      // centroid bound keys per range
     __shared__ float s_lo[MID_LIST][3], s_sc[MID_LIST][3];
     __shared__ uint32_t s_dim[MID_LIST], s_T[MID_LIST], s_P0[MID_LIST];
@@ -1324,10 +1339,10 @@ __global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) 
     __shared__ MidRange s_r[2][MID_LIST];
     __shared__ uint32_t s_n[2];
     __shared__ uint32_t s_wsum[MID_THREADS / 64];
-    // the node's subtree jobs, appended to the global list at the end with one counter
-    // update (a job holds >= 3 primitives: at most MID_MAX / 3 of them)
-    __shared__ uint4 s_job[MID_MAX / 3 + 1];  // first | count << 16, rank, node, parent
-    __shared__ uint16_t s_jdep[MID_MAX / 3 + 1];
+    // the node's first MID_JOBS_LDS subtree jobs, appended to the global list at the end
+    // with one counter update
+    __shared__ uint4 s_job[MID_JOBS_LDS];  // first | count << 16, rank, node, parent
+    __shared__ uint16_t s_jdep[MID_JOBS_LDS];
     __shared__ uint32_t s_nj, s_jbase;
     const OpenNode on = ((const OpenNode *)s.mid)[blockIdx.x];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -1541,8 +1556,13 @@ __global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) 
                     maxdepth = max(maxdepth, g.depth + 2);
                 } else if (cc <= (uint32_t)a.sub_max) {
                     const uint32_t jb = atomicAdd(&s_nj, 1u);
-                    s_job[jb] = make_uint4(cf | (cc << 16), cr, ci, g.node);
-                    s_jdep[jb] = (uint16_t)(g.depth + 1);
+                    if (jb < (uint32_t)MID_JOBS_LDS) {
+                        s_job[jb] = make_uint4(cf | (cc << 16), cr, ci, g.node);
+                        s_jdep[jb] = (uint16_t)(g.depth + 1);
+                    } else {
+                        ((OpenNode *)s.jobs)[atomicAdd(s.n_jobs, 1u)] =
+                            OpenNode{on.first + cf, cc, cr, ci, g.node, on.tree, g.depth + 1, 0};
+                    }
                 } else {
                     const uint32_t q = atomicAdd(&s_n[cur ^ 1u], 1u);
                     s_r[cur ^ 1u][q] = MidRange{cf, cc, cr, ci, g.depth + 1, g.node};
@@ -1566,9 +1586,10 @@ __global__ __launch_bounds__(MID_THREADS) void k_mid(BuildArgs a, BuildState s) 
         a.perm[on.first + k] = s_perm[k];
         for (int q = 0; q < 3; ++q) a.pc[q * a.n + on.first + k] = s_c[q][k];
     }
-    if (tid == 0) s_jbase = s_nj ? atomicAdd(s.n_jobs, s_nj) : 0u;
+    const uint32_t nj = min(s_nj, (uint32_t)MID_JOBS_LDS);
+    if (tid == 0) s_jbase = nj ? atomicAdd(s.n_jobs, nj) : 0u;
     __syncthreads();
-    for (uint32_t k = tid; k < s_nj; k += MID_THREADS) {
+    for (uint32_t k = tid; k < nj; k += MID_THREADS) {
         const uint4 j = s_job[k];
         ((OpenNode *)s.jobs)[s_jbase + k] =
             OpenNode{on.first + (j.x & 0xFFFFu), j.x >> 16, j.y, j.z, j.w, on.tree, s_jdep[k], 0};
