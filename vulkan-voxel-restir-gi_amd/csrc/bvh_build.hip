@@ -1281,7 +1281,7 @@ Builder::~Builder() { free_all(); }
 #endif
 static constexpr int MID_MAX = CHUNK;  // a node of the middle phase is one top-phase item
 static constexpr int MID_THREADS = 256;
-static constexpr int MID_LIST = MID_MAX / 129 + 1;  // ranges > SUB_MAX of one level are disjoint
+static constexpr int MID_LIST = MID_MAX / (SUB_MAX_DEFAULT + 1) + 1;  // ranges > SUB_MAX of one level are disjoint
 // ranges binned together (LDS for their bins), and the node's subtree jobs staged in LDS
 // (the rest are appended to the global list one by one): with the partition scratch in
 // the bin area, a workgroup holds about 49 KB of LDS, three per CU
@@ -1294,7 +1294,6 @@ static constexpr int MID_BATCH = VRGI_MID_BATCH;
 #endif
 static constexpr int MID_JOBS_LDS = VRGI_MID_JOBS_LDS;
 static constexpr int MID_SCRATCH = MID_BATCH * BIN_WORDS > MID_MAX ? MID_BATCH * BIN_WORDS : MID_MAX;
-static_assert(SUB_MAX_DEFAULT == 128, "MID_LIST counts ranges of more than 128 primitives");
 static_assert(MID_MAX <= 32767 && MID_LIST < 255, "16-bit positions, 8-bit range ids");
 static_assert(MID_MAX % MID_THREADS == 0, "every thread owns MID_MAX / MID_THREADS positions");
 __device__ __forceinline__ bool mid_node(const BuildArgs &a, uint32_t count, uint32_t tree) {
