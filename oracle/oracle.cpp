@@ -6,7 +6,7 @@
 // Every float expression below is written in the evaluation order of the cited
 // Rust / GLSL line, one IEEE-754 binary32 rounding per operation.
 //
-// Parity pinning (what checks each function; details in DESIGN.md):
+// Parity pinning (what checks each function; the tests named in tests/test_oracle_*.py):
 //   RNG                      known answers (tests/golden/rng_known_answers.json)
 //   build (A1-A6)            invariants + brute-force plane search + golden node arrays
 //   partition order          "parity unpinned": `partition` 0.1.2 source absent
@@ -292,7 +292,7 @@ void find_best_plane(size_t first, size_t count, const std::vector<size_t> &idx,
 // `partition` 0.1.2 (build.rs:145): l/r scan with swaps; returns the split index.
 // Closed form: with T = #true, the i-th false in [0,T) (left to right) trades
 // places with the i-th true in [T,n) (right to left). PARITY UNPINNED: restated
-// from the crate's published algorithm; its source is not in /root/reference.
+// from the crate's published algorithm; its source is not among the reference's files.
 size_t partition_inplace(size_t *d, size_t len, const std::vector<float> &cen, size_t dim,
                          float split) {
     if (len == 0) return 0;

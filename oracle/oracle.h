@@ -15,8 +15,8 @@
 // binned-SAH BVH built with build_bl_bvh semantics; ties between equal t are broken
 // by the lower global triangle id.
 //
-// Parity status per function is stated in DESIGN.md's oracle table; the functions
-// whose arithmetic lives outside /root/reference (the `partition` 0.1.2 crate order,
+// Parity status per function is listed at the top of oracle.cpp; the functions whose
+// arithmetic lives outside the reference's sources (the `partition` 0.1.2 crate order,
 // Vulkan barycentrics, GLSL built-ins) are "parity unpinned" against the reference.
 #pragma once
 #include <cstddef>
