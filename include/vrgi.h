@@ -149,7 +149,8 @@ int vrgi_destroy(vrgi_ctx *ctx);
  * (raytrace.rs:437-481) with the reference's default sampler: nearest texel, clamp to
  * edge, colour channels through the sRGB transfer function, alpha / 255. n_slots must
  * cover the material table; rgba8 == NULL returns to the constant materials of
- * vrgi_create. Not with ReSTIR DI (restir_mode 1) nor for a scene in a batch. */
+ * vrgi_create. Not with ReSTIR DI (restir_mode 1), nor while the scene is in a batch
+ * (a textured scene may join one afterwards: its batched frames read the atlas too). */
 int vrgi_set_textures(vrgi_ctx *ctx, uint32_t n_slots, uint32_t width, uint32_t height, const uint8_t *rgba8);
 /* utils::get_texture_luminances (utils.rs:223-235) over such an atlas: per slot, the mean
  * over its emissivity image of r + g + b in 8-bit units (the texture_luminances of

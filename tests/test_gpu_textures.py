@@ -3,7 +3,8 @@ emissivity and metallicity images at the hit's uv (raytrace.rs:437-481) through 
 reference's default sampler. Frames of the 64^3 world with a random atlas are bit-exact
 against the oracle, with and without per-frame rebuilds and the learned cache's albedo;
 clearing the atlas restores the constant materials; the luminance helper equals
-utils.rs:223-235 as the oracle states it; ReSTIR DI and batches refuse an atlas."""
+utils.rs:223-235 as the oracle states it; ReSTIR DI refuses an atlas, and a textured
+scene inside a batch renders the oracle's frames."""
 import numpy as np
 import pytest
 
@@ -97,13 +98,45 @@ def test_texture_luminances_and_refusals(world0):
     with vrgi.Context(vrgi.config(32, 32, 3, mode=1), world0["mats"], world0["lum"]) as c:
         with pytest.raises(RuntimeError):
             c.set_textures(atlas)
-    # a textured scene cannot join a batch
     c1 = vrgi.Context(vrgi.config(32, 32, 3), world0["mats"], world0["lum"])
     try:
-        c1.set_textures(atlas)
-        with pytest.raises(RuntimeError):
-            vrgi.Batch([c1], spp=1)
         with pytest.raises(RuntimeError):  # too few slots for the material table
             c1.set_textures(atlas[:2])
+        with vrgi.Batch([c1], spp=1):
+            with pytest.raises(RuntimeError):  # not while the scene is in a batch
+                c1.set_textures(atlas)
     finally:
+        c1.close()
+
+
+def test_textured_scene_in_batch(world0):
+    """The textured 64^3 world in a batch beside an untextured scene: three batched frames
+    (one graph each once the geometry replays) equal the oracle's textured frames."""
+    import oracle
+    import vrgi
+    W, H = 96, 64
+    atlas = _atlas(len(world0["mats"]), 11, world0["lum"])
+    s = oracle.Scene(world0["mats"], world0["lum"])
+    s.add_objects(world0["objs"])
+    s.set_textures(atlas)
+    s.build()
+    r = oracle.Renderer(W, H, 3)
+    cam = _camera0()
+    c0 = vrgi.Context(vrgi.config(W, H, 3), world0["mats"], world0["lum"])
+    c1 = vrgi.Context(vrgi.config(48, 32, 3), world0["mats"], world0["lum"])
+    try:
+        c0.add_objects(world0["objs"])
+        c0.set_textures(atlas)
+        c1.add_objects(world0["objs"][: len(world0["objs"]) // 2])
+        with vrgi.Batch([c0, c1], spp=1, rays_per_block=64, pixels_per_block=256) as bt:
+            for f in range(3):
+                say(f"textured scene in a batch, frame {f}")
+                bt.render([cam, cam], nee_type=1, spatial=5)
+                bt.sync()
+                ref = r.render(s, cam, f, nee_type=1, spatial=5)
+                assert c0.read(vrgi.BUF_RADIANCE).tobytes() == ref["l_out"][0].tobytes()
+                assert c0.read(vrgi.BUF_RESTIR).tobytes() == ref["restir"].tobytes()
+                assert c0.read(vrgi.BUF_BGRA8, np.uint8).tobytes() == ref["bgra8"].tobytes()
+    finally:
+        c0.close()
         c1.close()

@@ -2845,6 +2845,7 @@ void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list, bool repla
         G.d_lwide4.reserve(p.nl > 1 ? p.nl - 1 : 1);
         G.d_lanc.reserve((size_t)p.nl * VRGI_LANC_MAX);
         G.d_lanc_len.reserve(p.nl);
+        if (c->tex_on) G.d_tri_uv.reserve((size_t)3 * p.nt);
         if (!launch) continue;
         FlattenArgs f{};
         f.inst = c->d_inst.p;
@@ -2860,6 +2861,7 @@ void batch_build(vrgi_batch *bt, const std::vector<vrgi_ctx *> &list, bool repla
         f.power = bt->power.p;
         f.ids = bt->ids.p;
         f.prim_off = p.off;
+        if (c->tex_on) f.tri_uv = G.d_tri_uv.p;
         launch_flatten(s, f);
         FlattenArgs fl = f;
         fl.n_tris = p.nl;
@@ -3018,7 +3020,6 @@ const char *batch_scene_ok(const vrgi_ctx *c, const vrgi_ctx *first) {
     if (c->cfg.ray_sort_bits) return "ray sorting is not available in a batch";
     if (c->cfg.flags & VRGI_FLAG_KEEP_BOUNCES) return "VRGI_FLAG_KEEP_BOUNCES is not available in a batch";
     if (c->accum_on) return "FP64 accumulation is not available in a batch";
-    if (c->tex_on) return "a texture atlas is not available in a batch";
     return nullptr;
 }
 
