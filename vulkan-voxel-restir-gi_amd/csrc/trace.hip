@@ -79,13 +79,12 @@ __global__ void k_flatten_lights(const FlattenArgs f) {
 }
 
 // traversal copy: internal node of preorder rank k (= (left - 1) / 2) -> WideNode[k]
+// one thread per internal node rank: the children of rank r are nodes 2r + 1 and 2r + 2
+// (build.rs's insert_blas_leaf_node numbering), read as one 80-byte pair
 __global__ void k_wide(const vrgi_bvh_node *nodes, uint32_t n_nodes, WideNode *wide) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_nodes) return;
-    vrgi_bvh_node nd = nodes[i];
-    if (nd.left == VRGI_INVALID) return;
-    uint32_t rank = (nd.left - 1) / 2;
-    vrgi_bvh_node c0 = nodes[nd.left], c1 = nodes[nd.right_or_prim];
+    const uint32_t rank = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rank >= (n_nodes - 1) / 2) return;
+    const vrgi_bvh_node c0 = nodes[2 * rank + 1], c1 = nodes[2 * rank + 2];
     WideNode w;
     const float e = VRGI_BOX_PAD;
     w.a = make_float4(c0.min[0] - e, c0.min[1] - e, c0.min[2] - e, c0.max[0] + e);
@@ -1372,7 +1371,7 @@ void launch_flatten_lights(hipStream_t s, const FlattenArgs &f) {
     if (f.n_tris) k_flatten_lights<<<(f.n_tris + 255) / 256, 256, 0, s>>>(f);
 }
 void launch_wide(hipStream_t s, const vrgi_bvh_node *nodes, uint32_t n_nodes, WideNode *wide) {
-    if (n_nodes > 1) k_wide<<<(n_nodes + 255) / 256, 256, 0, s>>>(nodes, n_nodes, wide);
+    if (n_nodes > 1) k_wide<<<((n_nodes - 1) / 2 + 255) / 256, 256, 0, s>>>(nodes, n_nodes, wide);
 }
 void launch_trace_closest(hipStream_t s, const SceneView &sv, const float *o, const float *d,
                           uint32_t n, float tmin, float tmax, uint32_t *hit, float *t, float *uv) {
