@@ -18,16 +18,34 @@ namespace vrgi {
 
 // ---------------------------------------------------------------- flattening
 // world-space triangles of every instance (instances in key order), builder inputs
+// Instance of entry g of a list that the instances fill in order from their first entries
+// first(k): the last k with first(k) <= g. One thread of the block binary-searches the
+// block's first entry; each thread then steps forward from there (a block's 256 entries
+// span one or two instances of a world's chunks), instead of every thread searching.
+template <class F>
+__device__ __forceinline__ uint32_t block_instance(uint32_t g, uint32_t n_inst, F first) {
+    __shared__ uint32_t s_lo;
+    if (threadIdx.x == 0) {
+        const uint32_t g0 = blockIdx.x * blockDim.x;
+        uint32_t lo = 0, hi = n_inst;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (first(mid) <= g0) lo = mid;
+            else hi = mid;
+        }
+        s_lo = lo;
+    }
+    __syncthreads();
+    uint32_t lo = s_lo;
+    while (lo + 1 < n_inst && first(lo + 1) <= g) ++lo;
+    return lo;
+}
+
 __global__ void k_flatten(const FlattenArgs f) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    // instance of triangle g: last instance with tri_off <= g
+    const uint32_t lo = block_instance(g, f.n_inst, [&](uint32_t k) { return f.inst[k].tri_off; });
     if (g >= f.n_tris) return;
-    // instance of triangle g: last instance with tri_off <= g (binary search)
-    uint32_t lo = 0, hi = f.n_inst;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) / 2;
-        if (f.inst[mid].tri_off <= g) lo = mid;
-        else hi = mid;
-    }
     const DevInst &in = f.inst[lo];
     const vrgi_vertex3d *v = f.vtx[lo] + 3 * (g - in.tri_off);
     f3 w[3];
@@ -51,13 +69,8 @@ __global__ void k_flatten(const FlattenArgs f) {
 // world-space light triangles from the instances' light vertices
 __global__ void k_flatten_lights(const FlattenArgs f) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lo = block_instance(g, f.n_inst, [&](uint32_t k) { return f.light_off[k]; });
     if (g >= f.n_tris) return;
-    uint32_t lo = 0, hi = f.n_inst;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) / 2;
-        if (f.light_off[mid] <= g) lo = mid;
-        else hi = mid;
-    }
     const DevInst &in = f.inst[lo];
     uint32_t j = g - f.light_off[lo];
     const vrgi_light_vertex3d *v = f.lvtx[lo] + 3 * j;
