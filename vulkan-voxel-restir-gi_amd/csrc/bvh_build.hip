@@ -639,14 +639,27 @@ __global__ __launch_bounds__(TOP_THREADS) void k_part_b(BuildArgs a, BuildState 
     SplitInfo si = ((const SplitInfo *)s.split)[it.open];
     if (si.fallback) continue;
     const OpenNode on = ((const OpenNode *)s.open)[it.open];
-    for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
-        uint32_t i = it.beg + k;
-        uint32_t j = i - on.first;
-        if (j >= si.T) continue;
-        uint32_t rt = s.rank_tmp[i];
-        if (rt & 0x80000000u) continue;
-        uint32_t r = j - rt;
-        uint32_t p = s.pair[on.first + r];
+    // the thread's ranks, then the partners of its left-region falses, are fetched for
+    // all of its (at most PER_THREAD) positions at once: the swaps below then wait on one
+    // dependent load each instead of three
+    uint32_t part[PER_THREAD];
+#pragma unroll
+    for (int q = 0; q < PER_THREAD; ++q) {
+        const uint32_t k = (uint32_t)q * TOP_THREADS + threadIdx.x;
+        const uint32_t i = it.beg + k;
+        part[q] = k < it.len && i - on.first < si.T ? s.rank_tmp[i] : 0x80000000u;
+    }
+#pragma unroll
+    for (int q = 0; q < PER_THREAD; ++q) {
+        const uint32_t k = (uint32_t)q * TOP_THREADS + threadIdx.x;
+        const uint32_t j = it.beg + k - on.first;
+        part[q] = (part[q] & 0x80000000u) ? 0xFFFFFFFFu : s.pair[on.first + (j - part[q])];
+    }
+#pragma unroll
+    for (int q = 0; q < PER_THREAD; ++q) {
+        const uint32_t i = it.beg + (uint32_t)q * TOP_THREADS + threadIdx.x;
+        const uint32_t p = part[q];
+        if (p == 0xFFFFFFFFu) continue;
         uint32_t t0 = a.perm[i];
         a.perm[i] = a.perm[p];
         a.perm[p] = t0;
