@@ -285,8 +285,11 @@ __global__ __launch_bounds__(TOP_THREADS) void k_bounds(BuildArgs a, BuildState 
   for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     Item it = ((const Item *)s.items)[item];
     int mn[3] = {KEY_MIN_ID, KEY_MIN_ID, KEY_MIN_ID}, mx[3] = {KEY_MAX_ID, KEY_MAX_ID, KEY_MAX_ID};
-    for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS) {
-        uint32_t i = it.beg + k;
+    // all of the thread's (at most PER_THREAD) positions loaded at once: positions past
+    // the item read its last element, which the item holds anyway
+#pragma unroll
+    for (int q = 0; q < PER_THREAD; ++q) {
+        const uint32_t i = it.beg + min((uint32_t)q * TOP_THREADS + threadIdx.x, it.len - 1u);
         for (int d = 0; d < 3; ++d) {
             int key = fkey(a.pc[d * a.n + i]);
             mn[d] = min(mn[d], key);
@@ -369,14 +372,22 @@ __global__ __launch_bounds__(TOP_THREADS) void k_bin(BuildArgs a, BuildState s) 
     __syncthreads();
     int *mine_copy = sb + (lane % (uint32_t)nrep) * BIN_STRIDE;
     const uint32_t rounds = (it.len + TOP_THREADS - 1) / TOP_THREADS;  // uniform over the block
-    for (uint32_t r = 0; r < rounds; ++r) {
+    // the thread's primitive ids for all its rounds first (one latency), so that each
+    // round waits only on its box gather
+    uint32_t pp[PER_THREAD];
+#pragma unroll
+    for (int q = 0; q < PER_THREAD; ++q)
+        pp[q] = a.perm[it.beg + min((uint32_t)q * TOP_THREADS + threadIdx.x, it.len - 1u)];
+#pragma unroll
+    for (uint32_t r = 0; r < (uint32_t)PER_THREAD; ++r) {
+        if (r >= rounds) break;
         const uint32_t k = r * TOP_THREADS + threadIdx.x;
         const bool valid = k < it.len;
         int bk[6] = {KEY_MIN_ID, KEY_MIN_ID, KEY_MIN_ID, KEY_MAX_ID, KEY_MAX_ID, KEY_MAX_ID};
         uint32_t bb[3] = {0u, 0u, 0u};
         if (valid) {
             const uint32_t i = it.beg + k;
-            const uint32_t p = a.perm[i];
+            const uint32_t p = pp[r];
             const float4 b0 = a.box8[2 * p], b1 = a.box8[2 * p + 1];
             bk[0] = fkey(b0.x); bk[1] = fkey(b0.y); bk[2] = fkey(b0.z);
             bk[3] = fkey(b0.w); bk[4] = fkey(b1.x); bk[5] = fkey(b1.y);
