@@ -547,8 +547,12 @@ __global__ __launch_bounds__(TOP_THREADS) void k_count(BuildArgs a, BuildState s
         si = ((const SplitInfo *)s.split)[it.open];
     }
     uint32_t c = 0;
-    for (uint32_t k = threadIdx.x; k < it.len; k += TOP_THREADS)
-        c += a.pc[si.dim * a.n + it.beg + k] < si.split ? 1u : 0u;
+#pragma unroll
+    for (int q = 0; q < PER_THREAD; ++q) {  // all of the thread's positions loaded at once
+        const uint32_t k = (uint32_t)q * TOP_THREADS + threadIdx.x;
+        const float v = a.pc[si.dim * a.n + it.beg + min(k, it.len - 1u)];
+        c += k < it.len && v < si.split ? 1u : 0u;
+    }
     c = wave_sum(c);
     if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = c;
     __syncthreads();
